@@ -1,0 +1,213 @@
+"""Benchmark: one FP64 advection pass of MAESTROeX's hot path on one MI355X.
+
+Workload (BASELINE configuration "3-D 256^3 as 8 boxes of 128^3"): 256^3 cells
+as 8 boxes of 128^3 with 4 ghost cells, 5 advected scalars (rho', (rho h)',
+3 mass fractions) plus 3 velocity components, periodic in x and y, outflow in
+z, PPM type 1 (or --ppm 2).  One step is one pass of the path over the whole
+level with inputs resident in HBM:
+    MakeUtrans -> Addw0 -> VelPred -> Addw0 -> MakeEdgeScal (5 scalars)
+      -> MakeRhoXFlux -> MakeRhoHFlux
+value = valid cells / pass time, in Gcell-updates/s.
+
+roofline: the dominant kernel's algorithmic HBM bytes per launch over its
+average launch time, both measured here with HIP events on the library's
+stream (mx_adv_prof_*).  cpu_baseline: the CPU oracle (a line-faithful
+OpenMP restatement of the reference) timed on one 128^3 box of the same
+workload on this host.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s per FP64 advect step, % HBM peak; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per valid cell of one pass (FP64, valid region only,
+# compulsory traffic across the reference's stage boundaries): 3-D, nc = 5.
+PASS_BYTES_PER_CELL = {3: 488, 2: 352}
+
+
+def kernel_bytes_per_cell(name, nc, dim):
+    """Algorithmic bytes per valid cell of one launch of each kernel: its
+    compulsory inputs read once and its outputs written once (FP64)."""
+    D = dim
+    table = {
+        # MakeUtrans faces: read utilde (D) + ufull (D), write utrans (D)
+        "k_utrans": 8 * (3 * D),
+        # VelPred PPM: read utilde (D) + ufull (D), write Ip/Im (2 D per comp)
+        "k_vp_ppm": 8 * (2 * D + 2 * D * D),
+        # VelPred transverse: read Ip/Im (2 D D) + utrans (D), write 6 states
+        "k_vp_trans": 8 * (2 * D * D + D + 6),
+        # VelPred velocities: read states (2 D + 6) + utrans + force, write umac
+        "k_vp_vel": 8 * (2 * D + 6 + D + D + D),
+        # EdgeScal PPM per launch over nc comps: read s, umac (D); write Ip/Im (2 D)
+        "k_es_ppm": 8 * (nc * (1 + 2 * D) + D),
+        # EdgeScal transverse: read Ip/Im (2 D) + umac; write 6 states
+        "k_es_trans": 8 * (nc * (2 * D + 6) + D),
+        # EdgeScal edges: read Ip/Im, states, force, umac; write sedge (D)
+        "k_es_edge": 8 * (nc * (2 * D + 6 + 1 + D) + D),
+    }
+    return table.get(name)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ppm", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--max-grid", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    return ap.parse_args()
+
+
+def cpu_baseline(args):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from maestroex_amd.fabs import Backend
+    from maestroex_amd.workloads import Workload
+    cores = int(os.environ.get("MX_CPU_THREADS", "0")) or len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    pyoracle.set_threads(cores, fast=True)
+    n = args.cpu_n
+    w = Workload(dim=3, n=n, max_grid=n, ppm_type=args.ppm, backend=Backend("numpy"))
+    m = pyoracle.oracle_maestro(w.params, [w.boxes], fast=True)
+    m.advance_pass(0, w.pass_fields())  # warm-up (page-in, thread start)
+    times = []
+    for _ in range(args.cpu_reps):
+        t0 = time.perf_counter()
+        m.advance_pass(0, w.pass_fields())
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    return {
+        "value": round(n ** 3 / med / 1e9, 6),
+        "unit": "Gcell-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"one {n}^3 box of the same construction (4 ghosts, 5 scalars, ppm_type {args.ppm}), "
+                  f"median of {args.cpu_reps} passes after 1 warm-up, OpenMP {cores} threads, "
+                  f"g++ -O3 -fopenmp -ffp-contract=off; {med:.2f} s per pass",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    import maestroex_amd as mx
+    from maestroex_amd.workloads import Workload
+
+    dev = mx.device_backend("cuda:0")
+    t0 = time.perf_counter()
+    w = Workload(dim=3, n=args.n, max_grid=args.max_grid, ppm_type=args.ppm, backend=dev)
+    print(f"[bench] workload {args.n}^3 in {len(w.boxes)} boxes built in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    m = mx.gpu_maestro(w.params, [w.boxes])
+    lib, fns = mx.load()
+    fields = w.pass_fields()
+
+    for _ in range(args.warmup):
+        m.advance_pass(0, fields)
+    m.sync()
+    torch.cuda.synchronize()
+
+    # timed region: passes bracketed by events on the default stream (the
+    # library's stream); per-kernel events collected by the library
+    rc = fns["prof_enable"](m.ctx, 1)
+    assert rc == 0, rc
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(args.steps):
+        m.advance_pass(0, fields)
+    stop.record()
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(stop)
+    nk = fns["prof_read"](m.ctx, -1, None, None, None)
+    kernels = {}
+    for idx in range(nk):
+        nm = C.c_char_p()
+        ms = C.c_double()
+        cnt = C.c_longlong()
+        fns["prof_read"](m.ctx, idx, C.byref(nm), C.byref(ms), C.byref(cnt))
+        kernels[nm.value.decode()] = (ms.value, cnt.value)
+    fns["prof_enable"](m.ctx, 0)
+
+    cells = w.valid_cells()
+    ms_per_step = total_ms / args.steps
+    value = cells / (ms_per_step * 1e-3) / 1e9
+    pass_gbs = PASS_BYTES_PER_CELL[3] * cells / (ms_per_step * 1e-3) / 1e9
+
+    dom = max(kernels.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dcnt) = dom
+    avg_ms = dms / dcnt
+    # units one launch of the dominant kernel processes: the valid cells of
+    # all boxes (x the components the launch covers, folded into bytes/cell)
+    ncomp_launch = {"k_es_ppm": None, "k_es_trans": None, "k_es_edge": None}
+    if dname in ncomp_launch:
+        # MakeEdgeScal launches cover 3 species, then rho, then rhoh: average
+        nc = 5.0 / 3.0
+    else:
+        nc = 1
+    bpc = kernel_bytes_per_cell(dname, nc, 3)
+    achieved = bpc * cells / (avg_ms * 1e-3) / 1e9 if bpc else None
+    roofline = {
+        "bound": "hbm",
+        "kernel": dname,
+        "achieved": round(achieved, 2) if achieved else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+        "traffic": None,
+        "avg_launch_ms": round(avg_ms, 4),
+        "bytes_per_cell": bpc,
+        "pass_achieved": round(pass_gbs, 2),
+        "pass_frac": round(pass_gbs / HBM_PEAK_GBS, 4),
+    }
+    breakdown = {k: round(v[0] / args.steps, 4) for k, v in sorted(kernels.items(),
+                                                                    key=lambda kv: -kv[1][0])}
+    print(f"[bench] per-pass kernel ms: {breakdown}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3-D {args.n}^3 FP64 as {len(w.boxes)} boxes of {args.max_grid}^3, "
+                               f"4 ghosts, 5 scalars + 3 velocities, ppm_type {args.ppm}",
+                   "global_batch": cells, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

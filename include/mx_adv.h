@@ -1,0 +1,297 @@
+/*
+ * mx_adv.h -- C-ABI of the MI355X advection hot path (MAESTROeX-compatible).
+ *
+ * The path: PPM / slope reconstruction, transverse-corrected edge-state
+ * prediction (MakeUtrans, VelPred, MakeEdgeScal) and the conservative flux
+ * construction (MakeRhoXFlux, MakeRhoHFlux) with the base-state w0 / rho0
+ * terms.  Every entry point below replaces one member function of
+ * `class Maestro` (Source/Maestro.H in the reference); each declaration cites
+ * the function it replaces.
+ *
+ * Conventions
+ *   - All field data are FP64 and live where the caller put them: device
+ *     memory for the HIP library (libmaestroex_amd.so).  The caller owns and
+ *     pre-allocates every input and output, exactly as the reference callers
+ *     do (MaestroAdvection.cpp:41-55, MaestroAdvance.cpp:167-192).
+ *   - A FAB is described by an Array4-like view (mx_fab): element (i,j,k,n) is
+ *     p[(i-begin[0]) + (j-begin[1])*jstride + (k-begin[2])*kstride + n*nstride]
+ *     with i fastest, component outermost (AMReX Array4 order).  2-D data use
+ *     k == 0 and begin[2] == 0, end[2] == 1.
+ *   - A level's MultiFab is one mx_fab per box, in the order of the boxes
+ *     given to mx_adv_set_level (the level's BoxArray).  Face-centred data
+ *     (MAC velocities, edge states, fluxes) are one MultiFab per direction.
+ *   - One-dimensional base states (BaseState<Real>, Source/BaseState.H:80) are
+ *     (lev, r) arrays: element (lev, r) is p[lev*len + r].
+ *   - Ghost cells of inputs are filled by the caller, as the reference
+ *     assumes.  Functions that fill ghost faces in the reference
+ *     (MakeUtrans, Addw0) do so here too (single-level FillBoundary with
+ *     periodicity, then FillUmacGhost).
+ *   - Calls are asynchronous on the context's HIP stream; mx_adv_sync waits.
+ *   - Every function returns MX_OK (0) or a negative error code; no call
+ *     aborts the process (the reference calls amrex::Abort / Error instead).
+ */
+#ifndef MX_ADV_H
+#define MX_ADV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MX_MAX_LEV 8
+#define MX_MAX_SCAL 32
+
+/* return codes */
+enum {
+    MX_OK = 0,
+    MX_ERR_ARG = -1,         /* null pointer, bad count, bad parameter value */
+    MX_ERR_BOX = -2,         /* a FAB does not cover the region the stage needs */
+    MX_ERR_BC = -3,          /* invalid boundary type (VelPred's "invalid boundary type") */
+    MX_ERR_HIP = -4,         /* HIP runtime failure */
+    MX_ERR_UNSUPPORTED = -5, /* configuration the path does not implement */
+    MX_ERR_NOMEM = -6,       /* workspace allocation failed */
+    MX_ERR_NOLIB = -7        /* no usable GPU */
+};
+
+/* amrex::BCType values (mathematical boundary types, per component) */
+enum {
+    MX_BC_REFLECT_ODD = -1,
+    MX_BC_INT_DIR = 0,
+    MX_BC_REFLECT_EVEN = 1,
+    MX_BC_FOEXTRAP = 2,
+    MX_BC_EXT_DIR = 3,
+    MX_BC_HOEXTRAP = 4
+};
+
+/* amrex::PhysBCType values (physical boundary types, per face) */
+enum {
+    MX_PHYS_INTERIOR = 0,
+    MX_PHYS_INFLOW = 1,
+    MX_PHYS_OUTFLOW = 2,
+    MX_PHYS_SYMMETRY = 3,
+    MX_PHYS_SLIPWALL = 4,
+    MX_PHYS_NOSLIPWALL = 5
+};
+
+/* cell-centred index box, inclusive bounds (amrex::Box of a BoxArray) */
+typedef struct {
+    int lo[3];
+    int hi[3];
+} mx_box;
+
+/* Array4-like view of one FAB */
+typedef struct {
+    double* p;
+    long long jstride;
+    long long kstride;
+    long long nstride;
+    int begin[3];
+    int end[3]; /* exclusive */
+    int ncomp;
+} mx_fab;
+
+/* one level of a MultiFab: fab[b] belongs to box b of the level */
+typedef struct {
+    int nfab;
+    const mx_fab* fab;
+} mx_multifab;
+
+/* face-centred MultiFab triple (std::array<MultiFab, AMREX_SPACEDIM>) */
+typedef struct {
+    mx_multifab dir[3];
+} mx_face_multifab;
+
+/* BaseState<Real> with one variable: element (lev, r) is p[lev*len + r] */
+typedef struct {
+    double* p;
+    int nlev;
+    int len;
+} mx_basestate;
+
+/* amrex::BCRec */
+typedef struct {
+    int lo[3];
+    int hi[3];
+} mx_bcrec;
+
+/* The Maestro members the path reads implicitly (Maestro.H; defaults from
+   Source/param/_cpp_parameters). */
+typedef struct {
+    int dim;                 /* AMREX_SPACEDIM: 2 or 3 */
+    int ppm_type;            /* 0 (slopes), 1 (CW84), 2 (CS08) */
+    int slope_order;         /* 0, 2 or 4; used when ppm_type == 0 */
+    int ppm_trace_forces;    /* 0 or 1 */
+    int species_pred_type;   /* 1 rhoprime_and_X, 2 rhoX, 3 rho_and_X */
+    int enthalpy_pred_type;  /* 0 rhoh, 1 rhohprime, 2 h, 3 T_then_rhohprime,
+                                4 T_then_h, 5 hprime, 6 Tprime_then_h */
+    int spherical;           /* 0 planar, 1 spherical (3-D only) */
+    int use_exact_base_state;
+    int evolve_base_state;
+    double dt;
+    double rel_eps;          /* 1e-8 * umax (MaestroDt.cpp:481-483) */
+    int finest_level;        /* levels 0..finest_level */
+    int domain_lo[MX_MAX_LEV][3];
+    int domain_hi[MX_MAX_LEV][3];
+    double dx[MX_MAX_LEV][3];
+    int is_periodic[3];
+    int phys_bc_lo[3];       /* phys_bc (Maestro.H:1917) */
+    int phys_bc_hi[3];
+    int nscal;               /* Nscal (Source/param/_variables:8-14) */
+    int comp_rho;            /* Rho */
+    int comp_rhoh;           /* RhoH */
+    int comp_temp;           /* Temp */
+    int comp_pi;             /* Pi */
+    int first_spec;          /* FirstSpec */
+    int num_spec;            /* NumSpec; FirstAux.. are the remaining comps */
+} mx_adv_params;
+
+typedef struct mx_adv_ctx mx_adv_ctx;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Create a context holding the Maestro members above.  Takes no reference
+   counterpart beyond Maestro::Setup / ReadParameters (MaestroSetup.cpp). */
+int mx_adv_create(const mx_adv_params* params, mx_adv_ctx** out);
+int mx_adv_destroy(mx_adv_ctx* ctx);
+
+/* grids[lev]: the level's BoxArray (Maestro.H, AmrCore::grids). */
+int mx_adv_set_level(mx_adv_ctx* ctx, int lev, int nbox, const mx_box* boxes);
+
+/* dt and rel_eps, set per step in the reference by EstDt (MaestroDt.cpp:481-483). */
+int mx_adv_set_dt(mx_adv_ctx* ctx, double dt, double rel_eps);
+
+/* Maestro::BCSetup (MaestroSetup.cpp:276-559): the per-component BCRec
+   tables bcs_s (nscal entries) and bcs_u (dim entries) implied by phys_bc.
+   The context uses the same tables inside MakeUtrans, VelPred and the pass. */
+int mx_adv_bc_setup(const mx_adv_ctx* ctx, mx_bcrec* bcs_s, mx_bcrec* bcs_u);
+
+/* Stream for subsequent calls (hipStream_t, NULL = default stream). */
+int mx_adv_set_stream(mx_adv_ctx* ctx, void* hip_stream);
+int mx_adv_sync(mx_adv_ctx* ctx);
+const char* mx_adv_error_string(int code);
+
+/* Kernel timing (instrumentation; no counterpart in the reference, whose
+   timers are BL_PROFILE_VAR regions).  mx_adv_prof_enable(ctx, 1) clears the
+   counters and brackets every later kernel launch with a HIP event pair on
+   the context stream; mx_adv_prof_read(ctx, -1, ...) synchronises, folds the
+   events into per-kernel totals and returns the number of kernel names;
+   mx_adv_prof_read(ctx, i, &name, &ms, &n) returns kernel i's total time in
+   ms and launch count. */
+int mx_adv_prof_enable(mx_adv_ctx* ctx, int on);
+int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_ms,
+                     long long* launches);
+
+/* ---- unit-level kernels ------------------------------------------------- */
+
+/* Maestro::PPM (Maestro.H:1396-1406, MaestroPPM.cpp:6-2263).  bx is the box
+   Ip/Im are computed on; u,v,w are face velocities (is_umac != 0) or
+   cell-centred one-component tracing velocities. */
+int mx_adv_ppm(mx_adv_ctx* ctx, int lev, const mx_box* bx, const mx_fab* s,
+               const mx_fab* u, const mx_fab* v, const mx_fab* w, mx_fab* Ip,
+               mx_fab* Im, const mx_bcrec* bcs, int nbcs, int is_umac,
+               int comp, int bccomp);
+
+/* Maestro::Slopex / Slopey / Slopez (Maestro.H:1576-1593,
+   MaestroSlopes.cpp:6-600); dir selects the function. */
+int mx_adv_slope(mx_adv_ctx* ctx, int lev, int dir, const mx_box* bx,
+                 const mx_fab* s, mx_fab* sl, const mx_bcrec* bcs, int nbcs,
+                 int ncomp, int bc_start_comp, int slope_order);
+
+/* ---- level-wide stages (one level; finest_level == 0 ghost semantics) --- */
+
+/* Maestro::MakeUtrans (Maestro.H:1172-1177, MaestroMakeUtrans.cpp:6-648),
+   including its closing FillBoundary + FillUmacGhost.  w0_cart is the
+   cell-centred w0 (Maestro.H:1903); w0mac is read only when spherical. */
+int mx_adv_make_utrans(mx_adv_ctx* ctx, int lev, const mx_multifab* utilde,
+                       const mx_multifab* ufull, mx_face_multifab* utrans,
+                       const mx_face_multifab* w0mac,
+                       const mx_multifab* w0_cart);
+
+/* Maestro::Addw0 (Maestro.H:693-696, MaestroFill3dData.cpp:357-461),
+   including its closing FillBoundary + FillUmacGhost. */
+int mx_adv_addw0(mx_adv_ctx* ctx, int lev, mx_face_multifab* uedge,
+                 const mx_face_multifab* w0mac, const mx_basestate* w0,
+                 double mult);
+
+/* Maestro::VelPred (Maestro.H:1711-1718, MaestroVelPred.cpp:5-292). */
+int mx_adv_velpred(mx_adv_ctx* ctx, int lev, const mx_multifab* utilde,
+                   const mx_multifab* ufull, const mx_face_multifab* utrans,
+                   mx_face_multifab* umac, const mx_face_multifab* w0mac,
+                   const mx_multifab* force, const mx_multifab* w0_cart);
+
+/* Maestro::MakeEdgeScal (Maestro.H:913-919,
+   MaestroMakeEdgeScalars.cpp:6-302). */
+int mx_adv_make_edge_scal(mx_adv_ctx* ctx, int lev, const mx_multifab* state,
+                          mx_face_multifab* sedge,
+                          const mx_face_multifab* umac,
+                          const mx_multifab* force, int is_vel,
+                          const mx_bcrec* bcs, int nbccomp, int start_scomp,
+                          int start_bccomp, int num_comp,
+                          int is_conservative);
+
+/* Maestro::MakeRhoXFlux (Maestro.H:230-245, MaestroMakeFlux.cpp:18-465).
+   w0 is the Maestro member w0 (edge-centred BaseState). */
+int mx_adv_make_rhox_flux(
+    mx_adv_ctx* ctx, int lev, mx_face_multifab* sflux,
+    mx_multifab* etarhoflux, const mx_face_multifab* sedge,
+    const mx_face_multifab* umac, const mx_basestate* rho0_old,
+    const mx_basestate* rho0_edge_old, const mx_face_multifab* r0mac_old,
+    const mx_basestate* rho0_new, const mx_basestate* rho0_edge_new,
+    const mx_face_multifab* r0mac_new,
+    const mx_basestate* rho0_predicted_edge, const mx_basestate* w0,
+    int start_comp, int num_comp);
+
+/* Maestro::MakeRhoHFlux (Maestro.H:270-295, MaestroMakeFlux.cpp:467-1067). */
+int mx_adv_make_rhoh_flux(
+    mx_adv_ctx* ctx, int lev, mx_face_multifab* sflux,
+    const mx_face_multifab* sedge, const mx_face_multifab* umac,
+    const mx_face_multifab* w0mac, const mx_basestate* rho0_old,
+    const mx_basestate* rho0_edge_old, const mx_face_multifab* r0mac_old,
+    const mx_basestate* rho0_new, const mx_basestate* rho0_edge_new,
+    const mx_face_multifab* r0mac_new, const mx_basestate* rhoh0_old,
+    const mx_basestate* rhoh0_edge_old, const mx_face_multifab* rh0mac_old,
+    const mx_basestate* rhoh0_new, const mx_basestate* rhoh0_edge_new,
+    const mx_face_multifab* rh0mac_new, const mx_face_multifab* h0mac_old,
+    const mx_face_multifab* h0mac_new);
+
+/* Maestro::CelltoEdge (Maestro.H:401-402, MaestroCelltoEdge.cpp:5-55),
+   planar, single radial level, no coarse-fine chunks: edge[r] for
+   r = 0..nr on row lev of the BaseStates. */
+int mx_adv_cell_to_edge(mx_adv_ctx* ctx, int lev, int nr,
+                        const mx_basestate* s0_cell, mx_basestate* s0_edge);
+
+/* Maestro::FillUmacGhost (MaestroFillData.cpp:176-453) preceded by the
+   periodic FillBoundary of each face MultiFab, as MakeUtrans and Addw0 do
+   when finest_level == 0 (MaestroMakeUtrans.cpp:631-640). */
+int mx_adv_fill_face_ghosts(mx_adv_ctx* ctx, int lev,
+                            mx_face_multifab* uedge);
+
+/* ---- the whole pass ----------------------------------------------------- */
+
+/* One advection pass over one level, the unit of work of the benchmark:
+     MakeUtrans -> Addw0(utrans,+1) -> VelPred -> Addw0(umac,+1)
+       -> MakeEdgeScal(species) -> MakeEdgeScal(Rho) -> MakeEdgeScal(RhoH)
+       -> MakeRhoXFlux -> MakeRhoHFlux
+   in the order of AdvancePremac (MaestroAdvection.cpp:7-84),
+   DensityAdvance (MaestroDensityAdvance.cpp:115-129,182-184) and
+   EnthalpyAdvance (MaestroEnthalpyAdvance.cpp:145-146,229-233), without the
+   MAC projection and the producers of the forcing terms, which sit outside
+   the path.  scal is the scalar state in predictor form (rho', rhoh', X);
+   force terms, base states and outputs are caller-owned.  Base states are
+   planar (old == new). */
+typedef struct {
+    mx_multifab utilde, ufull, vel_force, w0_cart;
+    mx_multifab scal, scal_force;
+    mx_face_multifab w0mac;
+    mx_basestate w0, rho0, rho0_edge, rhoh0, rhoh0_edge, rho0_predicted_edge;
+    /* outputs */
+    mx_face_multifab utrans, umac, sedge, sflux;
+    mx_multifab etarhoflux;
+} mx_adv_pass_args;
+
+int mx_adv_advance_pass(mx_adv_ctx* ctx, int lev, const mx_adv_pass_args* a);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MX_ADV_H */

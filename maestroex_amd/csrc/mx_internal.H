@@ -1,0 +1,271 @@
+// mx_internal.H -- host-side context, workspace and launch helpers of
+// libmaestroex_amd.so (not part of the C-ABI).
+#ifndef MX_INTERNAL_H
+#define MX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "mx_adv.h"
+#include "mx_device.H"
+
+struct mx_adv_ctx {
+    mx_adv_params p;
+    std::vector<mx_box> grids[MX_MAX_LEV];
+    std::vector<mx_bcrec> bcs_s, bcs_u;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // device workspace: a bump allocator reset at the start of every entry
+    // point (all work is ordered on `stream`, so a later call reuses memory
+    // only after the earlier kernels have finished with it)
+    char* ws = nullptr;
+    size_t ws_cap = 0;
+    size_t ws_off = 0;
+
+    // device copy of the per-call box tables
+    char* tab = nullptr;
+    size_t tab_cap = 0;
+    std::vector<char> htab;
+
+    // FillBoundary plans, keyed by (lev, nodal dir, ghost width): the ghost
+    // shell of every box as slabs, each with the box's candidate sources
+    // (source box, periodic shift) in the order that decides shared faces
+    struct GhostSlab {
+        int box;
+        int lo[3], hi[3];
+        int cand_off, cand_n;
+    };
+    struct GhostCand {
+        int src_box;
+        int shift[3];
+    };
+    struct GhostPlan {
+        std::vector<GhostSlab> slabs;
+        std::vector<GhostCand> cands;
+    };
+    std::map<std::tuple<int, int, long long>, GhostPlan> plans;
+
+    // opt-in kernel timing: a HIP event pair around every launch
+    struct ProfRec {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    int prof_on = 0;
+    std::vector<ProfRec> prof_recs;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::string> prof_names;
+    std::vector<double> prof_ms;
+    std::vector<long long> prof_n;
+};
+
+namespace mxh {
+
+#define MX_HIP_CHECK(x)                                  \
+    do {                                                 \
+        hipError_t e_ = (x);                             \
+        if (e_ != hipSuccess) return MX_ERR_HIP;         \
+    } while (0)
+
+hipEvent_t prof_event(mx_adv_ctx* c);
+
+// launch on the context stream; with timing on, bracket it with events
+#define MX_LAUNCH(c, name, kern, grid, block, ...)                                  \
+    do {                                                                            \
+        hipEvent_t e0_ = nullptr;                                                   \
+        if ((c)->prof_on) {                                                         \
+            e0_ = mxh::prof_event(c);                                               \
+            MX_HIP_CHECK(hipEventRecord(e0_, (c)->stream));                         \
+        }                                                                           \
+        hipLaunchKernelGGL(kern, grid, block, 0, (c)->stream, __VA_ARGS__);         \
+        MX_HIP_CHECK(hipGetLastError());                                            \
+        if ((c)->prof_on) {                                                         \
+            hipEvent_t e1_ = mxh::prof_event(c);                                    \
+            MX_HIP_CHECK(hipEventRecord(e1_, (c)->stream));                         \
+            (c)->prof_recs.push_back({name, e0_, e1_});                             \
+        }                                                                           \
+    } while (0)
+
+inline mxd::DA to_da(const mx_fab& f) {
+    mxd::DA a;
+    a.p = f.p;
+    a.js = f.jstride;
+    a.ks = f.kstride;
+    a.ns = f.nstride;
+    a.b0 = f.begin[0];
+    a.b1 = f.begin[1];
+    a.b2 = f.begin[2];
+    return a;
+}
+
+inline mxd::DA null_da() {
+    mxd::DA a;
+    a.p = nullptr;
+    a.js = a.ks = a.ns = 0;
+    a.b0 = a.b1 = a.b2 = 0;
+    return a;
+}
+
+struct HBox {
+    int lo[3], hi[3];
+};
+
+inline HBox hbox(const mx_box& b) {
+    HBox h;
+    for (int d = 0; d < 3; ++d) {
+        h.lo[d] = b.lo[d];
+        h.hi[d] = b.hi[d];
+    }
+    return h;
+}
+inline HBox grow(HBox b, int dim, int n) {
+    for (int d = 0; d < dim; ++d) {
+        b.lo[d] -= n;
+        b.hi[d] += n;
+    }
+    return b;
+}
+inline HBox grow_dir(HBox b, int d, int n) {
+    b.lo[d] -= n;
+    b.hi[d] += n;
+    return b;
+}
+inline HBox grow_hi(HBox b, int d, int n) {
+    b.hi[d] += n;
+    return b;
+}
+inline HBox grow_lo(HBox b, int d, int n) {
+    b.lo[d] -= n;
+    return b;
+}
+inline HBox nodal(HBox b, int d) {
+    b.hi[d] += 1;
+    return b;
+}
+inline long long npts(const HBox& b) {
+    long long n = 1;
+    for (int d = 0; d < 3; ++d) n *= (long long)(b.hi[d] - b.lo[d] + 1);
+    return n;
+}
+
+// does FAB f cover region r for components [0, ncomp)?
+inline bool covers(const mx_fab& f, const HBox& r, int ncomp) {
+    if (!f.p) return false;
+    if (f.ncomp < ncomp) return false;
+    for (int d = 0; d < 3; ++d) {
+        if (r.lo[d] < f.begin[d] || r.hi[d] >= f.end[d]) return false;
+    }
+    return true;
+}
+
+mxd::KGeom kgeom(const mx_adv_ctx* c, int lev);
+
+// workspace
+int ws_reserve(mx_adv_ctx* c, size_t bytes);
+double* ws_take(mx_adv_ctx* c, size_t ndouble);
+
+// a temporary FAB per box over `region(box)` with ncomp components
+template <class RegionFn>
+std::vector<mxd::DA> tmp_fabs(mx_adv_ctx* c, const std::vector<mx_box>& grids, int ncomp,
+                              RegionFn region) {
+    std::vector<mxd::DA> out;
+    for (const mx_box& g : grids) {
+        HBox r = region(hbox(g));
+        long long n0 = r.hi[0] - r.lo[0] + 1, n1 = r.hi[1] - r.lo[1] + 1,
+                  n2 = r.hi[2] - r.lo[2] + 1;
+        mxd::DA a;
+        a.p = ws_take(c, (size_t)(n0 * n1 * n2 * ncomp));
+        a.js = n0;
+        a.ks = n0 * n1;
+        a.ns = n0 * n1 * n2;
+        a.b0 = r.lo[0];
+        a.b1 = r.lo[1];
+        a.b2 = r.lo[2];
+        out.push_back(a);
+    }
+    return out;
+}
+
+template <class RegionFn>
+size_t tmp_bytes(const std::vector<mx_box>& grids, int ncomp, RegionFn region) {
+    size_t s = 0;
+    for (const mx_box& g : grids) {
+        HBox r = region(hbox(g));
+        s += (size_t)npts(r) * ncomp * sizeof(double) + 256;
+    }
+    return s;
+}
+
+// per-call device table: append host structs, upload once, get device pointers
+struct Table {
+    mx_adv_ctx* c;
+    std::vector<size_t> offs;
+    explicit Table(mx_adv_ctx* ctx) : c(ctx) { c->htab.clear(); }
+    template <class T>
+    size_t add(const std::vector<T>& v) {
+        size_t off = (c->htab.size() + 255) & ~(size_t)255;
+        c->htab.resize(off + v.size() * sizeof(T));
+        std::memcpy(c->htab.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+    int upload();
+    template <class T>
+    const T* dev(size_t off) const {
+        return reinterpret_cast<const T*>(c->tab + off);
+    }
+};
+
+// launch geometry: x covers the largest per-box region, y = boxes, z = extra
+inline dim3 grid_for(long long maxpts, int nbox, int nz, int block) {
+    long long gx = (maxpts + block - 1) / block;
+    if (gx < 1) gx = 1;
+    return dim3((unsigned)gx, (unsigned)nbox, (unsigned)nz);
+}
+
+// stages shared between entry points
+int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge);
+int addw0(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge, const mx_face_multifab* w0mac,
+          const mx_basestate* w0, double mult);
+int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
+                const mx_face_multifab* utrans, const mx_face_multifab* w0mac,
+                const mx_multifab* w0_cart);
+int velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
+            const mx_face_multifab* utrans, const mx_face_multifab* umac,
+            const mx_face_multifab* w0mac, const mx_multifab* force, const mx_multifab* w0_cart);
+int make_edge_scal(mx_adv_ctx* c, int lev, const mx_multifab* state,
+                   const mx_face_multifab* sedge, const mx_face_multifab* umac,
+                   const mx_multifab* force, int is_vel, const mx_bcrec* bcs, int nbccomp,
+                   int start_scomp, int start_bccomp, int num_comp, int is_conservative);
+int make_rhox_flux(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux,
+                   const mx_multifab* etarhoflux, const mx_face_multifab* sedge,
+                   const mx_face_multifab* umac, const mx_basestate* rho0_old,
+                   const mx_basestate* rho0_edge_old, const mx_face_multifab* r0mac_old,
+                   const mx_basestate* rho0_new, const mx_basestate* rho0_edge_new,
+                   const mx_face_multifab* r0mac_new, const mx_basestate* rho0_predicted_edge,
+                   const mx_basestate* w0, int start_comp, int num_comp, int zero_eta);
+int make_rhoh_flux(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux,
+                   const mx_face_multifab* sedge, const mx_face_multifab* umac,
+                   const mx_basestate* rho0_old, const mx_basestate* rho0_edge_old,
+                   const mx_face_multifab* r0mac_old, const mx_basestate* rho0_new,
+                   const mx_basestate* rho0_edge_new, const mx_face_multifab* r0mac_new,
+                   const mx_basestate* rhoh0_old, const mx_basestate* rhoh0_edge_old,
+                   const mx_face_multifab* rh0mac_old, const mx_basestate* rhoh0_new,
+                   const mx_basestate* rhoh0_edge_new, const mx_face_multifab* rh0mac_new,
+                   const mx_face_multifab* h0mac_old, const mx_face_multifab* h0mac_new);
+
+void bc_setup_host(mx_adv_ctx* c);
+int check_level(const mx_adv_ctx* c, int lev);
+int check_mf(const mx_adv_ctx* c, int lev, const mx_multifab* m, int ncomp, int ngrow_needed,
+             int nodal_dir);
+int check_face(const mx_adv_ctx* c, int lev, const mx_face_multifab* f, int ncomp,
+               int ngrow_needed);
+
+}  // namespace mxh
+
+#endif

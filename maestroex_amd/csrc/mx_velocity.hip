@@ -1,0 +1,708 @@
+// mx_velocity.hip -- MakeUtrans and VelPred on the GPU.
+//
+// MakeUtrans (MaestroMakeUtrans.cpp:6-648): one kernel over the faces of
+// every box and direction.  Each face thread reconstructs the normal velocity
+// component in the normal direction only, on its two adjacent cells (the
+// reference runs PPM in all three directions and keeps one), applies the
+// physical-boundary overrides and solves the Riemann problem, with the
+// cell-centred w0 in the vertical direction (planar) or w0mac (spherical).
+//
+// VelPred (MaestroVelPred.cpp:5-1915): three kernels over all boxes --
+//   k_vp_ppm     PPM (or slopes) of every velocity component in every
+//                direction on the box grown by one (the Ipu..Imw temporaries);
+//   k_vp_trans   the six transverse states uimhyz..wimhyx (3-D), with the
+//                interface states ulx..urz and the upwind states uimhx..uimhz
+//                evaluated on the fly from the PPM temporaries;
+//   k_vp_vel     the MAC velocities umac/vmac/wmac on the valid faces.
+#include <algorithm>
+
+#include "mx_internal.H"
+
+using namespace mxd;
+using namespace mxh;
+
+namespace {
+
+constexpr int e(int d, int q) { return d == q ? 1 : 0; }
+
+// index of the transverse state on d-faces corrected by t (d != t)
+__host__ __device__ constexpr int pair_index(int d, int t) { return d * 2 + (t > d ? t - 1 : t); }
+
+// ---------------------------------------------------------------------------
+// MakeUtrans
+
+struct UtBox {
+    int lo[3], hi[3];
+    DA ut, uf, w0c;
+    DA tr[3];
+    DA w0m[3];
+};
+
+struct UtArgs {
+    int bclo[3][3], bchi[3][3];  // bcs_u[comp].lo/hi[dir]
+};
+
+template <int PPM, int D>
+__device__ __forceinline__ void utrans_face(const UtBox& B, const KGeom& g, const UtArgs& A,
+                                            int i, int j, int k) {
+    const int im = i - e(D, 0), jm = j - e(D, 1), km = k - e(D, 2);
+    const int pos = D == 0 ? i : (D == 1 ? j : k);
+    const int posm = pos - 1;
+    const double dt2 = 0.5 * g.dt;
+    const double hd = g.dx[D];
+    const int bclo = A.bclo[D][D], bchi = A.bchi[D][D];
+    const long long st = B.ut.stride(D);
+    double ul, ur;
+    if (PPM == 0) {
+        double v[9];
+        load_stencil<1>(B.ut.p + B.ut.off(im, jm, km, D), st, v);
+        double slL = slope_point<D>(v, g.slope_order, posm, g.domlo[D], g.domhi[D], bclo, bchi);
+        load_stencil<1>(B.ut.p + B.ut.off(i, j, k, D), st, v);
+        double slR = slope_point<D>(v, g.slope_order, pos, g.domlo[D], g.domhi[D], bclo, bchi);
+        ul = B.ut(im, jm, km, D) + (0.5 - (dt2 / hd) * dmax(0.0, B.uf(im, jm, km, D))) * slL;
+        ur = B.ut(i, j, k, D) - (0.5 + (dt2 / hd) * dmin(0.0, B.uf(i, j, k, D))) * slR;
+    } else {
+        double v[9], sm, sp, Ip, Im;
+        load_stencil<PPM>(B.ut.p + B.ut.off(im, jm, km, D), st, v);
+        ppm_edges<PPM, D>(v, posm, g.domlo[D], g.domhi[D], bclo, bchi, sm, sp);
+        double uc = B.uf(im, jm, km, D);
+        ppm_trace(v[4], sm, sp, uc, uc, g.dt, hd, g.rel_eps, Ip, Im);
+        ul = Ip;
+        load_stencil<PPM>(B.ut.p + B.ut.off(i, j, k, D), st, v);
+        ppm_edges<PPM, D>(v, pos, g.domlo[D], g.domhi[D], bclo, bchi, sm, sp);
+        uc = B.uf(i, j, k, D);
+        ppm_trace(v[4], sm, sp, uc, uc, g.dt, hd, g.rel_eps, Ip, Im);
+        ur = Im;
+    }
+
+    if (pos == g.domlo[D]) {
+        switch (g.phys_lo[D]) {
+            case MX_PHYS_INFLOW:
+                ul = B.ut(im, jm, km, D);
+                ur = B.ut(im, jm, km, D);
+                break;
+            case MX_PHYS_SLIPWALL:
+            case MX_PHYS_NOSLIPWALL:
+            case MX_PHYS_SYMMETRY:
+                if (g.dim == 2 && D == 1) {
+                    ur = 0.0;  // the 2-D y-lo branch zeroes vry twice and keeps vly
+                } else {
+                    ul = 0.0;
+                    ur = 0.0;
+                }
+                break;
+            case MX_PHYS_OUTFLOW:
+                ul = dmin(ur, 0.0);
+                ur = ul;
+                break;
+            default:
+                break;
+        }
+    } else if (pos == g.domhi[D] + 1) {
+        switch (g.phys_hi[D]) {
+            case MX_PHYS_INFLOW:
+                if (g.dim == 2) {
+                    ul = B.ut(i, j, k, D);
+                    ur = B.ut(i, j, k, D);
+                } else {
+                    ul = B.ut(i + e(D, 0), j + e(D, 1), k + e(D, 2), D);
+                    ur = B.ut(i + e(D, 0), j + e(D, 1), k + e(D, 2), D);
+                }
+                break;
+            case MX_PHYS_SLIPWALL:
+            case MX_PHYS_NOSLIPWALL:
+            case MX_PHYS_SYMMETRY:
+                ul = 0.0;
+                ur = 0.0;
+                break;
+            case MX_PHYS_OUTFLOW:
+                ul = dmax(ul, 0.0);
+                ur = ul;
+                break;
+            default:
+                break;
+        }
+    }
+
+    double res;
+    const bool sph = g.dim == 3 && g.spherical;
+    if (sph || D == g.dim - 1) {
+        const double w = sph ? B.w0m[D](i, j, k) : B.w0c(i, j, k, g.dim - 1);
+        bool test = (ul + w <= 0.0 && ur + w >= 0.0) || (dabs(ul + ur + 2.0 * w) < g.rel_eps);
+        res = 0.5 * (ul + ur) + w > 0.0 ? ul : ur;
+        res = test ? 0.0 : res;
+    } else {
+        bool test = (ul <= 0.0 && ur >= 0.0) || (dabs(ul + ur) < g.rel_eps);
+        res = 0.5 * (ul + ur) > 0.0 ? ul : ur;
+        res = test ? 0.0 : res;
+    }
+    B.tr[D](i, j, k) = res;
+}
+
+template <int PPM>
+__global__ void __launch_bounds__(256) k_utrans(const UtBox* __restrict__ boxes, KGeom g, UtArgs A) {
+    const UtBox& B = boxes[blockIdx.y];
+    const int d = blockIdx.z;
+    int lo[3] = {B.lo[0], B.lo[1], B.lo[2]};
+    int hi[3] = {B.hi[0], B.hi[1], B.hi[2]};
+    hi[d] += 1;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!region_point(idx, lo, hi, i, j, k)) return;
+    if (d == 0)
+        utrans_face<PPM, 0>(B, g, A, i, j, k);
+    else if (d == 1)
+        utrans_face<PPM, 1>(B, g, A, i, j, k);
+    else
+        utrans_face<PPM, 2>(B, g, A, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// VelPred
+
+struct VpBox {
+    int lo[3], hi[3];
+    DA ut, uf, fo, w0c;
+    DA tr[3], um[3], w0m[3];
+    DA Ip[3], Im[3];    // per velocity component: [dir] comps; ppm_type 0: slopes along dir
+    DA Ipf[3], Imf[3];  // traced forces
+    DA T[6];            // transverse states, pair_index(d, t)
+};
+
+struct VpArgs {
+    int bclo[3][3], bchi[3][3];  // bcs_u[comp].lo/hi[dir]
+};
+
+template <int PPM, int D>
+__device__ __forceinline__ void vp_ppm_dir(const VpBox& B, const KGeom& g, const VpArgs& A, int n,
+                                           int i, int j, int k) {
+    const int pos = D == 0 ? i : (D == 1 ? j : k);
+    double v[9];
+    if (PPM == 0) {
+        load_stencil<1>(B.ut.p + B.ut.off(i, j, k, n), B.ut.stride(D), v);
+        B.Ip[D](i, j, k, n) = slope_point<D>(v, g.slope_order, pos, g.domlo[D], g.domhi[D],
+                                             A.bclo[n][D], A.bchi[n][D]);
+        return;
+    }
+    double sm, sp, Ip, Im;
+    load_stencil<PPM>(B.ut.p + B.ut.off(i, j, k, n), B.ut.stride(D), v);
+    ppm_edges<PPM, D>(v, pos, g.domlo[D], g.domhi[D], A.bclo[n][D], A.bchi[n][D], sm, sp);
+    const double uc = B.uf(i, j, k, D);
+    ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[D], g.rel_eps, Ip, Im);
+    B.Ip[n](i, j, k, D) = Ip;
+    B.Im[n](i, j, k, D) = Im;
+    if (g.ppm_trace_forces == 1) {
+        load_stencil<PPM>(B.fo.p + B.fo.off(i, j, k, n), B.fo.stride(D), v);
+        ppm_edges<PPM, D>(v, pos, g.domlo[D], g.domhi[D], A.bclo[n][D], A.bchi[n][D], sm, sp);
+        ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[D], g.rel_eps, Ip, Im);
+        B.Ipf[n](i, j, k, D) = Ip;
+        B.Imf[n](i, j, k, D) = Im;
+    }
+}
+
+template <int PPM>
+__global__ void __launch_bounds__(256) k_vp_ppm(const VpBox* __restrict__ boxes, KGeom g, VpArgs A) {
+    const VpBox& B = boxes[blockIdx.y];
+    const int n = blockIdx.z;
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = B.lo[d] - (d < g.dim ? 1 : 0);
+        hi[d] = B.hi[d] + (d < g.dim ? 1 : 0);
+    }
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!region_point(idx, lo, hi, i, j, k)) return;
+    vp_ppm_dir<PPM, 0>(B, g, A, n, i, j, k);
+    vp_ppm_dir<PPM, 1>(B, g, A, n, i, j, k);
+    if (g.dim == 3) vp_ppm_dir<PPM, 2>(B, g, A, n, i, j, k);
+}
+
+// interface states ul/ur of velocity component n on D-faces (VelPredInterface)
+template <int D>
+__device__ __forceinline__ void vp_iface(const VpBox& B, const KGeom& g, int n, int i, int j,
+                                         int k, double& l, double& r) {
+    const int im = i - e(D, 0), jm = j - e(D, 1), km = k - e(D, 2);
+    if (g.ppm_type == 0) {
+        const double dt2 = 0.5 * g.dt;
+        const double hd = g.dx[D];
+        const DA& SL = B.Ip[D];
+        if (g.dim == 3) {
+            double maxu = 0.5 - dt2 * dmax(0.0, B.uf(im, jm, km, D)) / hd;
+            double minu = 0.5 + dt2 * dmin(0.0, B.uf(i, j, k, D)) / hd;
+            l = B.ut(im, jm, km, n) + maxu * SL(im, jm, km, n);
+            r = B.ut(i, j, k, n) - minu * SL(i, j, k, n);
+        } else {
+            double maxu = dmax(0.0, B.uf(im, jm, km, D));
+            double minu = dmin(0.0, B.uf(i, j, k, D));
+            l = B.ut(im, jm, km, n) + (0.5 - (dt2 / hd) * maxu) * SL(im, jm, km, n);
+            r = B.ut(i, j, k, n) - (0.5 + (dt2 / hd) * minu) * SL(i, j, k, n);
+        }
+    } else {
+        l = B.Ip[n](im, jm, km, D);
+        r = B.Im[n](i, j, k, D);
+    }
+    const int pos = D == 0 ? i : (D == 1 ? j : k);
+    if (pos == g.domlo[D]) {
+        switch (g.phys_lo[D]) {
+            case MX_PHYS_INFLOW:
+                l = B.ut(im, jm, km, n);
+                r = l;
+                break;
+            case MX_PHYS_SLIPWALL:
+            case MX_PHYS_SYMMETRY:
+                if (n == D) {
+                    l = 0.0;
+                    r = 0.0;
+                } else {
+                    l = r;
+                }
+                break;
+            case MX_PHYS_NOSLIPWALL:
+                l = 0.0;
+                r = 0.0;
+                break;
+            case MX_PHYS_OUTFLOW:
+                if (g.dim == 2 && D == 0) {
+                    r = l;  // 2-D x: urx(n) = ulx(n) after clamping urx(0)
+                } else {
+                    if (n == D) r = dmin(r, 0.0);
+                    l = r;
+                }
+                break;
+            default:
+                break;
+        }
+    } else if (pos == g.domhi[D] + 1) {
+        switch (g.phys_hi[D]) {
+            case MX_PHYS_INFLOW:
+                l = B.ut(i, j, k, n);
+                r = l;
+                break;
+            case MX_PHYS_SLIPWALL:
+            case MX_PHYS_SYMMETRY:
+                if (n == D) {
+                    l = 0.0;
+                    r = 0.0;
+                } else {
+                    r = l;
+                }
+                break;
+            case MX_PHYS_NOSLIPWALL:
+                l = 0.0;
+                r = 0.0;
+                break;
+            case MX_PHYS_OUTFLOW:
+                if (n == D) l = dmax(l, 0.0);
+                r = l;
+                break;
+            default:
+                break;
+        }
+    }
+}
+
+// upwind interface state uimh (VelPredInterface, MaestroVelPred.cpp:887-896)
+template <int D>
+__device__ __forceinline__ double vp_uimh(const VpBox& B, const KGeom& g, int n, int i, int j,
+                                          int k) {
+    double l, r;
+    vp_iface<D>(B, g, n, i, j, k, l, r);
+    const double t = B.tr[D](i, j, k);
+    double res = t > 0.0 ? l : r;
+    res = dabs(t) < g.rel_eps ? 0.5 * (l + r) : res;
+    return res;
+}
+
+// transverse state on D-faces of component O corrected by direction T
+template <int D, int T>
+__device__ __forceinline__ void vp_trans(const VpBox& B, const KGeom& g, int i, int j, int k) {
+    constexpr int O = 3 - D - T;
+    const int im = i - e(D, 0), jm = j - e(D, 1), km = k - e(D, 2);
+    const double dt6 = g.dt / 6.0;
+    const double ht = g.dx[T];
+    const DA& trt = B.tr[T];
+    double l, r;
+    vp_iface<D>(B, g, O, i, j, k, l, r);
+    l = l - (dt6 / ht) * (trt(im + e(T, 0), jm + e(T, 1), km + e(T, 2)) + trt(im, jm, km)) *
+                (vp_uimh<T>(B, g, O, im + e(T, 0), jm + e(T, 1), km + e(T, 2)) -
+                 vp_uimh<T>(B, g, O, im, jm, km));
+    r = r - (dt6 / ht) * (trt(i + e(T, 0), j + e(T, 1), k + e(T, 2)) + trt(i, j, k)) *
+                (vp_uimh<T>(B, g, O, i + e(T, 0), j + e(T, 1), k + e(T, 2)) -
+                 vp_uimh<T>(B, g, O, i, j, k));
+    const int pos = D == 0 ? i : (D == 1 ? j : k);
+    if (pos == g.domlo[D]) {
+        switch (g.phys_lo[D]) {
+            case MX_PHYS_INFLOW:
+                l = B.ut(im, jm, km, O);
+                r = l;
+                break;
+            case MX_PHYS_SLIPWALL:
+            case MX_PHYS_SYMMETRY:
+            case MX_PHYS_OUTFLOW:
+                l = r;
+                break;
+            case MX_PHYS_NOSLIPWALL:
+                l = 0.0;
+                r = 0.0;
+                break;
+            default:
+                break;
+        }
+    } else if (pos == g.domhi[D] + 1) {
+        switch (g.phys_hi[D]) {
+            case MX_PHYS_INFLOW:
+                l = B.ut(i, j, k, O);
+                r = l;
+                break;
+            case MX_PHYS_SLIPWALL:
+            case MX_PHYS_SYMMETRY:
+            case MX_PHYS_OUTFLOW:
+                r = l;
+                break;
+            case MX_PHYS_NOSLIPWALL:
+                l = 0.0;
+                r = 0.0;
+                break;
+            default:
+                break;
+        }
+    }
+    const double tv = B.tr[D](i, j, k);
+    double res = tv > 0.0 ? l : r;
+    res = dabs(tv) < g.rel_eps ? 0.5 * (l + r) : res;
+    B.T[pair_index(D, T)](i, j, k) = res;
+}
+
+__global__ void __launch_bounds__(256) k_vp_trans(const VpBox* __restrict__ boxes, KGeom g) {
+    const VpBox& B = boxes[blockIdx.y];
+    const int p = blockIdx.z;  // pair index
+    const int D = p / 2;
+    const int q = p % 2;
+    const int T = (q < D) ? q : q + 1;
+    const int O = 3 - D - T;
+    int lo[3] = {B.lo[0], B.lo[1], B.lo[2]};
+    int hi[3] = {B.hi[0], B.hi[1], B.hi[2]};
+    lo[O] -= 1;
+    hi[O] += 1;
+    hi[D] += 1;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!region_point(idx, lo, hi, i, j, k)) return;
+    switch (p) {
+        case 0: vp_trans<0, 1>(B, g, i, j, k); break;
+        case 1: vp_trans<0, 2>(B, g, i, j, k); break;
+        case 2: vp_trans<1, 0>(B, g, i, j, k); break;
+        case 3: vp_trans<1, 2>(B, g, i, j, k); break;
+        case 4: vp_trans<2, 0>(B, g, i, j, k); break;
+        default: vp_trans<2, 1>(B, g, i, j, k); break;
+    }
+}
+
+// one transverse correction term of VelPredVelocities for direction T
+template <int D, int T>
+__device__ __forceinline__ void vp_vel_term(const VpBox& B, const KGeom& g, int i, int j, int k,
+                                            double& vl, double& vr) {
+    const int im = i - e(D, 0), jm = j - e(D, 1), km = k - e(D, 2);
+    const double dt4 = 0.25 * g.dt;
+    const double ht = g.dx[T];
+    const DA& trt = B.tr[T];
+    double aL, bL, aR, bR;
+    if (g.dim == 3) {
+        const DA& Tt = B.T[pair_index(T, 3 - D - T)];
+        aL = Tt(im + e(T, 0), jm + e(T, 1), km + e(T, 2));
+        bL = Tt(im, jm, km);
+        aR = Tt(i + e(T, 0), j + e(T, 1), k + e(T, 2));
+        bR = Tt(i, j, k);
+    } else {
+        aL = vp_uimh<T>(B, g, D, im + e(T, 0), jm + e(T, 1), km + e(T, 2));
+        bL = vp_uimh<T>(B, g, D, im, jm, km);
+        aR = vp_uimh<T>(B, g, D, i + e(T, 0), j + e(T, 1), k + e(T, 2));
+        bR = vp_uimh<T>(B, g, D, i, j, k);
+    }
+    vl = vl - (dt4 / ht) * (trt(im + e(T, 0), jm + e(T, 1), km + e(T, 2)) + trt(im, jm, km)) *
+                  (aL - bL);
+    vr = vr - (dt4 / ht) * (trt(i + e(T, 0), j + e(T, 1), k + e(T, 2)) + trt(i, j, k)) * (aR - bR);
+}
+
+template <int D>
+__device__ __forceinline__ void vp_vel(const VpBox& B, const KGeom& g, int i, int j, int k) {
+    const int im = i - e(D, 0), jm = j - e(D, 1), km = k - e(D, 2);
+    const double dt2 = 0.5 * g.dt;
+    double fl, fr;
+    if (g.ppm_trace_forces == 0) {
+        fl = B.fo(im, jm, km, D);
+        fr = B.fo(i, j, k, D);
+    } else {
+        fl = B.Ipf[D](im, jm, km, D);
+        fr = B.Imf[D](i, j, k, D);
+    }
+    double vl, vr;
+    vp_iface<D>(B, g, D, i, j, k, vl, vr);
+    if (D == 0) {
+        vp_vel_term<0, 1>(B, g, i, j, k, vl, vr);
+        if (g.dim == 3) vp_vel_term<0, 2>(B, g, i, j, k, vl, vr);
+    } else if (D == 1) {
+        vp_vel_term<1, 0>(B, g, i, j, k, vl, vr);
+        if (g.dim == 3) vp_vel_term<1, 2>(B, g, i, j, k, vl, vr);
+    } else {
+        vp_vel_term<2, 0>(B, g, i, j, k, vl, vr);
+        vp_vel_term<2, 1>(B, g, i, j, k, vl, vr);
+    }
+    const double umacl = vl + dt2 * fl;
+    const double umacr = vr + dt2 * fr;
+
+    double res;
+    const bool sph = g.dim == 3 && g.spherical;
+    if (sph || D == g.dim - 1) {
+        const double w = sph ? B.w0m[D](i, j, k) : B.w0c(i, j, k, g.dim - 1);
+        bool test = (umacl + w <= 0.0 && umacr + w >= 0.0) ||
+                    (dabs(umacl + umacr + 2.0 * w) < g.rel_eps);
+        res = 0.5 * (umacl + umacr) + w > 0.0 ? umacl : umacr;
+        res = test ? 0.0 : res;
+    } else {
+        bool test = (umacl <= 0.0 && umacr >= 0.0) || (dabs(umacl + umacr) < g.rel_eps);
+        res = 0.5 * (umacl + umacr) > 0.0 ? umacl : umacr;
+        res = test ? 0.0 : res;
+    }
+    const int pos = D == 0 ? i : (D == 1 ? j : k);
+    if (pos == g.domlo[D]) {
+        switch (g.phys_lo[D]) {
+            case MX_PHYS_INFLOW: res = B.ut(im, jm, km, D); break;
+            case MX_PHYS_SLIPWALL:
+            case MX_PHYS_SYMMETRY:
+            case MX_PHYS_NOSLIPWALL: res = 0.0; break;
+            case MX_PHYS_OUTFLOW: res = dmin(umacr, 0.0); break;
+            default: break;
+        }
+    } else if (pos == g.domhi[D] + 1) {
+        switch (g.phys_hi[D]) {
+            case MX_PHYS_INFLOW: res = B.ut(i, j, k, D); break;
+            case MX_PHYS_SLIPWALL:
+            case MX_PHYS_SYMMETRY:
+            case MX_PHYS_NOSLIPWALL: res = 0.0; break;
+            case MX_PHYS_OUTFLOW: res = dmax(umacl, 0.0); break;
+            default: break;
+        }
+    }
+    B.um[D](i, j, k) = res;
+}
+
+__global__ void __launch_bounds__(256) k_vp_vel(const VpBox* __restrict__ boxes, KGeom g) {
+    const VpBox& B = boxes[blockIdx.y];
+    const int d = blockIdx.z;
+    int lo[3] = {B.lo[0], B.lo[1], B.lo[2]};
+    int hi[3] = {B.hi[0], B.hi[1], B.hi[2]};
+    hi[d] += 1;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!region_point(idx, lo, hi, i, j, k)) return;
+    if (d == 0)
+        vp_vel<0>(B, g, i, j, k);
+    else if (d == 1)
+        vp_vel<1>(B, g, i, j, k);
+    else
+        vp_vel<2>(B, g, i, j, k);
+}
+
+int check_phys(const mx_adv_ctx* c) {
+    for (int d = 0; d < c->p.dim; ++d) {
+        if (c->p.phys_bc_lo[d] < MX_PHYS_INTERIOR || c->p.phys_bc_lo[d] > MX_PHYS_NOSLIPWALL)
+            return MX_ERR_BC;
+        if (c->p.phys_bc_hi[d] < MX_PHYS_INTERIOR || c->p.phys_bc_hi[d] > MX_PHYS_NOSLIPWALL)
+            return MX_ERR_BC;
+    }
+    return MX_OK;
+}
+
+}  // namespace
+
+namespace mxh {
+
+int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
+                const mx_face_multifab* utrans, const mx_face_multifab* w0mac,
+                const mx_multifab* w0_cart) {
+    const int dim = c->p.dim;
+    const auto& grids = c->grids[lev];
+    const bool sph = dim == 3 && c->p.spherical;
+    std::vector<UtBox> boxes;
+    long long maxpts = 0;
+    for (size_t b = 0; b < grids.size(); ++b) {
+        UtBox B;
+        HBox g = hbox(grids[b]);
+        for (int q = 0; q < 3; ++q) {
+            B.lo[q] = g.lo[q];
+            B.hi[q] = g.hi[q];
+        }
+        B.ut = to_da(utilde->fab[b]);
+        B.uf = to_da(ufull->fab[b]);
+        B.w0c = (!sph && w0_cart) ? to_da(w0_cart->fab[b]) : null_da();
+        for (int d = 0; d < 3; ++d) {
+            B.tr[d] = d < dim ? to_da(utrans->dir[d].fab[b]) : null_da();
+            B.w0m[d] = (sph && d < dim) ? to_da(w0mac->dir[d].fab[b]) : null_da();
+        }
+        maxpts = std::max(maxpts, npts(nodal(g, 0)));
+        boxes.push_back(B);
+    }
+    UtArgs A;
+    for (int n = 0; n < 3; ++n)
+        for (int d = 0; d < 3; ++d) {
+            A.bclo[n][d] = c->bcs_u[n].lo[d];
+            A.bchi[n][d] = c->bcs_u[n].hi[d];
+        }
+    Table tab(c);
+    size_t off = tab.add(boxes);
+    int rc = tab.upload();
+    if (rc) return rc;
+    KGeom kg = kgeom(c, lev);
+    const int block = 256;
+    dim3 gr = grid_for(maxpts, (int)boxes.size(), dim, block);
+    const UtBox* dp = tab.dev<UtBox>(off);
+    if (c->p.ppm_type == 0)
+        MX_LAUNCH(c, "k_utrans", k_utrans<0>,  gr, dim3(block), dp, kg, A);
+    else if (c->p.ppm_type == 1)
+        MX_LAUNCH(c, "k_utrans", k_utrans<1>,  gr, dim3(block), dp, kg, A);
+    else
+        MX_LAUNCH(c, "k_utrans", k_utrans<2>,  gr, dim3(block), dp, kg, A);
+
+    if (c->p.finest_level != 0) return MX_ERR_UNSUPPORTED;
+    return fill_face_ghosts(c, lev, utrans);
+}
+
+int velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
+            const mx_face_multifab* utrans, const mx_face_multifab* umac,
+            const mx_face_multifab* w0mac, const mx_multifab* force, const mx_multifab* w0_cart) {
+    const int dim = c->p.dim;
+    const auto& grids = c->grids[lev];
+    const bool sph = dim == 3 && c->p.spherical;
+    const int ppm = c->p.ppm_type;
+    const bool tf = c->p.ppm_trace_forces == 1 && ppm != 0;
+    if (dim == 2 && c->p.ppm_trace_forces == 1) return MX_ERR_UNSUPPORTED;
+    int rc = check_phys(c);
+    if (rc) return rc;
+
+    auto grown1 = [&](HBox b) { return grow(b, dim, 1); };
+    // workspace: Ip/Im (or slopes) per component, traced forces, transverse states
+    size_t need = 2 * dim * tmp_bytes(grids, dim, grown1);
+    if (tf) need += 2 * dim * tmp_bytes(grids, dim, grown1);
+    if (dim == 3) need += 6 * tmp_bytes(grids, 1, grown1);
+    rc = ws_reserve(c, need);
+    if (rc) return rc;
+
+    std::vector<DA> Ip[3], Im[3], Ipf[3], Imf[3], T[6];
+    for (int n = 0; n < dim; ++n) {
+        Ip[n] = tmp_fabs(c, grids, dim, grown1);
+        Im[n] = tmp_fabs(c, grids, dim, grown1);
+        if (tf) {
+            Ipf[n] = tmp_fabs(c, grids, dim, grown1);
+            Imf[n] = tmp_fabs(c, grids, dim, grown1);
+        }
+    }
+    if (dim == 3)
+        for (int p = 0; p < 6; ++p) T[p] = tmp_fabs(c, grids, 1, grown1);
+
+    std::vector<VpBox> boxes;
+    long long maxpts = 0;
+    for (size_t b = 0; b < grids.size(); ++b) {
+        VpBox B;
+        HBox g = hbox(grids[b]);
+        for (int q = 0; q < 3; ++q) {
+            B.lo[q] = g.lo[q];
+            B.hi[q] = g.hi[q];
+        }
+        B.ut = to_da(utilde->fab[b]);
+        B.uf = to_da(ufull->fab[b]);
+        B.fo = to_da(force->fab[b]);
+        B.w0c = (!sph && w0_cart) ? to_da(w0_cart->fab[b]) : null_da();
+        for (int d = 0; d < 3; ++d) {
+            B.tr[d] = d < dim ? to_da(utrans->dir[d].fab[b]) : null_da();
+            B.um[d] = d < dim ? to_da(umac->dir[d].fab[b]) : null_da();
+            B.w0m[d] = (sph && d < dim) ? to_da(w0mac->dir[d].fab[b]) : null_da();
+            B.Ip[d] = d < dim ? Ip[d][b] : null_da();
+            B.Im[d] = d < dim ? Im[d][b] : null_da();
+            B.Ipf[d] = (tf && d < dim) ? Ipf[d][b] : null_da();
+            B.Imf[d] = (tf && d < dim) ? Imf[d][b] : null_da();
+        }
+        for (int p = 0; p < 6; ++p) B.T[p] = dim == 3 ? T[p][b] : null_da();
+        maxpts = std::max(maxpts, npts(grown1(g)));
+        boxes.push_back(B);
+    }
+    VpArgs A;
+    for (int n = 0; n < 3; ++n)
+        for (int d = 0; d < 3; ++d) {
+            A.bclo[n][d] = c->bcs_u[n].lo[d];
+            A.bchi[n][d] = c->bcs_u[n].hi[d];
+        }
+    Table tab(c);
+    size_t off = tab.add(boxes);
+    rc = tab.upload();
+    if (rc) return rc;
+    const VpBox* dp = tab.dev<VpBox>(off);
+    KGeom kg = kgeom(c, lev);
+    const int block = 256;
+    const int nb = (int)boxes.size();
+
+    dim3 gr = grid_for(maxpts, nb, dim, block);
+    if (ppm == 0)
+        MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm<0>,  gr, dim3(block), dp, kg, A);
+    else if (ppm == 1)
+        MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm<1>,  gr, dim3(block), dp, kg, A);
+    else
+        MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm<2>,  gr, dim3(block), dp, kg, A);
+
+
+    if (dim == 3) {
+        MX_LAUNCH(c, "k_vp_trans", k_vp_trans,  grid_for(maxpts, nb, 6, block), dim3(block), dp, kg);
+
+    }
+    MX_LAUNCH(c, "k_vp_vel", k_vp_vel,  grid_for(maxpts, nb, dim, block), dim3(block), dp,
+                       kg);
+
+    // AverageDownFaces(umac) is a no-op on a single level
+    return MX_OK;
+}
+
+}  // namespace mxh
+
+extern "C" {
+
+int mx_adv_make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde,
+                       const mx_multifab* ufull, mx_face_multifab* utrans,
+                       const mx_face_multifab* w0mac, const mx_multifab* w0_cart) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    const int dim = c->p.dim;
+    const int g = c->p.ppm_type == 2 ? 4 : 3;
+    if ((rc = check_mf(c, lev, utilde, dim, g, -1))) return rc;
+    if ((rc = check_mf(c, lev, ufull, dim, 1, -1))) return rc;
+    if ((rc = check_face(c, lev, utrans, 1, 1))) return rc;
+    if (dim == 3 && c->p.spherical) {
+        if ((rc = check_face(c, lev, w0mac, 1, 0))) return rc;
+    } else {
+        if ((rc = check_mf(c, lev, w0_cart, dim, 0, -1))) return rc;
+    }
+    return make_utrans(c, lev, utilde, ufull, utrans, w0mac, w0_cart);
+}
+
+int mx_adv_velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
+                   const mx_face_multifab* utrans, mx_face_multifab* umac,
+                   const mx_face_multifab* w0mac, const mx_multifab* force,
+                   const mx_multifab* w0_cart) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    const int dim = c->p.dim;
+    const int g = c->p.ppm_type == 2 ? 4 : 3;
+    if ((rc = check_mf(c, lev, utilde, dim, g, -1))) return rc;
+    if ((rc = check_mf(c, lev, ufull, dim, 1, -1))) return rc;
+    if ((rc = check_face(c, lev, utrans, 1, 1))) return rc;
+    if ((rc = check_face(c, lev, umac, 1, 0))) return rc;
+    const int fg = (c->p.ppm_trace_forces == 1) ? g : 1;
+    if ((rc = check_mf(c, lev, force, dim, fg, -1))) return rc;
+    if (dim == 3 && c->p.spherical) {
+        if ((rc = check_face(c, lev, w0mac, 1, 0))) return rc;
+    } else {
+        if ((rc = check_mf(c, lev, w0_cart, dim, 0, -1))) return rc;
+    }
+    return velpred(c, lev, utilde, ufull, utrans, umac, w0mac, force, w0_cart);
+}
+
+}  // extern "C"

@@ -1,0 +1,275 @@
+"""Level/BoxArray/MultiFab containers over numpy (host) or torch (device) arrays.
+
+A FAB for box [lo, hi] with `ng` ghosts is stored as an array of shape
+(ncomp, nz, ny, nx) -- i fastest, component outermost, the AMReX Array4
+order -- and described to the C-ABI by an `mx_fab` (maestroex_amd._abi.Fab).
+Face-centred data ("nodal" in one direction) have one more point in that
+direction, as in AMReX.  2-D data use nz == 1 and k == 0.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+class Geometry:
+    """geom[lev] + the boundary set-up of one level (Maestro::geom, phys_bc)."""
+
+    def __init__(self, dim, n_cells, dx, is_periodic, phys_bc_lo, phys_bc_hi, domain_lo=None):
+        self.dim = dim
+        lo = list(domain_lo) if domain_lo is not None else [0] * dim
+        self.domain_lo = (lo + [0, 0, 0])[:3]
+        hi = [lo[d] + n_cells[d] - 1 for d in range(dim)]
+        self.domain_hi = (hi + [0, 0, 0])[:3]
+        self.n_cells = list(n_cells) + [1] * (3 - dim)
+        self.dx = (list(dx) + [1.0, 1.0, 1.0])[:3]
+        self.is_periodic = (list(is_periodic) + [0, 0, 0])[:3]
+        self.phys_bc_lo = (list(phys_bc_lo) + [0, 0, 0])[:3]
+        self.phys_bc_hi = (list(phys_bc_hi) + [0, 0, 0])[:3]
+
+
+def split_domain(geom, max_grid):
+    """Chop the domain into boxes of at most `max_grid` cells per direction
+    (BoxArray::maxSize), ordered with x fastest."""
+    dim = geom.dim
+    mg = [max_grid] * 3 if np.isscalar(max_grid) else list(max_grid) + [1] * (3 - len(max_grid))
+    ranges = []
+    for d in range(3):
+        if d >= dim:
+            ranges.append([(0, 0)])
+            continue
+        lo, hi = geom.domain_lo[d], geom.domain_hi[d]
+        cuts = []
+        a = lo
+        while a <= hi:
+            b = min(hi, a + mg[d] - 1)
+            cuts.append((a, b))
+            a = b + 1
+        ranges.append(cuts)
+    boxes = []
+    for kz in ranges[2]:
+        for jy in ranges[1]:
+            for ix in ranges[0]:
+                boxes.append(((ix[0], jy[0], kz[0]), (ix[1], jy[1], kz[1])))
+    return boxes
+
+
+class Backend:
+    """numpy (host) or torch (device) array factory."""
+
+    def __init__(self, kind="numpy", device=None):
+        self.kind = kind
+        self.device = device
+        if kind == "torch":
+            import torch  # noqa: F401
+
+    def zeros(self, shape):
+        if self.kind == "numpy":
+            return np.zeros(shape, dtype=np.float64)
+        import torch
+        return torch.zeros(shape, dtype=torch.float64, device=self.device)
+
+    def full(self, shape, val):
+        if self.kind == "numpy":
+            return np.full(shape, val, dtype=np.float64)
+        import torch
+        return torch.full(shape, val, dtype=torch.float64, device=self.device)
+
+    def from_numpy(self, a):
+        if self.kind == "numpy":
+            return np.ascontiguousarray(a, dtype=np.float64).copy()
+        import torch
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(self.device)
+
+    def to_numpy(self, a):
+        if self.kind == "numpy":
+            return a
+        return a.detach().cpu().numpy()
+
+    def ptr(self, a):
+        if self.kind == "numpy":
+            return a.ctypes.data
+        return a.data_ptr()
+
+
+class MultiFab:
+    """One level of a cell- or face-centred MultiFab (one array per box)."""
+
+    def __init__(self, boxes, ncomp, ngrow, dim, backend, nodal_dir=-1, fill=0.0):
+        self.boxes = boxes
+        self.ncomp = ncomp
+        self.ngrow = ngrow
+        self.dim = dim
+        self.nodal_dir = nodal_dir
+        self.backend = backend
+        self.arrays = []
+        self.begin = []
+        for lo, hi in boxes:
+            lo = list(lo)
+            hi = list(hi)
+            if nodal_dir >= 0:
+                hi[nodal_dir] += 1
+            b = [lo[d] - ngrow if d < dim else 0 for d in range(3)]
+            e = [hi[d] + ngrow + 1 if d < dim else 1 for d in range(3)]
+            shape = (ncomp, e[2] - b[2], e[1] - b[1], e[0] - b[0])
+            self.arrays.append(backend.full(shape, fill) if fill != 0.0 else backend.zeros(shape))
+            self.begin.append(b)
+        self._desc = None
+
+    def fab_desc(self, b):
+        a = self.arrays[b]
+        beg = self.begin[b]
+        nc, nz, ny, nx = a.shape
+        f = _abi.Fab()
+        f.p = self.backend.ptr(a)
+        f.jstride = nx
+        f.kstride = nx * ny
+        f.nstride = nx * ny * nz
+        for d in range(3):
+            f.begin[d] = beg[d]
+        f.end[0] = beg[0] + nx
+        f.end[1] = beg[1] + ny
+        f.end[2] = beg[2] + nz
+        f.ncomp = nc
+        return f
+
+    def desc(self):
+        n = len(self.arrays)
+        fabs = (_abi.Fab * n)(*[self.fab_desc(b) for b in range(n)])
+        m = _abi.MultiFabDesc()
+        m.nfab = n
+        m.fab = C.cast(fabs, C.POINTER(_abi.Fab))
+        m._keep = fabs
+        return m
+
+    # --- index helpers -------------------------------------------------
+    def view(self, b, comp=None):
+        """Array of box b (all comps, or one) as (comp, k, j, i)."""
+        a = self.arrays[b]
+        return a if comp is None else a[comp]
+
+    def index(self, b, i, j, k):
+        beg = self.begin[b]
+        return (k - beg[2], j - beg[1], i - beg[0])
+
+    def set_from_global(self, b, gfun):
+        """Fill box b (valid + ghosts) from gfun(i_idx, j_idx, k_idx) -> (ncomp, ...)."""
+        a = self.arrays[b]
+        beg = self.begin[b]
+        nc, nz, ny, nx = a.shape
+        ii = np.arange(beg[0], beg[0] + nx)
+        jj = np.arange(beg[1], beg[1] + ny)
+        kk = np.arange(beg[2], beg[2] + nz)
+        vals = gfun(ii, jj, kk)
+        self.arrays[b] = self.backend.from_numpy(vals.reshape(a.shape))
+
+    def to_numpy(self):
+        return [np.array(self.backend.to_numpy(a)) for a in self.arrays]
+
+    def valid_region(self, b):
+        """(k, j, i) slices of the valid (non-ghost) points of box b."""
+        lo, hi = self.boxes[b]
+        hi = list(hi)
+        if self.nodal_dir >= 0:
+            hi[self.nodal_dir] += 1
+        beg = self.begin[b]
+        sl = []
+        for d in (2, 1, 0):
+            if d >= self.dim:
+                sl.append(slice(0, 1))
+            else:
+                sl.append(slice(lo[d] - beg[d], hi[d] - beg[d] + 1))
+        return tuple(sl)
+
+
+class FaceMultiFab:
+    def __init__(self, boxes, ncomp, ngrow, dim, backend, fill=0.0):
+        self.dim = dim
+        self.d = [MultiFab(boxes, ncomp, ngrow, dim, backend, nodal_dir=d, fill=fill) for d in range(dim)]
+
+    def __getitem__(self, d):
+        return self.d[d]
+
+    def desc(self):
+        f = _abi.FaceMultiFabDesc()
+        keep = []
+        for d in range(self.dim):
+            m = self.d[d].desc()
+            keep.append(m)
+            f.dir[d] = m
+        f._keep = keep
+        return f
+
+
+def empty_face_desc():
+    return _abi.FaceMultiFabDesc()
+
+
+def empty_mf_desc():
+    return _abi.MultiFabDesc()
+
+
+class BaseState:
+    """BaseState<Real> with one variable: data[lev, r]."""
+
+    def __init__(self, data, backend):
+        data = np.atleast_2d(np.asarray(data, dtype=np.float64))
+        self.nlev, self.len = data.shape
+        self.backend = backend
+        self.array = backend.from_numpy(data)
+
+    def desc(self):
+        s = _abi.BaseStateDesc()
+        s.p = self.backend.ptr(self.array)
+        s.nlev = self.nlev
+        s.len = self.len
+        return s
+
+    def to_numpy(self):
+        return np.array(self.backend.to_numpy(self.array))
+
+
+def make_params(geom, ppm_type=1, nspec=3, *, slope_order=4, ppm_trace_forces=0,
+                species_pred_type=1, enthalpy_pred_type=1, spherical=0,
+                use_exact_base_state=0, evolve_base_state=1, dt=0.0, rel_eps=0.0,
+                naux=0, extra_levels=()):
+    """mx_adv_params for one level (or several: extra_levels = [Geometry, ...])."""
+    p = _abi.Params()
+    p.dim = geom.dim
+    p.ppm_type = ppm_type
+    p.slope_order = slope_order
+    p.ppm_trace_forces = ppm_trace_forces
+    p.species_pred_type = species_pred_type
+    p.enthalpy_pred_type = enthalpy_pred_type
+    p.spherical = spherical
+    p.use_exact_base_state = use_exact_base_state
+    p.evolve_base_state = evolve_base_state
+    p.dt = dt
+    p.rel_eps = rel_eps
+    geoms = [geom] + list(extra_levels)
+    p.finest_level = len(geoms) - 1
+    for lev, g in enumerate(geoms):
+        for d in range(3):
+            p.domain_lo[lev][d] = g.domain_lo[d]
+            p.domain_hi[lev][d] = g.domain_hi[d]
+            p.dx[lev][d] = g.dx[d]
+    for d in range(3):
+        p.is_periodic[d] = geom.is_periodic[d]
+        p.phys_bc_lo[d] = geom.phys_bc_lo[d]
+        p.phys_bc_hi[d] = geom.phys_bc_hi[d]
+    # Rho, RhoH, Temp, Pi, FirstSpec.. (Source/param/_variables:8-14)
+    p.comp_rho, p.comp_rhoh, p.comp_temp, p.comp_pi = 0, 1, 2, 3
+    p.first_spec = 4
+    p.num_spec = nspec
+    p.nscal = 4 + nspec + naux
+    return p
+
+
+def boxes_desc(boxes):
+    arr = (_abi.Box * len(boxes))()
+    for b, (lo, hi) in enumerate(boxes):
+        for d in range(3):
+            arr[b].lo[d] = lo[d]
+            arr[b].hi[d] = hi[d]
+    return arr

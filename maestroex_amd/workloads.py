@@ -1,0 +1,234 @@
+"""Seeded synthetic inputs for the advection pass.
+
+The construction (one uint64 seed per workload):
+  * domain [0,1]^D, n^D cells, uniform dx = 1/n, ghost width `ng`;
+  * periodic in every direction but the vertical one (y in 2-D, z in 3-D),
+    which is an outflow boundary (foextrap ghost cells), unless
+    `all_periodic`;
+  * rho = exp(-r/0.3) * (1 + 0.1 exp(-d^2/0.05^2)): a Gaussian bubble at the
+    domain centre on a hydrostatic exponential background (multiplicative,
+    following the exp(-d^2/W^2) form of the reference's test_advect
+    initial data, Exec/unit_tests/test_advect/MaestroInitData.cpp:43-44);
+  * rho0 is the horizontal average of rho; the advected density is
+    rho' = rho - rho0 (PutInPertForm);
+  * h = 2 - r (a smooth enthalpy), rhoh = rho h, rhoh0 its horizontal average,
+    the advected enthalpy is (rho h)' = rhoh - rhoh0;
+  * NumSpec mass fractions uniform random in [0,1], normalised to sum 1 (the
+    advected species are X, after ConvertRhoXToX);
+  * velocity = sum of `nmodes` Fourier modes c_m cos(2 pi k_m.x + phi_m) with
+    c_m perpendicular to k_m (divergence free), integer wavenumbers in the
+    periodic directions, scaled to max|u| = 1;
+  * w0(r) = 0.01 * r_edge (planar base-state velocity at cell edges),
+    w0_cart its cell-centred average (Put1dArrayOnCart), ufull = utilde +
+    w0_cart;
+  * smooth velocity and scalar forcing terms;
+  * dt from CFL 0.5 on the full velocity, rel_eps = 1e-8 * umax.
+Ghost cells are filled as FillPatch would: periodic images in periodic
+directions, first-order extrapolation (copy of the nearest valid cell) at
+the outflow boundary.
+"""
+import numpy as np
+
+from . import _abi
+from .fabs import BaseState, FaceMultiFab, Geometry, MultiFab, make_params, split_domain
+
+
+def cell_to_edge(s0):
+    """Maestro::CelltoEdge (MaestroCelltoEdge.cpp:5-55), one radial level."""
+    nr = len(s0)
+    e = np.zeros(nr + 1)
+    for r in range(nr + 1):
+        if r == 0:
+            e[r] = s0[r]
+        elif r == 1:
+            e[r] = 0.5 * (s0[r - 1] + s0[r])
+        elif r == nr:
+            e[r] = s0[r - 1]
+        elif r == nr - 1:
+            e[r] = 0.5 * (s0[r] + s0[r - 1])
+        else:
+            tmp = 7.0 / 12.0 * (s0[r] + s0[r - 1]) - 1.0 / 12.0 * (s0[r + 1] + s0[r - 2])
+            lo = min(s0[r], s0[r - 1])
+            hi = max(s0[r], s0[r - 1])
+            e[r] = min(max(tmp, lo), hi)
+    return e
+
+
+class Workload:
+    """All inputs and outputs of one level of the pass, on one backend."""
+
+    def __init__(self, dim=3, n=32, max_grid=16, ppm_type=1, nspec=3, seed=1234, nmodes=None,
+                 ng=4, all_periodic=False, backend=None, ppm_trace_forces=0,
+                 phys_bc=None, zero_velocity=False):
+        from .fabs import Backend
+        self.backend = backend or Backend("numpy")
+        self.dim = dim
+        self.n = n
+        self.ng = ng
+        nm = nmodes if nmodes is not None else (8 if dim == 2 else 16)
+        vert = dim - 1
+        periodic = [1] * dim
+        lo_bc = [_abi.PHYS_INTERIOR] * dim
+        hi_bc = [_abi.PHYS_INTERIOR] * dim
+        if not all_periodic:
+            periodic[vert] = 0
+            lo_bc[vert] = _abi.PHYS_OUTFLOW
+            hi_bc[vert] = _abi.PHYS_OUTFLOW
+        if phys_bc is not None:
+            # explicit physical boundaries: {dir: (lo, hi)}; non-interior => not periodic
+            for d, (bl, bh) in phys_bc.items():
+                lo_bc[d], hi_bc[d] = bl, bh
+                periodic[d] = 1 if bl == _abi.PHYS_INTERIOR else 0
+        self.geom = Geometry(dim, [n] * dim, [1.0 / n] * dim, periodic, lo_bc, hi_bc)
+        self.boxes = split_domain(self.geom, max_grid)
+        rng = np.random.default_rng(seed)
+
+        nscal = 4 + nspec
+        N = n
+        dx = 1.0 / n
+        ext = N + 2 * ng
+        # global index arrays including ghosts, mapped to valid cells
+        gidx = np.arange(-ng, N + ng)
+
+        def mapped(d):
+            if periodic[d]:
+                return np.mod(gidx, N)
+            return np.clip(gidx, 0, N - 1)
+
+        mi = [mapped(d) for d in range(dim)]
+        xc = [(mi[d] + 0.5) * dx for d in range(dim)]
+
+        def grid(arrs):
+            # broadcast per-direction 1-D arrays to (z, y, x) or (y, x)
+            if dim == 3:
+                return arrs[0][None, None, :], arrs[1][None, :, None], arrs[2][:, None, None]
+            return arrs[0][None, :], arrs[1][:, None]
+
+        X = grid(xc)
+        r = X[vert]
+        dist2 = sum((X[d] - 0.5) ** 2 for d in range(dim))
+        rho = np.exp(-r / 0.3) * (1.0 + 0.1 * np.exp(-dist2 / 0.05 ** 2))
+        rho = np.broadcast_to(rho, (ext,) * dim).copy()
+        h = np.broadcast_to(2.0 - r, (ext,) * dim)
+        rhoh = rho * h
+        valid = tuple(slice(ng, ng + N) for _ in range(dim))
+        haxes = tuple(a for a in range(dim) if a != 0)  # all but vertical (axis 0 = vertical)
+        rho0 = rho[valid].mean(axis=haxes)
+        rhoh0 = rhoh[valid].mean(axis=haxes)
+        rmap = mi[vert]
+        shp = [1] * dim
+        shp[0] = ext
+        rhop = rho - rho0[rmap].reshape(shp)
+        rhohp = rhoh - rhoh0[rmap].reshape(shp)
+
+        Xs = rng.random((nspec,) + (N,) * dim)
+        Xs /= Xs.sum(axis=0, keepdims=True)
+        ix = np.ix_(*[mi[d] for d in reversed(range(dim))])
+        Xg = np.stack([Xs[k][ix] for k in range(nspec)])
+
+        # velocity: divergence-free Fourier modes
+        u = np.zeros((dim,) + (ext,) * dim)
+        if not zero_velocity:
+            for m in range(nm):
+                k = np.zeros(dim)
+                for d in range(dim):
+                    if periodic[d]:
+                        k[d] = rng.integers(-3, 4)
+                    else:
+                        k[d] = rng.uniform(-2.0, 2.0)
+                if not np.any(k):
+                    k[0] = 1.0
+                a = rng.normal(size=dim)
+                if dim == 2:
+                    cvec = np.array([-k[1], k[0]]) * rng.normal()
+                else:
+                    cvec = np.cross(k, a)
+                phi = rng.uniform(0, 2 * np.pi)
+                ph = [np.exp(1j * 2 * np.pi * k[d] * xc[d]) for d in range(dim)]
+                G = grid(ph)
+                mode = G[0] * G[1] * (G[2] if dim == 3 else 1.0) * np.exp(1j * phi)
+                cm = np.real(mode)
+                for d in range(dim):
+                    u[d] += cvec[d] * cm
+            umax0 = np.max(np.abs(u[(slice(None),) + valid]))
+            u /= umax0
+
+        nr = N
+        r_edge = np.arange(nr + 1) * dx
+        # a planar w0 is a function of height; with a periodic vertical
+        # direction it must vanish (the faces r = 0 and r = nr coincide)
+        w0 = np.zeros(nr + 1) if periodic[vert] else 0.01 * r_edge
+        w0c = 0.5 * (w0[:-1] + w0[1:])
+        w0_cart = np.zeros_like(u)
+        w0_cart[vert] = np.broadcast_to(w0c[rmap].reshape(shp), (ext,) * dim)
+        ufull = u + w0_cart
+
+        umax = float(np.max(np.abs(ufull[(slice(None),) + valid])))
+        if umax == 0.0:
+            umax = 1.0
+        self.dt = 0.5 * dx / umax
+        self.rel_eps = 1.0e-8 * umax
+
+        # forcing
+        arg = 2 * np.pi * sum((d + 1) * X[d] for d in range(dim))
+        vf = np.stack([0.05 * np.cos(arg + c) * np.ones((ext,) * dim) for c in range(dim)])
+        vf[vert] += -0.1 * rhop / rho
+        sf = np.stack([0.02 * np.sin(arg + 0.5 * c) * np.ones((ext,) * dim) for c in range(nscal)])
+
+        scal = np.zeros((nscal,) + (ext,) * dim)
+        scal[0] = rhop
+        scal[1] = rhohp
+        scal[4:4 + nspec] = Xg
+
+        self.params = make_params(self.geom, ppm_type=ppm_type, nspec=nspec, dt=self.dt,
+                                  rel_eps=self.rel_eps, ppm_trace_forces=ppm_trace_forces)
+        bk = self.backend
+        fng = ng if ppm_trace_forces else 1
+
+        def cellmf(data, ncomp, ngrow):
+            mf = MultiFab(self.boxes, ncomp, ngrow, dim, bk)
+            for b, (lo, hi) in enumerate(self.boxes):
+                sl = [slice(lo[d] - ngrow + ng, hi[d] + ngrow + 1 + ng) for d in reversed(range(dim))]
+                sub = data[(slice(None),) + tuple(sl)]
+                if dim == 2:
+                    sub = sub[:, None, :, :]
+                mf.arrays[b] = bk.from_numpy(sub)
+            return mf
+
+        self.utilde = cellmf(u, dim, ng)
+        self.ufull = cellmf(ufull, dim, ng)
+        self.w0_cart = cellmf(w0_cart, dim, min(2, ng))
+        self.vel_force = cellmf(vf, dim, fng)
+        self.scal = cellmf(scal, nscal, ng)
+        self.scal_force = cellmf(sf, nscal, fng)
+
+        self.w0 = BaseState(w0, bk)
+        self.rho0 = BaseState(rho0, bk)
+        self.rhoh0 = BaseState(rhoh0, bk)
+        self.rho0_edge = BaseState(cell_to_edge(rho0), bk)
+        self.rhoh0_edge = BaseState(cell_to_edge(rhoh0), bk)
+        self.rho0_predicted_edge = BaseState(cell_to_edge(rho0), bk)
+
+        self.nscal = nscal
+        self.alloc_outputs()
+
+    def alloc_outputs(self, fill=0.0):
+        bk = self.backend
+        dim = self.dim
+        self.utrans = FaceMultiFab(self.boxes, 1, 1, dim, bk, fill=fill)
+        self.umac = FaceMultiFab(self.boxes, 1, 1, dim, bk, fill=fill)
+        self.sedge = FaceMultiFab(self.boxes, self.nscal, 0, dim, bk, fill=fill)
+        self.sflux = FaceMultiFab(self.boxes, self.nscal, 0, dim, bk, fill=fill)
+        self.etarhoflux = MultiFab(self.boxes, 1, 1, dim, bk, nodal_dir=dim - 1, fill=fill)
+
+    def pass_fields(self):
+        return dict(utilde=self.utilde, ufull=self.ufull, vel_force=self.vel_force,
+                    w0_cart=self.w0_cart, scal=self.scal, scal_force=self.scal_force,
+                    w0mac=None, w0=self.w0, rho0=self.rho0, rho0_edge=self.rho0_edge,
+                    rhoh0=self.rhoh0, rhoh0_edge=self.rhoh0_edge,
+                    rho0_predicted_edge=self.rho0_predicted_edge, utrans=self.utrans,
+                    umac=self.umac, sedge=self.sedge, sflux=self.sflux,
+                    etarhoflux=self.etarhoflux)
+
+    def valid_cells(self):
+        return self.n ** self.dim

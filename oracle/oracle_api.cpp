@@ -1,0 +1,333 @@
+// oracle_api.cpp -- C entry points of the CPU oracle (liboracle.so).
+//
+// TEST INFRASTRUCTURE ONLY (see oracle_core.H).  The functions mirror the
+// mx_adv_* entry points of include/mx_adv.h one for one (prefix mxo_ instead
+// of mx_adv_), with every field in host memory, so that tests can run the
+// same call sequence on the oracle and on the HIP library and compare.
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle_core.H"
+
+namespace mxo {
+
+std::atomic<int> g_oob{0};
+
+void PPM(const Ctx& c, const IBox& bx, const Arr& s, const Arr& u,
+         const Arr& v, const Arr& w, const Arr& Ip, const Arr& Im,
+         const IBox& dom, const mx_bcrec* bcs, const double* dx, bool is_umac,
+         int comp, int bccomp);
+void Slope(int dir, int slope_order, const IBox& bx, const Arr& s,
+           const Arr& sl, const IBox& dom, const mx_bcrec* bcs, int ncomp,
+           int bc_start_comp);
+void BCSetup(const Ctx& c, std::vector<mx_bcrec>& bcs_s,
+             std::vector<mx_bcrec>& bcs_u);
+void FillFaceGhosts(const Ctx& c, int lev, const std::vector<Arr> uedge[3]);
+int MakeUtrans(const Ctx& c, int lev, const std::vector<Arr>& utilde,
+               const std::vector<Arr>& ufull, const std::vector<Arr> utrans[3],
+               const std::vector<Arr> w0mac[3], const std::vector<Arr>& w0_cart,
+               const std::vector<mx_bcrec>& bcs_u);
+int Addw0(const Ctx& c, int lev, const std::vector<Arr> uedge[3],
+          const std::vector<Arr> w0mac[3], const BaseArr& w0, double mult);
+int VelPred(const Ctx& c, int lev, const std::vector<Arr>& utilde,
+            const std::vector<Arr>& ufull, const std::vector<Arr> utrans[3],
+            const std::vector<Arr> umac[3], const std::vector<Arr> w0mac[3],
+            const std::vector<Arr>& force, const std::vector<Arr>& w0_cart,
+            const std::vector<mx_bcrec>& bcs_u);
+int MakeEdgeScal(const Ctx& c, int lev, const std::vector<Arr>& state,
+                 const std::vector<Arr> sedge[3], const std::vector<Arr> umac[3],
+                 const std::vector<Arr>& force, bool is_vel,
+                 const mx_bcrec* bcs, int start_scomp, int start_bccomp,
+                 int num_comp, bool is_conservative);
+int MakeRhoXFlux(const Ctx& c, int lev, const std::vector<Arr> sflux[3],
+                 const std::vector<Arr>& etarhoflux, const std::vector<Arr> sedge[3],
+                 const std::vector<Arr> umac[3], const BaseArr& rho0_old,
+                 const BaseArr& rho0_edge_old, const std::vector<Arr> r0mac_old[3],
+                 const BaseArr& rho0_new, const BaseArr& rho0_edge_new,
+                 const std::vector<Arr> r0mac_new[3], const BaseArr& rho0_predicted_edge,
+                 const BaseArr& w0, int start_comp, int num_comp);
+int MakeRhoHFlux(const Ctx& c, int lev, const std::vector<Arr> sflux[3],
+                 const std::vector<Arr> sedge[3], const std::vector<Arr> umac[3],
+                 const BaseArr& rho0_old, const BaseArr& rho0_edge_old,
+                 const std::vector<Arr> r0mac_old[3], const BaseArr& rho0_new,
+                 const BaseArr& rho0_edge_new, const std::vector<Arr> r0mac_new[3],
+                 const BaseArr& rhoh0_old, const BaseArr& rhoh0_edge_old,
+                 const std::vector<Arr> rh0mac_old[3], const BaseArr& rhoh0_new,
+                 const BaseArr& rhoh0_edge_new, const std::vector<Arr> rh0mac_new[3],
+                 const std::vector<Arr> h0mac_old[3], const std::vector<Arr> h0mac_new[3]);
+void CelltoEdge(int lev, int nr, const mx_basestate* cell, mx_basestate* edge);
+
+}  // namespace mxo
+
+using namespace mxo;
+
+namespace {
+
+struct Face {
+    std::vector<Arr> d[3];
+};
+Face faces(const mx_face_multifab* f) {
+    Face r;
+    if (f)
+        for (int d = 0; d < 3; ++d) r.d[d] = arrs(&f->dir[d]);
+    return r;
+}
+
+int finish(int rc) {
+    if (g_oob.exchange(0) != 0) return MX_ERR_BOX;
+    return rc;
+}
+
+int check_level(const Ctx* c, int lev) {
+    if (!c || lev < 0 || lev > c->p.finest_level) return MX_ERR_ARG;
+    if (c->grids[lev].empty()) return MX_ERR_ARG;
+    return MX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mxo_create(const mx_adv_params* p, Ctx** out) {
+    if (!p || !out) return MX_ERR_ARG;
+    if (p->dim != 2 && p->dim != 3) return MX_ERR_ARG;
+    if (p->ppm_type < 0 || p->ppm_type > 2) return MX_ERR_ARG;
+    if (p->finest_level < 0 || p->finest_level >= MX_MAX_LEV) return MX_ERR_ARG;
+    if (p->nscal <= 0 || p->nscal > MX_MAX_SCAL) return MX_ERR_ARG;
+    if (p->spherical && p->dim != 3) return MX_ERR_ARG;
+    Ctx* c = new Ctx();
+    c->p = *p;
+    *out = c;
+    return MX_OK;
+}
+
+int mxo_destroy(Ctx* c) {
+    delete c;
+    return MX_OK;
+}
+
+int mxo_set_level(Ctx* c, int lev, int nbox, const mx_box* boxes) {
+    if (!c || lev < 0 || lev > c->p.finest_level || nbox <= 0 || !boxes) return MX_ERR_ARG;
+    c->grids[lev].clear();
+    for (int b = 0; b < nbox; ++b) {
+        IBox bx = make_box(boxes[b].lo, boxes[b].hi);
+        if (c->p.dim == 2) bx.lo[2] = bx.hi[2] = 0;
+        c->grids[lev].push_back(bx);
+    }
+    return MX_OK;
+}
+
+int mxo_set_dt(Ctx* c, double dt, double rel_eps) {
+    if (!c) return MX_ERR_ARG;
+    c->p.dt = dt;
+    c->p.rel_eps = rel_eps;
+    return MX_OK;
+}
+
+int mxo_bc_setup(const Ctx* c, mx_bcrec* bcs_s, mx_bcrec* bcs_u) {
+    if (!c) return MX_ERR_ARG;
+    std::vector<mx_bcrec> s, u;
+    BCSetup(*c, s, u);
+    if (bcs_s) std::memcpy(bcs_s, s.data(), sizeof(mx_bcrec) * s.size());
+    if (bcs_u) std::memcpy(bcs_u, u.data(), sizeof(mx_bcrec) * c->p.dim);
+    return MX_OK;
+}
+
+int mxo_ppm(Ctx* c, int lev, const mx_box* bx, const mx_fab* s, const mx_fab* u,
+            const mx_fab* v, const mx_fab* w, mx_fab* Ip, mx_fab* Im,
+            const mx_bcrec* bcs, int nbcs, int is_umac, int comp, int bccomp) {
+    if (!c || !bx || !s || !u || !v || !Ip || !Im || !bcs) return MX_ERR_ARG;
+    if (c->p.dim == 3 && !w) return MX_ERR_ARG;
+    if (bccomp < 0 || bccomp >= nbcs) return MX_ERR_ARG;
+    IBox b = make_box(bx->lo, bx->hi);
+    if (c->p.dim == 2) b.lo[2] = b.hi[2] = 0;
+    Arr wa;
+    if (w) wa = from_fab(*w);
+    PPM(*c, b, from_fab(*s), from_fab(*u), from_fab(*v), wa, from_fab(*Ip),
+        from_fab(*Im), c->domain(lev), bcs, c->p.dx[lev], is_umac != 0, comp,
+        bccomp);
+    return finish(MX_OK);
+}
+
+int mxo_slope(Ctx* c, int lev, int dir, const mx_box* bx, const mx_fab* s,
+              mx_fab* sl, const mx_bcrec* bcs, int nbcs, int ncomp,
+              int bc_start_comp, int slope_order) {
+    if (!c || !bx || !s || !sl || !bcs || dir < 0 || dir >= c->p.dim) return MX_ERR_ARG;
+    if (bc_start_comp < 0 || bc_start_comp + ncomp > nbcs) return MX_ERR_ARG;
+    IBox b = make_box(bx->lo, bx->hi);
+    if (c->p.dim == 2) b.lo[2] = b.hi[2] = 0;
+    Slope(dir, slope_order, b, from_fab(*s), from_fab(*sl), c->domain(lev), bcs,
+          ncomp, bc_start_comp);
+    return finish(MX_OK);
+}
+
+int mxo_make_utrans(Ctx* c, int lev, const mx_multifab* utilde,
+                    const mx_multifab* ufull, mx_face_multifab* utrans,
+                    const mx_face_multifab* w0mac, const mx_multifab* w0_cart) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    std::vector<mx_bcrec> bs, bu;
+    BCSetup(*c, bs, bu);
+    Face tr = faces(utrans), wm = faces(w0mac);
+    rc = MakeUtrans(*c, lev, arrs(utilde), arrs(ufull), tr.d, wm.d, arrs(w0_cart), bu);
+    return finish(rc);
+}
+
+int mxo_addw0(Ctx* c, int lev, mx_face_multifab* uedge, const mx_face_multifab* w0mac,
+              const mx_basestate* w0, double mult) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    Face ue = faces(uedge), wm = faces(w0mac);
+    rc = Addw0(*c, lev, ue.d, wm.d, base(w0), mult);
+    return finish(rc);
+}
+
+int mxo_fill_face_ghosts(Ctx* c, int lev, mx_face_multifab* uedge) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    Face ue = faces(uedge);
+    FillFaceGhosts(*c, lev, ue.d);
+    return finish(MX_OK);
+}
+
+int mxo_velpred(Ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
+                const mx_face_multifab* utrans, mx_face_multifab* umac,
+                const mx_face_multifab* w0mac, const mx_multifab* force,
+                const mx_multifab* w0_cart) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    std::vector<mx_bcrec> bs, bu;
+    BCSetup(*c, bs, bu);
+    Face tr = faces(utrans), um = faces(umac), wm = faces(w0mac);
+    rc = VelPred(*c, lev, arrs(utilde), arrs(ufull), tr.d, um.d, wm.d, arrs(force),
+                 arrs(w0_cart), bu);
+    return finish(rc);
+}
+
+int mxo_make_edge_scal(Ctx* c, int lev, const mx_multifab* state, mx_face_multifab* sedge,
+                       const mx_face_multifab* umac, const mx_multifab* force, int is_vel,
+                       const mx_bcrec* bcs, int nbccomp, int start_scomp, int start_bccomp,
+                       int num_comp, int is_conservative) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!bcs || start_bccomp < 0 || start_bccomp + num_comp > nbccomp) return MX_ERR_ARG;
+    Face se = faces(sedge), um = faces(umac);
+    rc = MakeEdgeScal(*c, lev, arrs(state), se.d, um.d, arrs(force), is_vel != 0, bcs,
+                      start_scomp, start_bccomp, num_comp, is_conservative != 0);
+    return finish(rc);
+}
+
+int mxo_make_rhox_flux(Ctx* c, int lev, mx_face_multifab* sflux, mx_multifab* etarhoflux,
+                       const mx_face_multifab* sedge, const mx_face_multifab* umac,
+                       const mx_basestate* rho0_old, const mx_basestate* rho0_edge_old,
+                       const mx_face_multifab* r0mac_old, const mx_basestate* rho0_new,
+                       const mx_basestate* rho0_edge_new, const mx_face_multifab* r0mac_new,
+                       const mx_basestate* rho0_predicted_edge, const mx_basestate* w0,
+                       int start_comp, int num_comp) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    Face sf = faces(sflux), se = faces(sedge), um = faces(umac), ro = faces(r0mac_old),
+         rn = faces(r0mac_new);
+    rc = MakeRhoXFlux(*c, lev, sf.d, arrs(etarhoflux), se.d, um.d, base(rho0_old),
+                      base(rho0_edge_old), ro.d, base(rho0_new), base(rho0_edge_new), rn.d,
+                      base(rho0_predicted_edge), base(w0), start_comp, num_comp);
+    return finish(rc);
+}
+
+int mxo_make_rhoh_flux(Ctx* c, int lev, mx_face_multifab* sflux, const mx_face_multifab* sedge,
+                       const mx_face_multifab* umac, const mx_face_multifab* w0mac,
+                       const mx_basestate* rho0_old, const mx_basestate* rho0_edge_old,
+                       const mx_face_multifab* r0mac_old, const mx_basestate* rho0_new,
+                       const mx_basestate* rho0_edge_new, const mx_face_multifab* r0mac_new,
+                       const mx_basestate* rhoh0_old, const mx_basestate* rhoh0_edge_old,
+                       const mx_face_multifab* rh0mac_old, const mx_basestate* rhoh0_new,
+                       const mx_basestate* rhoh0_edge_new, const mx_face_multifab* rh0mac_new,
+                       const mx_face_multifab* h0mac_old, const mx_face_multifab* h0mac_new) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    (void)w0mac;
+    Face sf = faces(sflux), se = faces(sedge), um = faces(umac), ro = faces(r0mac_old),
+         rn = faces(r0mac_new), rho_ = faces(rh0mac_old), rhn = faces(rh0mac_new),
+         ho = faces(h0mac_old), hn = faces(h0mac_new);
+    rc = MakeRhoHFlux(*c, lev, sf.d, se.d, um.d, base(rho0_old), base(rho0_edge_old), ro.d,
+                      base(rho0_new), base(rho0_edge_new), rn.d, base(rhoh0_old),
+                      base(rhoh0_edge_old), rho_.d, base(rhoh0_new), base(rhoh0_edge_new),
+                      rhn.d, ho.d, hn.d);
+    return finish(rc);
+}
+
+int mxo_cell_to_edge(Ctx* c, int lev, int nr, const mx_basestate* s0_cell,
+                     mx_basestate* s0_edge) {
+    if (!c || !s0_cell || !s0_edge || nr < 4) return MX_ERR_ARG;
+    CelltoEdge(lev, nr, s0_cell, s0_edge);
+    return MX_OK;
+}
+
+// The pass of mx_adv_advance_pass (include/mx_adv.h).
+int mxo_advance_pass(Ctx* c, int lev, const mx_adv_pass_args* a) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!a) return MX_ERR_ARG;
+    if (c->p.spherical) return MX_ERR_UNSUPPORTED;
+    std::vector<mx_bcrec> bs, bu;
+    BCSetup(*c, bs, bu);
+    const mx_adv_params& p = c->p;
+
+    Face utrans = faces(&a->utrans), umac = faces(&a->umac), sedge = faces(&a->sedge),
+         sflux = faces(&a->sflux), w0mac = faces(&a->w0mac), none;
+    auto utilde = arrs(&a->utilde), ufull = arrs(&a->ufull), vel_force = arrs(&a->vel_force),
+         w0_cart = arrs(&a->w0_cart), scal = arrs(&a->scal), scal_force = arrs(&a->scal_force),
+         eta = arrs(&a->etarhoflux);
+
+    rc = MakeUtrans(*c, lev, utilde, ufull, utrans.d, w0mac.d, w0_cart, bu);
+    if (rc) return finish(rc);
+    rc = Addw0(*c, lev, utrans.d, w0mac.d, base(&a->w0), 1.0);
+    if (rc) return finish(rc);
+    rc = VelPred(*c, lev, utilde, ufull, utrans.d, umac.d, w0mac.d, vel_force, w0_cart, bu);
+    if (rc) return finish(rc);
+    rc = Addw0(*c, lev, umac.d, w0mac.d, base(&a->w0), 1.0);
+    if (rc) return finish(rc);
+    rc = MakeEdgeScal(*c, lev, scal, sedge.d, umac.d, scal_force, false, bs.data(),
+                      p.first_spec, p.first_spec, p.num_spec, false);
+    if (rc) return finish(rc);
+    rc = MakeEdgeScal(*c, lev, scal, sedge.d, umac.d, scal_force, false, bs.data(),
+                      p.comp_rho, p.comp_rho, 1, false);
+    if (rc) return finish(rc);
+    rc = MakeEdgeScal(*c, lev, scal, sedge.d, umac.d, scal_force, false, bs.data(),
+                      p.comp_rhoh, p.comp_rhoh, 1, false);
+    if (rc) return finish(rc);
+    // etarhoflux[lev].setVal(0.) (MaestroAdvance.cpp:442-445), ghosts included
+    for (size_t b = 0; b < eta.size(); ++b) {
+        const Arr& e = eta[b];
+        IBox whole;
+        for (int d = 0; d < 3; ++d) {
+            whole.lo[d] = e.b[d];
+            whole.hi[d] = e.e[d] - 1;
+        }
+        par_for(whole, [&](int i, int j, int k) { e(i, j, k) = 0.0; });
+    }
+    rc = MakeRhoXFlux(*c, lev, sflux.d, eta, sedge.d, umac.d, base(&a->rho0),
+                      base(&a->rho0_edge), none.d, base(&a->rho0), base(&a->rho0_edge), none.d,
+                      base(&a->rho0_predicted_edge), base(&a->w0), p.first_spec, p.num_spec);
+    if (rc) return finish(rc);
+    rc = MakeRhoHFlux(*c, lev, sflux.d, sedge.d, umac.d, base(&a->rho0), base(&a->rho0_edge),
+                      none.d, base(&a->rho0), base(&a->rho0_edge), none.d, base(&a->rhoh0),
+                      base(&a->rhoh0_edge), none.d, base(&a->rhoh0), base(&a->rhoh0_edge),
+                      none.d, none.d, none.d);
+    return finish(rc);
+}
+
+int mxo_omp_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+}  // extern "C"
