@@ -59,7 +59,7 @@ class Workload:
 
     def __init__(self, dim=3, n=32, max_grid=16, ppm_type=1, nspec=3, seed=1234, nmodes=None,
                  ng=4, all_periodic=False, backend=None, ppm_trace_forces=0,
-                 phys_bc=None, zero_velocity=False):
+                 phys_bc=None, zero_velocity=False, shift=(0, 0, 0)):
         from .fabs import Backend
         self.backend = backend or Backend("numpy")
         self.dim = dim
@@ -90,12 +90,18 @@ class Workload:
         # global index arrays including ghosts, mapped to valid cells
         gidx = np.arange(-ng, N + ng)
 
-        def mapped(d):
-            if periodic[d]:
-                return np.mod(gidx, N)
-            return np.clip(gidx, 0, N - 1)
+        shift = list(shift) + [0] * (3 - len(shift))
 
-        mi = [mapped(d) for d in range(dim)]
+        def mapped(d, sh):
+            g = gidx - sh
+            if periodic[d]:
+                return np.mod(g, N)
+            return np.clip(g, 0, N - 1)
+
+        mi = [mapped(d, shift[d]) for d in range(dim)]
+        # the horizontal averages come from the unshifted field, so that a
+        # periodic shift of the input leaves the base state bit-identical
+        mi0 = [mapped(d, 0) for d in range(dim)]
         xc = [(mi[d] + 0.5) * dx for d in range(dim)]
 
         def grid(arrs):
@@ -104,17 +110,20 @@ class Workload:
                 return arrs[0][None, None, :], arrs[1][None, :, None], arrs[2][:, None, None]
             return arrs[0][None, :], arrs[1][:, None]
 
+        def density(Xg):
+            rr = Xg[vert]
+            dist2 = sum((Xg[d] - 0.5) ** 2 for d in range(dim))
+            rho_ = np.exp(-rr / 0.3) * (1.0 + 0.1 * np.exp(-dist2 / 0.05 ** 2))
+            rho_ = np.broadcast_to(rho_, (ext,) * dim).copy()
+            return rho_, rho_ * np.broadcast_to(2.0 - rr, (ext,) * dim)
+
         X = grid(xc)
-        r = X[vert]
-        dist2 = sum((X[d] - 0.5) ** 2 for d in range(dim))
-        rho = np.exp(-r / 0.3) * (1.0 + 0.1 * np.exp(-dist2 / 0.05 ** 2))
-        rho = np.broadcast_to(rho, (ext,) * dim).copy()
-        h = np.broadcast_to(2.0 - r, (ext,) * dim)
-        rhoh = rho * h
+        rho, rhoh = density(X)
         valid = tuple(slice(ng, ng + N) for _ in range(dim))
         haxes = tuple(a for a in range(dim) if a != 0)  # all but vertical (axis 0 = vertical)
-        rho0 = rho[valid].mean(axis=haxes)
-        rhoh0 = rhoh[valid].mean(axis=haxes)
+        rho_u, rhoh_u = density(grid([(mi0[d] + 0.5) * dx for d in range(dim)]))
+        rho0 = rho_u[valid].mean(axis=haxes)
+        rhoh0 = rhoh_u[valid].mean(axis=haxes)
         rmap = mi[vert]
         shp = [1] * dim
         shp[0] = ext

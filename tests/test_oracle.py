@@ -1,0 +1,394 @@
+"""Pins the CPU oracle with checks that do not restate its formulas.
+
+* closed forms: constant data are reproduced exactly; on a linear or a
+  quadratic profile PPM type 1 and 2 reconstruct the exact face values
+  (4th-order interpolation is exact on cubics and neither limiter fires),
+  so the traced states equal the exact average of the profile over the
+  characteristic domain of dependence, and with a uniform normal velocity
+  and no transverse velocity the 3-D edge state reduces to that 1-D trace;
+* invariants: periodic translation (bit-exact), x-mirror with u -> -u
+  (bit-exact for PPM type 1: Ip and Im swap), agreement of the two boxes
+  that share a periodic face, F_Rho summed in component order;
+* behaviour of the reference's own unit test (test_advect): a Gaussian
+  advected one period with u = +-1 along x, y, z returns with the same
+  error in every direction;
+* Maestro::BCSetup's table (MaestroSetup.cpp:276-559) and error paths.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle
+from harness import global_face_array, oracle_for
+from maestroex_amd import _abi
+from maestroex_amd.fabs import Backend, FaceMultiFab, MultiFab
+from maestroex_amd.workloads import Workload
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NP = Backend("numpy")
+
+
+def fill_mf(mf, fn, comp):
+    """mf[comp] = fn(I, J, K) over every point (valid and ghost) of each FAB."""
+    for b, a in enumerate(mf.arrays):
+        beg = mf.begin[b]
+        nc, nz, ny, nx = a.shape
+        K, J, I = np.meshgrid(np.arange(beg[2], beg[2] + nz), np.arange(beg[1], beg[1] + ny),
+                              np.arange(beg[0], beg[0] + nx), indexing="ij")
+        a[comp] = fn(I, J, K)
+
+
+def single_box(n=16, ppm=1, dim=3, **kw):
+    return Workload(dim=dim, n=n, max_grid=n, ppm_type=ppm, all_periodic=True, backend=NP, **kw)
+
+
+def ppm_on(w, m, s_comp, vel_faces, bx=None):
+    lo, hi = w.boxes[0]
+    bx = bx or (lo, hi)
+    Ip = MultiFab(w.boxes, 3, 1, 3, NP)
+    Im = MultiFab(w.boxes, 3, 1, 3, NP)
+    bcs = (_abi.BCRec * 1)()
+    m.PPM(0, bx, (w.scal, 0), (vel_faces[0], 0), (vel_faces[1], 0), (vel_faces[2], 0), (Ip, 0),
+          (Im, 0), bcs, 1, s_comp, 0)
+    return Ip, Im
+
+
+def const_faces(w, u):
+    f = FaceMultiFab(w.boxes, 1, 1, 3, NP)
+    for d in range(3):
+        f[d].arrays[0][:] = u[d]
+    return f
+
+
+# ---------------------------------------------------------------- closed forms
+
+@pytest.mark.parametrize("ppm", [1, 2])
+def test_constant_field_ppm(ppm):
+    w = single_box(ppm=ppm)
+    m = oracle_for(w)
+    w.scal.arrays[0][0] = 3.25
+    Ip, Im = ppm_on(w, m, 0, const_faces(w, (0.7, -0.4, 0.3)))
+    valid = Ip.valid_region(0)
+    assert np.all(Ip.arrays[0][(slice(None),) + valid] == 3.25)
+    assert np.all(Im.arrays[0][(slice(None),) + valid] == 3.25)
+
+
+def _avg_poly(coef, a, b):
+    """exact average of sum c_p x^p over [a, b]"""
+    tot = 0.0
+    for p, c in enumerate(coef):
+        tot += c * (b ** (p + 1) - a ** (p + 1)) / (p + 1)
+    return tot / (b - a)
+
+
+@pytest.mark.parametrize("ppm", [1, 2])
+@pytest.mark.parametrize("coef", [(0.5, 1.3), (2.0, 0.1, 0.8)], ids=["linear", "quadratic"])
+def test_polynomial_trace_exact(ppm, coef):
+    """Cell averages of a monotone polynomial in x: Ip(i) is the exact average
+    of the polynomial over [x_{i+1/2} - u dt, x_{i+1/2}] (u > 0), Im(i) over
+    [x_{i-1/2}, x_{i-1/2} + |u| dt] (u < 0)."""
+    n = 16
+    w = single_box(n=n, ppm=ppm)
+    m = oracle_for(w)
+    h = 1.0 / n
+    # shift x so the profile is monotone on and around the box
+    x0 = 2.0
+
+    def avg(I, J, K):
+        return np.vectorize(lambda i: _avg_poly(coef, x0 + i * h, x0 + (i + 1) * h))(I)
+
+    fill_mf(w.scal, avg, 0)
+    u = 0.8
+    dt = w.params.dt
+    for sgn in (1.0, -1.0):
+        Ip, Im = ppm_on(w, m, 0, const_faces(w, (sgn * u, 0.0, 0.0)))
+        sig = u * dt / h
+        for i in range(2, n - 2):
+            xl, xr = x0 + i * h, x0 + (i + 1) * h
+            a = Ip.arrays[0][0, 5, 5, i + 1]  # Ip has one ghost cell: index = i + 1
+            b = Im.arrays[0][0, 5, 5, i + 1]
+            if sgn > 0:
+                exact = _avg_poly(coef, xr - sig * h, xr)
+                assert abs(a - exact) <= 1e-12 * abs(exact), (i, a, exact)
+            else:
+                exact = _avg_poly(coef, xl, xl + sig * h)
+                assert abs(b - exact) <= 1e-12 * abs(exact), (i, b, exact)
+
+
+@pytest.mark.parametrize("ppm", [1, 2])
+def test_uniform_velocity_edge_state_is_1d_trace(ppm):
+    """MakeEdgeScal with umac = (u, 0, 0): the transverse corrections vanish
+    and sedge_x(i) = Ip_x(i-1) + dt/2 f, the exact characteristic average of
+    a quadratic profile plus half a step of forcing."""
+    n = 16
+    w = single_box(n=n, ppm=ppm)
+    m = oracle_for(w)
+    h = 1.0 / n
+    coef = (1.0, 0.3, 0.5)
+    x0 = 1.5
+    fill_mf(w.scal, lambda I, J, K: np.vectorize(
+        lambda i: _avg_poly(coef, x0 + i * h, x0 + (i + 1) * h))(I), 4)
+    f = 0.125
+    w.scal_force.arrays[0][4] = f
+    u = 0.6
+    umac = const_faces(w, (u, 0.0, 0.0))
+    bcs_s, _ = m.bc_setup()
+    m.MakeEdgeScal(0, w.scal, w.sedge, umac, w.scal_force, False, bcs_s, 4, 4, 1, False)
+    dt = w.params.dt
+    sig = u * dt / h
+    se = w.sedge[0].arrays[0][4]
+    for i in range(3, n - 2):
+        xr = x0 + i * h  # face i is the right face of cell i-1
+        exact = _avg_poly(coef, xr - sig * h, xr) + 0.5 * dt * f
+        got = se[5, 5, i]
+        assert abs(got - exact) <= 1e-12 * abs(exact), (i, got, exact)
+
+
+def test_constant_scalars_pass():
+    """Constant scalars with no forcing: every edge state equals the value and
+    the fluxes are the closed forms of MakeRhoXFlux / MakeRhoHFlux."""
+    w = Workload(dim=3, n=16, max_grid=8, ppm_type=1, backend=NP)
+    vals = {0: 0.05, 1: -0.2, 4: 0.3, 5: 0.5, 6: 0.2}
+    for b in range(len(w.boxes)):
+        for c, v in vals.items():
+            w.scal.arrays[b][c] = v
+        w.scal_force.arrays[b][:] = 0.0
+    m = oracle_for(w)
+    m.advance_pass(0, w.pass_fields())
+    rho0 = w.rho0.to_numpy()[0]
+    rho0e = w.rho0_edge.to_numpy()[0]
+    rhoh0 = w.rhoh0.to_numpy()[0]
+    rhoh0e = w.rhoh0_edge.to_numpy()[0]
+    for d in range(3):
+        for b, (lo, hi) in enumerate(w.boxes):
+            reg = w.sedge[d].valid_region(b)
+            se = w.sedge[d].arrays[b]
+            for c, v in vals.items():
+                assert np.all(se[c][reg] == v)
+            um = w.umac[d].arrays[b][0][w.umac[d].valid_region(b)]
+            sf = w.sflux[d].arrays[b]
+            kk = np.arange(lo[2], hi[2] + 1 + (1 if d == 2 else 0))[:, None, None]
+            r0 = (0.5 * (rho0e[kk] + rho0e[kk])) if d == 2 else 0.5 * (rho0[kk] + rho0[kk])
+            rh0 = (0.5 * (rhoh0e[kk] + rhoh0e[kk])) if d == 2 else 0.5 * (rhoh0[kk] + rhoh0[kk])
+            tot = 0.0
+            for c in (4, 5, 6):
+                exp = um * (r0 + vals[0]) * vals[c]
+                assert np.array_equal(sf[c][reg], exp)
+                tot = tot + exp
+            assert np.array_equal(sf[0][reg], 0.0 + tot)
+            assert np.array_equal(sf[1][reg], um * (rh0 + vals[1]))
+
+
+# ---------------------------------------------------------------- invariants
+
+def test_translation_invariance():
+    """Periodic domain, input shifted by (3, 2) cells in x and y -> every
+    output shifted by the same amount, bit for bit."""
+    base = dict(dim=3, n=16, max_grid=8, ppm_type=2, all_periodic=True, backend=NP)
+    w0 = Workload(**base)
+    oracle_for(w0).advance_pass(0, w0.pass_fields())
+    w1 = Workload(shift=(3, 2, 0), **base)
+    oracle_for(w1).advance_pass(0, w1.pass_fields())
+    for name in ("umac", "sedge", "sflux"):
+        for d in range(3):
+            for comp in ([0] if name == "umac" else [0, 1, 4, 6]):
+                a = global_face_array(getattr(w0, name)[d], 3, d, [16] * 3, comp)
+                b = global_face_array(getattr(w1, name)[d], 3, d, [16] * 3, comp)
+                if d == 0:
+                    assert np.array_equal(np.roll(a[:, :, :16], (2, 3), axis=(1, 2)), b[:, :, :16])
+                elif d == 1:
+                    assert np.array_equal(np.roll(a[:, :16, :], (2, 3), axis=(1, 2)), b[:, :16, :])
+                else:
+                    assert np.array_equal(np.roll(a, (2, 3), axis=(1, 2)), b)
+
+
+def test_periodic_shared_faces_agree():
+    """The face at x = 1 (stored by the hi boxes) and x = 0 (lo boxes) are the
+    same periodic face: computed independently, they agree bit for bit."""
+    w = Workload(dim=3, n=16, max_grid=8, ppm_type=1, all_periodic=True, backend=NP)
+    oracle_for(w).advance_pass(0, w.pass_fields())
+    for name in ("umac", "sedge", "sflux"):
+        for d in range(3):
+            if name == "sflux" and d == 2:
+                continue  # vertical fluxes carry the height-dependent rho0_edge(r)
+            comps = [0] if name == "umac" else [0, 1, 4, 5, 6]
+            for comp in comps:
+                g = global_face_array(getattr(w, name)[d], 3, d, [16] * 3, comp)
+                idx = [slice(None)] * 3
+                ax = 2 - d
+                lo = list(idx)
+                hi = list(idx)
+                lo[ax] = 0
+                hi[ax] = 16
+                assert np.array_equal(g[tuple(lo)], g[tuple(hi)]), (name, d, comp)
+
+
+def test_mirror_symmetry_ppm1():
+    """x -> -x with u_x -> -u_x swaps Ip and Im (PPM type 1, bit-exact)."""
+    w = single_box(n=16, ppm=1)
+    m = oracle_for(w)
+    rng = np.random.default_rng(5)
+    s = w.scal.arrays[0]
+    s[0] = rng.random(s[0].shape)
+    vel = FaceMultiFab(w.boxes, 1, 1, 3, NP)
+    for d in range(3):
+        vel[d].arrays[0][:] = rng.normal(size=vel[d].arrays[0].shape)
+    Ip, Im = ppm_on(w, m, 0, vel)
+    # mirrored problem
+    s[0] = s[0][..., ::-1].copy()
+    vel[0].arrays[0][:] = -vel[0].arrays[0][..., ::-1]
+    for d in (1, 2):
+        vel[d].arrays[0][:] = vel[d].arrays[0][..., ::-1]
+    Ip2, Im2 = ppm_on(w, m, 0, vel)
+    reg = (slice(1, -1),) * 3
+    A, B, A2, B2 = (x.arrays[0] for x in (Ip, Im, Ip2, Im2))
+    assert np.array_equal(A2[0][..., ::-1][reg], B[0][reg])
+    assert np.array_equal(B2[0][..., ::-1][reg], A[0][reg])
+    for d in (1, 2):
+        assert np.array_equal(A2[d][..., ::-1][reg], A[d][reg])
+        assert np.array_equal(B2[d][..., ::-1][reg], B[d][reg])
+
+
+def test_rho_flux_is_ordered_species_sum():
+    w = Workload(dim=3, n=16, max_grid=8, ppm_type=2, backend=NP)
+    oracle_for(w).advance_pass(0, w.pass_fields())
+    for d in range(3):
+        for b in range(len(w.boxes)):
+            reg = w.sflux[d].valid_region(b)
+            sf = w.sflux[d].arrays[b]
+            acc = np.zeros_like(sf[0][reg])
+            for c in (4, 5, 6):
+                acc = acc + sf[c][reg]
+            assert np.array_equal(sf[0][reg], acc)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_advect_direction_independence(dim):
+    """test_advect (Exec/unit_tests/test_advect/MaestroEvolve.cpp:82-210): a
+    Gaussian advected one period by u = +-1 along each axis returns with the
+    same L2 error for every direction and sign (no directional bias)."""
+    n = 16
+    errs = []
+    for d in range(dim):
+        for sgn in (1.0, -1.0):
+            w = Workload(dim=dim, n=n, max_grid=n, ppm_type=1, all_periodic=True, backend=NP,
+                         zero_velocity=True)
+            m = oracle_for(w)
+            h = 1.0 / n
+            dt = 0.5 * h
+            m.set_dt(dt, 1e-8)
+            umac = FaceMultiFab(w.boxes, 1, 1, dim, NP)
+            umac[d].arrays[0][:] = sgn
+            for b in range(len(w.boxes)):
+                w.scal_force.arrays[b][:] = 0.0
+            ng = w.ng
+            X = (np.arange(n) + 0.5) * h
+            grids = np.meshgrid(*([X] * dim), indexing="ij")
+            r2 = sum((g - 0.5) ** 2 for g in grids)
+            s0 = np.exp(-r2 / 0.1 ** 2)
+            s0 = s0.transpose() if dim == 3 else s0.T
+            s = s0.copy()
+            bcs_s, _ = m.bc_setup()
+            for step in range(2 * n):
+                full = np.pad(s, ng, mode="wrap")
+                if dim == 2:
+                    full = full[None]
+                w.scal.arrays[0][4] = full
+                m.MakeEdgeScal(0, w.scal, w.sedge, umac, w.scal_force, False, bcs_s, 4, 4, 1,
+                               False)
+                se = w.sedge[d].arrays[0][4]
+                if dim == 2:
+                    se = se[0]
+                F = sgn * se
+                ax = dim - 1 - d
+                lo = [slice(None)] * dim
+                hi = [slice(None)] * dim
+                lo[ax] = slice(0, n)
+                hi[ax] = slice(1, n + 1)
+                s = s - dt / h * (F[tuple(hi)] - F[tuple(lo)])
+            errs.append(float(np.sqrt(np.mean((s - s0) ** 2))))
+    errs = np.array(errs)
+    assert np.all(np.abs(errs - errs[0]) <= 1e-12 * errs[0]), errs
+    assert 0 < errs[0] < 0.1
+
+
+# ---------------------------------------------------------------- BCSetup, ABI
+
+def test_bc_setup_table():
+    """MaestroSetup.cpp:276-559: velocity and scalar BCRecs per phys_bc."""
+    P = _abi
+    w = Workload(dim=3, n=8, max_grid=8, ppm_type=1, backend=NP,
+                 phys_bc={0: (P.PHYS_INFLOW, P.PHYS_OUTFLOW), 1: (P.PHYS_SYMMETRY, P.PHYS_SLIPWALL),
+                          2: (P.PHYS_NOSLIPWALL, P.PHYS_OUTFLOW)})
+    m = oracle_for(w)
+    bs, bu = m.bc_setup()
+    exp_u = {  # (dir, side) -> (all comps, comp == dir)
+        (0, 0): (P.BC_EXT_DIR, P.BC_EXT_DIR), (0, 1): (P.BC_FOEXTRAP, P.BC_FOEXTRAP),
+        (1, 0): (P.BC_REFLECT_EVEN, P.BC_REFLECT_ODD), (1, 1): (P.BC_HOEXTRAP, P.BC_EXT_DIR),
+        (2, 0): (P.BC_EXT_DIR, P.BC_EXT_DIR), (2, 1): (P.BC_FOEXTRAP, P.BC_FOEXTRAP)}
+    exp_s = {  # (dir, side) -> (Rho/RhoH/Temp/species, Pi)
+        (0, 0): (P.BC_EXT_DIR, P.BC_FOEXTRAP), (0, 1): (P.BC_FOEXTRAP, P.BC_EXT_DIR),
+        (1, 0): (P.BC_REFLECT_EVEN, P.BC_REFLECT_EVEN), (1, 1): (P.BC_HOEXTRAP, P.BC_FOEXTRAP),
+        (2, 0): (P.BC_HOEXTRAP, P.BC_FOEXTRAP), (2, 1): (P.BC_FOEXTRAP, P.BC_EXT_DIR)}
+    for (d, side), (allc, own) in exp_u.items():
+        for comp in range(3):
+            v = (bu[comp].lo if side == 0 else bu[comp].hi)[d]
+            assert v == (own if comp == d else allc), (d, side, comp)
+    for (d, side), (sc, pi) in exp_s.items():
+        for comp in range(w.nscal):
+            v = (bs[comp].lo if side == 0 else bs[comp].hi)[d]
+            assert v == (pi if comp == 3 else sc), (d, side, comp)
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "mx_adv.h")).read()
+    return sorted(set(re.findall(r"\b(mx_adv_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header_symbols():
+    import maestroex_amd as mx
+    assert os.path.exists(mx.LIB_PATH), "libmaestroex_amd.so not built"
+    lib = C.CDLL(mx.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_library_without_gpu_fails_loudly():
+    """No GPU in this container: context creation reports MX_ERR_NOLIB and
+    the package raises instead of falling back to the CPU."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    import maestroex_amd as mx
+    w = Workload(dim=3, n=8, max_grid=8, ppm_type=1, backend=NP)
+    with pytest.raises(_abi.MxError) as ei:
+        mx.gpu_maestro(w.params, [w.boxes])
+    assert ei.value.code == _abi.MX_ERR_NOLIB
+    with pytest.raises(RuntimeError):
+        mx.device_backend()
+
+
+def test_oracle_reports_short_fabs():
+    """A state with too few ghost cells for PPM is reported (MX_ERR_BOX)."""
+    w = Workload(dim=3, n=8, max_grid=8, ppm_type=2, backend=NP)
+    m = oracle_for(w)
+    small = MultiFab(w.boxes, w.nscal, 1, 3, NP)
+    bcs_s, _ = m.bc_setup()
+    with pytest.raises(_abi.MxError) as ei:
+        m.MakeEdgeScal(0, small, w.sedge, w.umac, w.scal_force, False, bcs_s, 4, 4, 1, False)
+    assert ei.value.code == _abi.MX_ERR_BOX
+
+
+def test_oracle_rejects_2d_traced_forces():
+    """2-D VelPred with ppm_trace_forces reads never-written temporaries in the
+    reference (MaestroVelPred.cpp:128-133, 556): reported as unsupported."""
+    w = Workload(dim=2, n=16, max_grid=16, ppm_type=1, backend=NP, ppm_trace_forces=1)
+    m = oracle_for(w)
+    with pytest.raises(_abi.MxError) as ei:
+        m.advance_pass(0, w.pass_fields())
+    assert ei.value.code == _abi.MX_ERR_UNSUPPORTED
