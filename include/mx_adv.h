@@ -180,6 +180,17 @@ int mx_adv_prof_enable(mx_adv_ctx* ctx, int on);
 int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_ms,
                      long long* launches);
 
+/* Kernel selection and launch shape (no counterpart in the reference, which
+   has one CPU loop nest per stage).  Every choice computes bit-identical
+   results; they differ in speed only.  Keys:
+     "edge_scal_fused"  1: 3-D MakeEdgeScal with ppm_type 1/2 and no traced
+                        forces runs as one z-marching kernel;
+                        0 (default): the three-kernel form (PPM, transverse,
+                        edges).
+     "edge_scal_tz"     planes per z-chunk of the fused kernel (default 32).
+   Unknown keys and out-of-range values return MX_ERR_ARG. */
+int mx_adv_set_tuning(mx_adv_ctx* ctx, const char* key, long long value);
+
 /* ---- unit-level kernels ------------------------------------------------- */
 
 /* Maestro::PPM (Maestro.H:1396-1406, MaestroPPM.cpp:6-2263).  bx is the box

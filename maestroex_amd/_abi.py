@@ -165,6 +165,7 @@ LIB_ONLY = {
     "error_string": [_i],
     "prof_enable": [_vp, _i],
     "prof_read": [_vp, _i, P(C.c_char_p), P(C.c_double), P(C.c_longlong)],
+    "set_tuning": [_vp, C.c_char_p, C.c_longlong],
 }
 
 

@@ -721,6 +721,21 @@ int mx_adv_prof_enable(mx_adv_ctx* c, int on) {
     return MX_OK;
 }
 
+int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
+    if (!c || !key) return MX_ERR_ARG;
+    const std::string k(key);
+    if (k == "edge_scal_fused") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->fused_edge = (int)value;
+    } else if (k == "edge_scal_tz") {
+        if (value < 1 || value > 4096) return MX_ERR_ARG;
+        c->fused_tz = (int)value;
+    } else {
+        return MX_ERR_ARG;
+    }
+    return MX_OK;
+}
+
 int mx_adv_prof_read(mx_adv_ctx* c, int idx, const char** name, double* total_ms,
                      long long* launches) {
     if (!c) return MX_ERR_ARG;

@@ -63,6 +63,10 @@ struct mx_adv_ctx {
     std::vector<std::string> prof_names;
     std::vector<double> prof_ms;
     std::vector<long long> prof_n;
+
+    // kernel selection (mx_adv_set_tuning)
+    int fused_edge = 0;
+    int fused_tz = 32;
 };
 
 namespace mxh {
@@ -258,6 +262,13 @@ int make_rhoh_flux(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux,
                    const mx_face_multifab* rh0mac_old, const mx_basestate* rhoh0_new,
                    const mx_basestate* rhoh0_edge_new, const mx_face_multifab* rh0mac_new,
                    const mx_face_multifab* h0mac_old, const mx_face_multifab* h0mac_new);
+
+// fused 3-D MakeEdgeScal (ppm_type 1/2, forces not traced)
+bool fused_edge_scal_ok(const mx_adv_ctx* c, int is_conservative);
+int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
+                         const mx_face_multifab* sedge, const mx_face_multifab* umac,
+                         const mx_multifab* force, int is_vel, const mx_bcrec* bcs,
+                         int start_scomp, int start_bccomp, int num_comp, int is_conservative);
 
 void bc_setup_host(mx_adv_ctx* c);
 int check_level(const mx_adv_ctx* c, int lev);

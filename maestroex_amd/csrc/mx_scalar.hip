@@ -630,6 +630,9 @@ int make_edge_scal(mx_adv_ctx* c, int lev, const mx_multifab* state,
                    const mx_multifab* force, int is_vel, const mx_bcrec* bcs, int nbccomp,
                    int start_scomp, int start_bccomp, int num_comp, int is_conservative) {
     (void)nbccomp;
+    if (fused_edge_scal_ok(c, is_conservative))
+        return make_edge_scal_fused(c, lev, state, sedge, umac, force, is_vel, bcs, start_scomp,
+                                    start_bccomp, num_comp, is_conservative);
     const int dim = c->p.dim;
     const auto& grids = c->grids[lev];
     const int ppm = c->p.ppm_type;

@@ -540,7 +540,8 @@ int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mult
             B.tr[d] = d < dim ? to_da(utrans->dir[d].fab[b]) : null_da();
             B.w0m[d] = (sph && d < dim) ? to_da(w0mac->dir[d].fab[b]) : null_da();
         }
-        maxpts = std::max(maxpts, npts(nodal(g, 0)));
+        // the grid covers the largest face region of any direction
+        for (int d = 0; d < dim; ++d) maxpts = std::max(maxpts, npts(nodal(g, d)));
         boxes.push_back(B);
     }
     UtArgs A;

@@ -67,6 +67,10 @@ class Maestro:
         self._check(self.fns["bc_setup"](self.ctx, bs, bu), "bc_setup")
         return bs, bu
 
+    def set_tuning(self, key, value):
+        """Kernel selection / launch shape; results are identical for every choice."""
+        self._check(self.fns["set_tuning"](self.ctx, key.encode(), int(value)), "set_tuning")
+
     def sync(self):
         if "sync" in self.fns:
             self._check(self.fns["sync"](self.ctx), "sync")

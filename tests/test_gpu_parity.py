@@ -65,6 +65,35 @@ def test_pass_parity(name, cfg):
     assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} etarhoflux")
 
 
+KERNEL_CHOICE_CASES = [
+    ("3d_ppm1", dict(dim=3, n=16, max_grid=8, ppm_type=1)),
+    ("3d_uneven_ppm2", dict(dim=3, n=24, max_grid=(16, 8, 12), ppm_type=2)),
+    ("3d_walls_ppm2", CASES[10][1]),
+    ("3d_walls_ppm1_wide", dict(dim=3, n=40, max_grid=(40, 20, 40), ppm_type=1,
+                                phys_bc={0: (P.PHYS_INFLOW, P.PHYS_OUTFLOW),
+                                         1: (P.PHYS_SLIPWALL, P.PHYS_NOSLIPWALL),
+                                         2: (P.PHYS_SYMMETRY, P.PHYS_OUTFLOW)})),
+]
+
+
+@pytest.mark.parametrize("fused,tz", [(0, 32), (1, 1), (1, 5), (1, 32)])
+@pytest.mark.parametrize("name,cfg", KERNEL_CHOICE_CASES, ids=[c[0] for c in KERNEL_CHOICE_CASES])
+def test_pass_kernel_choices(name, cfg, fused, tz):
+    """The three-kernel and the fused z-marching MakeEdgeScal, the latter
+    with z-chunks of 1, 5 and 32 planes (chunk edges inside and at the box
+    faces, tiles narrower than a box and wider than one), all bit-exact."""
+    print(f"[parity] kernel choice {name} fused={fused} tz={tz}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    m.set_tuning("edge_scal_fused", fused)
+    m.set_tuning("edge_scal_tz", tz)
+    _run_pass(m, wg)
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} sedge")
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
+
+
 @pytest.mark.parametrize("is_vel,is_cons,ppm", [(1, 0, 1), (1, 0, 2), (0, 1, 1), (0, 1, 2), (1, 0, 0),
                                                 (0, 1, 0)])
 @pytest.mark.parametrize("dim", [2, 3])
