@@ -52,6 +52,8 @@ def kernel_bytes_per_cell(name, nc, dim):
         "k_es_trans": 8 * (nc * (2 * D + 6) + D),
         # EdgeScal edges: read Ip/Im, states, force, umac; write sedge (D)
         "k_es_edge": 8 * (nc * (2 * D + 6 + 1 + D) + D),
+        # fused MakeEdgeScal: read s, force, umac (D); write sedge (D)
+        "k_es_fused": 8 * (nc * (2 + D) + D),
     }
     return table.get(name)
 
@@ -67,6 +69,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="kernel choice passed to mx_adv_set_tuning (repeatable)")
     return ap.parse_args()
 
 
@@ -117,6 +121,9 @@ def main():
     print(f"[bench] workload {args.n}^3 in {len(w.boxes)} boxes built in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     m = mx.gpu_maestro(w.params, [w.boxes])
+    for kv in args.tune:
+        key, val = kv.split("=")
+        m.set_tuning(key, int(val))
     lib, fns = mx.load()
     fields = w.pass_fields()
 
@@ -158,7 +165,7 @@ def main():
     avg_ms = dms / dcnt
     # units one launch of the dominant kernel processes: the valid cells of
     # all boxes (x the components the launch covers, folded into bytes/cell)
-    ncomp_launch = {"k_es_ppm": None, "k_es_trans": None, "k_es_edge": None}
+    ncomp_launch = {"k_es_ppm": None, "k_es_trans": None, "k_es_edge": None, "k_es_fused": None}
     if dname in ncomp_launch:
         # MakeEdgeScal launches cover 3 species, then rho, then rhoh: average
         nc = 5.0 / 3.0

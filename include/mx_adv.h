@@ -183,10 +183,10 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
 /* Kernel selection and launch shape (no counterpart in the reference, which
    has one CPU loop nest per stage).  Every choice computes bit-identical
    results; they differ in speed only.  Keys:
-     "edge_scal_fused"  1: 3-D MakeEdgeScal with ppm_type 1/2 and no traced
-                        forces runs as one z-marching kernel;
-                        0 (default): the three-kernel form (PPM, transverse,
-                        edges).
+     "edge_scal_fused"  1 (default): 3-D MakeEdgeScal with ppm_type 1/2 and
+                        no traced forces runs as one z-marching kernel;
+                        0: the three-kernel form (PPM, transverse, edges),
+                        which also serves ppm_type 0, traced forces and 2-D.
      "edge_scal_tz"     planes per z-chunk of the fused kernel (default 32).
    Unknown keys and out-of-range values return MX_ERR_ARG. */
 int mx_adv_set_tuning(mx_adv_ctx* ctx, const char* key, long long value);
@@ -269,6 +269,37 @@ int mx_adv_make_rhoh_flux(
    r = 0..nr on row lev of the BaseStates. */
 int mx_adv_cell_to_edge(mx_adv_ctx* ctx, int lev, int nr,
                         const mx_basestate* s0_cell, mx_basestate* s0_edge);
+
+/* Radial geometry of the base state (BaseGeom, Source/BaseGeom.H): the
+   radial levels 0..nlev-1 (finest_radial_level + 1), their cell counts
+   nr(n) and widths dr(n), and the disjoint chunks [r_start, r_end] each
+   level covers (numdisjointchunks, r_start_coord, r_end_coord; chunk i of
+   level n is entry i - 1). */
+#define MX_MAX_CHUNK 16
+typedef struct {
+    int nlev;
+    int nr_fine;
+    int nr[MX_MAX_LEV];
+    double dr[MX_MAX_LEV];
+    int nchunks[MX_MAX_LEV];
+    int r_start[MX_MAX_LEV][MX_MAX_CHUNK];
+    int r_end[MX_MAX_LEV][MX_MAX_CHUNK];
+} mx_radial_geom;
+
+/* Maestro::MakeEdgeState1d (Maestro.H:1050-1060,
+   MaestroMakeEdgeState.cpp:5-981): the time-centred edge states of a
+   base-state array s (nlev rows of nr_fine) traced with the base-state
+   velocity w0 (the Maestro member, edge-centred) and the source force;
+   sedge has rows of nr_fine + 1.  Spherical contexts use row 0 only
+   (MakeEdgeState1dSphr); planar ones every level and chunk of rg, with the
+   coarse-fine edge synchronisation of MakeEdgeState1dPlanar.  ppm_type,
+   slope_order, dt and rel_eps come from the context.  Returns MX_ERR_ARG
+   where the reference aborts (a planar ppm_type 1/2 chunk that starts at
+   r = 2 or ends at nr - 3). */
+int mx_adv_make_edge_state_1d(mx_adv_ctx* ctx, const mx_radial_geom* rg,
+                              const mx_basestate* s, mx_basestate* sedge,
+                              const mx_basestate* force,
+                              const mx_basestate* w0);
 
 /* Maestro::FillUmacGhost (MaestroFillData.cpp:176-453) preceded by the
    periodic FillBoundary of each face MultiFab, as MakeUtrans and Addw0 do

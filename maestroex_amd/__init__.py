@@ -18,6 +18,18 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmaestroex_amd.so")
 _LIB = None
 
 
+def _torch_first():
+    """Initialise torch's HIP runtime before this library's.  torch ships
+    its own HIP runtime; when libmaestroex_amd.so had initialised the device
+    first, torch in the same process reported no GPU.  Python callers that
+    pair the library with torch tensors therefore start torch first."""
+    try:
+        import torch
+    except ImportError:
+        return
+    torch.cuda.is_available()
+
+
 def load():
     """Load libmaestroex_amd.so and bind the C-ABI; raises if it is missing."""
     global _LIB
@@ -26,6 +38,7 @@ def load():
             raise RuntimeError(
                 f"{LIB_PATH} is not built; run `python -m maestroex_amd.build` "
                 "(the advection path has no CPU fallback)")
+        _torch_first()
         lib = C.CDLL(LIB_PATH)
         fns = _abi.bind(lib, "mx_adv_", extra=_abi.LIB_ONLY)
         _LIB = (lib, fns)

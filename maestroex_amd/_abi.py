@@ -102,6 +102,40 @@ class Params(C.Structure):
     ]
 
 
+MX_MAX_CHUNK = 16
+
+
+class RadialGeom(C.Structure):
+    """mx_radial_geom: radial levels, cell counts, widths and chunks."""
+    _fields_ = [
+        ("nlev", C.c_int),
+        ("nr_fine", C.c_int),
+        ("nr", C.c_int * MX_MAX_LEV),
+        ("dr", C.c_double * MX_MAX_LEV),
+        ("nchunks", C.c_int * MX_MAX_LEV),
+        ("r_start", (C.c_int * MX_MAX_CHUNK) * MX_MAX_LEV),
+        ("r_end", (C.c_int * MX_MAX_CHUNK) * MX_MAX_LEV),
+    ]
+
+
+def radial_geom(nr_fine, dr, levels=None):
+    """One radial level covering [0, nr_fine) (levels=None), or the given
+    levels: [(nr, dr, [(r_start, r_end), ...]), ...]."""
+    g = RadialGeom()
+    g.nr_fine = nr_fine
+    if levels is None:
+        levels = [(nr_fine, dr, [(0, nr_fine - 1)])]
+    g.nlev = len(levels)
+    for n, (nr, d, chunks) in enumerate(levels):
+        g.nr[n] = nr
+        g.dr[n] = d
+        g.nchunks[n] = len(chunks)
+        for i, (a, b) in enumerate(chunks):
+            g.r_start[n][i] = a
+            g.r_end[n][i] = b
+    return g
+
+
 class PassArgs(C.Structure):
     _fields_ = [
         ("utilde", MultiFabDesc),
@@ -155,6 +189,8 @@ SIGNATURES = {
                        P(BaseStateDesc), P(FaceMultiFabDesc), P(BaseStateDesc), P(BaseStateDesc),
                        P(FaceMultiFabDesc), P(FaceMultiFabDesc), P(FaceMultiFabDesc)],
     "cell_to_edge": [_vp, _i, _i, P(BaseStateDesc), P(BaseStateDesc)],
+    "make_edge_state_1d": [_vp, P(RadialGeom), P(BaseStateDesc), P(BaseStateDesc),
+                           P(BaseStateDesc), P(BaseStateDesc)],
     "advance_pass": [_vp, _i, P(PassArgs)],
 }
 

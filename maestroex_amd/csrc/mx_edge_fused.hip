@@ -1,25 +1,41 @@
 // mx_edge_fused.hip -- fused 3-D MakeEdgeScal: PPM, predictor, transverse
-// and edge states of one scalar in a single kernel.
+// and edge states of one scalar in a single z-marching kernel.
 //
-// A block owns a TX x TY tile of cell columns of one box and marches up a
-// chunk [z0, z1] of its planes.  Every per-plane quantity lives in LDS on the
-// tile grown by one cell in x and y (E = (TX+2) x (TY+2) points); at step q:
-//   A  the scalar plane q (with the PPM stencil halo) is staged in LDS; each
-//      column owner shifts its register window of s along z;
-//   B  PPM in z of cell q (register window) and in x, y of plane q (LDS
-//      stencils), traced with the MAC velocities;
-//   C  predictor states slx/srx, sly/sry of plane q with their boundary
-//      overrides and the upwind states simhx, simhy; slz/srz and simhz of the
-//      z-face q were formed in B;
-//   D  the six corner-coupled states: simhxy, simhyx on plane q, simhzx,
-//      simhzy on z-face q, simhxz, simhyz on plane q-1;
-//   E  the edge states of plane q-1 (x and y faces) and of z-face q.
-// Ring buffers keep the two newest planes of the quantities read across
-// planes; the predictor states a thread reads only at its own face stay in
-// its registers.  The arithmetic of every point is that of
-// MaestroMakeEdgeScalars.cpp (Predictor :662-863, Transverse :865-1359,
-// Edges :1361-1667) and MaestroPPM.cpp, in the same operand order, so the
-// results are bit-identical to the per-stage kernels and to the CPU path.
+// A block owns a 32 x 8 tile of cell columns of one box, grown by one cell in
+// x and y (34 x 10 columns, one thread each), and marches up a chunk [z0, z1]
+// of the box's planes, one cell plane q per step (plus q = z0 - 1 and
+// q = z1 + 1, whose states the chunk's first and last faces need).
+//
+// The reference formulas (MaestroMakeEdgeScalars.cpp: Predictor :662-863,
+// Transverse :865-1359, Edges :1361-1667; MaestroPPM.cpp) are regrouped, not
+// changed:
+//   * a PPM face value (type 1: the limited 4th-order interpolant; type 2: the
+//     Colella-Sekora face value) is computed once per face and read by both
+//     cells beside it; sigma = |u| dt / dx likewise, once per face;
+//   * each transverse correction of a corner-coupled state,
+//     dt/6h (u_T(+) + u_T(-)) (simh_T(+) - simh_T(-)), depends only on the cell
+//     it is taken across, so it is computed once per cell and direction T and
+//     used by every face state that crosses that cell (the left state of the
+//     face above it and the right state of the face below); the edge-state
+//     corrections likewise;
+//   * every such term is evaluated by the reference's expression in the
+//     reference's operand order, and subtracted in the reference's order, so
+//     all results are bit-identical to the per-point form.
+// PPM boundary stencils (ext_dir / hoextrap within two or three cells of the
+// domain edge) fall back to the full per-cell routine of mx_device.H.
+//
+// Per step q a block runs seven phases separated by barriers:
+//   A  stage plane q of s, the x- and y-MAC velocities and their sigma in LDS;
+//      shift the column's z window (s, w, force) in registers;
+//   B1 PPM face values of plane q (x, y) into LDS; PPM of cell q in z
+//      (register window) and the upwind state simhz on z-face q;
+//   B2 PPM of plane q in x and y (cell limiter + trace);
+//   C  predictor states and upwind states simhx, simhy of plane q;
+//   D1 per-cell transverse terms: x, y of plane q, z of plane q-1;
+//   D2 corner-coupled states simhxy, simhyx (plane q), simhxz, simhyz (plane
+//      q-1), simhzx, simhzy (z-face q, in registers);
+//   E1 per-cell edge terms of plane q-1 (LDS) and of plane q (registers);
+//   E2 edge states of plane q-1 (x and y faces) and of z-face q.
 #include <algorithm>
 
 #include "mx_internal.H"
@@ -28,6 +44,13 @@ using namespace mxd;
 using namespace mxh;
 
 namespace {
+
+typedef __attribute__((address_space(1))) const double gcd;
+typedef __attribute__((address_space(1))) double gd;
+
+constexpr int TX = 32;
+constexpr int TY = 8;
+constexpr int NT = 384;
 
 struct FBox {
     int lo[3], hi[3];
@@ -43,7 +66,6 @@ struct FArgs {
     int start_scomp;
     int num_comp;
     int is_vel;
-    int is_conservative;
     int tz;  // planes per chunk
 };
 
@@ -80,10 +102,6 @@ __device__ __forceinline__ void bc_pair(int pos, int dlo, int dhi, int bclo, int
     }
 }
 
-__device__ __forceinline__ bool bc_needs_s(int pos, int dlo, int dhi, int bclo, int bchi) {
-    return (pos == dlo && bclo == MX_BC_EXT_DIR) || (pos == dhi + 1 && bchi == MX_BC_EXT_DIR);
-}
-
 __device__ __forceinline__ double upwind(double l, double r, double u, double thr) {
     double res = (u > 0.0) ? l : r;
     res = (dabs(u) > thr) ? res : 0.5 * (l + r);
@@ -118,62 +136,84 @@ __device__ __forceinline__ double bc_edge(double res, int pos, int dlo, int dhi,
     return res;
 }
 
-__device__ __forceinline__ double divu_at(const FBox& B, double dx0, int i, int j, int k) {
-    double dv = B.um[0](i + 1, j, k) - B.um[0](i, j, k) + B.um[1](i, j + 1, k) -
-                B.um[1](i, j, k) + B.um[2](i, j, k + 1) - B.um[2](i, j, k);
-    dv /= dx0;
-    return dv;
+// PPM type 1 face value between cells sa (left) and sb (right) from the two
+// cells' van Leer slopes (MaestroPPM.cpp:86-99: sm of the right cell, which
+// equals sp of the left cell)
+__device__ __forceinline__ double face1(double sa, double sb, double dla, double dlb) {
+    double e = 0.5 * (sb + sa) - (dlb - dla) / 6.0;
+    e = dmax(e, dmin(sb, sa));
+    e = dmin(e, dmax(sb, sa));
+    return e;
 }
 
-// corner-coupled update of one side (MakeEdgeScalTransverse): u1/u0 the
-// transverse MAC velocities at the +1 and 0 faces, h1/h0 the upwind states
-template <bool CONS>
-__device__ __forceinline__ double trans_side(double x, double dt3h, double dt6h, double dt3,
-                                             double u1, double u0, double h1, double h0, double s,
-                                             double dv) {
-    if (CONS) {
-        return x - dt3h * (h1 * u1 - h0 * u0) - dt3 * s * dv + dt3h * s * (u1 - u0);
+// does the cell at `pos` take a PPM boundary stencil?
+template <int PPM>
+__device__ __forceinline__ bool ppm_special(int pos, int dlo, int dhi, int bclo, int bchi) {
+    constexpr int R = PPM == 2 ? 2 : 1;
+    return (is_ho(bclo) && pos >= dlo && pos <= dlo + R) ||
+           (is_ho(bchi) && pos <= dhi && pos >= dhi - R);
+}
+
+// the characteristic trace of ppm_trace with sigma of each face given
+__device__ __forceinline__ void trace_sig(double s0, double sm, double sp, double u_hi,
+                                          double u_lo, double sig_hi, double sig_lo,
+                                          double rel_eps, double& Ip, double& Im) {
+    double s6 = 6.0 * s0 - 3.0 * (sm + sp);
+    if (u_hi > rel_eps) {
+        Ip = sp - 0.5 * sig_hi * (sp - sm - (1.0 - 2.0 / 3.0 * sig_hi) * s6);
     } else {
-        return x - dt6h * (u1 + u0) * (h1 - h0);
+        Ip = s0;
+    }
+    if (u_lo < -rel_eps) {
+        Im = sm + 0.5 * sig_lo * (sp - sm + (1.0 - 2.0 / 3.0 * sig_lo) * s6);
+    } else {
+        Im = s0;
     }
 }
 
+// transverse / edge corrections of one side: x - term (advective) or
+// ((x - a) - b) + c (conservative)
 template <bool CONS>
-__device__ __forceinline__ double edge_side(double x, double dt2h, double dt4h, double u1,
-                                            double u0, double h1, double h0) {
-    if (CONS) {
-        return x - dt2h * (h1 * u1 - h0 * u0);
-    } else {
-        return x - dt4h * (u1 + u0) * (h1 - h0);
-    }
+__device__ __forceinline__ double apply3(double x, double a, double b, double c) {
+    if (CONS) return ((x - a) - b) + c;
+    return x - a;
 }
 
-template <int PPM, int TX, int TY, bool CONS>
-__global__ void __launch_bounds__(384) k_es_fused(const FBox* __restrict__ boxes,
-                                                  const FComp* __restrict__ comps, KGeom g,
-                                                  FArgs A) {
+template <int PPM, bool CONS>
+__global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
+                                                 const FComp* __restrict__ comps, KGeom g,
+                                                 FArgs A) {
     constexpr int G = PPM == 2 ? 3 : 2;
     constexpr int W = TX + 2;
-    constexpr int NE = (TX + 2) * (TY + 2);
-    constexpr int SW = TX + 2 + 2 * G;
-    constexpr int SH = TY + 2 + 2 * G;
-    constexpr int NT = 384;
+    constexpr int H = TY + 2;
+    constexpr int NE = W * H;
+    constexpr int SW = W + 2 * G;
+    constexpr int SH = H + 2 * G;
+    constexpr int UW = W + 1;  // x-faces per row of the x-velocity plane
+    constexpr int VH = H + 1;  // y-faces per column of the y-velocity plane
+    constexpr int FO = PPM == 2 ? 1 : 0;
+    constexpr int FXW = W + 1 + 2 * FO;  // x-face values per row
+    constexpr int FYH = H + 1 + 2 * FO;  // y-face values per column
+    constexpr int NP2 = CONS ? 8 * NE : 4 * NE;
     static_assert(NE <= NT, "tile too large for the block");
+    static_assert(FXW * H + W * FYH <= 4 * NE, "face values do not fit the pool");
 
     __shared__ double S[SW * SH];
-    __shared__ double PX[NE], MX[NE], PY[NE], MY[NE];
-    __shared__ double HX[2][NE], HY[2][NE], HZ[2][NE];
-    __shared__ double TXY[2][NE], TYX[2][NE], TZX[2][NE], TZY[2][NE];
-    __shared__ double TXZ[NE], TYZ[NE];
+    __shared__ double UL[3][UW * H];
+    __shared__ double VL[3][W * VH];
+    __shared__ double SGX[UW * H];
+    __shared__ double SGY[W * VH];
+    __shared__ double SC[NE];
+    __shared__ double FL[NE];
+    __shared__ double P1[4 * NE];
+    __shared__ double P2[NP2];
 
-    const FBox& B = boxes[blockIdx.y];
-    const int vc = blockIdx.z;
-    const FComp& Cb = comps[vc];
-    const int comp = A.start_scomp + vc;
-
-    const int nbx = B.hi[0] - B.lo[0] + 1;
-    const int nby = B.hi[1] - B.lo[1] + 1;
-    const int nbz = B.hi[2] - B.lo[2] + 1;
+    const int bid = blockIdx.y;
+    const int blo0 = boxes[bid].lo[0], blo1 = boxes[bid].lo[1], blo2 = boxes[bid].lo[2];
+    const int bhi0 = boxes[bid].hi[0], bhi1 = boxes[bid].hi[1], bhi2 = boxes[bid].hi[2];
+    const int nbx = bhi0 - blo0 + 1;
+    const int nby = bhi1 - blo1 + 1;
+    const int nbz = bhi2 - blo2 + 1;
     const int ntx = (nbx + TX - 1) / TX;
     const int nty = (nby + TY - 1) / TY;
     const int nzc = (nbz + A.tz - 1) / A.tz;
@@ -182,319 +222,486 @@ __global__ void __launch_bounds__(384) k_es_fused(const FBox* __restrict__ boxes
     const int tx = tile % ntx;
     const int ty = (tile / ntx) % nty;
     const int tzc = tile / (ntx * nty);
-    const int X0 = B.lo[0] + tx * TX;
-    const int Y0 = B.lo[1] + ty * TY;
-    const int nx = min(TX, B.hi[0] - X0 + 1);
-    const int ny = min(TY, B.hi[1] - Y0 + 1);
-    const int z0 = B.lo[2] + tzc * A.tz;
-    const int z1 = min(z0 + A.tz - 1, B.hi[2]);
-    const bool last_z = (z1 == B.hi[2]);
-    const bool last_x = (X0 + nx - 1 == B.hi[0]);
-    const bool last_y = (Y0 + ny - 1 == B.hi[1]);
+    const int X0 = blo0 + tx * TX;
+    const int Y0 = blo1 + ty * TY;
+    const int nx = min(TX, bhi0 - X0 + 1);
+    const int ny = min(TY, bhi1 - Y0 + 1);
+    const int z0 = blo2 + tzc * A.tz;
+    const int z1 = min(z0 + A.tz - 1, bhi2);
+    const bool last_x = (X0 + nx - 1 == bhi0);
+    const bool last_y = (Y0 + ny - 1 == bhi1);
+    const bool last_z = (z1 == bhi2);
+
+    const int vc = blockIdx.z;
+    const int comp = A.start_scomp + vc;
+    const int bcx0 = comps[vc].bclo[0], bcx1 = comps[vc].bchi[0];
+    const int bcy0 = comps[vc].bclo[1], bcy1 = comps[vc].bchi[1];
+    const int bcz0 = comps[vc].bclo[2], bcz1 = comps[vc].bchi[2];
+    const int dlx = g.domlo[0], dhx = g.domhi[0];
+    const int dly = g.domlo[1], dhy = g.domhi[1];
+    const int dlz = g.domlo[2], dhz = g.domhi[2];
+
+    // views (the strides fit 32 bits; checked on the host)
+    const DA sv = boxes[bid].s;
+    const DA fv = boxes[bid].fo;
+    const DA uv = boxes[bid].um[0];
+    const DA vv = boxes[bid].um[1];
+    const DA wv = boxes[bid].um[2];
+    const DA e0 = boxes[bid].se[0];
+    const DA e1 = boxes[bid].se[1];
+    const DA e2 = boxes[bid].se[2];
+    const int sjs = (int)sv.js, sks = (int)sv.ks;
+    const int ujs = (int)uv.js, uks = (int)uv.ks;
+    const int vjs = (int)vv.js, vks = (int)vv.ks;
 
     const int tid = threadIdx.x;
-    const int a = tid % W;  // local cell x + 1
-    const int b = tid / W;  // local cell y + 1
+    const int a = tid % W;  // local column: cell (X0 + a - 1, Y0 + b - 1)
+    const int b = tid / W;
     const bool col = (tid < NE) && (a <= nx + 1) && (b <= ny + 1);
     const int I = X0 + a - 1;
     const int J = Y0 + b - 1;
+    const int si = (a + G) + (b + G) * SW;
+    const int ui = a + b * UW;
+    const int vi = a + b * W;
 
     const double dt = g.dt;
+    const double rel_eps = g.rel_eps;
     const double dt2 = 0.5 * dt;
     const double dt3 = dt / 3.0;
     const double dt4 = 0.25 * dt;
     const double dt6 = dt / 6.0;
     const double hx = g.dx[0], hy = g.dx[1], hz = g.dx[2];
-    const double dt6hx = dt6 / hx, dt6hy = dt6 / hy, dt6hz = dt6 / hz;
-    const double dt3hx = dt3 / hx, dt3hy = dt3 / hy, dt3hz = dt3 / hz;
-    const double dt4hx = dt4 / hx, dt4hy = dt4 / hy, dt4hz = dt4 / hz;
+    // per-direction factors of the transverse (t) and edge (e) terms
+    const double tfx = CONS ? dt3 / hx : dt6 / hx;
+    const double tfy = CONS ? dt3 / hy : dt6 / hy;
+    const double tfz = CONS ? dt3 / hz : dt6 / hz;
+    const double efx = CONS ? dt2 / hx : dt4 / hx;
+    const double efy = CONS ? dt2 / hy : dt4 / hy;
+    const double efz = CONS ? dt2 / hz : dt4 / hz;
     const double dt2hx = dt2 / hx, dt2hy = dt2 / hy, dt2hz = dt2 / hz;
     const bool vn0 = A.is_vel && comp == 0;
     const bool vn1 = A.is_vel && comp == 1;
     const bool vn2 = A.is_vel && comp == 2;
-    const DA& s = B.s;
-    const DA& U = B.um[0];
-    const DA& V = B.um[1];
-    const DA& Wm = B.um[2];
 
-    // register state of the owned column / faces
-    double v[9];
-    double Ipz_prev = 0.0, slz = 0.0, srz = 0.0;
-    double slx = 0.0, srx = 0.0, slx_p = 0.0, srx_p = 0.0;
-    double sly = 0.0, sry = 0.0, sly_p = 0.0, sry_p = 0.0;
-
-    // v[jj] = s(I, J, z0 - 6 + jj): after the first shift v[4] = s(z0 - 1)
-    for (int jj = 0; jj < 9; ++jj) v[jj] = 0.0;
+    // plane pointers (block-uniform) at plane z0 - 1
+    gcd* sPl = (gcd*)sv.p + sv.off(X0 - 1 - G, Y0 - 1 - G, z0 - 1, comp);
+    gcd* uPl = (gcd*)uv.p + uv.off(X0 - 1, Y0 - 1, z0 - 1);
+    gcd* vPl = (gcd*)vv.p + vv.off(X0 - 1, Y0 - 1, z0 - 1);
+    // column pointers
+    gcd* sCol = (gcd*)sv.p;
+    gcd* fCol = (gcd*)fv.p;
+    gcd* wCol = (gcd*)wv.p;
+    gd* e0Col = (gd*)e0.p;
+    gd* e1Col = (gd*)e1.p;
+    gd* e2Col = (gd*)e2.p;
     if (col) {
-        for (int jj = 5 - G; jj <= 4 + G; ++jj) v[jj] = s(I, J, z0 - 6 + jj, comp);
+        sCol += sv.off(I, J, z0 - 1 + G, comp);  // next z-window load: s(q + G)
+        fCol += fv.off(I, J, z0 - 1, comp);
+        wCol += wv.off(I, J, z0 - 1);
+        e0Col += e0.off(I, J, z0 - 1, comp);
+        e1Col += e1.off(I, J, z0 - 1, comp);
+        e2Col += e2.off(I, J, z0 - 1, comp);
     }
+    const long long fks = fv.ks, wks = wv.ks, e0ks = e0.ks, e1ks = e1.ks, e2ks = e2.ks;
+
+    // column state
+    double v[9];
+    for (int jj = 0; jj < 9; ++jj) v[jj] = 0.0;
+    double w_q = 0.0, w_p = 0.0, w_hi = 0.0, sig_lo = 0.0;
+    double fo_q = 0.0, fo_p = 0.0;
+    double zf0 = 0.0, zf1 = 0.0, zf2 = 0.0, zsl = 0.0;  // z face values / slope queue
+    double Ipz_prev = 0.0, slz = 0.0, srz = 0.0, hz_q = 0.0, hz_p = 0.0;
+    double slx = 0.0, srx = 0.0, sly = 0.0, sry = 0.0;
+    double slx_p = 0.0, srx_p = 0.0, sly_p = 0.0, sry_p = 0.0;
+    double tx_q = 0.0, tx_p = 0.0, ty_q = 0.0, ty_p = 0.0;     // own transverse terms
+    double cx_q = 0.0, cx_p = 0.0, cy_q = 0.0, cy_p = 0.0;     // conservative parts
+    double bq = 0.0, bp = 0.0;
+    double tzx_q = 0.0, tzx_p = 0.0, tzy_q = 0.0, tzy_p = 0.0;  // simhzx, simhzy
+    double exy_q = 0.0, exy_p = 0.0, eyx_q = 0.0, eyx_p = 0.0;  // own edge terms
+    if (col) {
+        for (int jj = 5 - G; jj <= 4 + G; ++jj) v[jj] = sv(I, J, z0 - 6 + jj, comp);
+        w_q = wv(I, J, z0 - 1);
+        sig_lo = dabs(w_q) * dt / hz;
+    }
+    int slot_q = 0, slot_p = 2;
 
     for (int q = z0 - 1; q <= z1 + 1; ++q) {
-        const int cq = q & 1;
-        const int cp = cq ^ 1;
-        const int p = q - 1;
+        const bool zface = q >= z0;        // z-face q is in the chunk's range
+        const bool pplane = q >= z0 + 1;   // plane p = q - 1 is a chunk plane
+        const bool first = q == z0 - 1;
+        gcd* uq = uPl;
+        gcd* vq = vPl;
+        double* ULq = UL[slot_q];
+        double* VLq = VL[slot_q];
+        const double* ULp = UL[slot_p];
+        const double* VLp = VL[slot_p];
 
-        // ---- A: stage plane q, shift the z window
+        // ---- A: stage plane q; shift the column window
         for (int idx = tid; idx < SW * SH; idx += NT) {
             const int sx = idx % SW, sy = idx / SW;
-            const int lx = sx - 1 - G, ly = sy - 1 - G;
-            if (lx <= nx + G && ly <= ny + G) S[idx] = s(X0 + lx, Y0 + ly, q, comp);
+            if (sx <= nx + 1 + 2 * G && sy <= ny + 1 + 2 * G) S[idx] = sPl[sx + sy * sjs];
+        }
+        for (int idx = tid; idx < UW * H; idx += NT) {
+            const int f = idx % UW, r = idx / UW;
+            if (f <= nx + 2 && r <= ny + 1) {
+                const double u = uq[f + r * ujs];
+                ULq[idx] = u;
+                SGX[idx] = dabs(u) * dt / hx;
+            }
+        }
+        for (int idx = tid; idx < W * VH; idx += NT) {
+            const int c = idx % W, r = idx / W;
+            if (c <= nx + 1 && r <= ny + 2) {
+                const double u = vq[c + r * vjs];
+                VLq[idx] = u;
+                SGY[idx] = dabs(u) * dt / hy;
+            }
         }
         if (col) {
 #pragma unroll
             for (int jj = 0; jj < 8; ++jj) v[jj] = v[jj + 1];
-            v[4 + G] = s(I, J, q + G, comp);
+            v[4 + G] = *sCol;
+            sCol += sks;
+            w_hi = wCol[wks];
+            fo_q = *fCol;
+            fCol += fks;
         }
+        sPl += sks;
+        uPl += uks;
+        vPl += vks;
         __syncthreads();
 
-        // ---- B: PPM in z (register window) and in x, y (LDS plane)
+        // ---- B1: PPM face values of plane q; PPM of cell q in z
+        {
+            double* FX = P1;
+            double* FY = P1 + FXW * H;
+            for (int idx = tid; idx < FXW * H; idx += NT) {
+                const int f = idx % FXW - FO, r = idx / FXW;  // face f: cells f-1 | f
+                if (f <= nx + 2 + FO && r <= ny + 1) {
+                    const double* sr = S + (f + G) + (r + G) * SW;
+                    if (PPM == 1) {
+                        const double sa2 = sr[-2], sa = sr[-1], sb = sr[0], sb2 = sr[1];
+                        FX[idx] = face1(sa, sb, vl_slope(sa2, sa, sb), vl_slope(sa, sb, sb2));
+                    } else {
+                        FX[idx] = cs_face(sr[-2], sr[-1], sr[0], sr[1]);
+                    }
+                }
+            }
+            for (int idx = tid; idx < W * FYH; idx += NT) {
+                const int c = idx % W, f = idx / W - FO;  // face f: cells f-1 | f
+                if (c <= nx + 1 && f <= ny + 2 + FO) {
+                    const double* sr = S + (c + G) + (f + G) * SW;
+                    if (PPM == 1) {
+                        const double sa2 = sr[-2 * SW], sa = sr[-SW], sb = sr[0], sb2 = sr[SW];
+                        FY[idx] = face1(sa, sb, vl_slope(sa2, sa, sb), vl_slope(sa, sb, sb2));
+                    } else {
+                        FY[idx] = cs_face(sr[-2 * SW], sr[-SW], sr[0], sr[SW]);
+                    }
+                }
+            }
+        }
+        double Ipz = 0.0, Imz = 0.0;
         if (col) {
-            double sm, sp, Ip, Im;
-            const double wlo = Wm(I, J, q), whi = Wm(I, J, q + 1);
-            ppm_edges<PPM, 2>(v, q, g.domlo[2], g.domhi[2], Cb.bclo[2], Cb.bchi[2], sm, sp);
-            ppm_trace(v[4], sm, sp, whi, wlo, dt, hz, g.rel_eps, Ip, Im);
-            if (q >= z0) {
+            SC[tid] = v[3];
+            FL[tid] = fo_p;
+            double sm, sp;
+            if (PPM == 1) {
+                // zf0 = face q, zsl = slope(q); new: slope(q+1), face q+1
+                if (first) {
+                    const double d0 = vl_slope(v[2], v[3], v[4]);
+                    zsl = vl_slope(v[3], v[4], v[5]);
+                    zf0 = face1(v[3], v[4], d0, zsl);
+                }
+                const double d1 = vl_slope(v[4], v[5], v[6]);
+                zf1 = face1(v[4], v[5], zsl, d1);
+                zsl = d1;
+                sm = zf0;
+                sp = zf1;
+                cw_limit(v[4], sm, sp);
+            } else {
+                // zf0, zf1, zf2 = faces q-1, q, q+1; new: face q+2
+                if (first) {
+                    zf0 = cs_face(v[1], v[2], v[3], v[4]);
+                    zf1 = cs_face(v[2], v[3], v[4], v[5]);
+                    zf2 = cs_face(v[3], v[4], v[5], v[6]);
+                }
+                const double f3 = cs_face(v[4], v[5], v[6], v[7]);
+                cs_cell(v, zf0, zf1, zf2, f3, 1.e-10, sm, sp);
+                zf0 = zf1;
+                zf1 = zf2;
+                zf2 = f3;
+            }
+            if (ppm_special<PPM>(q, dlz, dhz, bcz0, bcz1))
+                ppm_edges<PPM, 2>(v, q, dlz, dhz, bcz0, bcz1, sm, sp);
+            if (PPM == 1) zf0 = zf1;
+            const double sig_hi = dabs(w_hi) * dt / hz;
+            trace_sig(v[4], sm, sp, w_hi, w_q, sig_hi, sig_lo, rel_eps, Ipz, Imz);
+            sig_lo = sig_hi;
+            if (zface) {
                 slz = Ipz_prev;
-                srz = Im;
-                bc_pair(q, g.domlo[2], g.domhi[2], Cb.bclo[2], Cb.bchi[2], vn2, v[3], v[4], slz,
-                        srz);
-                HZ[cq][tid] = upwind(slz, srz, wlo, 0.0);
+                srz = Imz;
+                bc_pair(q, dlz, dhz, bcz0, bcz1, vn2, v[3], v[4], slz, srz);
+                hz_q = upwind(slz, srz, w_q, 0.0);
             }
-            Ipz_prev = Ip;
-
-            const int si = (a + G) + (b + G) * SW;
-            double w9[9];
-            w9[0] = w9[8] = 0.0;
-#pragma unroll
-            for (int o = -3; o <= 3; ++o) {
-                if (o >= -G && o <= G) w9[4 + o] = S[si + o];
-                else w9[4 + o] = 0.0;
-            }
-            ppm_edges<PPM, 0>(w9, I, g.domlo[0], g.domhi[0], Cb.bclo[0], Cb.bchi[0], sm, sp);
-            ppm_trace(w9[4], sm, sp, U(I + 1, J, q), U(I, J, q), dt, hx, g.rel_eps, Ip, Im);
-            PX[tid] = Ip;
-            MX[tid] = Im;
-#pragma unroll
-            for (int o = -3; o <= 3; ++o) {
-                if (o >= -G && o <= G) w9[4 + o] = S[si + o * SW];
-                else w9[4 + o] = 0.0;
-            }
-            ppm_edges<PPM, 1>(w9, J, g.domlo[1], g.domhi[1], Cb.bclo[1], Cb.bchi[1], sm, sp);
-            ppm_trace(w9[4], sm, sp, V(I, J + 1, q), V(I, J, q), dt, hy, g.rel_eps, Ip, Im);
-            PY[tid] = Ip;
-            MY[tid] = Im;
+            Ipz_prev = Ipz;
         }
         __syncthreads();
 
-        // ---- C: predictor states and upwind states on x- and y-faces of plane q
+        // ---- B2: PPM of plane q in x and y
+        if (col) {
+            const double* FX = P1;
+            const double* FY = P1 + FXW * H;
+            double sm, sp, Ip, Im;
+            double w9[9];
+#pragma unroll
+            for (int o = -4; o <= 4; ++o) w9[4 + o] = (o >= -G && o <= G) ? S[si + o] : 0.0;
+            const int fx = (a + FO) + b * FXW;
+            if (PPM == 1) {
+                sm = FX[fx];
+                sp = FX[fx + 1];
+                cw_limit(w9[4], sm, sp);
+            } else {
+                cs_cell(w9, FX[fx - 1], FX[fx], FX[fx + 1], FX[fx + 2], 1.e-10, sm, sp);
+            }
+            if (ppm_special<PPM>(I, dlx, dhx, bcx0, bcx1))
+                ppm_edges<PPM, 0>(w9, I, dlx, dhx, bcx0, bcx1, sm, sp);
+            trace_sig(w9[4], sm, sp, ULq[ui + 1], ULq[ui], SGX[ui + 1], SGX[ui], rel_eps, Ip,
+                      Im);
+            P2[tid] = Ip;
+            P2[NE + tid] = Im;
+#pragma unroll
+            for (int o = -4; o <= 4; ++o) w9[4 + o] = (o >= -G && o <= G) ? S[si + o * SW] : 0.0;
+            const int fy = a + (b + FO) * W;
+            if (PPM == 1) {
+                sm = FY[fy];
+                sp = FY[fy + W];
+                cw_limit(w9[4], sm, sp);
+            } else {
+                cs_cell(w9, FY[fy - W], FY[fy], FY[fy + W], FY[fy + 2 * W], 1.e-10, sm, sp);
+            }
+            if (ppm_special<PPM>(J, dly, dhy, bcy0, bcy1))
+                ppm_edges<PPM, 1>(w9, J, dly, dhy, bcy0, bcy1, sm, sp);
+            trace_sig(w9[4], sm, sp, VLq[vi + W], VLq[vi], SGY[vi + W], SGY[vi], rel_eps, Ip,
+                      Im);
+            P2[2 * NE + tid] = Ip;
+            P2[3 * NE + tid] = Im;
+        }
+        __syncthreads();
+
+        // ---- C: predictor and upwind states on the x- and y-faces of plane q
         if (col && a >= 1) {
-            const int si = (a + G) + (b + G) * SW;
-            slx = PX[tid - 1];
-            srx = MX[tid];
-            bc_pair(I, g.domlo[0], g.domhi[0], Cb.bclo[0], Cb.bchi[0], vn0, S[si - 1], S[si], slx,
-                    srx);
-            HX[cq][tid] = upwind(slx, srx, U(I, J, q), 0.0);
+            slx = P2[tid - 1];
+            srx = P2[NE + tid];
+            bc_pair(I, dlx, dhx, bcx0, bcx1, vn0, S[si - 1], S[si], slx, srx);
+            P1[tid] = upwind(slx, srx, ULq[ui], 0.0);
         }
         if (col && b >= 1) {
-            const int si = (a + G) + (b + G) * SW;
-            sly = PY[tid - W];
-            sry = MY[tid];
-            bc_pair(J, g.domlo[1], g.domhi[1], Cb.bclo[1], Cb.bchi[1], vn1, S[si - SW], S[si],
-                    sly, sry);
-            HY[cq][tid] = upwind(sly, sry, V(I, J, q), 0.0);
+            sly = P2[2 * NE + tid - W];
+            sry = P2[3 * NE + tid];
+            bc_pair(J, dly, dhy, bcy0, bcy1, vn1, S[si - SW], S[si], sly, sry);
+            P1[NE + tid] = upwind(sly, sry, VLq[vi], 0.0);
         }
         __syncthreads();
 
-        // ---- D: corner-coupled states
-        // simhxy on x-faces of plane q
-        if (col && a >= 1 && b >= 1 && b <= ny) {
-            const int si = (a + G) + (b + G) * SW;
-            double sL = 0.0, sR = 0.0, dvL = 0.0, dvR = 0.0;
+        // ---- D1: per-cell transverse terms
+        if (col) {
+            const double ux1 = ULq[ui + 1], ux0 = ULq[ui];
+            const double vy1 = VLq[vi + W], vy0 = VLq[vi];
             if (CONS) {
-                sL = S[si - 1];
-                sR = S[si];
-                dvL = divu_at(B, hx, I - 1, J, q);
-                dvR = divu_at(B, hx, I, J, q);
+                double dv = ux1 - ux0 + vy1 - vy0 + w_hi - w_q;
+                dv /= hx;
+                bq = dt3 * v[4] * dv;
+                P2[6 * NE + tid] = bq;
+                P2[7 * NE + tid] = bp;
             }
-            double l = trans_side<CONS>(slx, dt3hy, dt6hy, dt3, V(I - 1, J + 1, q), V(I - 1, J, q),
-                                        HY[cq][tid - 1 + W], HY[cq][tid - 1], sL, dvL);
-            double r = trans_side<CONS>(srx, dt3hy, dt6hy, dt3, V(I, J + 1, q), V(I, J, q),
-                                        HY[cq][tid + W], HY[cq][tid], sR, dvR);
-            bc_pair(I, g.domlo[0], g.domhi[0], Cb.bclo[0], Cb.bchi[0], vn0, S[si - 1], S[si], l, r);
-            TXY[cq][tid] = upwind(l, r, U(I, J, q), g.rel_eps);
-        }
-        // simhyx on y-faces of plane q
-        if (col && b >= 1 && a >= 1 && a <= nx) {
-            const int si = (a + G) + (b + G) * SW;
-            double sL = 0.0, sR = 0.0, dvL = 0.0, dvR = 0.0;
-            if (CONS) {
-                sL = S[si - SW];
-                sR = S[si];
-                dvL = divu_at(B, hx, I, J - 1, q);
-                dvR = divu_at(B, hx, I, J, q);
-            }
-            double l = trans_side<CONS>(sly, dt3hx, dt6hx, dt3, U(I + 1, J - 1, q), U(I, J - 1, q),
-                                        HX[cq][tid - W + 1], HX[cq][tid - W], sL, dvL);
-            double r = trans_side<CONS>(sry, dt3hx, dt6hx, dt3, U(I + 1, J, q), U(I, J, q),
-                                        HX[cq][tid + 1], HX[cq][tid], sR, dvR);
-            bc_pair(J, g.domlo[1], g.domhi[1], Cb.bclo[1], Cb.bchi[1], vn1, S[si - SW], S[si], l,
-                    r);
-            TYX[cq][tid] = upwind(l, r, V(I, J, q), g.rel_eps);
-        }
-        if (q >= z0 + 1) {
-            // simhxz on x-faces of plane p = q - 1
-            if (col && a >= 1) {
-                double sL = 0.0, sR = 0.0, dvL = 0.0, dvR = 0.0;
-                if (CONS || bc_needs_s(I, g.domlo[0], g.domhi[0], Cb.bclo[0], Cb.bchi[0])) {
-                    sL = s(I - 1, J, p, comp);
-                    sR = s(I, J, p, comp);
-                }
+            if (a >= 1 && a <= nx) {
+                const double h1 = P1[tid + 1], h0 = P1[tid];
                 if (CONS) {
-                    dvL = divu_at(B, hx, I - 1, J, p);
-                    dvR = divu_at(B, hx, I, J, p);
+                    tx_q = tfx * (h1 * ux1 - h0 * ux0);
+                    cx_q = tfx * v[4] * (ux1 - ux0);
+                    P2[3 * NE + tid] = cx_q;
+                } else {
+                    tx_q = tfx * (ux1 + ux0) * (h1 - h0);
                 }
-                double l = trans_side<CONS>(slx_p, dt3hz, dt6hz, dt3, Wm(I - 1, J, q),
-                                            Wm(I - 1, J, p), HZ[cq][tid - 1], HZ[cp][tid - 1], sL,
-                                            dvL);
-                double r = trans_side<CONS>(srx_p, dt3hz, dt6hz, dt3, Wm(I, J, q), Wm(I, J, p),
-                                            HZ[cq][tid], HZ[cp][tid], sR, dvR);
-                bc_pair(I, g.domlo[0], g.domhi[0], Cb.bclo[0], Cb.bchi[0], vn0, sL, sR, l, r);
-                TXZ[tid] = upwind(l, r, U(I, J, p), g.rel_eps);
+                P2[tid] = tx_q;
             }
-            // simhyz on y-faces of plane p
-            if (col && b >= 1) {
-                double sL = 0.0, sR = 0.0, dvL = 0.0, dvR = 0.0;
-                if (CONS || bc_needs_s(J, g.domlo[1], g.domhi[1], Cb.bclo[1], Cb.bchi[1])) {
-                    sL = s(I, J - 1, p, comp);
-                    sR = s(I, J, p, comp);
-                }
+            if (b >= 1 && b <= ny) {
+                const double h1 = P1[NE + tid + W], h0 = P1[NE + tid];
                 if (CONS) {
-                    dvL = divu_at(B, hx, I, J - 1, p);
-                    dvR = divu_at(B, hx, I, J, p);
+                    ty_q = tfy * (h1 * vy1 - h0 * vy0);
+                    cy_q = tfy * v[4] * (vy1 - vy0);
+                    P2[4 * NE + tid] = cy_q;
+                } else {
+                    ty_q = tfy * (vy1 + vy0) * (h1 - h0);
                 }
-                double l = trans_side<CONS>(sly_p, dt3hz, dt6hz, dt3, Wm(I, J - 1, q),
-                                            Wm(I, J - 1, p), HZ[cq][tid - W], HZ[cp][tid - W], sL,
-                                            dvL);
-                double r = trans_side<CONS>(sry_p, dt3hz, dt6hz, dt3, Wm(I, J, q), Wm(I, J, p),
-                                            HZ[cq][tid], HZ[cp][tid], sR, dvR);
-                bc_pair(J, g.domlo[1], g.domhi[1], Cb.bclo[1], Cb.bchi[1], vn1, sL, sR, l, r);
-                TYZ[tid] = upwind(l, r, V(I, J, p), g.rel_eps);
+                P2[NE + tid] = ty_q;
             }
-        }
-        if (q >= z0) {
-            double dvL = 0.0, dvR = 0.0;
-            if (CONS && col) {
-                dvL = divu_at(B, hx, I, J, q - 1);
-                dvR = divu_at(B, hx, I, J, q);
-            }
-            // simhzx on z-face q
-            if (col && a >= 1 && a <= nx) {
-                double l = trans_side<CONS>(slz, dt3hx, dt6hx, dt3, U(I + 1, J, q - 1),
-                                            U(I, J, q - 1), HX[cp][tid + 1], HX[cp][tid], v[3],
-                                            dvL);
-                double r = trans_side<CONS>(srz, dt3hx, dt6hx, dt3, U(I + 1, J, q), U(I, J, q),
-                                            HX[cq][tid + 1], HX[cq][tid], v[4], dvR);
-                bc_pair(q, g.domlo[2], g.domhi[2], Cb.bclo[2], Cb.bchi[2], vn2, v[3], v[4], l, r);
-                TZX[cq][tid] = upwind(l, r, Wm(I, J, q), g.rel_eps);
-            }
-            // simhzy on z-face q
-            if (col && b >= 1 && b <= ny) {
-                double l = trans_side<CONS>(slz, dt3hy, dt6hy, dt3, V(I, J + 1, q - 1),
-                                            V(I, J, q - 1), HY[cp][tid + W], HY[cp][tid], v[3],
-                                            dvL);
-                double r = trans_side<CONS>(srz, dt3hy, dt6hy, dt3, V(I, J + 1, q), V(I, J, q),
-                                            HY[cq][tid + W], HY[cq][tid], v[4], dvR);
-                bc_pair(q, g.domlo[2], g.domhi[2], Cb.bclo[2], Cb.bchi[2], vn2, v[3], v[4], l, r);
-                TZY[cq][tid] = upwind(l, r, Wm(I, J, q), g.rel_eps);
+            if (pplane) {
+                if (CONS) {
+                    P2[2 * NE + tid] = tfz * (hz_q * w_q - hz_p * w_p);
+                    P2[5 * NE + tid] = tfz * v[3] * (w_q - w_p);
+                } else {
+                    P2[2 * NE + tid] = tfz * (w_q + w_p) * (hz_q - hz_p);
+                }
             }
         }
         __syncthreads();
 
-        // ---- E: edge states
-        if (q >= z0 + 1) {
-            // x-faces of plane p
-            if (col && a >= 1 && b >= 1 && b <= ny && (a <= nx || last_x)) {
-                double l = slx_p, r = srx_p;
-                l = edge_side<CONS>(l, dt2hy, dt4hy, V(I - 1, J + 1, p), V(I - 1, J, p),
-                                    TYZ[tid - 1 + W], TYZ[tid - 1]);
-                r = edge_side<CONS>(r, dt2hy, dt4hy, V(I, J + 1, p), V(I, J, p), TYZ[tid + W],
-                                    TYZ[tid]);
-                l = edge_side<CONS>(l, dt2hz, dt4hz, Wm(I - 1, J, q), Wm(I - 1, J, p),
-                                    TZY[cq][tid - 1], TZY[cp][tid - 1]);
-                r = edge_side<CONS>(r, dt2hz, dt4hz, Wm(I, J, q), Wm(I, J, p), TZY[cq][tid],
-                                    TZY[cp][tid]);
-                double sL = 0.0, sR = 0.0;
-                if (CONS || bc_needs_s(I, g.domlo[0], g.domhi[0], Cb.bclo[0], Cb.bchi[0])) {
-                    sL = s(I - 1, J, p, comp);
-                    sR = s(I, J, p, comp);
-                }
-                if (CONS) {
-                    l = l - dt2hx * sL * (U(I, J, p) - U(I - 1, J, p));
-                    r = r - dt2hx * sR * (U(I + 1, J, p) - U(I, J, p));
-                }
-                const double sedgel = l + dt2 * B.fo(I - 1, J, p, comp);
-                const double sedger = r + dt2 * B.fo(I, J, p, comp);
-                double res = upwind(sedgel, sedger, U(I, J, p), g.rel_eps);
-                res = bc_edge(res, I, g.domlo[0], g.domhi[0], Cb.bclo[0], Cb.bchi[0], vn0, sL, sR,
-                              sedgel, sedger);
-                B.se[0](I, J, p, comp) = res;
-            }
-            // y-faces of plane p
-            if (col && b >= 1 && a >= 1 && a <= nx && (b <= ny || last_y)) {
-                double l = sly_p, r = sry_p;
-                l = edge_side<CONS>(l, dt2hx, dt4hx, U(I + 1, J - 1, p), U(I, J - 1, p),
-                                    TXZ[tid - W + 1], TXZ[tid - W]);
-                r = edge_side<CONS>(r, dt2hx, dt4hx, U(I + 1, J, p), U(I, J, p), TXZ[tid + 1],
-                                    TXZ[tid]);
-                l = edge_side<CONS>(l, dt2hz, dt4hz, Wm(I, J - 1, q), Wm(I, J - 1, p),
-                                    TZX[cq][tid - W], TZX[cp][tid - W]);
-                r = edge_side<CONS>(r, dt2hz, dt4hz, Wm(I, J, q), Wm(I, J, p), TZX[cq][tid],
-                                    TZX[cp][tid]);
-                double sL = 0.0, sR = 0.0;
-                if (CONS || bc_needs_s(J, g.domlo[1], g.domhi[1], Cb.bclo[1], Cb.bchi[1])) {
-                    sL = s(I, J - 1, p, comp);
-                    sR = s(I, J, p, comp);
-                }
-                if (CONS) {
-                    l = l - dt2hy * sL * (V(I, J, p) - V(I, J - 1, p));
-                    r = r - dt2hy * sR * (V(I, J + 1, p) - V(I, J, p));
-                }
-                const double sedgel = l + dt2 * B.fo(I, J - 1, p, comp);
-                const double sedger = r + dt2 * B.fo(I, J, p, comp);
-                double res = upwind(sedgel, sedger, V(I, J, p), g.rel_eps);
-                res = bc_edge(res, J, g.domlo[1], g.domhi[1], Cb.bclo[1], Cb.bchi[1], vn1, sL, sR,
-                              sedgel, sedger);
-                B.se[1](I, J, p, comp) = res;
+        // ---- D2: corner-coupled states
+        if (col && a >= 1 && b >= 1 && b <= ny) {  // simhxy, x-faces of plane q
+            double l = apply3<CONS>(slx, P2[NE + tid - 1], CONS ? P2[6 * NE + tid - 1] : 0.0,
+                                    CONS ? P2[4 * NE + tid - 1] : 0.0);
+            double r = apply3<CONS>(srx, P2[NE + tid], CONS ? P2[6 * NE + tid] : 0.0,
+                                    CONS ? P2[4 * NE + tid] : 0.0);
+            bc_pair(I, dlx, dhx, bcx0, bcx1, vn0, S[si - 1], S[si], l, r);
+            P1[tid] = upwind(l, r, ULq[ui], rel_eps);
+        }
+        if (col && b >= 1 && a >= 1 && a <= nx) {  // simhyx, y-faces of plane q
+            double l = apply3<CONS>(sly, P2[tid - W], CONS ? P2[6 * NE + tid - W] : 0.0,
+                                    CONS ? P2[3 * NE + tid - W] : 0.0);
+            double r = apply3<CONS>(sry, P2[tid], CONS ? P2[6 * NE + tid] : 0.0,
+                                    CONS ? P2[3 * NE + tid] : 0.0);
+            bc_pair(J, dly, dhy, bcy0, bcy1, vn1, S[si - SW], S[si], l, r);
+            P1[NE + tid] = upwind(l, r, VLq[vi], rel_eps);
+        }
+        if (pplane && col && a >= 1) {  // simhxz, x-faces of plane p
+            double l = apply3<CONS>(slx_p, P2[2 * NE + tid - 1], CONS ? P2[7 * NE + tid - 1] : 0.0,
+                                    CONS ? P2[5 * NE + tid - 1] : 0.0);
+            double r = apply3<CONS>(srx_p, P2[2 * NE + tid], CONS ? P2[7 * NE + tid] : 0.0,
+                                    CONS ? P2[5 * NE + tid] : 0.0);
+            bc_pair(I, dlx, dhx, bcx0, bcx1, vn0, SC[tid - 1], SC[tid], l, r);
+            P1[2 * NE + tid] = upwind(l, r, ULp[ui], rel_eps);
+        }
+        if (pplane && col && b >= 1) {  // simhyz, y-faces of plane p
+            double l = apply3<CONS>(sly_p, P2[2 * NE + tid - W], CONS ? P2[7 * NE + tid - W] : 0.0,
+                                    CONS ? P2[5 * NE + tid - W] : 0.0);
+            double r = apply3<CONS>(sry_p, P2[2 * NE + tid], CONS ? P2[7 * NE + tid] : 0.0,
+                                    CONS ? P2[5 * NE + tid] : 0.0);
+            bc_pair(J, dly, dhy, bcy0, bcy1, vn1, SC[tid - W], SC[tid], l, r);
+            P1[3 * NE + tid] = upwind(l, r, VLp[vi], rel_eps);
+        }
+        if (zface && col && a >= 1 && a <= nx) {  // simhzx, z-face q
+            double l = apply3<CONS>(slz, tx_p, bp, cx_p);
+            double r = apply3<CONS>(srz, tx_q, bq, cx_q);
+            bc_pair(q, dlz, dhz, bcz0, bcz1, vn2, v[3], v[4], l, r);
+            tzx_q = upwind(l, r, w_q, rel_eps);
+        }
+        if (zface && col && b >= 1 && b <= ny) {  // simhzy, z-face q
+            double l = apply3<CONS>(slz, ty_p, bp, cy_p);
+            double r = apply3<CONS>(srz, ty_q, bq, cy_q);
+            bc_pair(q, dlz, dhz, bcz0, bcz1, vn2, v[3], v[4], l, r);
+            tzy_q = upwind(l, r, w_q, rel_eps);
+        }
+        __syncthreads();
+
+        // ---- E1: per-cell edge terms
+        if (col && a >= 1 && a <= nx && b >= 1 && b <= ny) {
+            const double ux1 = ULq[ui + 1], ux0 = ULq[ui];
+            const double vy1 = VLq[vi + W], vy0 = VLq[vi];
+            const double t1 = P1[tid + 1], t0 = P1[tid];
+            const double y1 = P1[NE + tid + W], y0 = P1[NE + tid];
+            if (CONS) {
+                exy_q = efx * (t1 * ux1 - t0 * ux0);
+                eyx_q = efy * (y1 * vy1 - y0 * vy0);
+            } else {
+                exy_q = efx * (ux1 + ux0) * (t1 - t0);
+                eyx_q = efy * (vy1 + vy0) * (y1 - y0);
             }
         }
-        // z-face q
-        if ((q >= z0 && q <= z1) || (q == z1 + 1 && last_z)) {
-            if (col && a >= 1 && a <= nx && b >= 1 && b <= ny) {
-                double l = slz, r = srz;
-                l = edge_side<CONS>(l, dt2hx, dt4hx, U(I + 1, J, p), U(I, J, p), TXY[cp][tid + 1],
-                                    TXY[cp][tid]);
-                r = edge_side<CONS>(r, dt2hx, dt4hx, U(I + 1, J, q), U(I, J, q), TXY[cq][tid + 1],
-                                    TXY[cq][tid]);
-                l = edge_side<CONS>(l, dt2hy, dt4hy, V(I, J + 1, p), V(I, J, p), TYX[cp][tid + W],
-                                    TYX[cp][tid]);
-                r = edge_side<CONS>(r, dt2hy, dt4hy, V(I, J + 1, q), V(I, J, q), TYX[cq][tid + W],
-                                    TYX[cq][tid]);
-                if (CONS) {
-                    l = l - dt2hz * v[3] * (Wm(I, J, q) - Wm(I, J, p));
-                    r = r - dt2hz * v[4] * (Wm(I, J, q + 1) - Wm(I, J, q));
-                }
-                const double sedgel = l + dt2 * B.fo(I, J, p, comp);
-                const double sedger = r + dt2 * B.fo(I, J, q, comp);
-                double res = upwind(sedgel, sedger, Wm(I, J, q), g.rel_eps);
-                res = bc_edge(res, q, g.domlo[2], g.domhi[2], Cb.bclo[2], Cb.bchi[2], vn2, v[3],
-                              v[4], sedgel, sedger);
-                B.se[2](I, J, q, comp) = res;
+        if (pplane && col) {
+            if (b >= 1 && b <= ny) {
+                const double vy1 = VLp[vi + W], vy0 = VLp[vi];
+                const double t1 = P1[3 * NE + tid + W], t0 = P1[3 * NE + tid];
+                P2[tid] = CONS ? efy * (t1 * vy1 - t0 * vy0) : efy * (vy1 + vy0) * (t1 - t0);
+                P2[NE + tid] = CONS ? efz * (tzy_q * w_q - tzy_p * w_p)
+                                    : efz * (w_q + w_p) * (tzy_q - tzy_p);
+            }
+            if (a >= 1 && a <= nx) {
+                const double ux1 = ULp[ui + 1], ux0 = ULp[ui];
+                const double t1 = P1[2 * NE + tid + 1], t0 = P1[2 * NE + tid];
+                P2[2 * NE + tid] = CONS ? efx * (t1 * ux1 - t0 * ux0) : efx * (ux1 + ux0) * (t1 - t0);
+                P2[3 * NE + tid] = CONS ? efz * (tzx_q * w_q - tzx_p * w_p)
+                                        : efz * (w_q + w_p) * (tzx_q - tzx_p);
             }
         }
+        __syncthreads();
+
+        // ---- E2: edge states
+        if (pplane && col && a >= 1 && b >= 1 && b <= ny && (a <= nx || last_x)) {  // x-faces, p
+            double l = slx_p - P2[tid - 1];
+            l = l - P2[NE + tid - 1];
+            double r = srx_p - P2[tid];
+            r = r - P2[NE + tid];
+            const double sL = SC[tid - 1], sR = SC[tid];
+            const double u0 = ULp[ui];
+            if (CONS) {
+                l = l - dt2hx * sL * (u0 - ULp[ui - 1]);
+                r = r - dt2hx * sR * (ULp[ui + 1] - u0);
+            }
+            const double sedgel = l + dt2 * FL[tid - 1];
+            const double sedger = r + dt2 * FL[tid];
+            double res = upwind(sedgel, sedger, u0, rel_eps);
+            res = bc_edge(res, I, dlx, dhx, bcx0, bcx1, vn0, sL, sR, sedgel, sedger);
+            e0Col[-e0ks] = res;  // plane p
+        }
+        if (pplane && col && b >= 1 && a >= 1 && a <= nx && (b <= ny || last_y)) {  // y-faces, p
+            double l = sly_p - P2[2 * NE + tid - W];
+            l = l - P2[3 * NE + tid - W];
+            double r = sry_p - P2[2 * NE + tid];
+            r = r - P2[3 * NE + tid];
+            const double sL = SC[tid - W], sR = SC[tid];
+            const double u0 = VLp[vi];
+            if (CONS) {
+                l = l - dt2hy * sL * (u0 - VLp[vi - W]);
+                r = r - dt2hy * sR * (VLp[vi + W] - u0);
+            }
+            const double sedgel = l + dt2 * FL[tid - W];
+            const double sedger = r + dt2 * FL[tid];
+            double res = upwind(sedgel, sedger, u0, rel_eps);
+            res = bc_edge(res, J, dly, dhy, bcy0, bcy1, vn1, sL, sR, sedgel, sedger);
+            e1Col[-e1ks] = res;  // plane p
+        }
+        if (((zface && q <= z1) || (q == z1 + 1 && last_z)) && col && a >= 1 && a <= nx &&
+            b >= 1 && b <= ny) {  // z-face q
+            double l = slz - exy_p;
+            l = l - eyx_p;
+            double r = srz - exy_q;
+            r = r - eyx_q;
+            if (CONS) {
+                l = l - dt2hz * v[3] * (w_q - w_p);
+                r = r - dt2hz * v[4] * (w_hi - w_q);
+            }
+            const double sedgel = l + dt2 * fo_p;
+            const double sedger = r + dt2 * fo_q;
+            double res = upwind(sedgel, sedger, w_q, rel_eps);
+            res = bc_edge(res, q, dlz, dhz, bcz0, bcz1, vn2, v[3], v[4], sedgel, sedger);
+            *e2Col = res;  // z-face q
+        }
+
+        // ---- advance the column state to the next plane
         slx_p = slx;
         srx_p = srx;
         sly_p = sly;
         sry_p = sry;
+        hz_p = hz_q;
+        tx_p = tx_q;
+        ty_p = ty_q;
+        cx_p = cx_q;
+        cy_p = cy_q;
+        bp = bq;
+        tzx_p = tzx_q;
+        tzy_p = tzy_q;
+        exy_p = exy_q;
+        eyx_p = eyx_q;
+        w_p = w_q;
+        w_q = w_hi;
+        fo_p = fo_q;
+        wCol += wks;
+        e0Col += e0ks;
+        e1Col += e1ks;
+        e2Col += e2ks;
+        slot_p = slot_q;
+        slot_q = slot_q == 2 ? 0 : slot_q + 1;
     }
 }
 
@@ -512,9 +719,9 @@ int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
                          const mx_face_multifab* sedge, const mx_face_multifab* umac,
                          const mx_multifab* force, int is_vel, const mx_bcrec* bcs,
                          int start_scomp, int start_bccomp, int num_comp, int is_conservative) {
-    constexpr int TX = 32, TY = 8;
     const auto& grids = c->grids[lev];
     const int tz = c->fused_tz > 0 ? c->fused_tz : 32;
+    const long long lim = 0x7fffffffLL;
     std::vector<FBox> boxes;
     long long maxtiles = 0;
     for (size_t b = 0; b < grids.size(); ++b) {
@@ -530,6 +737,10 @@ int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
             B.um[d] = to_da(umac->dir[d].fab[b]);
             B.se[d] = to_da(sedge->dir[d].fab[b]);
         }
+        // the kernel keeps the plane strides in 32 bits
+        const DA* views[] = {&B.s, &B.fo, &B.um[0], &B.um[1], &B.um[2]};
+        for (const DA* d : views)
+            if (d->js > lim || d->ks > lim) return MX_ERR_UNSUPPORTED;
         const long long nt = (long long)((g.hi[0] - g.lo[0] + TX) / TX) *
                              ((g.hi[1] - g.lo[1] + TY) / TY) * ((g.hi[2] - g.lo[2] + tz) / tz);
         maxtiles = std::max(maxtiles, nt);
@@ -545,7 +756,6 @@ int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
     A.start_scomp = start_scomp;
     A.num_comp = num_comp;
     A.is_vel = is_vel;
-    A.is_conservative = is_conservative;
     A.tz = tz;
     Table tab(c);
     size_t ob = tab.add(boxes);
@@ -556,18 +766,18 @@ int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
     const FComp* dc = tab.dev<FComp>(oc);
     KGeom kg = kgeom(c, lev);
     dim3 grid((unsigned)maxtiles, (unsigned)grids.size(), (unsigned)num_comp);
-    dim3 block(384);
+    dim3 block(NT);
     const bool cons = is_conservative != 0;
     if (c->p.ppm_type == 1) {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, TX, TY, true>), grid, block, db, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true>), grid, block, db, dc, kg, A);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, TX, TY, false>), grid, block, db, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false>), grid, block, db, dc, kg, A);
     } else {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, TX, TY, true>), grid, block, db, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true>), grid, block, db, dc, kg, A);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, TX, TY, false>), grid, block, db, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false>), grid, block, db, dc, kg, A);
     }
     return MX_OK;
 }

@@ -65,7 +65,7 @@ struct mx_adv_ctx {
     std::vector<long long> prof_n;
 
     // kernel selection (mx_adv_set_tuning)
-    int fused_edge = 0;
+    int fused_edge = 1;
     int fused_tz = 32;
 };
 

@@ -147,6 +147,14 @@ class Maestro:
         self._check(self.fns["cell_to_edge"](self.ctx, lev, nr, *[C.byref(x) for x in a]),
                     "CelltoEdge")
 
+    def MakeEdgeState1d(self, rg, s, sedge, force, w0):
+        """Maestro::MakeEdgeState1d; rg is an _abi.RadialGeom, w0 the Maestro
+        member w0 (edge-centred)."""
+        a = [_bs(s), _bs(sedge), _bs(force), _bs(w0)]
+        self._check(self.fns["make_edge_state_1d"](self.ctx, C.byref(rg),
+                                                   *[C.byref(x) for x in a]),
+                    "MakeEdgeState1d")
+
     def advance_pass(self, lev, f):
         """mx_adv_advance_pass with the fields of dict `f` (see workloads.PassFields)."""
         a = _abi.PassArgs()

@@ -59,6 +59,8 @@ int MakeRhoHFlux(const Ctx& c, int lev, const std::vector<Arr> sflux[3],
                  const BaseArr& rhoh0_edge_new, const std::vector<Arr> rh0mac_new[3],
                  const std::vector<Arr> h0mac_old[3], const std::vector<Arr> h0mac_new[3]);
 void CelltoEdge(int lev, int nr, const mx_basestate* cell, mx_basestate* edge);
+int MakeEdgeState1d(const Ctx* c, const mx_radial_geom* rg, const mx_basestate* s,
+                    mx_basestate* sedge, const mx_basestate* force, const mx_basestate* w0);
 
 }  // namespace mxo
 
@@ -264,6 +266,46 @@ int mxo_cell_to_edge(Ctx* c, int lev, int nr, const mx_basestate* s0_cell,
     if (!c || !s0_cell || !s0_edge || nr < 4) return MX_ERR_ARG;
     CelltoEdge(lev, nr, s0_cell, s0_edge);
     return MX_OK;
+}
+
+// argument checks of mx_adv_make_edge_state_1d (include/mx_adv.h)
+static int edge1d_check(const Ctx* c, const mx_radial_geom* rg, const mx_basestate* s,
+                        const mx_basestate* sedge, const mx_basestate* force,
+                        const mx_basestate* w0) {
+    if (!c || !rg || !s || !sedge || !force || !w0) return MX_ERR_ARG;
+    if (!s->p || !sedge->p || !force->p || !w0->p) return MX_ERR_ARG;
+    const int nlev = c->p.spherical ? 1 : rg->nlev;
+    if (nlev < 1 || nlev > MX_MAX_LEV || rg->nr_fine < 4) return MX_ERR_ARG;
+    if (s->len < rg->nr_fine || force->len < rg->nr_fine || sedge->len < rg->nr_fine + 1 ||
+        w0->len < rg->nr_fine + 1)
+        return MX_ERR_ARG;
+    if (s->nlev < nlev || force->nlev < nlev || sedge->nlev < nlev || w0->nlev < nlev)
+        return MX_ERR_ARG;
+    if (c->p.spherical) return rg->dr[0] > 0.0 ? MX_OK : MX_ERR_ARG;
+    for (int n = 0; n < nlev; ++n) {
+        const int nr = rg->nr[n];
+        if (nr < 4 || nr > rg->nr_fine || !(rg->dr[n] > 0.0)) return MX_ERR_ARG;
+        if (rg->nchunks[n] < 0 || rg->nchunks[n] > MX_MAX_CHUNK) return MX_ERR_ARG;
+        if (n + 1 < nlev && rg->nchunks[n + 1] < rg->nchunks[n]) return MX_ERR_UNSUPPORTED;
+        for (int i = 0; i < rg->nchunks[n]; ++i) {
+            const int lo = rg->r_start[n][i], hi = rg->r_end[n][i];
+            if (lo < 0 || hi < lo || hi > nr - 1) return MX_ERR_ARG;
+            // "make_edge_state assumes blocking_factor > 1" (MaestroMakeEdgeState.cpp:427-436)
+            if ((c->p.ppm_type == 1 || c->p.ppm_type == 2) && (lo == 2 || hi == nr - 3))
+                return MX_ERR_ARG;
+            if (n > 0 && (lo / 2 > rg->nr[n - 1] || (hi + 1) / 2 > rg->nr[n - 1]))
+                return MX_ERR_ARG;
+        }
+    }
+    return MX_OK;
+}
+
+int mxo_make_edge_state_1d(Ctx* c, const mx_radial_geom* rg, const mx_basestate* s,
+                           mx_basestate* sedge, const mx_basestate* force,
+                           const mx_basestate* w0) {
+    int rc = edge1d_check(c, rg, s, sedge, force, w0);
+    if (rc) return rc;
+    return MakeEdgeState1d(c, rg, s, sedge, force, w0);
 }
 
 // The pass of mx_adv_advance_pass (include/mx_adv.h).

@@ -96,19 +96,23 @@ def test_pass_kernel_choices(name, cfg, fused, tz):
 
 @pytest.mark.parametrize("is_vel,is_cons,ppm", [(1, 0, 1), (1, 0, 2), (0, 1, 1), (0, 1, 2), (1, 0, 0),
                                                 (0, 1, 0)])
-@pytest.mark.parametrize("dim", [2, 3])
-def test_edge_scal_variants(dim, is_vel, is_cons, ppm):
+@pytest.mark.parametrize("dim,fused", [(2, 0), (3, 0), (3, 1)])
+def test_edge_scal_variants(dim, fused, is_vel, is_cons, ppm):
     """MakeEdgeScal on the velocity (VelocityAdvance: is_vel, bcs_u) and in
-    conservative form, with walls on every side."""
+    conservative form, with walls on every side; in 3-D with the three-kernel
+    and with the fused kernel (ppm_type 0 always takes the three kernels)."""
     n = 32 if dim == 2 else 16
     bc = {0: (P.PHYS_SLIPWALL, P.PHYS_INFLOW), 1: (P.PHYS_OUTFLOW, P.PHYS_NOSLIPWALL)}
     if dim == 3:
         bc[2] = (P.PHYS_SYMMETRY, P.PHYS_OUTFLOW)
     cfg = dict(dim=dim, n=n, max_grid=n // 2, ppm_type=ppm, phys_bc=bc)
-    print(f"[parity] edge_scal dim={dim} is_vel={is_vel} cons={is_cons} ppm={ppm}", flush=True)
+    print(f"[parity] edge_scal dim={dim} fused={fused} is_vel={is_vel} cons={is_cons} ppm={ppm}",
+          flush=True)
     wc, wg = pair(**cfg)
     res = []
-    for w, m in ((wc, oracle_for(wc)), (wg, gpu_for(wg))):
+    mg = gpu_for(wg)
+    mg.set_tuning("edge_scal_fused", fused)
+    for w, m in ((wc, oracle_for(wc)), (wg, mg)):
         # a MAC velocity with filled ghosts from the first two stages
         m.MakeUtrans(0, w.utilde, w.ufull, w.utrans, None, w.w0_cart)
         m.VelPred(0, w.utilde, w.ufull, w.utrans, w.umac, None, w.vel_force, w.w0_cart)

@@ -392,3 +392,105 @@ def test_oracle_rejects_2d_traced_forces():
     with pytest.raises(_abi.MxError) as ei:
         m.advance_pass(0, w.pass_fields())
     assert ei.value.code == _abi.MX_ERR_UNSUPPORTED
+
+
+# ---------------------------------------------------------------------------
+# MakeEdgeState1d (MaestroMakeEdgeState.cpp)
+
+from edge1d_cases import dummy_boxes, params as e1_params, random_fields, run as e1_run
+from edge1d_cases import two_level_geom
+
+
+def e1_oracle(ppm, sph=0, so=4, **kw):
+    p = e1_params(ppm, sph, so, **kw)
+    return p, pyoracle.oracle_maestro(p, [dummy_boxes()])
+
+
+@pytest.mark.parametrize("sph", [0, 1])
+@pytest.mark.parametrize("ppm,so", [(0, 0), (0, 2), (0, 4), (1, 4), (2, 4)])
+def test_edge_state_1d_constant(ppm, so, sph):
+    """Constant s and force: every edge state is s + dt/2 f exactly, for any
+    w0 (the slopes, face values and limiters all vanish)."""
+    nr = 32
+    p, m = e1_oracle(ppm, sph, so)
+    rg = _abi.radial_geom(nr, 1.0 / nr)
+    s = np.full((1, nr), 1.7)
+    f = np.full((1, nr), 0.3)
+    w0 = np.random.default_rng(5).uniform(-1, 1, (1, nr + 1))
+    e = e1_run(m, rg, s, f, w0, NP)
+    dth = 0.5 * p.dt
+    np.testing.assert_array_equal(e, np.full_like(e, 1.7 + dth * 0.3))
+
+
+@pytest.mark.parametrize("sph", [0, 1])
+@pytest.mark.parametrize("ppm", [1, 2])
+def test_edge_state_1d_zero_w0(ppm, sph):
+    """w0 = 0: the trace returns the cell value on both sides, and the
+    Riemann solver averages the two neighbours (a = s + dt/2 f):
+    sedge(r) = (a(r) + a(r-1)) / 2, a(0) and a(nr-1) at the ends."""
+    nr = 40
+    p, m = e1_oracle(ppm, sph)
+    rg = _abi.radial_geom(nr, 1.0 / nr)
+    s, f, _ = random_fields(1, nr, 11)
+    w0 = np.zeros((1, nr + 1))
+    e = e1_run(m, rg, s, f, w0, NP)[0]
+    a = s[0] + (0.5 * p.dt) * f[0]
+    ref = np.empty(nr + 1)
+    ref[1:nr] = 0.5 * (a[:-1] + a[1:])
+    ref[0] = a[0]
+    ref[nr] = a[nr - 1]
+    np.testing.assert_array_equal(e, ref)
+
+
+@pytest.mark.parametrize("sph", [0, 1])
+@pytest.mark.parametrize("ppm", [1, 2])
+@pytest.mark.parametrize("W", [0.7, -0.7])
+def test_edge_state_1d_linear_trace(ppm, sph, W):
+    """A linear profile s = a + b r with uniform w0 = W: both PPM types
+    reconstruct it exactly and no limiter fires, so an interior edge state
+    is the exact characteristic average s_upwind +- b (1 - sigma) / 2."""
+    nr, dr, dt = 48, 1.0 / 48, 0.01
+    p, m = e1_oracle(ppm, sph, dt=dt)
+    rg = _abi.radial_geom(nr, dr)
+    b = 0.25
+    s = (2.0 + b * np.arange(nr))[None, :]
+    f = np.zeros((1, nr))
+    w0 = np.full((1, nr + 1), W)
+    e = e1_run(m, rg, s, f, w0, NP)[0]
+    sig = abs(W) * dt / dr
+    r = np.arange(5, nr - 4)
+    ref = s[0, r - 1] + b * (1 - sig) / 2 if W > 0 else s[0, r] - b * (1 - sig) / 2
+    np.testing.assert_allclose(e[r], ref, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("ppm", [1, 2])
+def test_edge_state_1d_coarse_fine_sync(ppm):
+    """Two radial levels, w0 = 0: at the coarse-fine edges each level takes
+    its outer state from the other level (MaestroMakeEdgeState.cpp:927-950),
+    so both levels hold (a_fine + a_coarse) / 2 there."""
+    p, m = e1_oracle(ppm)
+    rg = two_level_geom()
+    s, f, _ = random_fields(2, 64, 3)
+    w0 = np.zeros((2, 65))
+    e = e1_run(m, rg, s, f, w0, NP)
+    a = s + (0.5 * p.dt) * f
+    assert e[1, 16] == 0.5 * (a[1, 16] + a[0, 7])
+    assert e[0, 8] == 0.5 * (a[1, 16] + a[0, 7])
+    assert e[1, 48] == 0.5 * (a[0, 24] + a[1, 47])
+    assert e[0, 24] == 0.5 * (a[0, 24] + a[1, 47])
+    # interior of each level: the single-level closed form
+    assert e[1, 30] == 0.5 * (a[1, 30] + a[1, 29])
+    assert e[0, 3] == 0.5 * (a[0, 3] + a[0, 2])
+    # fine entries outside the chunk are untouched
+    assert np.isnan(e[1, 10]) and np.isnan(e[1, 60])
+
+
+def test_edge_state_1d_rejects_blocking_factor_one():
+    """A planar ppm chunk starting at r = 2 aborts in the reference
+    (MaestroMakeEdgeState.cpp:427-436); the oracle returns MX_ERR_ARG."""
+    p, m = e1_oracle(1)
+    rg = _abi.radial_geom(64, 1.0 / 64, levels=[(32, 1.0 / 32, [(0, 31)]),
+                                                (64, 1.0 / 64, [(2, 40)])])
+    s, f, w0 = random_fields(2, 64, 1)
+    with pytest.raises(_abi.MxError):
+        e1_run(m, rg, s, f, w0, NP)
