@@ -71,7 +71,22 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="kernel choice passed to mx_adv_set_tuning (repeatable)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
+
+
+def committed_traffic(kernel, workload_tag):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    profiles/traffic_<kernel>_<workload_tag>.md (written by
+    maestroex_amd.profiling from two rocprofv3 --pmc runs), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload_tag}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(args):
@@ -127,6 +142,14 @@ def main():
     lib, fns = mx.load()
     fields = w.pass_fields()
 
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     for _ in range(args.warmup):
         m.advance_pass(0, fields)
     m.sync()
@@ -180,7 +203,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-        "traffic": None,
+        "traffic": committed_traffic(dname, f"{args.n}c_ppm{args.ppm}"),
         "avg_launch_ms": round(avg_ms, 4),
         "bytes_per_cell": bpc,
         "pass_achieved": round(pass_gbs, 2),
