@@ -318,12 +318,16 @@ int mx_adv_fill_face_ghosts(mx_adv_ctx* ctx, int lev,
    EnthalpyAdvance (MaestroEnthalpyAdvance.cpp:145-146,229-233), without the
    MAC projection and the producers of the forcing terms, which sit outside
    the path.  scal is the scalar state in predictor form (rho', rhoh', X);
-   force terms, base states and outputs are caller-owned.  Base states are
-   planar (old == new). */
+   force terms, base states and outputs are caller-owned.  Old and new base
+   states are the same arrays.  Planar runs read the 1-D base states;
+   spherical runs read w0mac and the face base states r0mac (rho0), rh0mac
+   (rho0 h0) and h0mac (h0) of MakeS0mac (MaestroFill3dData.cpp:464-) in
+   MakeRhoXFlux / MakeRhoHFlux. */
 typedef struct {
     mx_multifab utilde, ufull, vel_force, w0_cart;
     mx_multifab scal, scal_force;
     mx_face_multifab w0mac;
+    mx_face_multifab r0mac, rh0mac, h0mac;
     mx_basestate w0, rho0, rho0_edge, rhoh0, rhoh0_edge, rho0_predicted_edge;
     /* outputs */
     mx_face_multifab utrans, umac, sedge, sflux;

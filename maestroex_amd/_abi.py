@@ -145,6 +145,9 @@ class PassArgs(C.Structure):
         ("scal", MultiFabDesc),
         ("scal_force", MultiFabDesc),
         ("w0mac", FaceMultiFabDesc),
+        ("r0mac", FaceMultiFabDesc),
+        ("rh0mac", FaceMultiFabDesc),
+        ("h0mac", FaceMultiFabDesc),
         ("w0", BaseStateDesc),
         ("rho0", BaseStateDesc),
         ("rho0_edge", BaseStateDesc),

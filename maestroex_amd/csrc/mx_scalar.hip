@@ -989,8 +989,20 @@ int mx_adv_advance_pass(mx_adv_ctx* c, int lev, const mx_adv_pass_args* a) {
     int rc = check_level(c, lev);
     if (rc) return rc;
     if (!a) return MX_ERR_ARG;
-    if (c->p.spherical) return MX_ERR_UNSUPPORTED;
     const mx_adv_params& p = c->p;
+    const bool sph = p.dim == 3 && p.spherical;
+    if (sph) {
+        if ((rc = check_face(c, lev, &a->w0mac, 1, 0))) return rc;
+        if ((rc = check_face(c, lev, &a->r0mac, 1, 0))) return rc;
+        if (p.use_exact_base_state) {
+            if ((rc = check_face(c, lev, &a->rh0mac, 1, 0))) return rc;
+        } else {
+            if ((rc = check_face(c, lev, &a->h0mac, 1, 0))) return rc;
+        }
+    }
+    const mx_face_multifab* r0m = sph ? &a->r0mac : nullptr;
+    const mx_face_multifab* rh0m = sph ? &a->rh0mac : nullptr;
+    const mx_face_multifab* h0m = sph ? &a->h0mac : nullptr;
     const int dim = p.dim;
     const int g = p.ppm_type == 2 ? 4 : 3;
     const int fg = p.ppm_trace_forces == 1 ? g : 1;
@@ -1031,12 +1043,12 @@ int mx_adv_advance_pass(mx_adv_ctx* c, int lev, const mx_adv_pass_args* a) {
                              p.comp_rhoh, p.comp_rhoh, 1, 0)))
         return rc;
     if ((rc = make_rhox_flux(c, lev, &a->sflux, &a->etarhoflux, &a->sedge, &a->umac, &a->rho0,
-                             &a->rho0_edge, nullptr, &a->rho0, &a->rho0_edge, nullptr,
+                             &a->rho0_edge, r0m, &a->rho0, &a->rho0_edge, r0m,
                              &a->rho0_predicted_edge, &a->w0, p.first_spec, p.num_spec, 1)))
         return rc;
-    return make_rhoh_flux(c, lev, &a->sflux, &a->sedge, &a->umac, &a->rho0, &a->rho0_edge,
-                          nullptr, &a->rho0, &a->rho0_edge, nullptr, &a->rhoh0, &a->rhoh0_edge,
-                          nullptr, &a->rhoh0, &a->rhoh0_edge, nullptr, nullptr, nullptr);
+    return make_rhoh_flux(c, lev, &a->sflux, &a->sedge, &a->umac, &a->rho0, &a->rho0_edge, r0m,
+                          &a->rho0, &a->rho0_edge, r0m, &a->rhoh0, &a->rhoh0_edge, rh0m,
+                          &a->rhoh0, &a->rhoh0_edge, rh0m, h0m, h0m);
 }
 
 }  // extern "C"

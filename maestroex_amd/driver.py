@@ -163,7 +163,7 @@ class Maestro:
             d = _mf(f[name])
             keep.append(d)
             setattr(a, name, d)
-        for name in ("w0mac", "utrans", "umac", "sedge", "sflux"):
+        for name in ("w0mac", "r0mac", "rh0mac", "h0mac", "utrans", "umac", "sedge", "sflux"):
             d = _face(f.get(name))
             keep.append(d)
             setattr(a, name, d)

@@ -59,7 +59,8 @@ class Workload:
 
     def __init__(self, dim=3, n=32, max_grid=16, ppm_type=1, nspec=3, seed=1234, nmodes=None,
                  ng=4, all_periodic=False, backend=None, ppm_trace_forces=0,
-                 phys_bc=None, zero_velocity=False, shift=(0, 0, 0)):
+                 phys_bc=None, zero_velocity=False, shift=(0, 0, 0), spherical=False,
+                 use_exact_base_state=0, species_pred_type=1, enthalpy_pred_type=1):
         from .fabs import Backend
         self.backend = backend or Backend("numpy")
         self.dim = dim
@@ -70,7 +71,13 @@ class Workload:
         periodic = [1] * dim
         lo_bc = [_abi.PHYS_INTERIOR] * dim
         hi_bc = [_abi.PHYS_INTERIOR] * dim
-        if not all_periodic:
+        if spherical:
+            # a star in a box: outflow on every side (3-D only)
+            assert dim == 3
+            periodic = [0] * dim
+            lo_bc = [_abi.PHYS_OUTFLOW] * dim
+            hi_bc = [_abi.PHYS_OUTFLOW] * dim
+        elif not all_periodic:
             periodic[vert] = 0
             lo_bc[vert] = _abi.PHYS_OUTFLOW
             hi_bc[vert] = _abi.PHYS_OUTFLOW
@@ -110,8 +117,13 @@ class Workload:
                 return arrs[0][None, None, :], arrs[1][None, :, None], arrs[2][:, None, None]
             return arrs[0][None, :], arrs[1][:, None]
 
+        def radius(Xg):
+            if spherical:
+                return np.sqrt(sum((Xg[d] - 0.5) ** 2 for d in range(dim)))
+            return Xg[vert]
+
         def density(Xg):
-            rr = Xg[vert]
+            rr = radius(Xg)
             dist2 = sum((Xg[d] - 0.5) ** 2 for d in range(dim))
             rho_ = np.exp(-rr / 0.3) * (1.0 + 0.1 * np.exp(-dist2 / 0.05 ** 2))
             rho_ = np.broadcast_to(rho_, (ext,) * dim).copy()
@@ -166,10 +178,19 @@ class Workload:
         r_edge = np.arange(nr + 1) * dx
         # a planar w0 is a function of height; with a periodic vertical
         # direction it must vanish (the faces r = 0 and r = nr coincide)
-        w0 = np.zeros(nr + 1) if periodic[vert] else 0.01 * r_edge
+        w0 = np.zeros(nr + 1) if (periodic[vert] or spherical) else 0.01 * r_edge
         w0c = 0.5 * (w0[:-1] + w0[1:])
         w0_cart = np.zeros_like(u)
-        w0_cart[vert] = np.broadcast_to(w0c[rmap].reshape(shp), (ext,) * dim)
+        if spherical:
+            # radial w0(r) = 0.05 r (1 - r) along the unit radial vector
+            rc = radius(X)
+            wr = 0.05 * rc * (1.0 - rc)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                for d in range(dim):
+                    w0_cart[d] = np.broadcast_to(np.where(rc > 0, wr * (X[d] - 0.5) / rc, 0.0),
+                                                 (ext,) * dim)
+        else:
+            w0_cart[vert] = np.broadcast_to(w0c[rmap].reshape(shp), (ext,) * dim)
         ufull = u + w0_cart
 
         umax = float(np.max(np.abs(ufull[(slice(None),) + valid])))
@@ -190,7 +211,11 @@ class Workload:
         scal[4:4 + nspec] = Xg
 
         self.params = make_params(self.geom, ppm_type=ppm_type, nspec=nspec, dt=self.dt,
-                                  rel_eps=self.rel_eps, ppm_trace_forces=ppm_trace_forces)
+                                  rel_eps=self.rel_eps, ppm_trace_forces=ppm_trace_forces,
+                                  spherical=1 if spherical else 0,
+                                  use_exact_base_state=use_exact_base_state,
+                                  species_pred_type=species_pred_type,
+                                  enthalpy_pred_type=enthalpy_pred_type)
         bk = self.backend
         fng = ng if ppm_trace_forces else 1
 
@@ -218,6 +243,35 @@ class Workload:
         self.rhoh0_edge = BaseState(cell_to_edge(rhoh0), bk)
         self.rho0_predicted_edge = BaseState(cell_to_edge(rho0), bk)
 
+        # spherical: w0 and the base states on faces (MakeW0mac / MakeS0mac
+        # style: the radial profiles evaluated at each face centre)
+        self.w0mac = self.r0mac = self.rh0mac = self.h0mac = None
+        if spherical:
+            def face_field(fn):
+                fm = FaceMultiFab(self.boxes, 1, 1, dim, bk)
+                for d in range(dim):
+                    for b in range(len(self.boxes)):
+                        def g(ii, jj, kk, d=d):
+                            idx = [ii, jj, kk]
+                            xs = [(idx[q] + (0.0 if q == d else 0.5)) * dx for q in range(3)]
+                            Z, Y, Xx = np.meshgrid(xs[2], xs[1], xs[0], indexing="ij")
+                            return fn(d, (Xx, Y, Z))[None]
+                        fm[d].set_from_global(b, g)
+                return fm
+
+            def rad(P):
+                return np.sqrt(sum((P[q] - 0.5) ** 2 for q in range(3)))
+
+            def w0f(d, P):
+                r = rad(P)
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    return np.where(r > 0, 0.05 * r * (1.0 - r) * (P[d] - 0.5) / r, 0.0)
+
+            self.w0mac = face_field(w0f)
+            self.r0mac = face_field(lambda d, P: np.exp(-rad(P) / 0.3))
+            self.h0mac = face_field(lambda d, P: 2.0 - rad(P))
+            self.rh0mac = face_field(lambda d, P: np.exp(-rad(P) / 0.3) * (2.0 - rad(P)))
+
         self.nscal = nscal
         self.alloc_outputs()
 
@@ -233,7 +287,8 @@ class Workload:
     def pass_fields(self):
         return dict(utilde=self.utilde, ufull=self.ufull, vel_force=self.vel_force,
                     w0_cart=self.w0_cart, scal=self.scal, scal_force=self.scal_force,
-                    w0mac=None, w0=self.w0, rho0=self.rho0, rho0_edge=self.rho0_edge,
+                    w0mac=self.w0mac, r0mac=self.r0mac, rh0mac=self.rh0mac, h0mac=self.h0mac,
+                    w0=self.w0, rho0=self.rho0, rho0_edge=self.rho0_edge,
                     rhoh0=self.rhoh0, rhoh0_edge=self.rhoh0_edge,
                     rho0_predicted_edge=self.rho0_predicted_edge, utrans=self.utrans,
                     umac=self.umac, sedge=self.sedge, sflux=self.sflux,

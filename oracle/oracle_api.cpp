@@ -313,13 +313,19 @@ int mxo_advance_pass(Ctx* c, int lev, const mx_adv_pass_args* a) {
     int rc = check_level(c, lev);
     if (rc) return rc;
     if (!a) return MX_ERR_ARG;
-    if (c->p.spherical) return MX_ERR_UNSUPPORTED;
     std::vector<mx_bcrec> bs, bu;
     BCSetup(*c, bs, bu);
     const mx_adv_params& p = c->p;
 
     Face utrans = faces(&a->utrans), umac = faces(&a->umac), sedge = faces(&a->sedge),
          sflux = faces(&a->sflux), w0mac = faces(&a->w0mac), none;
+    const bool sph = c->p.dim == 3 && c->p.spherical;
+    Face r0mac, rh0mac, h0mac;
+    if (sph) {
+        r0mac = faces(&a->r0mac);
+        rh0mac = faces(&a->rh0mac);
+        h0mac = faces(&a->h0mac);
+    }
     auto utilde = arrs(&a->utilde), ufull = arrs(&a->ufull), vel_force = arrs(&a->vel_force),
          w0_cart = arrs(&a->w0_cart), scal = arrs(&a->scal), scal_force = arrs(&a->scal_force),
          eta = arrs(&a->etarhoflux);
@@ -352,13 +358,13 @@ int mxo_advance_pass(Ctx* c, int lev, const mx_adv_pass_args* a) {
         par_for(whole, [&](int i, int j, int k) { e(i, j, k) = 0.0; });
     }
     rc = MakeRhoXFlux(*c, lev, sflux.d, eta, sedge.d, umac.d, base(&a->rho0),
-                      base(&a->rho0_edge), none.d, base(&a->rho0), base(&a->rho0_edge), none.d,
+                      base(&a->rho0_edge), r0mac.d, base(&a->rho0), base(&a->rho0_edge), r0mac.d,
                       base(&a->rho0_predicted_edge), base(&a->w0), p.first_spec, p.num_spec);
     if (rc) return finish(rc);
     rc = MakeRhoHFlux(*c, lev, sflux.d, sedge.d, umac.d, base(&a->rho0), base(&a->rho0_edge),
-                      none.d, base(&a->rho0), base(&a->rho0_edge), none.d, base(&a->rhoh0),
-                      base(&a->rhoh0_edge), none.d, base(&a->rhoh0), base(&a->rhoh0_edge),
-                      none.d, none.d, none.d);
+                      r0mac.d, base(&a->rho0), base(&a->rho0_edge), r0mac.d, base(&a->rhoh0),
+                      base(&a->rhoh0_edge), rh0mac.d, base(&a->rhoh0), base(&a->rhoh0_edge),
+                      rh0mac.d, h0mac.d, h0mac.d);
     return finish(rc);
 }
 

@@ -43,6 +43,11 @@ CASES = [
     ("2d_walls_ppm2", dict(dim=2, n=32, max_grid=16, ppm_type=2,
                            phys_bc={0: (P.PHYS_INFLOW, P.PHYS_SYMMETRY),
                                     1: (P.PHYS_NOSLIPWALL, P.PHYS_OUTFLOW)})),
+    ("3d_spherical_ppm1", dict(dim=3, n=16, max_grid=8, ppm_type=1, spherical=True)),
+    ("3d_spherical_ppm2_uneven", dict(dim=3, n=24, max_grid=(16, 8, 12), ppm_type=2,
+                                      spherical=True)),
+    ("3d_spherical_exact", dict(dim=3, n=16, max_grid=8, ppm_type=2, spherical=True,
+                                use_exact_base_state=1)),
 ]
 
 
@@ -138,10 +143,14 @@ def test_edge_scal_variants(dim, fused, is_vel, is_cons, ppm):
 
 
 @pytest.mark.parametrize("spt,ept", [(1, 1), (2, 0), (3, 2), (1, 2), (1, 3), (2, 4), (1, 5), (3, 6)])
-def test_flux_pred_types(spt, ept):
-    """MakeRhoXFlux / MakeRhoHFlux for every species and enthalpy predictor type."""
-    cfg = dict(dim=3, n=16, max_grid=8, ppm_type=1)
-    print(f"[parity] flux species_pred_type={spt} enthalpy_pred_type={ept}", flush=True)
+@pytest.mark.parametrize("geom", ["planar", "spherical", "spherical_exact"])
+def test_flux_pred_types(spt, ept, geom):
+    """MakeRhoXFlux / MakeRhoHFlux for every species and enthalpy predictor
+    type, planar and spherical (face base states, with and without
+    use_exact_base_state)."""
+    cfg = dict(dim=3, n=16, max_grid=8, ppm_type=1, spherical=geom != "planar",
+               use_exact_base_state=1 if geom == "spherical_exact" else 0)
+    print(f"[parity] flux {geom} species_pred_type={spt} enthalpy_pred_type={ept}", flush=True)
     wc, wg = pair(**cfg)
     outs = []
     for w, mk in ((wc, oracle_for), (wg, gpu_for)):

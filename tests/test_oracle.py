@@ -182,6 +182,47 @@ def test_constant_scalars_pass():
             assert np.array_equal(sf[1][reg], um * (rh0 + vals[1]))
 
 
+@pytest.mark.parametrize("exact", [0, 1])
+def test_constant_scalars_spherical_pass(exact):
+    """Spherical: constant scalars give constant edge states, and the fluxes
+    take the face base states: F_X = u (rho0mac + rho') X (pred type 1),
+    F_rhoh = u (rho0mac h0mac + (rho h)') (rhohprime, face base states
+    averaged old/new), or u (rh0mac + (rho h)') with use_exact_base_state."""
+    w = Workload(dim=3, n=16, max_grid=8, ppm_type=2, backend=NP, spherical=True,
+                 use_exact_base_state=exact)
+    vals = {0: 0.05, 1: -0.2, 4: 0.3, 5: 0.5, 6: 0.2}
+    for b in range(len(w.boxes)):
+        for c, v in vals.items():
+            w.scal.arrays[b][c] = v
+        w.scal_force.arrays[b][:] = 0.0
+    m = oracle_for(w)
+    m.advance_pass(0, w.pass_fields())
+    for d in range(3):
+        for b in range(len(w.boxes)):
+            reg = w.sedge[d].valid_region(b)
+            se = w.sedge[d].arrays[b]
+            for c, v in vals.items():
+                assert np.all(se[c][reg] == v)
+            um = w.umac[d].arrays[b][0][w.umac[d].valid_region(b)]
+            r0 = w.r0mac[d].arrays[b][0][w.r0mac[d].valid_region(b)]
+            h0 = w.h0mac[d].arrays[b][0][w.h0mac[d].valid_region(b)]
+            rh0 = w.rh0mac[d].arrays[b][0][w.rh0mac[d].valid_region(b)]
+            sf = w.sflux[d].arrays[b]
+            r0e = 0.5 * r0 + 0.5 * r0
+            tot = 0.0
+            for c in (4, 5, 6):
+                exp = um * (r0e + vals[0]) * vals[c]
+                assert np.array_equal(sf[c][reg], exp)
+                tot = tot + exp
+            assert np.array_equal(sf[0][reg], 0.0 + tot)
+            if exact:
+                exp_h = um * ((0.5 * rh0 + 0.5 * rh0) + vals[1])
+            else:
+                exp_h = um * (r0e * (0.5 * h0 + 0.5 * h0) + vals[1])
+            assert np.array_equal(sf[1][reg], exp_h)
+    assert all(np.all(a == 0.0) for a in w.etarhoflux.to_numpy())
+
+
 # ---------------------------------------------------------------- invariants
 
 def test_translation_invariance():
