@@ -188,6 +188,8 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
                         0: the three-kernel form (PPM, transverse, edges),
                         which also serves ppm_type 0, traced forces and 2-D.
      "edge_scal_tz"     planes per z-chunk of the fused kernel (default 32).
+     "edge_scal_tile"   cell columns per block of the fused kernel:
+                        0 (default) 32 x 8, 1: 16 x 16.
    Unknown keys and out-of-range values return MX_ERR_ARG. */
 int mx_adv_set_tuning(mx_adv_ctx* ctx, const char* key, long long value);
 
@@ -207,7 +209,15 @@ int mx_adv_slope(mx_adv_ctx* ctx, int lev, int dir, const mx_box* bx,
                  const mx_fab* s, mx_fab* sl, const mx_bcrec* bcs, int nbcs,
                  int ncomp, int bc_start_comp, int slope_order);
 
-/* ---- level-wide stages (one level; finest_level == 0 ghost semantics) --- */
+/* ---- level-wide stages -------------------------------------------------- */
+/* Every stage works on one level.  Where the reference fills ghost faces
+   (MakeUtrans, Addw0), this library does what the reference does when
+   finest_level == 0 (FillBoundary within the level, periodic images
+   included, then FillUmacGhost at the physical boundaries of the level's
+   domain) on every level.  With finest_level > 0 the reference instead
+   fills coarse-fine ghost faces from the coarser level (FillPatch,
+   MaestroMakeUtrans.cpp:641-647); here those faces keep the values the
+   caller's arrays hold. */
 
 /* Maestro::MakeUtrans (Maestro.H:1172-1177, MaestroMakeUtrans.cpp:6-648),
    including its closing FillBoundary + FillUmacGhost.  w0_cart is the
@@ -335,6 +345,67 @@ typedef struct {
 } mx_adv_pass_args;
 
 int mx_adv_advance_pass(mx_adv_ctx* ctx, int lev, const mx_adv_pass_args* a);
+
+/* ---- a whole level hierarchy (batched) ---------------------------------- */
+/* No single counterpart in the reference: the reference runs each stage over
+   levels 0..finest_level in turn (MaestroAdvection.cpp:7-84 loops inside every
+   function).  Here args[lev] holds level lev's fields, levels 0..nlev-1; each
+   level follows the single-level semantics of mx_adv_advance_pass (ghost
+   cells filled by the caller; FillBoundary within the level), so the result
+   on every level equals mx_adv_advance_pass on that level alone.  The scalar
+   edge states of all levels go to one fused launch per MakeEdgeScal call
+   (species, Rho, RhoH), over one box/tile work list. */
+int mx_adv_advance_pass_levels(mx_adv_ctx* ctx, int nlev, const mx_adv_pass_args* args);
+
+/* The same hierarchy pass captured once as a HIP graph and replayed with one
+   graph launch on the context stream.  Creation runs the pass twice eagerly
+   (its outputs are written).  The graph holds the device addresses of the
+   caller's buffers as given: replay it only while they stay allocated, and
+   destroy it before the context. */
+typedef struct mx_adv_graph mx_adv_graph;
+int mx_adv_pass_graph_create(mx_adv_ctx* ctx, int nlev, const mx_adv_pass_args* args,
+                             mx_adv_graph** out);
+int mx_adv_pass_graph_launch(mx_adv_graph* g);
+int mx_adv_pass_graph_destroy(mx_adv_graph* g);
+
+/* Autotuner of the fused edge-state kernel (no counterpart in the
+   reference): runs the hierarchy pass once, then times the species edge-state
+   launch for every tile shape (32x8, 16x16 columns) and z-chunk (8, 16, 32,
+   64 planes), keeps the fastest in the context ("edge_scal_tile",
+   "edge_scal_tz") and reports every candidate. */
+#define MX_TUNE_MAX 16
+typedef struct {
+    int ncand;
+    int tile[MX_TUNE_MAX];
+    int tz[MX_TUNE_MAX];
+    double ms[MX_TUNE_MAX];
+    int best; /* index of the chosen candidate */
+} mx_adv_tune_report;
+int mx_adv_autotune(mx_adv_ctx* ctx, int nlev, const mx_adv_pass_args* args,
+                    mx_adv_tune_report* report);
+
+/* ---- differentiation ---------------------------------------------------- */
+/* A differentiable scalar advection step (no single counterpart: the
+   composition of MakeEdgeScal, MakeRhoXFlux and UpdateScal for a species
+   (MaestroDensityAdvance.cpp:115-122, MaestroMakeFlux.cpp:118-140,
+   MaestroAdvection.cpp:165-183) with a fixed MAC velocity and no forcing):
+     sedge = MakeEdgeScal(s, umac, force 0, advective form), ppm_type 1
+     snew  = s + dt (-div(umac sedge) + 0)
+   on a context whose level 0 is one fully periodic box [0, n)^3.  s holds
+   ncomp components with >= 3 ghost cells, umac[0..2] one face FAB per
+   direction with >= 1 ghost face.  The call fills the ghost cells of s and
+   umac by periodic copy and sets face n of umac to face 0 (one periodic face
+   carries one value). */
+int mx_adv_scal_step(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, mx_fab* snew, int ncomp);
+
+/* Hand-written adjoint of mx_adv_scal_step (no counterpart in the
+   reference): given gbar = dL/dsnew on the valid cells, writes sbar = dL/ds
+   on the valid cells and ubar[d] = dL/dumac[d] on faces 0..n-1 (face n is
+   set to 0: it is face 0).  Piecewise differentiation: every limiter and
+   upwind branch is taken as in the forward step.  Deterministic (no
+   atomics); the workspace holds about 93 ncomp + 6 arrays of n^3 doubles. */
+int mx_adv_scal_step_vjp(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, const mx_fab* gbar,
+                         mx_fab* sbar, mx_fab* ubar, int ncomp);
 
 #ifdef __cplusplus
 }

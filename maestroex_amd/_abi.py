@@ -197,8 +197,23 @@ SIGNATURES = {
     "advance_pass": [_vp, _i, P(PassArgs)],
 }
 
+MX_TUNE_MAX = 16
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("ncand", C.c_int), ("tile", C.c_int * MX_TUNE_MAX), ("tz", C.c_int * MX_TUNE_MAX),
+                ("ms", C.c_double * MX_TUNE_MAX), ("best", C.c_int)]
+
+
 # entry points only the HIP library has
 LIB_ONLY = {
+    "advance_pass_levels": [_vp, _i, P(PassArgs)],
+    "pass_graph_create": [_vp, _i, P(PassArgs), P(_vp)],
+    "pass_graph_launch": [_vp],
+    "pass_graph_destroy": [_vp],
+    "autotune": [_vp, _i, P(PassArgs), P(TuneReport)],
+    "scal_step": [_vp, P(Fab), P(Fab), P(Fab), _i],
+    "scal_step_vjp": [_vp, P(Fab), P(Fab), P(Fab), P(Fab), P(Fab), _i],
     "set_stream": [_vp, _vp],
     "sync": [_vp],
     "error_string": [_i],

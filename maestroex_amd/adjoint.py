@@ -1,0 +1,208 @@
+"""Differentiable scalar advection on a periodic box (mx_adv_scal_step and its
+hand-written adjoint mx_adv_scal_step_vjp), as torch.autograd.Functions.
+
+* `ScalStepper(n, ncomp, dt, rel_eps)`: a C-ABI context for one fully
+  periodic box [0, n)^3 (dx = 1/n, ppm_type 1) with device buffers of the
+  step; `step(s, u)` and `vjp(s, u, gbar)` take dense torch tensors:
+  s (ncomp, n, n, n) indexed [c, k, j, i] and u = (ux, uy, uz), each
+  (n, n, n) holding the faces 0..n-1 of its direction (face n is face 0).
+* `ScalStepFn.apply(stepper, s, ux, uy, uz)`: one step, differentiable in s
+  and the velocity.
+* `Rollout.apply(stepper, nsteps, every, s0, ux, uy, uz)`: nsteps steps with
+  a fixed velocity, storing only every `every`-th state; the backward pass
+  recomputes each segment forward from its checkpoint and runs the adjoint
+  steps in reverse, accumulating the velocity gradient over the steps.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from .fabs import Geometry, make_params
+
+
+def periodic_mac_velocity(n, seed=0, nmodes=16, umax=1.0):
+    """A discretely divergence-free MAC velocity on the periodic box, the
+    discrete curl of a random smooth vector potential on cell edges, scaled to
+    max |u| = umax.  Returns (ux, uy, uz), each (n, n, n) [k, j, i]."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(n) / n
+    h = np.arange(n) / n + 0.5 / n
+    A = []
+    for comp in range(3):
+        a = np.zeros((n, n, n))
+        # component comp lives at the mid-point along comp, on nodes elsewhere
+        cz = h if comp == 2 else x
+        cy = h if comp == 1 else x
+        cx = h if comp == 0 else x
+        Z, Y, X = np.meshgrid(cz, cy, cx, indexing="ij")
+        for _ in range(nmodes):
+            k = rng.integers(-3, 4, size=3)
+            if not k.any():
+                k[0] = 1
+            a += rng.normal() * np.cos(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z)
+                                       + rng.uniform(0, 2 * np.pi))
+        A.append(a)
+    Ax, Ay, Az = A
+    dx = 1.0 / n
+    r = lambda a, ax: np.roll(a, -1, axis=ax)  # value at index + 1
+    ux = (r(Az, 1) - Az) / dx - (r(Ay, 0) - Ay) / dx
+    uy = (r(Ax, 0) - Ax) / dx - (r(Az, 2) - Az) / dx
+    uz = (r(Ay, 2) - Ay) / dx - (r(Ax, 1) - Ax) / dx
+    s = umax / max(np.abs(ux).max(), np.abs(uy).max(), np.abs(uz).max())
+    return ux * s, uy * s, uz * s
+
+
+def periodic_divergence(u):
+    ux, uy, uz = u
+    n = ux.shape[0]
+    return (np.roll(ux, -1, 2) - ux + np.roll(uy, -1, 1) - uy + np.roll(uz, -1, 0) - uz) * n
+
+
+class ScalStepper:
+    """Device buffers and a C-ABI context for the step on [0, n)^3."""
+
+    NG = 3
+
+    def __init__(self, n, ncomp, dt, rel_eps, device="cuda:0"):
+        import torch
+        import maestroex_amd as mx
+        self.torch = torch
+        self.n, self.ncomp, self.dt, self.rel_eps = n, ncomp, dt, rel_eps
+        g = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 1], [0, 0, 0], [0, 0, 0])
+        p = make_params(g, ppm_type=1, nspec=ncomp, dt=dt, rel_eps=rel_eps)
+        self.m = mx.gpu_maestro(p, [[((0, 0, 0), (n - 1, n - 1, n - 1))]])
+        self.lib, self.fns = mx.load()
+        dev = torch.device(device)
+        ng = self.NG
+        e = n + 2 * ng
+        f64 = torch.float64
+        self.s = torch.zeros((ncomp, e, e, e), dtype=f64, device=dev)
+        self.snew = torch.zeros((ncomp, n, n, n), dtype=f64, device=dev)
+        self.gbar = torch.zeros((ncomp, n, n, n), dtype=f64, device=dev)
+        self.sbar = torch.zeros((ncomp, n, n, n), dtype=f64, device=dev)
+        self.u = []
+        self.ubar = []
+        for d in range(3):
+            shp = [n + 2, n + 2, n + 2]
+            shp[2 - d] = n + 3  # nodal in d, one ghost face each side
+            self.u.append(torch.zeros(shp, dtype=f64, device=dev))
+            shpb = [n, n, n]
+            shpb[2 - d] = n + 1
+            self.ubar.append(torch.zeros(shpb, dtype=f64, device=dev))
+        self._fab_s = self._fab(self.s, [-ng] * 3, ncomp)
+        self._fab_snew = self._fab(self.snew, [0] * 3, ncomp)
+        self._fab_gbar = self._fab(self.gbar, [0] * 3, ncomp)
+        self._fab_sbar = self._fab(self.sbar, [0] * 3, ncomp)
+        self._fab_u = (_abi.Fab * 3)(*[self._fab(self.u[d], [-1] * 3, 1) for d in range(3)])
+        self._fab_ubar = (_abi.Fab * 3)(*[self._fab(self.ubar[d], [0] * 3, 1) for d in range(3)])
+
+    @staticmethod
+    def _fab(t, begin, nc):
+        f = _abi.Fab()
+        nz, ny, nx = t.shape[-3:]
+        f.p = t.data_ptr()
+        f.jstride = nx
+        f.kstride = nx * ny
+        f.nstride = nx * ny * nz
+        for d in range(3):
+            f.begin[d] = begin[d]
+        f.end[0], f.end[1], f.end[2] = begin[0] + nx, begin[1] + ny, begin[2] + nz
+        f.ncomp = nc
+        return f
+
+    def _load(self, s, u):
+        n, ng = self.n, self.NG
+        self.s[:, ng:ng + n, ng:ng + n, ng:ng + n].copy_(s)
+        for d in range(3):
+            sl = [slice(1, 1 + n)] * 3
+            self.u[d][tuple(sl)].copy_(u[d])
+
+    def _check(self, rc, where):
+        if rc != _abi.MX_OK:
+            raise _abi.MxError(rc, where)
+
+    def step(self, s, u):
+        self._load(s, u)
+        self._check(self.fns["scal_step"](self.m.ctx, C.byref(self._fab_s), self._fab_u,
+                                          C.byref(self._fab_snew), self.ncomp), "scal_step")
+        self.m.sync()
+        return self.snew.clone()
+
+    def vjp(self, s, u, gbar):
+        self._load(s, u)
+        self.gbar.copy_(gbar)
+        self._check(self.fns["scal_step_vjp"](self.m.ctx, C.byref(self._fab_s), self._fab_u,
+                                              C.byref(self._fab_gbar), C.byref(self._fab_sbar),
+                                              self._fab_ubar, self.ncomp), "scal_step_vjp")
+        self.m.sync()
+        n = self.n
+        ub = [self.ubar[d][:n, :n, :n].clone() for d in range(3)]
+        return self.sbar.clone(), ub
+
+
+def _autograd():
+    import torch
+
+    class ScalStepFn(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, stepper, s, ux, uy, uz):
+            ctx.stepper = stepper
+            ctx.save_for_backward(s, ux, uy, uz)
+            return stepper.step(s, (ux, uy, uz))
+
+        @staticmethod
+        def backward(ctx, g):
+            s, ux, uy, uz = ctx.saved_tensors
+            sb, ub = ctx.stepper.vjp(s, (ux, uy, uz), g.contiguous())
+            return None, sb, ub[0], ub[1], ub[2]
+
+    class Rollout(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, stepper, nsteps, every, s0, ux, uy, uz):
+            u = (ux, uy, uz)
+            ckpt = [s0.clone()]
+            s = s0
+            for k in range(nsteps):
+                s = stepper.step(s, u)
+                if (k + 1) % every == 0 and k + 1 < nsteps:
+                    ckpt.append(s.clone())
+            ctx.stepper, ctx.nsteps, ctx.every = stepper, nsteps, every
+            ctx.ckpt = ckpt
+            ctx.save_for_backward(ux, uy, uz)
+            return s
+
+        @staticmethod
+        def backward(ctx, g):
+            ux, uy, uz = ctx.saved_tensors
+            u = (ux, uy, uz)
+            st, nsteps, every = ctx.stepper, ctx.nsteps, ctx.every
+            gb = g.contiguous()
+            ubar = [torch.zeros_like(x) for x in u]
+            nseg = (nsteps + every - 1) // every
+            for seg in reversed(range(nseg)):
+                k0 = seg * every
+                k1 = min(k0 + every, nsteps)
+                states = [ctx.ckpt[seg]]
+                for _ in range(k0, k1 - 1):
+                    states.append(st.step(states[-1], u))
+                for s in reversed(states):
+                    gb, ub = st.vjp(s, u, gb)
+                    for d in range(3):
+                        ubar[d] += ub[d]
+            return None, None, None, gb, ubar[0], ubar[1], ubar[2]
+
+    return ScalStepFn, Rollout
+
+
+_FNS = {}
+
+
+def __getattr__(name):
+    # the autograd classes are built on first use, so that importing this
+    # module does not import torch
+    if name in ("ScalStepFn", "Rollout"):
+        if not _FNS:
+            _FNS["ScalStepFn"], _FNS["Rollout"] = _autograd()
+        return _FNS[name]
+    raise AttributeError(name)

@@ -63,7 +63,20 @@ double* ws_take(mx_adv_ctx* c, size_t ndouble) {
 
 int Table::upload() {
     size_t bytes = c->htab.size();
-    if (bytes == 0) return MX_OK;
+    if (c->tab_mode == 2) {
+        // replay under capture: the recorded table of this position (the
+        // same call sequence over the same descriptors; the bytes are not
+        // compared, the host structs carry uninitialised padding)
+        if (c->rec_next >= c->rec_tabs.size()) return MX_ERR_ARG;
+        if (c->rec_tabs[c->rec_next].size() != bytes) return MX_ERR_ARG;
+        base = c->rec_arena + c->rec_offs[c->rec_next++];
+        return MX_OK;
+    }
+    if (c->tab_mode == 1) c->rec_tabs.push_back(c->htab);
+    if (bytes == 0) {
+        base = c->tab;
+        return MX_OK;
+    }
     if (bytes > c->tab_cap) {
         MX_HIP_CHECK(hipStreamSynchronize(c->stream));
         if (c->tab) MX_HIP_CHECK(hipFree(c->tab));
@@ -77,6 +90,7 @@ int Table::upload() {
         c->tab_cap = cap;
     }
     MX_HIP_CHECK(hipMemcpyAsync(c->tab, c->htab.data(), bytes, hipMemcpyHostToDevice, c->stream));
+    base = c->tab;
     return MX_OK;
 }
 
@@ -579,7 +593,6 @@ int addw0(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge, const mx_face_m
     MX_LAUNCH(c, "k_addw0", k_addw0,  grid_for(maxpts, (int)tasks.size(), 1, block), dim3(block), tab.dev<AddTask>(off), c->p.spherical ? nullptr : w0->p,
                        c->p.spherical ? 0LL : (long long)lev * w0->len, mult, c->p.spherical);
 
-    if (c->p.finest_level != 0) return MX_ERR_UNSUPPORTED;
     return fill_face_ghosts(c, lev, uedge);
 }
 
@@ -730,6 +743,12 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
     } else if (k == "edge_scal_tz") {
         if (value < 1 || value > 4096) return MX_ERR_ARG;
         c->fused_tz = (int)value;
+    } else if (k == "edge_scal_tile") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->fused_tile = (int)value;
+    } else if (k == "edge_scal_per_box") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->fused_per_box = (int)value;
     } else {
         return MX_ERR_ARG;
     }
@@ -842,7 +861,6 @@ int mx_adv_fill_face_ghosts(mx_adv_ctx* c, int lev, mx_face_multifab* uedge) {
     if (rc) return rc;
     rc = check_face(c, lev, uedge, 1, 1);
     if (rc) return rc;
-    if (c->p.finest_level != 0) return MX_ERR_UNSUPPORTED;
     return fill_face_ghosts(c, lev, uedge);
 }
 

@@ -48,14 +48,20 @@ namespace {
 typedef __attribute__((address_space(1))) const double gcd;
 typedef __attribute__((address_space(1))) double gd;
 
-constexpr int TX = 32;
-constexpr int TY = 8;
 constexpr int NT = 384;
 
 struct FBox {
     int lo[3], hi[3];
     DA s, fo;
     DA um[3], se[3];
+    // geometry of the box's level
+    double dx[3];
+    int domlo[3], domhi[3];
+};
+
+// one block's work: a tile of one box, chunk tzc of its planes
+struct FTile {
+    int box, tx, ty, tzc;
 };
 
 struct FComp {
@@ -179,8 +185,9 @@ __device__ __forceinline__ double apply3(double x, double a, double b, double c)
     return x - a;
 }
 
-template <int PPM, bool CONS>
+template <int PPM, bool CONS, int TX, int TY>
 __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
+                                                 const FTile* __restrict__ tiles,
                                                  const FComp* __restrict__ comps, KGeom g,
                                                  FArgs A) {
     constexpr int G = PPM == 2 ? 3 : 2;
@@ -208,20 +215,13 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
     __shared__ double P1[4 * NE];
     __shared__ double P2[NP2];
 
-    const int bid = blockIdx.y;
+    const FTile tl = tiles[blockIdx.x];
+    const int bid = tl.box;
     const int blo0 = boxes[bid].lo[0], blo1 = boxes[bid].lo[1], blo2 = boxes[bid].lo[2];
     const int bhi0 = boxes[bid].hi[0], bhi1 = boxes[bid].hi[1], bhi2 = boxes[bid].hi[2];
-    const int nbx = bhi0 - blo0 + 1;
-    const int nby = bhi1 - blo1 + 1;
-    const int nbz = bhi2 - blo2 + 1;
-    const int ntx = (nbx + TX - 1) / TX;
-    const int nty = (nby + TY - 1) / TY;
-    const int nzc = (nbz + A.tz - 1) / A.tz;
-    const int tile = blockIdx.x;
-    if (tile >= ntx * nty * nzc) return;
-    const int tx = tile % ntx;
-    const int ty = (tile / ntx) % nty;
-    const int tzc = tile / (ntx * nty);
+    const int tx = tl.tx;
+    const int ty = tl.ty;
+    const int tzc = tl.tzc;
     const int X0 = blo0 + tx * TX;
     const int Y0 = blo1 + ty * TY;
     const int nx = min(TX, bhi0 - X0 + 1);
@@ -237,9 +237,9 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
     const int bcx0 = comps[vc].bclo[0], bcx1 = comps[vc].bchi[0];
     const int bcy0 = comps[vc].bclo[1], bcy1 = comps[vc].bchi[1];
     const int bcz0 = comps[vc].bclo[2], bcz1 = comps[vc].bchi[2];
-    const int dlx = g.domlo[0], dhx = g.domhi[0];
-    const int dly = g.domlo[1], dhy = g.domhi[1];
-    const int dlz = g.domlo[2], dhz = g.domhi[2];
+    const int dlx = boxes[bid].domlo[0], dhx = boxes[bid].domhi[0];
+    const int dly = boxes[bid].domlo[1], dhy = boxes[bid].domhi[1];
+    const int dlz = boxes[bid].domlo[2], dhz = boxes[bid].domhi[2];
 
     // views (the strides fit 32 bits; checked on the host)
     const DA sv = boxes[bid].s;
@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
     const double dt3 = dt / 3.0;
     const double dt4 = 0.25 * dt;
     const double dt6 = dt / 6.0;
-    const double hx = g.dx[0], hy = g.dx[1], hz = g.dx[2];
+    const double hx = boxes[bid].dx[0], hy = boxes[bid].dx[1], hz = boxes[bid].dx[2];
     // per-direction factors of the transverse (t) and edge (e) terms
     const double tfx = CONS ? dt3 / hx : dt6 / hx;
     const double tfy = CONS ? dt3 / hy : dt6 / hy;
@@ -715,37 +715,50 @@ bool fused_edge_scal_ok(const mx_adv_ctx* c, int is_conservative) {
            c->p.ppm_trace_forces == 0 && c->fused_edge;
 }
 
-int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
-                         const mx_face_multifab* sedge, const mx_face_multifab* umac,
-                         const mx_multifab* force, int is_vel, const mx_bcrec* bcs,
-                         int start_scomp, int start_bccomp, int num_comp, int is_conservative) {
-    const auto& grids = c->grids[lev];
-    const int tz = c->fused_tz > 0 ? c->fused_tz : 32;
+namespace {
+
+template <int TX, int TY>
+int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int is_vel,
+                 const mx_bcrec* bcs, int start_scomp, int start_bccomp, int num_comp,
+                 int is_conservative) {
     const long long lim = 0x7fffffffLL;
     std::vector<FBox> boxes;
-    long long maxtiles = 0;
-    for (size_t b = 0; b < grids.size(); ++b) {
-        FBox B;
-        HBox g = hbox(grids[b]);
-        for (int q = 0; q < 3; ++q) {
-            B.lo[q] = g.lo[q];
-            B.hi[q] = g.hi[q];
+    std::vector<FTile> tiles;
+    for (const EdgeLevel& L : levs) {
+        const auto& grids = c->grids[L.lev];
+        for (size_t b = 0; b < grids.size(); ++b) {
+            if (L.box >= 0 && (size_t)L.box != b) continue;
+            FBox B;
+            HBox g = hbox(grids[b]);
+            for (int q = 0; q < 3; ++q) {
+                B.lo[q] = g.lo[q];
+                B.hi[q] = g.hi[q];
+                B.dx[q] = c->p.dx[L.lev][q];
+                B.domlo[q] = c->p.domain_lo[L.lev][q];
+                B.domhi[q] = c->p.domain_hi[L.lev][q];
+            }
+            B.s = to_da(L.state->fab[b]);
+            B.fo = to_da(L.force->fab[b]);
+            for (int d = 0; d < 3; ++d) {
+                B.um[d] = to_da(L.umac->dir[d].fab[b]);
+                B.se[d] = to_da(L.sedge->dir[d].fab[b]);
+            }
+            // the kernel keeps the plane strides in 32 bits
+            const DA* views[] = {&B.s, &B.fo, &B.um[0], &B.um[1], &B.um[2]};
+            for (const DA* d : views)
+                if (d->js > lim || d->ks > lim) return MX_ERR_UNSUPPORTED;
+            const int ntx = (g.hi[0] - g.lo[0] + TX) / TX;
+            const int nty = (g.hi[1] - g.lo[1] + TY) / TY;
+            const int nzc = (g.hi[2] - g.lo[2] + tz) / tz;
+            const int id = (int)boxes.size();
+            for (int z = 0; z < nzc; ++z)
+                for (int y = 0; y < nty; ++y)
+                    for (int x = 0; x < ntx; ++x) tiles.push_back({id, x, y, z});
+            boxes.push_back(B);
         }
-        B.s = to_da(state->fab[b]);
-        B.fo = to_da(force->fab[b]);
-        for (int d = 0; d < 3; ++d) {
-            B.um[d] = to_da(umac->dir[d].fab[b]);
-            B.se[d] = to_da(sedge->dir[d].fab[b]);
-        }
-        // the kernel keeps the plane strides in 32 bits
-        const DA* views[] = {&B.s, &B.fo, &B.um[0], &B.um[1], &B.um[2]};
-        for (const DA* d : views)
-            if (d->js > lim || d->ks > lim) return MX_ERR_UNSUPPORTED;
-        const long long nt = (long long)((g.hi[0] - g.lo[0] + TX) / TX) *
-                             ((g.hi[1] - g.lo[1] + TY) / TY) * ((g.hi[2] - g.lo[2] + tz) / tz);
-        maxtiles = std::max(maxtiles, nt);
-        boxes.push_back(B);
     }
+    if (tiles.empty()) return MX_OK;
+    if (tiles.size() > 0x7fffffffULL) return MX_ERR_UNSUPPORTED;
     std::vector<FComp> comps(num_comp);
     for (int vc = 0; vc < num_comp; ++vc)
         for (int d = 0; d < 3; ++d) {
@@ -759,27 +772,69 @@ int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
     A.tz = tz;
     Table tab(c);
     size_t ob = tab.add(boxes);
+    size_t ot = tab.add(tiles);
     size_t oc = tab.add(comps);
     int rc = tab.upload();
     if (rc) return rc;
     const FBox* db = tab.dev<FBox>(ob);
+    const FTile* dtl = tab.dev<FTile>(ot);
     const FComp* dc = tab.dev<FComp>(oc);
-    KGeom kg = kgeom(c, lev);
-    dim3 grid((unsigned)maxtiles, (unsigned)grids.size(), (unsigned)num_comp);
+    KGeom kg = kgeom(c, levs[0].lev);
+    dim3 grid((unsigned)tiles.size(), 1, (unsigned)num_comp);
     dim3 block(NT);
     const bool cons = is_conservative != 0;
     if (c->p.ppm_type == 1) {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true>), grid, block, db, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true, TX, TY>), grid, block, db, dtl, dc, kg, A);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false>), grid, block, db, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false, TX, TY>), grid, block, db, dtl, dc, kg, A);
     } else {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true>), grid, block, db, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true, TX, TY>), grid, block, db, dtl, dc, kg, A);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false>), grid, block, db, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false, TX, TY>), grid, block, db, dtl, dc, kg, A);
     }
     return MX_OK;
+}
+
+}  // namespace
+
+int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int is_vel,
+                                const mx_bcrec* bcs, int start_scomp, int start_bccomp,
+                                int num_comp, int is_conservative) {
+    if (levs.empty()) return MX_OK;
+    const int tz = c->fused_tz > 0 ? c->fused_tz : 32;
+    if (c->fused_per_box) {
+        // comparison mode: the same work as one launch per box
+        std::vector<EdgeLevel> one(1);
+        for (const EdgeLevel& L : levs)
+            for (size_t b = 0; b < c->grids[L.lev].size(); ++b) {
+                one[0] = L;
+                one[0].box = (int)b;
+                int rc = c->fused_tile == 1
+                             ? launch_fused<16, 16>(c, one, tz, is_vel, bcs, start_scomp,
+                                                    start_bccomp, num_comp, is_conservative)
+                             : launch_fused<32, 8>(c, one, tz, is_vel, bcs, start_scomp,
+                                                   start_bccomp, num_comp, is_conservative);
+                if (rc) return rc;
+            }
+        return MX_OK;
+    }
+    if (c->fused_tile == 1)
+        return launch_fused<16, 16>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
+                                    num_comp, is_conservative);
+    return launch_fused<32, 8>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp, num_comp,
+                               is_conservative);
+}
+
+int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
+                         const mx_face_multifab* sedge, const mx_face_multifab* umac,
+                         const mx_multifab* force, int is_vel, const mx_bcrec* bcs,
+                         int start_scomp, int start_bccomp, int num_comp, int is_conservative) {
+    std::vector<EdgeLevel> levs(1);
+    levs[0] = {lev, state, sedge, umac, force};
+    return make_edge_scal_fused_levels(c, levs, is_vel, bcs, start_scomp, start_bccomp, num_comp,
+                                       is_conservative);
 }
 
 }  // namespace mxh

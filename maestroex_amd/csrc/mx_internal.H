@@ -67,6 +67,17 @@ struct mx_adv_ctx {
     // kernel selection (mx_adv_set_tuning)
     int fused_edge = 1;
     int fused_tz = 32;
+    int fused_tile = 0;     // 0: 32 x 8 columns, 1: 16 x 16
+    int fused_per_box = 0;  // 1: one fused launch per box (comparison mode)
+
+    // box tables under graph capture: 0 eager (one device table, copied at
+    // every call); 1 record (eager, and keep each table's bytes in order);
+    // 2 replay (each table is the next recorded one, already in rec_arena)
+    int tab_mode = 0;
+    std::vector<std::vector<char>> rec_tabs;
+    std::vector<size_t> rec_offs;
+    char* rec_arena = nullptr;
+    size_t rec_next = 0;
 };
 
 namespace mxh {
@@ -209,7 +220,7 @@ size_t tmp_bytes(const std::vector<mx_box>& grids, int ncomp, RegionFn region) {
 // per-call device table: append host structs, upload once, get device pointers
 struct Table {
     mx_adv_ctx* c;
-    std::vector<size_t> offs;
+    char* base = nullptr;  // device copy, set by upload()
     explicit Table(mx_adv_ctx* ctx) : c(ctx) { c->htab.clear(); }
     template <class T>
     size_t add(const std::vector<T>& v) {
@@ -221,7 +232,7 @@ struct Table {
     int upload();
     template <class T>
     const T* dev(size_t off) const {
-        return reinterpret_cast<const T*>(c->tab + off);
+        return reinterpret_cast<const T*>(base + off);
     }
 };
 
@@ -263,12 +274,28 @@ int make_rhoh_flux(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux,
                    const mx_basestate* rhoh0_edge_new, const mx_face_multifab* rh0mac_new,
                    const mx_face_multifab* h0mac_old, const mx_face_multifab* h0mac_new);
 
-// fused 3-D MakeEdgeScal (ppm_type 1/2, forces not traced)
+// fused 3-D MakeEdgeScal (ppm_type 1/2, forces not traced); the _levels form
+// runs the boxes of several levels in one launch
+struct EdgeLevel {
+    int lev;
+    const mx_multifab* state;
+    const mx_face_multifab* sedge;
+    const mx_face_multifab* umac;
+    const mx_multifab* force;
+    int box = -1;  // one box of the level, or -1 for all
+};
 bool fused_edge_scal_ok(const mx_adv_ctx* c, int is_conservative);
+int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int is_vel,
+                                const mx_bcrec* bcs, int start_scomp, int start_bccomp,
+                                int num_comp, int is_conservative);
 int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
                          const mx_face_multifab* sedge, const mx_face_multifab* umac,
                          const mx_multifab* force, int is_vel, const mx_bcrec* bcs,
                          int start_scomp, int start_bccomp, int num_comp, int is_conservative);
+
+// the pass (mx_adv_advance_pass and its multi-level / graph forms)
+int pass_check(mx_adv_ctx* c, int lev, const mx_adv_pass_args* a);
+int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_args* a);
 
 void bc_setup_host(mx_adv_ctx* c);
 int check_level(const mx_adv_ctx* c, int lev);

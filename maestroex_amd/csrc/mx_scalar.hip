@@ -877,6 +877,107 @@ bool base_ok(const mx_basestate* s, int lev, int need_len) {
 
 }  // namespace
 
+namespace mxh {
+
+// argument checks of the pass on one level (mx_adv_advance_pass)
+int pass_check(mx_adv_ctx* c, int lev, const mx_adv_pass_args* a) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!a) return MX_ERR_ARG;
+    const mx_adv_params& p = c->p;
+    const bool sph = p.dim == 3 && p.spherical;
+    if (sph) {
+        if ((rc = check_face(c, lev, &a->w0mac, 1, 0))) return rc;
+        if ((rc = check_face(c, lev, &a->r0mac, 1, 0))) return rc;
+        if (p.use_exact_base_state) {
+            if ((rc = check_face(c, lev, &a->rh0mac, 1, 0))) return rc;
+        } else {
+            if ((rc = check_face(c, lev, &a->h0mac, 1, 0))) return rc;
+        }
+    }
+    const int dim = p.dim;
+    const int g = p.ppm_type == 2 ? 4 : 3;
+    const int fg = p.ppm_trace_forces == 1 ? g : 1;
+    const int nr = p.domain_hi[lev][dim - 1] + 1;
+    if ((rc = check_mf(c, lev, &a->utilde, dim, g, -1))) return rc;
+    if ((rc = check_mf(c, lev, &a->ufull, dim, 1, -1))) return rc;
+    if ((rc = check_mf(c, lev, &a->vel_force, dim, fg, -1))) return rc;
+    if ((rc = check_mf(c, lev, &a->w0_cart, dim, 0, -1))) return rc;
+    if ((rc = check_mf(c, lev, &a->scal, p.nscal, g, -1))) return rc;
+    if ((rc = check_mf(c, lev, &a->scal_force, p.nscal, fg, -1))) return rc;
+    if ((rc = check_face(c, lev, &a->utrans, 1, 1))) return rc;
+    if ((rc = check_face(c, lev, &a->umac, 1, 1))) return rc;
+    if ((rc = check_face(c, lev, &a->sedge, p.nscal, 0))) return rc;
+    if ((rc = check_face(c, lev, &a->sflux, p.nscal, 0))) return rc;
+    if ((rc = check_mf(c, lev, &a->etarhoflux, 1, 0, dim - 1))) return rc;
+    if (!base_ok(&a->w0, lev, nr + 1) || !base_ok(&a->rho0, lev, nr) ||
+        !base_ok(&a->rho0_edge, lev, nr + 1) || !base_ok(&a->rhoh0, lev, nr) ||
+        !base_ok(&a->rhoh0_edge, lev, nr + 1) || !base_ok(&a->rho0_predicted_edge, lev, nr + 1))
+        return MX_ERR_ARG;
+    if (dim == 2 && p.ppm_trace_forces == 1) return MX_ERR_UNSUPPORTED;
+    return MX_OK;
+}
+
+// The pass over levels lev0 .. lev0 + nlev - 1 (a[i] holds level lev0 + i).
+// Levels are independent, so the velocity stages run level by level, the
+// scalar edge states of every level go to one fused launch per component
+// group (MakeEdgeScal's species, Rho, RhoH calls) and the fluxes follow.
+int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_args* a) {
+    if (!c || !a || nlev < 1 || lev0 < 0 || lev0 + nlev - 1 > c->p.finest_level)
+        return MX_ERR_ARG;
+    int rc;
+    for (int i = 0; i < nlev; ++i)
+        if ((rc = pass_check(c, lev0 + i, &a[i]))) return rc;
+    const mx_adv_params& p = c->p;
+    const bool sph = p.dim == 3 && p.spherical;
+    const mx_bcrec* bs = c->bcs_s.data();
+    const int nbs = (int)c->bcs_s.size();
+    for (int i = 0; i < nlev; ++i) {
+        const int lev = lev0 + i;
+        const mx_adv_pass_args* A = &a[i];
+        if ((rc = make_utrans(c, lev, &A->utilde, &A->ufull, &A->utrans, &A->w0mac, &A->w0_cart)))
+            return rc;
+        if ((rc = addw0(c, lev, &A->utrans, &A->w0mac, &A->w0, 1.0))) return rc;
+        if ((rc = velpred(c, lev, &A->utilde, &A->ufull, &A->utrans, &A->umac, &A->w0mac,
+                          &A->vel_force, &A->w0_cart)))
+            return rc;
+        if ((rc = addw0(c, lev, &A->umac, &A->w0mac, &A->w0, 1.0))) return rc;
+    }
+    const int groups[3][2] = {{p.first_spec, p.num_spec}, {p.comp_rho, 1}, {p.comp_rhoh, 1}};
+    if (fused_edge_scal_ok(c, 0)) {
+        std::vector<EdgeLevel> levs;
+        for (int i = 0; i < nlev; ++i)
+            levs.push_back({lev0 + i, &a[i].scal, &a[i].sedge, &a[i].umac, &a[i].scal_force});
+        for (const auto& gr : groups)
+            if ((rc = make_edge_scal_fused_levels(c, levs, 0, bs, gr[0], gr[0], gr[1], 0)))
+                return rc;
+    } else {
+        for (int i = 0; i < nlev; ++i)
+            for (const auto& gr : groups)
+                if ((rc = make_edge_scal(c, lev0 + i, &a[i].scal, &a[i].sedge, &a[i].umac,
+                                         &a[i].scal_force, 0, bs, nbs, gr[0], gr[0], gr[1], 0)))
+                    return rc;
+    }
+    for (int i = 0; i < nlev; ++i) {
+        const int lev = lev0 + i;
+        const mx_adv_pass_args* A = &a[i];
+        const mx_face_multifab* r0m = sph ? &A->r0mac : nullptr;
+        const mx_face_multifab* rh0m = sph ? &A->rh0mac : nullptr;
+        const mx_face_multifab* h0m = sph ? &A->h0mac : nullptr;
+        if ((rc = make_rhox_flux(c, lev, &A->sflux, &A->etarhoflux, &A->sedge, &A->umac,
+                                 &A->rho0, &A->rho0_edge, r0m, &A->rho0, &A->rho0_edge, r0m,
+                                 &A->rho0_predicted_edge, &A->w0, p.first_spec, p.num_spec, 1)))
+            return rc;
+        if ((rc = make_rhoh_flux(c, lev, &A->sflux, &A->sedge, &A->umac, &A->rho0, &A->rho0_edge,
+                                 r0m, &A->rho0, &A->rho0_edge, r0m, &A->rhoh0, &A->rhoh0_edge,
+                                 rh0m, &A->rhoh0, &A->rhoh0_edge, rh0m, h0m, h0m)))
+            return rc;
+    }
+    return MX_OK;
+}
+
+}  // namespace mxh
+
 extern "C" {
 
 int mx_adv_bc_setup(const mx_adv_ctx* c, mx_bcrec* bcs_s, mx_bcrec* bcs_u) {
@@ -988,67 +1089,7 @@ int mx_adv_make_rhoh_flux(mx_adv_ctx* c, int lev, mx_face_multifab* sflux,
 int mx_adv_advance_pass(mx_adv_ctx* c, int lev, const mx_adv_pass_args* a) {
     int rc = check_level(c, lev);
     if (rc) return rc;
-    if (!a) return MX_ERR_ARG;
-    const mx_adv_params& p = c->p;
-    const bool sph = p.dim == 3 && p.spherical;
-    if (sph) {
-        if ((rc = check_face(c, lev, &a->w0mac, 1, 0))) return rc;
-        if ((rc = check_face(c, lev, &a->r0mac, 1, 0))) return rc;
-        if (p.use_exact_base_state) {
-            if ((rc = check_face(c, lev, &a->rh0mac, 1, 0))) return rc;
-        } else {
-            if ((rc = check_face(c, lev, &a->h0mac, 1, 0))) return rc;
-        }
-    }
-    const mx_face_multifab* r0m = sph ? &a->r0mac : nullptr;
-    const mx_face_multifab* rh0m = sph ? &a->rh0mac : nullptr;
-    const mx_face_multifab* h0m = sph ? &a->h0mac : nullptr;
-    const int dim = p.dim;
-    const int g = p.ppm_type == 2 ? 4 : 3;
-    const int fg = p.ppm_trace_forces == 1 ? g : 1;
-    const int nr = p.domain_hi[lev][dim - 1] + 1;
-    if ((rc = check_mf(c, lev, &a->utilde, dim, g, -1))) return rc;
-    if ((rc = check_mf(c, lev, &a->ufull, dim, 1, -1))) return rc;
-    if ((rc = check_mf(c, lev, &a->vel_force, dim, fg, -1))) return rc;
-    if ((rc = check_mf(c, lev, &a->w0_cart, dim, 0, -1))) return rc;
-    if ((rc = check_mf(c, lev, &a->scal, p.nscal, g, -1))) return rc;
-    if ((rc = check_mf(c, lev, &a->scal_force, p.nscal, fg, -1))) return rc;
-    if ((rc = check_face(c, lev, &a->utrans, 1, 1))) return rc;
-    if ((rc = check_face(c, lev, &a->umac, 1, 1))) return rc;
-    if ((rc = check_face(c, lev, &a->sedge, p.nscal, 0))) return rc;
-    if ((rc = check_face(c, lev, &a->sflux, p.nscal, 0))) return rc;
-    if ((rc = check_mf(c, lev, &a->etarhoflux, 1, 0, dim - 1))) return rc;
-    if (!base_ok(&a->w0, lev, nr + 1) || !base_ok(&a->rho0, lev, nr) ||
-        !base_ok(&a->rho0_edge, lev, nr + 1) || !base_ok(&a->rhoh0, lev, nr) ||
-        !base_ok(&a->rhoh0_edge, lev, nr + 1) || !base_ok(&a->rho0_predicted_edge, lev, nr + 1))
-        return MX_ERR_ARG;
-    if (dim == 2 && p.ppm_trace_forces == 1) return MX_ERR_UNSUPPORTED;
-
-    const mx_bcrec* bs = c->bcs_s.data();
-    const int nbs = (int)c->bcs_s.size();
-    if ((rc = make_utrans(c, lev, &a->utilde, &a->ufull, &a->utrans, &a->w0mac, &a->w0_cart)))
-        return rc;
-    if ((rc = addw0(c, lev, &a->utrans, &a->w0mac, &a->w0, 1.0))) return rc;
-    if ((rc = velpred(c, lev, &a->utilde, &a->ufull, &a->utrans, &a->umac, &a->w0mac,
-                      &a->vel_force, &a->w0_cart)))
-        return rc;
-    if ((rc = addw0(c, lev, &a->umac, &a->w0mac, &a->w0, 1.0))) return rc;
-    if ((rc = make_edge_scal(c, lev, &a->scal, &a->sedge, &a->umac, &a->scal_force, 0, bs, nbs,
-                             p.first_spec, p.first_spec, p.num_spec, 0)))
-        return rc;
-    if ((rc = make_edge_scal(c, lev, &a->scal, &a->sedge, &a->umac, &a->scal_force, 0, bs, nbs,
-                             p.comp_rho, p.comp_rho, 1, 0)))
-        return rc;
-    if ((rc = make_edge_scal(c, lev, &a->scal, &a->sedge, &a->umac, &a->scal_force, 0, bs, nbs,
-                             p.comp_rhoh, p.comp_rhoh, 1, 0)))
-        return rc;
-    if ((rc = make_rhox_flux(c, lev, &a->sflux, &a->etarhoflux, &a->sedge, &a->umac, &a->rho0,
-                             &a->rho0_edge, r0m, &a->rho0, &a->rho0_edge, r0m,
-                             &a->rho0_predicted_edge, &a->w0, p.first_spec, p.num_spec, 1)))
-        return rc;
-    return make_rhoh_flux(c, lev, &a->sflux, &a->sedge, &a->umac, &a->rho0, &a->rho0_edge, r0m,
-                          &a->rho0, &a->rho0_edge, r0m, &a->rhoh0, &a->rhoh0_edge, rh0m,
-                          &a->rhoh0, &a->rhoh0_edge, rh0m, h0m, h0m);
+    return advance_pass_levels(c, lev, 1, a);
 }
 
 }  // extern "C"

@@ -565,7 +565,6 @@ int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mult
     else
         MX_LAUNCH(c, "k_utrans", k_utrans<2>,  gr, dim3(block), dp, kg, A);
 
-    if (c->p.finest_level != 0) return MX_ERR_UNSUPPORTED;
     return fill_face_ghosts(c, lev, utrans);
 }
 

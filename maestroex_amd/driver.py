@@ -155,10 +155,10 @@ class Maestro:
                                                    *[C.byref(x) for x in a]),
                     "MakeEdgeState1d")
 
-    def advance_pass(self, lev, f):
-        """mx_adv_advance_pass with the fields of dict `f` (see workloads.PassFields)."""
+    @staticmethod
+    def pass_args(f, keep):
+        """PassArgs of the fields of dict `f` (descriptors kept alive in `keep`)."""
         a = _abi.PassArgs()
-        keep = []
         for name in ("utilde", "ufull", "vel_force", "w0_cart", "scal", "scal_force", "etarhoflux"):
             d = _mf(f[name])
             keep.append(d)
@@ -169,7 +169,64 @@ class Maestro:
             setattr(a, name, d)
         for name in ("w0", "rho0", "rho0_edge", "rhoh0", "rhoh0_edge", "rho0_predicted_edge"):
             setattr(a, name, _bs(f[name]))
+        return a
+
+    def advance_pass(self, lev, f):
+        """mx_adv_advance_pass with the fields of dict `f` (see Workload.pass_fields)."""
+        keep = []
+        a = self.pass_args(f, keep)
         self._check(self.fns["advance_pass"](self.ctx, lev, C.byref(a)), "advance_pass")
+
+    def _levels_args(self, fields):
+        keep = []
+        arr = (_abi.PassArgs * len(fields))(*[self.pass_args(f, keep) for f in fields])
+        return arr, keep
+
+    def advance_pass_levels(self, fields):
+        """The pass over levels 0..len(fields)-1 (fields[lev] per level)."""
+        arr, keep = self._levels_args(fields)
+        self._check(self.fns["advance_pass_levels"](self.ctx, len(fields), arr),
+                    "advance_pass_levels")
+
+    def pass_graph(self, fields):
+        """Capture the hierarchy pass as a HIP graph; returns a PassGraph."""
+        arr, keep = self._levels_args(fields)
+        g = C.c_void_p()
+        self._check(self.fns["pass_graph_create"](self.ctx, len(fields), arr, C.byref(g)),
+                    "pass_graph_create")
+        return PassGraph(self, g, keep)
+
+    def autotune(self, fields):
+        """Pick the fused edge kernel's tile shape and z-chunk; returns the
+        candidates as a list of (tile, tz, ms) and the chosen index."""
+        arr, keep = self._levels_args(fields)
+        rep = _abi.TuneReport()
+        self._check(self.fns["autotune"](self.ctx, len(fields), arr, C.byref(rep)), "autotune")
+        cands = [(rep.tile[i], rep.tz[i], rep.ms[i]) for i in range(rep.ncand)]
+        return cands, rep.best
+
+
+class PassGraph:
+    """A captured hierarchy pass (mx_adv_pass_graph_*)."""
+
+    def __init__(self, m, handle, keep):
+        self.m = m
+        self.handle = handle
+        self._keep = keep
+
+    def launch(self):
+        self.m._check(self.m.fns["pass_graph_launch"](self.handle), "pass_graph_launch")
+
+    def close(self):
+        if self.handle:
+            self.m.fns["pass_graph_destroy"](self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def bcrec_array(recs):

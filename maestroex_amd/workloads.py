@@ -296,3 +296,248 @@ class Workload:
 
     def valid_cells(self):
         return self.n ** self.dim
+
+
+# ---------------------------------------------------------------------------
+# An uneven AMR hierarchy (every level a list of boxes, fields per box)
+
+def _hash_uniform(seed, lev, comp, i, j, k):
+    """Deterministic uniform [0, 1) per (seed, level, component, cell): a
+    splitmix64 finaliser over the packed integer coordinates."""
+    with np.errstate(over="ignore"):
+        x = (np.uint64(seed) * np.uint64(0x9E3779B97F4A7C15)
+             + np.uint64(lev) * np.uint64(0xBF58476D1CE4E5B9)
+             + np.uint64(comp) * np.uint64(0x94D049BB133111EB))
+        x = (x + (i.astype(np.uint64) << np.uint64(42)) + (j.astype(np.uint64) << np.uint64(21))
+             + k.astype(np.uint64))
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return (x >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+
+
+def _partition(rng, n, lo, hi):
+    """Split [0, n) into consecutive segments of length lo..hi."""
+    segs = []
+    start = 0
+    while start < n:
+        rest = n - start
+        if rest <= hi:
+            if rest >= lo or not segs:
+                segs.append((start, n - 1))
+                break
+            # merge a short remainder into the previous segment
+            a, _ = segs.pop()
+            segs.append((a, n - 1))
+            break
+        L = int(rng.integers(lo, hi + 1))
+        if rest - L < lo:
+            L = rest - lo
+        segs.append((start, start + L - 1))
+        start += L
+    return segs
+
+
+def _random_boxes(rng, n, count, lo, hi, region_frac=1.0, tries=200000):
+    """`count` non-overlapping boxes with random edges lo..hi inside the
+    first region_frac of [0, n)^3 along every axis."""
+    extent = max(hi, int(round(n * region_frac)))
+    occ = []
+    boxes = []
+    t = 0
+    while len(boxes) < count and t < tries:
+        t += 1
+        e = [int(rng.integers(lo, hi + 1)) for _ in range(3)]
+        a = [int(rng.integers(0, extent - e[d] + 1)) for d in range(3)]
+        b = [a[d] + e[d] - 1 for d in range(3)]
+        if any(all(a[d] <= q[1][d] and q[0][d] <= b[d] for d in range(3)) for q in occ):
+            continue
+        occ.append((a, b))
+        boxes.append((tuple(a), tuple(b)))
+    return boxes
+
+
+class Hierarchy:
+    """An uneven AMR hierarchy with the field construction of Workload,
+    evaluated per box (no full-level arrays), refinement ratio 2.
+
+    level 0: n0^3 split into boxes whose edges are lo0..hi0 along each axis;
+    level l > 0: `counts[l-1]` non-overlapping boxes with random edges
+    `edges[l-1]` inside the first `fracs[l-1]` of the level's index space.
+    Periodic in x and y, outflow in z; w0 = 0.01 z on every level; one dt
+    (CFL 0.5 on the finest level) for all levels.  Every level follows the
+    single-level semantics of the pass: ghost cells hold the analytic fields
+    (periodic images / nearest valid cell in z), as FillPatch from a fine
+    enough level would provide.
+    """
+
+    def __init__(self, n0=32, level0_edges=(8, 16), counts=(6, 8), edges=((8, 16), (4, 12)),
+                 fracs=(0.6, 0.4), ppm_type=1, nspec=3, seed=7, nmodes=16, ng=4, backend=None,
+                 verbose=False):
+        from .fabs import Backend
+        self.verbose = verbose
+        self.backend = bk = backend or Backend("numpy")
+        self.dim = 3
+        rng = np.random.default_rng(seed)
+        nlev = 1 + len(counts)
+        self.nlev = nlev
+        self.ns = [n0 * (2 ** l) for l in range(nlev)]
+        segs = [_partition(rng, n0, *level0_edges) for _ in range(3)]
+        b0 = [((x[0], y[0], z[0]), (x[1], y[1], z[1])) for z in segs[2] for y in segs[1]
+              for x in segs[0]]
+        self.boxes = [b0]
+        for l in range(1, nlev):
+            self.boxes.append(_random_boxes(rng, self.ns[l], counts[l - 1], *edges[l - 1],
+                                            region_frac=fracs[l - 1]))
+        geoms = [Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 0],
+                          [_abi.PHYS_INTERIOR, _abi.PHYS_INTERIOR, _abi.PHYS_OUTFLOW],
+                          [_abi.PHYS_INTERIOR, _abi.PHYS_INTERIOR, _abi.PHYS_OUTFLOW])
+                 for n in self.ns]
+
+        # divergence-free Fourier modes, integer wavenumbers in x, y
+        self.modes = []
+        for _ in range(nmodes):
+            k = np.array([rng.integers(-3, 4), rng.integers(-3, 4), rng.uniform(-2.0, 2.0)],
+                         dtype=float)
+            if not np.any(k):
+                k[0] = 1.0
+            cvec = np.cross(k, rng.normal(size=3))
+            self.modes.append((k, cvec, rng.uniform(0, 2 * np.pi)))
+        # scale so that max |u| over the finest level's cell centres is 1
+        # (estimated on a coarse sample of the domain)
+        xs = (np.arange(64) + 0.5) / 64
+        X = np.meshgrid(xs, xs, xs, indexing="ij")
+        self.uscale = 1.0
+        self.uscale = 1.0 / np.max(np.abs(self._velocity(X)))
+        umax = 1.0 + 0.01
+        dxf = 1.0 / self.ns[-1]
+        self.dt = 0.5 * dxf / umax
+        self.rel_eps = 1.0e-8 * umax
+        self.nscal = 4 + nspec
+        self.nspec = nspec
+        self.ng = ng
+        self.seed = seed
+        self.params = make_params(geoms[0], ppm_type=ppm_type, nspec=nspec, dt=self.dt,
+                                  rel_eps=self.rel_eps, extra_levels=geoms[1:])
+        self.levels = [self._level(l) for l in range(nlev)]
+
+    # analytic fields -----------------------------------------------------
+    def _velocity(self, X):
+        u = np.zeros((3,) + np.broadcast(*X).shape)
+        for k, cvec, phi in self.modes:
+            arg = 2 * np.pi * (k[0] * X[0] + k[1] * X[1] + k[2] * X[2]) + phi
+            c = np.cos(arg)
+            for d in range(3):
+                u[d] += cvec[d] * c
+        return u * self.uscale
+
+    def _velocity_cached(self, X, key):
+        # utilde and ufull of one box are evaluated on the same cells
+        if getattr(self, "_vkey", None) != key:
+            self._vkey = key
+            self._vval = self._velocity(X)
+        return self._vval
+
+    def _level(self, l):
+        n = self.ns[l]
+        dx = 1.0 / n
+        bk = self.backend
+        boxes = self.boxes[l]
+        ng = self.ng
+        nscal = self.nscal
+        out = {}
+
+        def cellmf(fn, ncomp, ngrow):
+            mf = MultiFab(boxes, ncomp, ngrow, 3, bk)
+            for b, (lo, hi) in enumerate(boxes):
+                idx = [np.arange(lo[d] - ngrow, hi[d] + ngrow + 1) for d in range(3)]
+                # ghost cells: periodic images in x, y; nearest valid cell in z
+                mi = [np.mod(idx[0], n), np.mod(idx[1], n), np.clip(idx[2], 0, n - 1)]
+                K, J, I = np.meshgrid(mi[2], mi[1], mi[0], indexing="ij")
+                Xc = ((I + 0.5) * dx, (J + 0.5) * dx, (K + 0.5) * dx)
+                self._box_key = (l, b, ngrow)
+                mf.arrays[b] = bk.from_numpy(np.ascontiguousarray(fn(Xc, I, J, K)))
+            return mf
+
+        def rho_and_rhoh(Xc):
+            z = Xc[2]
+            dist2 = sum((Xc[d] - 0.5) ** 2 for d in range(3))
+            rho = np.exp(-z / 0.3) * (1.0 + 0.1 * np.exp(-dist2 / 0.05 ** 2))
+            return rho, rho * (2.0 - z)
+
+        def scal(Xc, I, J, K):
+            rho, rhoh = rho_and_rhoh(Xc)
+            z = Xc[2]
+            rho0 = np.exp(-z / 0.3)
+            s = np.zeros((nscal,) + rho.shape)
+            s[0] = rho - rho0
+            s[1] = rhoh - rho0 * (2.0 - z)
+            xs = np.stack([_hash_uniform(self.seed, l, c, I, J, K) for c in range(self.nspec)])
+            s[4:] = xs / xs.sum(axis=0, keepdims=True)
+            return s
+
+        def w0c(Xc):
+            w = np.zeros((3,) + np.broadcast(*Xc).shape)
+            w[2] = 0.01 * Xc[2]
+            return w
+
+        def vel(Xc, I, J, K):
+            return self._velocity_cached(Xc, self._box_key)
+
+        def ufull(Xc, I, J, K):
+            return self._velocity_cached(Xc, self._box_key) + w0c(Xc)
+
+        def vforce(Xc, I, J, K):
+            arg = 2 * np.pi * (Xc[0] + 2 * Xc[1] + 3 * Xc[2])
+            f = np.stack([0.05 * np.cos(arg + c) for c in range(3)])
+            rho, _ = rho_and_rhoh(Xc)
+            f[2] += -0.1 * (rho - np.exp(-Xc[2] / 0.3)) / rho
+            return f
+
+        def sforce(Xc, I, J, K):
+            arg = 2 * np.pi * (Xc[0] + 2 * Xc[1] + 3 * Xc[2])
+            return np.stack([0.02 * np.sin(arg + 0.5 * c) for c in range(nscal)])
+
+        fng = 1
+        if self.verbose:
+            print(f"[hierarchy] level {l}: {len(boxes)} boxes", flush=True)
+        out["utilde"] = cellmf(vel, 3, ng)
+        out["ufull"] = cellmf(ufull, 3, ng)
+        self._vkey = None
+        out["w0_cart"] = cellmf(lambda Xc, I, J, K: w0c(Xc), 3, min(2, ng))
+        out["vel_force"] = cellmf(vforce, 3, fng)
+        out["scal"] = cellmf(scal, nscal, ng)
+        out["scal_force"] = cellmf(sforce, nscal, fng)
+        zc = (np.arange(n) + 0.5) * dx
+        ze = np.arange(n + 1) * dx
+        nrow = self.ns[-1] + 1  # one row length for every level's base states
+        rows = lambda v: np.pad(v, (0, nrow - len(v)))[None, :]
+        base = np.zeros((self.nlev, nrow))
+
+        def bs(v):
+            a = base.copy()
+            a[l] = rows(v)[0]
+            return BaseState(a, bk)
+
+        rho0 = np.exp(-zc / 0.3)
+        rhoh0 = rho0 * (2.0 - zc)
+        out["w0"] = bs(0.01 * ze)
+        out["rho0"] = bs(rho0)
+        out["rhoh0"] = bs(rhoh0)
+        out["rho0_edge"] = bs(cell_to_edge(rho0))
+        out["rhoh0_edge"] = bs(cell_to_edge(rhoh0))
+        out["rho0_predicted_edge"] = bs(cell_to_edge(rho0))
+        out["w0mac"] = out["r0mac"] = out["rh0mac"] = out["h0mac"] = None
+        out["utrans"] = FaceMultiFab(boxes, 1, 1, 3, bk)
+        out["umac"] = FaceMultiFab(boxes, 1, 1, 3, bk)
+        out["sedge"] = FaceMultiFab(boxes, nscal, 0, 3, bk)
+        out["sflux"] = FaceMultiFab(boxes, nscal, 0, 3, bk)
+        out["etarhoflux"] = MultiFab(boxes, 1, 1, 3, bk, nodal_dir=2)
+        return out
+
+    def pass_fields(self, l):
+        return self.levels[l]
+
+    def valid_cells(self):
+        return sum(int(np.prod([hi[d] - lo[d] + 1 for d in range(3)]))
+                   for bl in self.boxes for (lo, hi) in bl)

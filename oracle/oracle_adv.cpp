@@ -521,7 +521,6 @@ int MakeUtrans(const Ctx& c, int lev, const std::vector<Arr>& utilde,
         }
     }
 
-    if (c.p.finest_level != 0) return MX_ERR_UNSUPPORTED;
     FillFaceGhosts(c, lev, utrans);
     return MX_OK;
 }
@@ -553,7 +552,6 @@ int Addw0(const Ctx& c, int lev, const std::vector<Arr> uedge[3],
             }
         }
     }
-    if (c.p.finest_level != 0) return MX_ERR_UNSUPPORTED;
     FillFaceGhosts(c, lev, uedge);
     return MX_OK;
 }

@@ -1,0 +1,273 @@
+// oracle_step.cpp -- CPU oracle of the differentiable scalar advection step
+// and its forward-mode derivative (test infrastructure only, see
+// oracle_core.H).
+//
+// The step, on one fully periodic box [0, N)^3 with a fixed MAC velocity
+// (face N is face 0), for each scalar component:
+//   sedge = MakeEdgeScal(s, umac, force = 0, advective form, ppm_type 1)
+//           (MaestroMakeEdgeScalars.cpp: PPM, Predictor :662-863, Transverse
+//            :865-1359, Edges :1361-1667; interior formulas only, periodic)
+//   F     = umac * sedge                     (MakeRhoXFlux, predict_rhoX,
+//                                             MaestroMakeFlux.cpp:118-140)
+//   snew  = s + dt * (-div F + 0)            (UpdateScal, MaestroAdvection.cpp:165-183)
+// written once for a number type T: double gives the step itself, Dual (a
+// value and one directional derivative) gives the Jacobian-vector product
+// with the limiter branches taken at the primal values.
+#include <cmath>
+#include <vector>
+
+#include "oracle_core.H"
+
+namespace mxo {
+
+namespace {
+
+struct Dual {
+    double v, d;
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) {
+    return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)};
+}
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+inline Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+inline Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+inline Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+inline Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+
+inline double val(double x) { return x; }
+inline double val(Dual x) { return x.v; }
+inline double mk(double x, double) { return x; }
+template <class T>
+inline T cst(double x);
+template <>
+inline double cst<double>(double x) { return x; }
+template <>
+inline Dual cst<Dual>(double x) { return {x, 0.0}; }
+
+// amrex::min / max (the first argument wins ties), |x|, copysign(1, x)
+template <class T>
+inline T tmin(T a, T b) { return (val(b) < val(a)) ? b : a; }
+template <class T>
+inline T tmax(T a, T b) { return (val(a) < val(b)) ? b : a; }
+inline double tabs(double a) { return std::fabs(a); }
+inline Dual tabs(Dual a) { return std::signbit(a.v) ? Dual{-a.v, -a.d} : a; }
+template <class T>
+inline double sgn1(T a) { return std::copysign(1.0, val(a)); }
+
+template <class T>
+T vl(T a, T b, T c) {  // van Leer slope (MaestroPPM.cpp:46-57)
+    T dsc = 0.5 * (c - a);
+    T dsl = 2.0 * (b - a);
+    T dsr = 2.0 * (c - b);
+    T r = cst<T>(0.0);
+    if (val(dsl * dsr) > 0.0) r = sgn1(dsc) * tmin(tabs(dsc), tmin(tabs(dsl), tabs(dsr)));
+    return r;
+}
+
+// PPM type 1 sm, sp of the cell with stencil v[-2..2] (interior formulas)
+template <class T>
+void ppm1(const T* v, T& sm, T& sp) {
+    T dl = vl(v[-2], v[-1], v[0]);
+    T dc = vl(v[-1], v[0], v[1]);
+    T dr = vl(v[0], v[1], v[2]);
+    sm = 0.5 * (v[0] + v[-1]) - (dc - dl) / 6.0;
+    sm = tmax(sm, tmin(v[0], v[-1]));
+    sm = tmin(sm, tmax(v[0], v[-1]));
+    sp = 0.5 * (v[1] + v[0]) - (dr - dc) / 6.0;
+    sp = tmax(sp, tmin(v[1], v[0]));
+    sp = tmin(sp, tmax(v[1], v[0]));
+    const T s0 = v[0];
+    if (val((sp - s0) * (s0 - sm)) <= 0.0) {
+        sp = s0;
+        sm = s0;
+    } else if (val(tabs(sp - s0)) >= val(2.0 * tabs(sm - s0))) {
+        sp = 3.0 * s0 - 2.0 * sm;
+    } else if (val(tabs(sm - s0)) >= val(2.0 * tabs(sp - s0))) {
+        sm = 3.0 * s0 - 2.0 * sp;
+    }
+}
+
+template <class T>
+void trace(T s0, T sm, T sp, T uhi, T ulo, double dt, double h, double rel_eps, T& Ip, T& Im) {
+    T s6 = 6.0 * s0 - 3.0 * (sm + sp);
+    T sigma = tabs(uhi) * dt / h;
+    if (val(uhi) > rel_eps)
+        Ip = sp - 0.5 * sigma * (sp - sm - (1.0 - 2.0 / 3.0 * sigma) * s6);
+    else
+        Ip = s0;
+    sigma = tabs(ulo) * dt / h;
+    if (val(ulo) < -rel_eps)
+        Im = sm + 0.5 * sigma * (sp - sm + (1.0 - 2.0 / 3.0 * sigma) * s6);
+    else
+        Im = s0;
+}
+
+template <class T>
+T upwind(T l, T r, T u, double thr) {
+    T res = (val(u) > 0.0) ? l : r;
+    res = (val(tabs(u)) > thr) ? res : 0.5 * (l + r);
+    return res;
+}
+
+struct Grid {
+    int n;
+    long long at(int i, int j, int k) const {
+        auto w = [&](int x) { return ((x % n) + n) % n; };
+        return (long long)w(i) + (long long)n * ((long long)w(j) + (long long)n * w(k));
+    }
+};
+
+inline void step_e(int d, int& a, int& b, int& c) {
+    a = d == 0;
+    b = d == 1;
+    c = d == 2;
+}
+
+// one step of one component; s, u[3] and the output are N^3 arrays
+template <class T>
+void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double dt,
+               const double dx[3], double rel_eps) {
+    const int n = g.n;
+    const long long N3 = (long long)n * n * n;
+    std::vector<T> Ip[3], Im[3], simh[3], corner[6], se[3];
+    for (int d = 0; d < 3; ++d) {
+        Ip[d].resize(N3);
+        Im[d].resize(N3);
+        simh[d].resize(N3);
+        se[d].resize(N3);
+    }
+    for (auto& c : corner) c.resize(N3);
+    auto pair_index = [](int d, int t) { return d * 2 + (t > d ? t - 1 : t); };
+
+    // PPM in every direction (faces are indexed by the cell on their high side)
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) {
+                    int a, b, c;
+                    step_e(d, a, b, c);
+                    T v[5];
+                    for (int o = -2; o <= 2; ++o) v[o + 2] = s[g.at(i + o * a, j + o * b, k + o * c)];
+                    T sm, sp;
+                    ppm1(v + 2, sm, sp);
+                    trace(v[2], sm, sp, u[d][g.at(i + a, j + b, k + c)], u[d][g.at(i, j, k)], dt,
+                          dx[d], rel_eps, Ip[d][g.at(i, j, k)], Im[d][g.at(i, j, k)]);
+                }
+    // predictor and upwind states on faces (3-D threshold 0)
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) {
+                    int a, b, c;
+                    step_e(d, a, b, c);
+                    const long long f = g.at(i, j, k);
+                    simh[d][f] = upwind(Ip[d][g.at(i - a, j - b, k - c)], Im[d][f], u[d][f], 0.0);
+                }
+    // corner-coupled states simh_dt on d-faces, corrected by t
+    const double dt6 = dt / 6.0;
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d)
+                    for (int t = 0; t < 3; ++t) {
+                        if (t == d) continue;
+                        int a, b, c, p, q, r;
+                        step_e(d, a, b, c);
+                        step_e(t, p, q, r);
+                        const long long f = g.at(i, j, k);
+                        const long long Lc = g.at(i - a, j - b, k - c);  // cell below the face
+                        const long long L1 = g.at(i - a + p, j - b + q, k - c + r);
+                        const long long R1 = g.at(i + p, j + q, k + r);
+                        const double ht = dx[t];
+                        T l = Ip[d][Lc];
+                        T rr = Im[d][f];
+                        l = l - (dt6 / ht) * (u[t][L1] + u[t][Lc]) * (simh[t][L1] - simh[t][Lc]);
+                        rr = rr - (dt6 / ht) * (u[t][R1] + u[t][f]) * (simh[t][R1] - simh[t][f]);
+                        corner[pair_index(d, t)][f] = upwind(l, rr, u[d][f], rel_eps);
+                    }
+    // edge states: d-face terms in the order of the reference (x: y, z;
+    // y: x, z; z: x, y), force zero
+    const double dt4 = 0.25 * dt;
+    const double dt2 = 0.5 * dt;
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) {
+                    int a, b, c;
+                    step_e(d, a, b, c);
+                    const long long f = g.at(i, j, k);
+                    const long long Lc = g.at(i - a, j - b, k - c);
+                    T l = Ip[d][Lc];
+                    T rr = Im[d][f];
+                    for (int t = 0; t < 3; ++t) {
+                        if (t == d) continue;
+                        const int o = 3 - d - t;
+                        int p, q, r;
+                        step_e(t, p, q, r);
+                        const std::vector<T>& ct = corner[pair_index(t, o)];
+                        const long long L1 = g.at(i - a + p, j - b + q, k - c + r);
+                        const long long R1 = g.at(i + p, j + q, k + r);
+                        const double ht = dx[t];
+                        l = l - (dt4 / ht) * (u[t][L1] + u[t][Lc]) * (ct[L1] - ct[Lc]);
+                        rr = rr - (dt4 / ht) * (u[t][R1] + u[t][f]) * (ct[R1] - ct[f]);
+                    }
+                    T sedgel = l + dt2 * 0.0;
+                    T sedger = rr + dt2 * 0.0;
+                    se[d][f] = upwind(sedgel, sedger, u[d][f], rel_eps);
+                }
+    // fluxes and the update
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const long long cc = g.at(i, j, k);
+                T fx1 = u[0][g.at(i + 1, j, k)] * se[0][g.at(i + 1, j, k)];
+                T fx0 = u[0][cc] * se[0][cc];
+                T fy1 = u[1][g.at(i, j + 1, k)] * se[1][g.at(i, j + 1, k)];
+                T fy0 = u[1][cc] * se[1][cc];
+                T fz1 = u[2][g.at(i, j, k + 1)] * se[2][g.at(i, j, k + 1)];
+                T fz0 = u[2][cc] * se[2][cc];
+                T divterm = (fx1 - fx0) / dx[0];
+                divterm = divterm + (fy1 - fy0) / dx[1];
+                divterm = divterm + (fz1 - fz0) / dx[2];
+                snew[cc] = s[cc] + dt * (-divterm + 0.0);
+            }
+}
+
+}  // namespace
+
+// the step (T = double) or its directional derivative (ds, du != null):
+// s, snew: ncomp x N^3; u[d]: N^3 (face N is face 0)
+void ScalStep(int n, int ncomp, double dt, const double dx[3], double rel_eps, const double* s,
+              const double* const u[3], double* snew, const double* ds,
+              const double* const du[3], double* dsnew) {
+    const Grid g{n};
+    const long long N3 = (long long)n * n * n;
+    if (!ds) {
+        for (int c = 0; c < ncomp; ++c) step_comp<double>(g, s + c * N3, u, snew + c * N3, dt, dx, rel_eps);
+        return;
+    }
+    std::vector<Dual> ud[3];
+    const Dual* up[3];
+    for (int d = 0; d < 3; ++d) {
+        ud[d].resize(N3);
+        for (long long x = 0; x < N3; ++x) ud[d][x] = {u[d][x], du[d] ? du[d][x] : 0.0};
+        up[d] = ud[d].data();
+    }
+    std::vector<Dual> sd(N3), out(N3);
+    for (int c = 0; c < ncomp; ++c) {
+        for (long long x = 0; x < N3; ++x) sd[x] = {s[c * N3 + x], ds[c * N3 + x]};
+        step_comp<Dual>(g, sd.data(), up, out.data(), dt, dx, rel_eps);
+        for (long long x = 0; x < N3; ++x) {
+            snew[c * N3 + x] = out[x].v;
+            dsnew[c * N3 + x] = out[x].d;
+        }
+    }
+}
+
+}  // namespace mxo

@@ -48,3 +48,44 @@ def oracle_maestro(params, levels, fast=False):
 def set_threads(n, fast=True):
     lib, fns = load(fast)
     return fns["omp_threads"](n)
+
+
+def _dptr(a):
+    import numpy as np
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def scal_step(s, u, dt, dx, rel_eps, ds=None, du=None):
+    """The differentiable scalar step on a periodic N^3 box (oracle_step.cpp).
+    s: (ncomp, N, N, N) indexed [c, k, j, i]; u: three (N, N, N) face arrays
+    (face index = the cell above it; face N is face 0).  Returns snew, or
+    (snew, dsnew) when the direction ds (and optionally du) is given."""
+    import numpy as np
+    lib, _ = load(False)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    u = [np.ascontiguousarray(x, dtype=np.float64) for x in u]
+    ncomp, n = s.shape[0], s.shape[1]
+    dxa = (C.c_double * 3)(*dx)
+    snew = np.empty_like(s)
+    if ds is None:
+        f = lib.mxo_scal_step
+        f.restype = C.c_int
+        rc = f(n, ncomp, C.c_double(dt), dxa, C.c_double(rel_eps), _dptr(s), *[_dptr(x) for x in u],
+               _dptr(snew))
+        if rc:
+            raise _abi.MxError(rc, "oracle.scal_step")
+        return snew
+    ds = np.ascontiguousarray(ds, dtype=np.float64)
+    dun = [None if (du is None or du[d] is None) else np.ascontiguousarray(du[d], dtype=np.float64)
+           for d in range(3)]
+    dsnew = np.empty_like(s)
+    f = lib.mxo_scal_step_jvp
+    f.restype = C.c_int
+    null = C.POINTER(C.c_double)()
+    rc = f(n, ncomp, C.c_double(dt), dxa, C.c_double(rel_eps), _dptr(s), *[_dptr(x) for x in u],
+           _dptr(ds), *[(_dptr(x) if x is not None else null) for x in dun], _dptr(snew),
+           _dptr(dsnew))
+    if rc:
+        raise _abi.MxError(rc, "oracle.scal_step_jvp")
+    return snew, dsnew
