@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--max-grid", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape of the fused edge kernel")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -237,6 +239,19 @@ def adjoint_main(args):
     fwd_ms = time_ms(lambda: st.step(s, u), args.steps, args.warmup)
     g = torch.ones_like(s)
     adj_ms = time_ms(lambda: st.vjp(s, u, g), args.steps, args.warmup)
+    # per-kernel times of one forward and one adjoint step (HIP events)
+    fns = st.fns
+    fns["prof_enable"](st.m.ctx, 1)
+    st.step(s, u)
+    st.vjp(s, u, g)
+    nk = fns["prof_read"](st.m.ctx, -1, None, None, None)
+    kern = {}
+    for idx in range(nk):
+        nm, ms, cnt = C.c_char_p(), C.c_double(), C.c_longlong()
+        fns["prof_read"](st.m.ctx, idx, C.byref(nm), C.byref(ms), C.byref(cnt))
+        kern[nm.value.decode()] = round(ms.value, 4)
+    fns["prof_enable"](st.m.ctx, 0)
+    print(f"[bench] step kernels ms: {kern}", file=sys.stderr, flush=True)
 
     def rollout():
         sd = s.clone().requires_grad_(True)
@@ -293,11 +308,19 @@ def main():
     print(f"[bench] workload {args.n}^3 in {len(w.boxes)} boxes built in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     m = mx.gpu_maestro(w.params, [w.boxes])
+    lib, fns = mx.load()
+    fields = w.pass_fields()
+    tuned = None
+    if not args.no_autotune:
+        # the autotuner picks the fused edge kernel's tile and z-chunk on
+        # this workload (untimed, before the warm-up)
+        cands, best = m.autotune([fields])
+        tuned = {"tile": ["32x8", "16x16"][cands[best][0]], "tz": cands[best][1]}
+        print(f"[bench] autotune {[(c[0], c[1], round(c[2], 3)) for c in cands]} -> {tuned}",
+              file=sys.stderr, flush=True)
     for kv in args.tune:
         key, val = kv.split("=")
         m.set_tuning(key, int(val))
-    lib, fns = mx.load()
-    fields = w.pass_fields()
 
     if args.calib_copy_mib:
         n = args.calib_copy_mib * (1 << 20) // 8
@@ -389,7 +412,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"3-D {args.n}^3 FP64 as {len(w.boxes)} boxes of {args.max_grid}^3, "
                                f"4 ghosts, 5 scalars + 3 velocities, ppm_type {args.ppm}",
-                   "global_batch": cells, "parallelism": "single GPU"},
+                   "global_batch": cells, "parallelism": "single GPU",
+                   "edge_kernel_launch": tuned or "default 32x8, tz 32"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -137,6 +137,8 @@ struct SlabDesc {
     int lo[3], hi[3];
     int cand_off, cand_n;
     int ncomp;
+    int vlo[3], vhi[3];  // valid faces of the destination box
+    int plo[3], phi[3];  // faces not outside a non-periodic domain side
 };
 
 struct CandDesc {
@@ -161,6 +163,16 @@ __global__ void k_fill_ghost_faces(const SlabDesc* __restrict__ slabs,
             return;
         }
     }
+    // no box of the level covers it: a coarse-fine face of a finer level takes
+    // the value of the box's nearest valid face; faces outside a non-periodic
+    // domain side are left to FillUmacGhost
+    if (i < t.plo[0] || i > t.phi[0] || j < t.plo[1] || j > t.phi[1] || k < t.plo[2] ||
+        k > t.phi[2])
+        return;
+    const int ri = min(max(i, t.vlo[0]), t.vhi[0]);
+    const int rj = min(max(j, t.vlo[1]), t.vhi[1]);
+    const int rk = min(max(k, t.vlo[2]), t.vhi[2]);
+    for (int n = 0; n < t.ncomp; ++n) t.dst(i, j, k, n) = t.dst(ri, rj, rk, n);
 }
 
 // FillUmacGhost (MaestroFillData.cpp:176-453) for faces normal to D; the
@@ -388,7 +400,6 @@ const mx_adv_ctx::GhostPlan& ghost_plan(mx_adv_ctx* c, int lev, int ndir, const 
             }
         }
         const int n = (int)plan.cands.size() - off;
-        if (n == 0) continue;
         // the ghost shell fb \ valid as up to 2*dim slabs
         for (int d = 0; d < dim; ++d) {
             for (int side = 0; side < 2; ++side) {
@@ -456,6 +467,14 @@ int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge) {
             sd.cand_off = sl.cand_off;
             sd.cand_n = sl.cand_n;
             sd.ncomp = m.fab[sl.box].ncomp;
+            HBox vb = nodal(hbox(grids[sl.box]), d);
+            for (int q = 0; q < 3; ++q) {
+                sd.vlo[q] = vb.lo[q];
+                sd.vhi[q] = vb.hi[q];
+                const bool bounded = q < dim && !c->p.is_periodic[q];
+                sd.plo[q] = bounded ? c->p.domain_lo[lev][q] : -(1 << 30);
+                sd.phi[q] = bounded ? c->p.domain_hi[lev][q] + (q == d ? 1 : 0) : (1 << 30);
+            }
             maxpts = std::max(maxpts, npts(r));
             slabs.push_back(sd);
         }

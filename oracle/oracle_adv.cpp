@@ -114,6 +114,9 @@ void BCSetup(const Ctx& c, std::vector<mx_bcrec>& bcs_s,
 // periodic boundary) takes the value of the first source in the order
 // (source box ascending, shift (sx, sy, sz) ascending from -1 to 1); when the
 // sources agree, as they do for consistent data, the order is immaterial.
+// A ghost face that no box of the level covers (a coarse-fine face of a finer
+// level, which the reference fills by FillPatch from the coarser level) takes
+// the value of the box's nearest valid face.
 void FillBoundaryFace(const Ctx& c, int lev, const std::vector<Arr>& f,
                       int ndir) {
     const int dim = c.dim();
@@ -140,6 +143,12 @@ void FillBoundaryFace(const Ctx& c, int lev, const std::vector<Arr>& f,
             fabbox.lo[d] = dst.b[d];
             fabbox.hi[d] = dst.e[d] - 1;
         }
+        const long long fx = fabbox.hi[0] - fabbox.lo[0] + 1, fy = fabbox.hi[1] - fabbox.lo[1] + 1,
+                        fz = fabbox.hi[2] - fabbox.lo[2] + 1;
+        std::vector<char> covered((size_t)(fx * fy * fz), 0);
+        auto at = [&](int i, int j, int k) {
+            return (size_t)((i - fabbox.lo[0]) + fx * ((j - fabbox.lo[1]) + fy * (k - fabbox.lo[2])));
+        };
         // sources in reverse order, so that the first source is written last
         for (size_t s1 = grids.size(); s1 > 0; --s1) {
             const size_t s = s1 - 1;
@@ -161,9 +170,29 @@ void FillBoundaryFace(const Ctx& c, int lev, const std::vector<Arr>& f,
                             for (int i = in.lo[0]; i <= in.hi[0]; ++i) {
                                 if (contains(valid, i, j, k)) continue;
                                 dst(i, j, k, n) = src(i - sh[0], j - sh[1], k - sh[2], n);
+                                covered[at(i, j, k)] = 1;
                             }
             }
         }
+        // ghost faces no box of the level covers and that are not outside a
+        // non-periodic domain side (coarse-fine faces of a finer level):
+        // first-order extrapolation from the box's own nearest valid face
+        for (int k = fabbox.lo[2]; k <= fabbox.hi[2]; ++k)
+            for (int j = fabbox.lo[1]; j <= fabbox.hi[1]; ++j)
+                for (int i = fabbox.lo[0]; i <= fabbox.hi[0]; ++i) {
+                    if (contains(valid, i, j, k) || covered[at(i, j, k)]) continue;
+                    const int q[3] = {i, j, k};
+                    bool phys = false;
+                    for (int d = 0; d < dim; ++d) {
+                        if (c.p.is_periodic[d]) continue;
+                        const int hi = dom.hi[d] + (d == ndir ? 1 : 0);
+                        if (q[d] < dom.lo[d] || q[d] > hi) phys = true;
+                    }
+                    if (phys) continue;
+                    int r[3];
+                    for (int d = 0; d < 3; ++d) r[d] = std::min(std::max(q[d], valid.lo[d]), valid.hi[d]);
+                    for (int n = 0; n < dst.nc; ++n) dst(i, j, k, n) = dst(r[0], r[1], r[2], n);
+                }
     }
 }
 
