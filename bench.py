@@ -330,24 +330,18 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    for _ in range(args.warmup):
-        m.advance_pass(0, fields)
-    m.sync()
-    torch.cuda.synchronize()
-
-    # timed region: passes bracketed by events on the default stream (the
-    # library's stream); per-kernel events collected by the library
+    # timed regions, each bracketed by events on the default stream (the
+    # library's stream): (1) eager passes; (2) the pass as one HIP graph
+    # launch (the measured value); (3) eager passes with the library's
+    # per-kernel events, for the kernel breakdown and the roofline
+    eager_ms = time_ms(lambda: m.advance_pass(0, fields), args.steps, args.warmup)
+    graph = m.pass_graph([fields])
+    total_ms = time_ms(graph.launch, args.steps, args.warmup) * args.steps
+    graph.close()
     rc = fns["prof_enable"](m.ctx, 1)
     assert rc == 0, rc
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
     for _ in range(args.steps):
         m.advance_pass(0, fields)
-    stop.record()
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(stop)
     nk = fns["prof_read"](m.ctx, -1, None, None, None)
     kernels = {}
     for idx in range(nk):
@@ -413,6 +407,8 @@ def main():
         "config": {"workload": f"3-D {args.n}^3 FP64 as {len(w.boxes)} boxes of {args.max_grid}^3, "
                                f"4 ghosts, 5 scalars + 3 velocities, ppm_type {args.ppm}",
                    "global_batch": cells, "parallelism": "single GPU",
+                   "pass_launch": "one HIP graph launch per pass",
+                   "eager_ms_per_step": round(eager_ms, 4),
                    "edge_kernel_launch": tuned or "default 32x8, tz 32"},
         "roofline": roofline,
         "cpu_baseline": cpu,

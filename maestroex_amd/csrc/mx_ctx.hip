@@ -74,24 +74,64 @@ int Table::upload() {
     }
     if (c->tab_mode == 1) c->rec_tabs.push_back(c->htab);
     if (bytes == 0) {
-        base = c->tab;
+        base = nullptr;
         return MX_OK;
     }
-    if (bytes > c->tab_cap) {
+    // FNV-1a over the bytes, then a full comparison
+    unsigned long long h = 1469598103934665603ULL;
+    for (size_t i = 0; i < bytes; ++i) h = (h ^ (unsigned char)c->htab[i]) * 1099511628211ULL;
+    for (const auto& e : c->tab_cache) {
+        if (e.hash == h && e.bytes.size() == bytes &&
+            std::memcmp(e.bytes.data(), c->htab.data(), bytes) == 0) {
+            base = e.dev;
+            return MX_OK;
+        }
+    }
+    constexpr size_t kCacheEntries = 4096, kCacheBytes = (size_t)256 << 20;
+    if (c->tab_cache.size() >= kCacheEntries || c->tab_cache_bytes + bytes > kCacheBytes) {
+        // full: queued kernels may still read the cached copies
         MX_HIP_CHECK(hipStreamSynchronize(c->stream));
-        if (c->tab) MX_HIP_CHECK(hipFree(c->tab));
-        c->tab = nullptr;
+        free_tab_cache(c);
+    }
+    char* dev = nullptr;
+    if (hipMalloc((void**)&dev, bytes) != hipSuccess) return MX_ERR_NOMEM;
+    c->tab_cache.push_back({h, c->htab, dev});
+    c->tab_cache_bytes += bytes;
+    if (c->tab_slots.empty()) c->tab_slots.resize(mx_adv_ctx::kTabSlots);
+    mx_adv_ctx::TabSlot& t = c->tab_slots[c->tab_next];
+    c->tab_next = (c->tab_next + 1) % mx_adv_ctx::kTabSlots;
+    if (t.pending) {
+        MX_HIP_CHECK(hipEventSynchronize(t.done));
+        t.pending = false;
+    }
+    if (!t.done) MX_HIP_CHECK(hipEventCreateWithFlags(&t.done, hipEventDisableTiming));
+    if (bytes > t.cap) {
+        free_tab_slot(t);
         size_t cap = std::max(bytes * 2, (size_t)1 << 16);
-        if (hipMalloc((void**)&c->tab, cap) != hipSuccess) {
-            c->tab = nullptr;
-            c->tab_cap = 0;
+        if (hipHostMalloc((void**)&t.host, cap, hipHostMallocDefault) != hipSuccess) {
+            t.host = nullptr;
             return MX_ERR_NOMEM;
         }
-        c->tab_cap = cap;
+        t.cap = cap;
     }
-    MX_HIP_CHECK(hipMemcpyAsync(c->tab, c->htab.data(), bytes, hipMemcpyHostToDevice, c->stream));
-    base = c->tab;
+    std::memcpy(t.host, c->htab.data(), bytes);
+    MX_HIP_CHECK(hipMemcpyAsync(dev, t.host, bytes, hipMemcpyHostToDevice, c->stream));
+    MX_HIP_CHECK(hipEventRecord(t.done, c->stream));
+    t.pending = true;
+    base = dev;
     return MX_OK;
+}
+
+void free_tab_cache(mx_adv_ctx* c) {
+    for (auto& e : c->tab_cache) (void)hipFree(e.dev);
+    c->tab_cache.clear();
+    c->tab_cache_bytes = 0;
+}
+
+void free_tab_slot(mx_adv_ctx::TabSlot& t) {
+    if (t.host) (void)hipHostFree(t.host);
+    t.host = nullptr;
+    t.cap = 0;
 }
 
 int check_level(const mx_adv_ctx* c, int lev) {
@@ -139,12 +179,14 @@ struct SlabDesc {
     int ncomp;
     int vlo[3], vhi[3];  // valid faces of the destination box
     int plo[3], phi[3];  // faces not outside a non-periodic domain side
+    int pad_ = 0;
 };
 
 struct CandDesc {
     DA src;
     int vlo[3], vhi[3];  // valid faces of the source box
     int shift[3];        // source index = destination index - shift
+    int pad_ = 0;
 };
 
 // one ghost face: copy from the first candidate source that holds it
@@ -542,6 +584,7 @@ struct AddTask {
     DA wm;  // spherical: w0mac of this direction
     int lo[3], hi[3];
     int r_dir;  // planar: radial index direction
+    int pad_ = 0;
 };
 
 __global__ void k_addw0(const AddTask* __restrict__ tasks, const double* __restrict__ w0,
@@ -807,9 +850,13 @@ int mx_adv_destroy(mx_adv_ctx* c) {
     if (!c) return MX_OK;
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     c->ev_pool.clear();
-    if (c->ws || c->tab) (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     if (c->ws) (void)hipFree(c->ws);
-    if (c->tab) (void)hipFree(c->tab);
+    for (auto& t : c->tab_slots) {
+        free_tab_slot(t);
+        if (t.done) (void)hipEventDestroy(t.done);
+    }
+    free_tab_cache(c);
     delete c;
     return MX_OK;
 }

@@ -23,6 +23,7 @@ struct DA {
     double* p;
     long long js, ks, ns;
     int b0, b1, b2;
+    int pad_ = 0;  // explicit, so that copies of a view compare equal byte for byte
     __device__ __forceinline__ double& operator()(int i, int j, int k, int n = 0) const {
         return p[(long long)(i - b0) + (long long)(j - b1) * js + (long long)(k - b2) * ks +
                  (long long)n * ns];

@@ -185,7 +185,11 @@ __device__ __forceinline__ double apply3(double x, double a, double b, double c)
     return x - a;
 }
 
-template <int PPM, bool CONS, int TX, int TY>
+// XYF: no x- or y-boundary condition acts on the tile (the host puts a tile
+// in this class when its stencils keep clear of every physical x/y domain
+// side, or when those sides carry no boundary override), so the per-lane
+// boundary branches of those directions are left out.
+template <int PPM, bool CONS, int TX, int TY, bool XYF>
 __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
                                                  const FTile* __restrict__ tiles,
                                                  const FComp* __restrict__ comps, KGeom g,
@@ -463,7 +467,7 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
             } else {
                 cs_cell(w9, FX[fx - 1], FX[fx], FX[fx + 1], FX[fx + 2], 1.e-10, sm, sp);
             }
-            if (ppm_special<PPM>(I, dlx, dhx, bcx0, bcx1))
+            if (!XYF && ppm_special<PPM>(I, dlx, dhx, bcx0, bcx1))
                 ppm_edges<PPM, 0>(w9, I, dlx, dhx, bcx0, bcx1, sm, sp);
             trace_sig(w9[4], sm, sp, ULq[ui + 1], ULq[ui], SGX[ui + 1], SGX[ui], rel_eps, Ip,
                       Im);
@@ -479,7 +483,7 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
             } else {
                 cs_cell(w9, FY[fy - W], FY[fy], FY[fy + W], FY[fy + 2 * W], 1.e-10, sm, sp);
             }
-            if (ppm_special<PPM>(J, dly, dhy, bcy0, bcy1))
+            if (!XYF && ppm_special<PPM>(J, dly, dhy, bcy0, bcy1))
                 ppm_edges<PPM, 1>(w9, J, dly, dhy, bcy0, bcy1, sm, sp);
             trace_sig(w9[4], sm, sp, VLq[vi + W], VLq[vi], SGY[vi + W], SGY[vi], rel_eps, Ip,
                       Im);
@@ -492,13 +496,13 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
         if (col && a >= 1) {
             slx = P2[tid - 1];
             srx = P2[NE + tid];
-            bc_pair(I, dlx, dhx, bcx0, bcx1, vn0, S[si - 1], S[si], slx, srx);
+            if (!XYF) bc_pair(I, dlx, dhx, bcx0, bcx1, vn0, S[si - 1], S[si], slx, srx);
             P1[tid] = upwind(slx, srx, ULq[ui], 0.0);
         }
         if (col && b >= 1) {
             sly = P2[2 * NE + tid - W];
             sry = P2[3 * NE + tid];
-            bc_pair(J, dly, dhy, bcy0, bcy1, vn1, S[si - SW], S[si], sly, sry);
+            if (!XYF) bc_pair(J, dly, dhy, bcy0, bcy1, vn1, S[si - SW], S[si], sly, sry);
             P1[NE + tid] = upwind(sly, sry, VLq[vi], 0.0);
         }
         __syncthreads();
@@ -553,7 +557,7 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
                                     CONS ? P2[4 * NE + tid - 1] : 0.0);
             double r = apply3<CONS>(srx, P2[NE + tid], CONS ? P2[6 * NE + tid] : 0.0,
                                     CONS ? P2[4 * NE + tid] : 0.0);
-            bc_pair(I, dlx, dhx, bcx0, bcx1, vn0, S[si - 1], S[si], l, r);
+            if (!XYF) bc_pair(I, dlx, dhx, bcx0, bcx1, vn0, S[si - 1], S[si], l, r);
             P1[tid] = upwind(l, r, ULq[ui], rel_eps);
         }
         if (col && b >= 1 && a >= 1 && a <= nx) {  // simhyx, y-faces of plane q
@@ -561,7 +565,7 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
                                     CONS ? P2[3 * NE + tid - W] : 0.0);
             double r = apply3<CONS>(sry, P2[tid], CONS ? P2[6 * NE + tid] : 0.0,
                                     CONS ? P2[3 * NE + tid] : 0.0);
-            bc_pair(J, dly, dhy, bcy0, bcy1, vn1, S[si - SW], S[si], l, r);
+            if (!XYF) bc_pair(J, dly, dhy, bcy0, bcy1, vn1, S[si - SW], S[si], l, r);
             P1[NE + tid] = upwind(l, r, VLq[vi], rel_eps);
         }
         if (pplane && col && a >= 1) {  // simhxz, x-faces of plane p
@@ -569,7 +573,7 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
                                     CONS ? P2[5 * NE + tid - 1] : 0.0);
             double r = apply3<CONS>(srx_p, P2[2 * NE + tid], CONS ? P2[7 * NE + tid] : 0.0,
                                     CONS ? P2[5 * NE + tid] : 0.0);
-            bc_pair(I, dlx, dhx, bcx0, bcx1, vn0, SC[tid - 1], SC[tid], l, r);
+            if (!XYF) bc_pair(I, dlx, dhx, bcx0, bcx1, vn0, SC[tid - 1], SC[tid], l, r);
             P1[2 * NE + tid] = upwind(l, r, ULp[ui], rel_eps);
         }
         if (pplane && col && b >= 1) {  // simhyz, y-faces of plane p
@@ -577,7 +581,7 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
                                     CONS ? P2[5 * NE + tid - W] : 0.0);
             double r = apply3<CONS>(sry_p, P2[2 * NE + tid], CONS ? P2[7 * NE + tid] : 0.0,
                                     CONS ? P2[5 * NE + tid] : 0.0);
-            bc_pair(J, dly, dhy, bcy0, bcy1, vn1, SC[tid - W], SC[tid], l, r);
+            if (!XYF) bc_pair(J, dly, dhy, bcy0, bcy1, vn1, SC[tid - W], SC[tid], l, r);
             P1[3 * NE + tid] = upwind(l, r, VLp[vi], rel_eps);
         }
         if (zface && col && a >= 1 && a <= nx) {  // simhzx, z-face q
@@ -632,16 +636,17 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
             l = l - P2[NE + tid - 1];
             double r = srx_p - P2[tid];
             r = r - P2[NE + tid];
-            const double sL = SC[tid - 1], sR = SC[tid];
             const double u0 = ULp[ui];
             if (CONS) {
-                l = l - dt2hx * sL * (u0 - ULp[ui - 1]);
-                r = r - dt2hx * sR * (ULp[ui + 1] - u0);
+                l = l - dt2hx * SC[tid - 1] * (u0 - ULp[ui - 1]);
+                r = r - dt2hx * SC[tid] * (ULp[ui + 1] - u0);
             }
             const double sedgel = l + dt2 * FL[tid - 1];
             const double sedger = r + dt2 * FL[tid];
             double res = upwind(sedgel, sedger, u0, rel_eps);
-            res = bc_edge(res, I, dlx, dhx, bcx0, bcx1, vn0, sL, sR, sedgel, sedger);
+            if (!XYF)
+                res = bc_edge(res, I, dlx, dhx, bcx0, bcx1, vn0, SC[tid - 1], SC[tid], sedgel,
+                              sedger);
             e0Col[-e0ks] = res;  // plane p
         }
         if (pplane && col && b >= 1 && a >= 1 && a <= nx && (b <= ny || last_y)) {  // y-faces, p
@@ -649,16 +654,17 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
             l = l - P2[3 * NE + tid - W];
             double r = sry_p - P2[2 * NE + tid];
             r = r - P2[3 * NE + tid];
-            const double sL = SC[tid - W], sR = SC[tid];
             const double u0 = VLp[vi];
             if (CONS) {
-                l = l - dt2hy * sL * (u0 - VLp[vi - W]);
-                r = r - dt2hy * sR * (VLp[vi + W] - u0);
+                l = l - dt2hy * SC[tid - W] * (u0 - VLp[vi - W]);
+                r = r - dt2hy * SC[tid] * (VLp[vi + W] - u0);
             }
             const double sedgel = l + dt2 * FL[tid - W];
             const double sedger = r + dt2 * FL[tid];
             double res = upwind(sedgel, sedger, u0, rel_eps);
-            res = bc_edge(res, J, dly, dhy, bcy0, bcy1, vn1, sL, sR, sedgel, sedger);
+            if (!XYF)
+                res = bc_edge(res, J, dly, dhy, bcy0, bcy1, vn1, SC[tid - W], SC[tid], sedgel,
+                              sedger);
             e1Col[-e1ks] = res;  // plane p
         }
         if (((zface && q <= z1) || (q == z1 + 1 && last_z)) && col && a >= 1 && a <= nx &&
@@ -717,13 +723,56 @@ bool fused_edge_scal_ok(const mx_adv_ctx* c, int is_conservative) {
 
 namespace {
 
+// does boundary code `bc` override anything (everything but INT_DIR)?
+bool bc_acts(int bc) {
+    return bc == MX_BC_EXT_DIR || bc == MX_BC_HOEXTRAP || bc == MX_BC_FOEXTRAP ||
+           bc == MX_BC_REFLECT_EVEN || bc == MX_BC_REFLECT_ODD;
+}
+
+// A tile is x/y-free when, in x and in y, each domain side either carries no
+// override for any component or lies beyond the tile's reach: the kernel
+// evaluates cells [X0 - 1, X0 + nx] (PPM boundary stencils act within three
+// cells of a side) and faces [X0, X0 + nx] (face overrides act on the side's
+// face).
+bool tile_xy_free(const int lo0, int n, int dlo, int dhi, bool act_lo, bool act_hi) {
+    const bool free_lo = !act_lo || lo0 - 1 > dlo + 2;
+    const bool free_hi = !act_hi || lo0 + n < dhi - 2;
+    return free_lo && free_hi;
+}
+
+template <int TX, int TY, bool XYF>
+int launch_one(mx_adv_ctx* c, const FBox* db, const FTile* dtl, size_t ntile, const FComp* dc,
+               const KGeom& kg, const FArgs& A, int num_comp, bool cons) {
+    if (ntile == 0) return MX_OK;
+    dim3 grid((unsigned)ntile, 1, (unsigned)num_comp);
+    dim3 block(NT);
+    if (c->p.ppm_type == 1) {
+        if (cons)
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true, TX, TY, XYF>), grid, block, db, dtl, dc, kg, A);
+        else
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false, TX, TY, XYF>), grid, block, db, dtl, dc, kg, A);
+    } else {
+        if (cons)
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true, TX, TY, XYF>), grid, block, db, dtl, dc, kg, A);
+        else
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false, TX, TY, XYF>), grid, block, db, dtl, dc, kg, A);
+    }
+    return MX_OK;
+}
+
 template <int TX, int TY>
 int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int is_vel,
                  const mx_bcrec* bcs, int start_scomp, int start_bccomp, int num_comp,
                  int is_conservative) {
     const long long lim = 0x7fffffffLL;
     std::vector<FBox> boxes;
-    std::vector<FTile> tiles;
+    std::vector<FTile> tiles, btiles;  // x/y-free tiles, then the others
+    bool act[2][2] = {{false, false}, {false, false}};
+    for (int vc = 0; vc < num_comp; ++vc)
+        for (int d = 0; d < 2; ++d) {
+            act[d][0] = act[d][0] || bc_acts(bcs[start_bccomp + vc].lo[d]);
+            act[d][1] = act[d][1] || bc_acts(bcs[start_bccomp + vc].hi[d]);
+        }
     for (const EdgeLevel& L : levs) {
         const auto& grids = c->grids[L.lev];
         for (size_t b = 0; b < grids.size(); ++b) {
@@ -753,10 +802,20 @@ int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int 
             const int id = (int)boxes.size();
             for (int z = 0; z < nzc; ++z)
                 for (int y = 0; y < nty; ++y)
-                    for (int x = 0; x < ntx; ++x) tiles.push_back({id, x, y, z});
+                    for (int x = 0; x < ntx; ++x) {
+                        const int x0 = g.lo[0] + x * TX, y0 = g.lo[1] + y * TY;
+                        const int nx = std::min(TX, g.hi[0] - x0 + 1);
+                        const int ny = std::min(TY, g.hi[1] - y0 + 1);
+                        const bool f =
+                            tile_xy_free(x0, nx, B.domlo[0], B.domhi[0], act[0][0], act[0][1]) &&
+                            tile_xy_free(y0, ny, B.domlo[1], B.domhi[1], act[1][0], act[1][1]);
+                        (f ? tiles : btiles).push_back({id, x, y, z});
+                    }
             boxes.push_back(B);
         }
     }
+    const size_t nfree = tiles.size();
+    tiles.insert(tiles.end(), btiles.begin(), btiles.end());
     if (tiles.empty()) return MX_OK;
     if (tiles.size() > 0x7fffffffULL) return MX_ERR_UNSUPPORTED;
     std::vector<FComp> comps(num_comp);
@@ -780,21 +839,11 @@ int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int 
     const FTile* dtl = tab.dev<FTile>(ot);
     const FComp* dc = tab.dev<FComp>(oc);
     KGeom kg = kgeom(c, levs[0].lev);
-    dim3 grid((unsigned)tiles.size(), 1, (unsigned)num_comp);
-    dim3 block(NT);
     const bool cons = is_conservative != 0;
-    if (c->p.ppm_type == 1) {
-        if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true, TX, TY>), grid, block, db, dtl, dc, kg, A);
-        else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false, TX, TY>), grid, block, db, dtl, dc, kg, A);
-    } else {
-        if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true, TX, TY>), grid, block, db, dtl, dc, kg, A);
-        else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false, TX, TY>), grid, block, db, dtl, dc, kg, A);
-    }
-    return MX_OK;
+    rc = launch_one<TX, TY, true>(c, db, dtl, nfree, dc, kg, A, num_comp, cons);
+    if (rc) return rc;
+    return launch_one<TX, TY, false>(c, db, dtl + nfree, tiles.size() - nfree, dc, kg, A,
+                                     num_comp, cons);
 }
 
 }  // namespace

@@ -28,10 +28,30 @@ struct mx_adv_ctx {
     size_t ws_cap = 0;
     size_t ws_off = 0;
 
-    // device copy of the per-call box tables
-    char* tab = nullptr;
-    size_t tab_cap = 0;
+    // per-call box tables: built in htab; a new table is copied through a
+    // ring of pinned staging buffers, so that an upload never waits for the
+    // stream to drain (a pageable copy would).  A slot is reused after its
+    // previous copy has completed (its event).
+    struct TabSlot {
+        char* host = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    static constexpr int kTabSlots = 32;
+    std::vector<TabSlot> tab_slots;
+    int tab_next = 0;
     std::vector<char> htab;
+    // device copies of the tables already uploaded, found by content: a pass
+    // over unchanged grids and arrays builds the same tables every call and
+    // then copies nothing.  Entries are never written after their upload.
+    struct TabEntry {
+        unsigned long long hash;
+        std::vector<char> bytes;
+        char* dev;
+    };
+    std::vector<TabEntry> tab_cache;
+    size_t tab_cache_bytes = 0;
 
     // FillBoundary plans, keyed by (lev, nodal dir, ghost width): the ghost
     // shell of every box as slabs, each with the box's candidate sources
@@ -183,6 +203,8 @@ mxd::KGeom kgeom(const mx_adv_ctx* c, int lev);
 
 // workspace
 int ws_reserve(mx_adv_ctx* c, size_t bytes);
+void free_tab_slot(mx_adv_ctx::TabSlot& t);
+void free_tab_cache(mx_adv_ctx* c);
 double* ws_take(mx_adv_ctx* c, size_t ndouble);
 
 // a temporary FAB per box over `region(box)` with ncomp components
