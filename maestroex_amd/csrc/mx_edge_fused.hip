@@ -190,7 +190,7 @@ __device__ __forceinline__ double apply3(double x, double a, double b, double c)
 // side, or when those sides carry no boundary override), so the per-lane
 // boundary branches of those directions are left out.
 template <int PPM, bool CONS, int TX, int TY, bool XYF>
-__global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) k_es_fused(const FBox* __restrict__ boxes,
                                                  const FTile* __restrict__ tiles,
                                                  const FComp* __restrict__ comps, KGeom g,
                                                  FArgs A) {
@@ -329,50 +329,105 @@ __global__ void __launch_bounds__(NT) k_es_fused(const FBox* __restrict__ boxes,
     }
     int slot_q = 0, slot_p = 2;
 
+    // Plane q + 1's loads are issued during step q and land in registers
+    // (the barriers wait for LDS traffic only), so the global-memory latency
+    // overlaps the step's work.  Each thread owns fixed slots of the staged
+    // planes: s (NS per thread), the x- and y-velocities (NU, NV).
+    constexpr int NS = (SW * SH + NT - 1) / NT;
+    constexpr int NU = (UW * H + NT - 1) / NT;
+    constexpr int NV = (W * VH + NT - 1) / NT;
+    bool okS[NS], okU[NU], okV[NV];
+    int offS[NS], offU[NU], offV[NV];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int idx = tid + k * NT;
+        const int sx = idx % SW, sy = idx / SW;
+        okS[k] = idx < SW * SH && sx <= nx + 1 + 2 * G && sy <= ny + 1 + 2 * G;
+        offS[k] = sx + sy * sjs;
+    }
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+        const int idx = tid + k * NT;
+        const int f = idx % UW, r = idx / UW;
+        okU[k] = idx < UW * H && f <= nx + 2 && r <= ny + 1;
+        offU[k] = f + r * ujs;
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int idx = tid + k * NT;
+        const int cc = idx % W, r = idx / W;
+        okV[k] = idx < W * VH && cc <= nx + 1 && r <= ny + 2;
+        offV[k] = cc + r * vjs;
+    }
+    double pS[NS], pU[NU], pV[NV];
+    double pcs = 0.0, pw = 0.0, pf = 0.0;
+    // plane z0 - 1
+#pragma unroll
+    for (int k = 0; k < NS; ++k) pS[k] = okS[k] ? sPl[offS[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) pU[k] = okU[k] ? uPl[offU[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) pV[k] = okV[k] ? vPl[offV[k]] : 0.0;
+    if (col) {
+        pcs = *sCol;
+        pw = wCol[wks];
+        pf = *fCol;
+    }
+
     for (int q = z0 - 1; q <= z1 + 1; ++q) {
         const bool zface = q >= z0;        // z-face q is in the chunk's range
         const bool pplane = q >= z0 + 1;   // plane p = q - 1 is a chunk plane
         const bool first = q == z0 - 1;
-        gcd* uq = uPl;
-        gcd* vq = vPl;
         double* ULq = UL[slot_q];
         double* VLq = VL[slot_q];
         const double* ULp = UL[slot_p];
         const double* VLp = VL[slot_p];
 
-        // ---- A: stage plane q; shift the column window
-        for (int idx = tid; idx < SW * SH; idx += NT) {
-            const int sx = idx % SW, sy = idx / SW;
-            if (sx <= nx + 1 + 2 * G && sy <= ny + 1 + 2 * G) S[idx] = sPl[sx + sy * sjs];
-        }
-        for (int idx = tid; idx < UW * H; idx += NT) {
-            const int f = idx % UW, r = idx / UW;
-            if (f <= nx + 2 && r <= ny + 1) {
-                const double u = uq[f + r * ujs];
-                ULq[idx] = u;
-                SGX[idx] = dabs(u) * dt / hx;
+        // ---- A: stage plane q (loaded during the previous step); shift the
+        //      column window; issue the loads of plane q + 1
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (okS[k]) S[tid + k * NT] = pS[k];
+#pragma unroll
+        for (int k = 0; k < NU; ++k)
+            if (okU[k]) {
+                ULq[tid + k * NT] = pU[k];
+                SGX[tid + k * NT] = dabs(pU[k]) * dt / hx;
             }
-        }
-        for (int idx = tid; idx < W * VH; idx += NT) {
-            const int c = idx % W, r = idx / W;
-            if (c <= nx + 1 && r <= ny + 2) {
-                const double u = vq[c + r * vjs];
-                VLq[idx] = u;
-                SGY[idx] = dabs(u) * dt / hy;
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            if (okV[k]) {
+                VLq[tid + k * NT] = pV[k];
+                SGY[tid + k * NT] = dabs(pV[k]) * dt / hy;
             }
-        }
         if (col) {
 #pragma unroll
             for (int jj = 0; jj < 8; ++jj) v[jj] = v[jj + 1];
-            v[4 + G] = *sCol;
-            sCol += sks;
-            w_hi = wCol[wks];
-            fo_q = *fCol;
-            fCol += fks;
+            v[4 + G] = pcs;
+            w_hi = pw;
+            fo_q = pf;
         }
-        sPl += sks;
-        uPl += uks;
-        vPl += vks;
+        if (q <= z1) {
+            sPl += sks;
+            uPl += uks;
+            vPl += vks;
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                if (okS[k]) pS[k] = sPl[offS[k]];
+#pragma unroll
+            for (int k = 0; k < NU; ++k)
+                if (okU[k]) pU[k] = uPl[offU[k]];
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                if (okV[k]) pV[k] = vPl[offV[k]];
+            if (col) {
+                sCol += sks;
+                fCol += fks;
+                pcs = *sCol;
+                pw = wCol[2 * wks];
+                pf = *fCol;
+            }
+        }
         __syncthreads();
 
         // ---- B1: PPM face values of plane q; PPM of cell q in z
