@@ -164,6 +164,10 @@ def amr_main(args):
     for kv in args.tune:
         key, val = kv.split("=")
         m.set_tuning(key, int(val))
+        if key == "edge_scal_tile":
+            shape["tile"] = ["32x8", "16x16"][int(val)]
+        elif key == "edge_scal_tz":
+            shape["tz"] = int(val)
     cands, best = m.autotune(fields)
     tuned = {"tile": ["32x8", "16x16"][cands[best][0]], "tz": cands[best][1]}
     print(f"[bench] autotune {cands} -> {tuned}", file=sys.stderr, flush=True)
@@ -311,16 +315,22 @@ def main():
     lib, fns = mx.load()
     fields = w.pass_fields()
     tuned = None
+    shape = {"tile": "32x8", "tz": 32}  # the library's defaults
     if not args.no_autotune:
         # the autotuner picks the fused edge kernel's tile and z-chunk on
         # this workload (untimed, before the warm-up)
         cands, best = m.autotune([fields])
         tuned = {"tile": ["32x8", "16x16"][cands[best][0]], "tz": cands[best][1]}
+        shape = dict(tuned)
         print(f"[bench] autotune {[(c[0], c[1], round(c[2], 3)) for c in cands]} -> {tuned}",
               file=sys.stderr, flush=True)
     for kv in args.tune:
         key, val = kv.split("=")
         m.set_tuning(key, int(val))
+        if key == "edge_scal_tile":
+            shape["tile"] = ["32x8", "16x16"][int(val)]
+        elif key == "edge_scal_tz":
+            shape["tz"] = int(val)
 
     if args.calib_copy_mib:
         n = args.calib_copy_mib * (1 << 20) // 8
@@ -377,7 +387,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-        "traffic": committed_traffic(dname, f"{args.n}c_ppm{args.ppm}"),
+        "traffic": committed_traffic(
+            dname, f"{args.n}c_ppm{args.ppm}_{shape['tile']}_tz{shape['tz']}"),
         "avg_launch_ms": round(avg_ms, 4),
         "bytes_per_cell": bpc,
         "pass_achieved": round(pass_gbs, 2),
@@ -409,7 +420,7 @@ def main():
                    "global_batch": cells, "parallelism": "single GPU",
                    "pass_launch": "one HIP graph launch per pass",
                    "eager_ms_per_step": round(eager_ms, 4),
-                   "edge_kernel_launch": tuned or "default 32x8, tz 32"},
+                   "edge_kernel_launch": shape, "autotuned": tuned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
