@@ -17,12 +17,13 @@
 // states, corner-coupled states, edge states, on dense periodic N^3 arrays)
 // and reverse sweeps that mirror the forward stages.  Every reverse stage
 // gathers the adjoints of its inputs from the adjoints of the outputs that
-// use them, so no atomics are used and the result is deterministic:
-//   B1 flux and update          -> edge-state and velocity adjoints
-//   B2 edge-state upwinding     -> per-cell edge-term adjoints
-//   B3 edge terms, corner upwinding -> corner, predictor, velocity adjoints
-//   B4 corner terms             -> per-cell transverse-term adjoints
-//   B5 transverse terms, predictor upwinding -> predictor, velocity adjoints
+// use them, so no atomics are used and the result is deterministic.
+// Adjoints that are cheap to recompute from their neighbours' inputs are
+// recomputed where they are gathered instead of stored:
+//   B1 flux, update and edge-state upwinding -> per-cell edge-term adjoints
+//   B4 corner terms (recomputed)    -> per-cell transverse-term adjoints
+//   B3 + B5 the face adjoints of the predictor states and velocity: flux,
+//      corner and transverse terms, summed where B6 and B10 read them
 //   B6 characteristic trace and CW limiter (per cell and direction)
 //   B7 PPM face values          B8 van Leer slopes   B9 state adjoint
 //   B10 velocity adjoint summed over components.
@@ -41,14 +42,16 @@ constexpr double C23 = 2.0 / 3.0;
 
 __host__ __device__ constexpr int pidx(int d, int t) { return d * 2 + (t > d ? t - 1 : t); }
 
+// periodic indexing of a dense N^3 array (N^3 fits 31 bits; checked on the
+// host, so all index arithmetic is 32-bit)
 struct Per {
     int n;
     __device__ __forceinline__ int w(int x) const { return x < 0 ? x + n : (x >= n ? x - n : x); }
-    __device__ __forceinline__ long long at(int i, int j, int k) const {
-        return (long long)w(i) + (long long)n * ((long long)w(j) + (long long)n * w(k));
+    __device__ __forceinline__ int at(int i, int j, int k) const {
+        return w(i) + n * (w(j) + n * w(k));
     }
     // neighbour of (i,j,k) at offset o along direction d
-    __device__ __forceinline__ long long nb(int i, int j, int k, int d, int o) const {
+    __device__ __forceinline__ int nb(int i, int j, int k, int d, int o) const {
         return at(i + (d == 0 ? o : 0), j + (d == 1 ? o : 0), k + (d == 2 ? o : 0));
     }
 };
@@ -58,10 +61,9 @@ struct AB {
     const double* U[3];
     const double* G;   // adjoint of snew
     double *IP[3], *IM[3], *SIMH[3], *COR[6], *SE[3];
-    double *LB[3], *RB[3], *EB[6], *CLB[6], *CRB[6], *TB[3], *SLB[3], *SRB[3];
+    double *EB[6], *TB[3];
     double *S0B[3], *FLB[3], *FRB[3], *UHI[3], *ULO[3];
     double *DSA[3], *DSB[3], *DLA[3], *DLB[3], *QM[3], *Q0[3], *QP[3];
-    double* UB[3];     // per-component velocity adjoint, ncomp x N^3
     double* SB;        // state adjoint, ncomp x N^3
     long long cs;      // N^3
     int n, ncomp;
@@ -69,12 +71,14 @@ struct AB {
 };
 
 __device__ __forceinline__ bool point(const AB& A, int& i, int& j, int& k, long long& p) {
-    p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= A.cs) return false;
-    const int n = A.n;
-    i = (int)(p % n);
-    j = (int)((p / n) % n);
-    k = (int)(p / ((long long)n * n));
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)A.cs) return false;
+    const unsigned n = (unsigned)A.n;
+    const unsigned r = q / n;
+    i = (int)(q - r * n);
+    k = (int)(r / n);
+    j = (int)(r - (unsigned)k * n);
+    p = q;
     return true;
 }
 
@@ -108,6 +112,7 @@ __global__ void __launch_bounds__(NTB) k_tape_ppm(AB A) {
     const Per P{A.n};
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
+    double ip[3], imv[3];
     for (int d = 0; d < 3; ++d) {
         double v[9];
         for (int o = -4; o <= 4; ++o) v[4 + o] = (o >= -2 && o <= 2) ? S[P.nb(i, j, k, d, o)] : 0.0;
@@ -117,8 +122,14 @@ __global__ void __launch_bounds__(NTB) k_tape_ppm(AB A) {
         else ppm_edges<1, 2>(v, k, 0, A.n - 1, MX_BC_INT_DIR, MX_BC_INT_DIR, sm, sp);
         ppm_trace(v[4], sm, sp, A.U[d][P.nb(i, j, k, d, 1)], A.U[d][p], A.dt, A.dx[d], A.rel_eps,
                   Ip, Im);
-        A.IP[d][co + p] = Ip;
-        A.IM[d][co + p] = Im;
+        ip[d] = Ip;
+        imv[d] = Im;
+    }
+    // every result is stored after the last load (stores and loads share
+    // one in-order counter, so a load issued after a store waits for it)
+    for (int d = 0; d < 3; ++d) {
+        A.IP[d][co + p] = ip[d];
+        A.IM[d][co + p] = imv[d];
     }
 }
 
@@ -129,9 +140,10 @@ __global__ void __launch_bounds__(NTB) k_tape_simh(AB A) {
     if (!point(A, i, j, k, p)) return;
     const Per P{A.n};
     const long long co = (long long)blockIdx.y * A.cs;
+    double h[3];
     for (int d = 0; d < 3; ++d)
-        A.SIMH[d][co + p] =
-            upwind(A.IP[d][co + P.nb(i, j, k, d, -1)], A.IM[d][co + p], A.U[d][p], 0.0);
+        h[d] = upwind(A.IP[d][co + P.nb(i, j, k, d, -1)], A.IM[d][co + p], A.U[d][p], 0.0);
+    for (int d = 0; d < 3; ++d) A.SIMH[d][co + p] = h[d];
 }
 
 // transverse term of cell c in direction t (dt/6h_t (u+ + u-) (simh+ - simh-))
@@ -150,6 +162,7 @@ __global__ void __launch_bounds__(NTB) k_tape_corner(AB A) {
     if (!point(A, i, j, k, p)) return;
     const Per P{A.n};
     const long long co = (long long)blockIdx.y * A.cs;
+    double cor[6];
     for (int d = 0; d < 3; ++d) {
         const int im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
         for (int t = 0; t < 3; ++t) {
@@ -158,9 +171,10 @@ __global__ void __launch_bounds__(NTB) k_tape_corner(AB A) {
             double r = A.IM[d][co + p];
             l = l - tt(A, P, co, im, jm, km, t);
             r = r - tt(A, P, co, i, j, k, t);
-            A.COR[pidx(d, t)][co + p] = upwind(l, r, A.U[d][p], A.rel_eps);
+            cor[pidx(d, t)] = upwind(l, r, A.U[d][p], A.rel_eps);
         }
     }
+    for (int q = 0; q < 6; ++q) A.COR[q][co + p] = cor[q];
 }
 
 // edge term of cell c for the corner state on t-faces corrected by o
@@ -181,6 +195,7 @@ __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
     const Per P{A.n};
     const long long co = (long long)blockIdx.y * A.cs;
     const double dt2 = 0.5 * A.dt;
+    double se[3];
     for (int d = 0; d < 3; ++d) {
         const int im = i - (d == 0), jm = j - (d == 1), km = k - (d == 2);
         double l = A.IP[d][co + P.at(im, jm, km)];
@@ -193,121 +208,148 @@ __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
         }
         const double sedgel = l + dt2 * 0.0;
         const double sedger = r + dt2 * 0.0;
-        A.SE[d][co + p] = upwind(sedgel, sedger, A.U[d][p], A.rel_eps);
+        se[d] = upwind(sedgel, sedger, A.U[d][p], A.rel_eps);
     }
+    for (int d = 0; d < 3; ++d) A.SE[d][co + p] = se[d];
 }
 
 // ---------------------------------------------------------------- reverse
-// B1: flux and update; edge-state upwinding
+// adjoint of the edge state on d-face c through the flux (Fb u) and its
+// upwinding weights
+__device__ __forceinline__ double edge_bar(const AB& A, const Per& P, long long co, int i, int j,
+                                           int k, int d, double& wl, double& wr) {
+    const long long c = P.at(i, j, k);
+    const double Fb = A.dt / A.dx[d] * (A.G[co + c] - A.G[co + P.nb(i, j, k, d, -1)]);
+    const double u = A.U[d][c];
+    weights(u, A.rel_eps, wl, wr);
+    return Fb * u;
+}
+
+// B1: flux and update, edge-state upwinding -> the per-cell edge-term
+// adjoints (each edge term of cell p enters the left state of d-face p + d
+// and the right state of d-face p, d = 3-t-o)
 __global__ void __launch_bounds__(NTB) k_adj_flux(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
     const Per P{A.n};
     const long long co = (long long)blockIdx.y * A.cs;
+    double lb1[3], rb0[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double Fb = A.dt / A.dx[d] * (A.G[co + p] - A.G[co + P.nb(i, j, k, d, -1)]);
-        const double u = A.U[d][p];
-        const double seb = Fb * u;
-        A.UB[d][co + p] = Fb * A.SE[d][co + p];
         double wl, wr;
-        weights(u, A.rel_eps, wl, wr);
-        A.LB[d][co + p] = wl * seb;
-        A.RB[d][co + p] = wr * seb;
-        A.SLB[d][co + p] = wl * seb;
-        A.SRB[d][co + p] = wr * seb;
+        const double seb = edge_bar(A, P, co, i, j, k, d, wl, wr);
+        rb0[d] = wr * seb;
+        double wl1, wr1;
+        const double seb1 = edge_bar(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, wl1, wr1);
+        lb1[d] = wl1 * seb1;
     }
-}
-
-// B2: per-cell edge-term adjoints (each term feeds the d-faces, d = 3-t-o)
-__global__ void __launch_bounds__(NTB) k_adj_eterm(AB A) {
-    int i, j, k;
-    long long p;
-    if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
-    const long long co = (long long)blockIdx.y * A.cs;
+#pragma unroll
     for (int t = 0; t < 3; ++t)
+#pragma unroll
         for (int o = 0; o < 3; ++o) {
             if (o == t) continue;
             const int d = 3 - t - o;
-            A.EB[pidx(t, o)][co + p] = -A.LB[d][co + P.nb(i, j, k, d, 1)] - A.RB[d][co + p];
+            A.EB[pidx(t, o)][co + p] = -lb1[d] - rb0[d];
         }
 }
 
-// B3: edge terms -> corner-state and velocity adjoints; corner upwinding
-__global__ void __launch_bounds__(NTB) k_adj_corner(AB A) {
-    int i, j, k;
-    long long p;
-    if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
-    const long long co = (long long)blockIdx.y * A.cs;
-    const double dt4 = 0.25 * A.dt;
-    for (int t = 0; t < 3; ++t) {
-        const double b = dt4 / A.dx[t];
-        const long long fm = P.nb(i, j, k, t, -1), fp = P.nb(i, j, k, t, 1);
-        const double u0 = A.U[t][p], um = A.U[t][fm], up = A.U[t][fp];
-        double ub = 0.0, slb = 0.0, srb = 0.0;
-        for (int o = 0; o < 3; ++o) {
-            if (o == t) continue;
-            const int q = pidx(t, o);
+// B3 + B5: the adjoints on t-face (i,j,k) of its left and right predictor
+// states (slb, srb) and, with want_u, of its velocity for this component
+// (ub): each the sum, in this order, of the flux term (edge-state
+// upwinding), the corner-state terms (edge terms, corner upwinding) and the
+// transverse term (predictor upwinding)
+__device__ __forceinline__ void face_totals(const AB& A, const Per& P, long long co, int i, int j,
+                                            int k, int t, bool want_u, double& slb, double& srb,
+                                            double& ub) {
+    const long long f = P.at(i, j, k);
+    const long long fm = P.nb(i, j, k, t, -1), fp = P.nb(i, j, k, t, 1);
+    const double u0 = A.U[t][f], um = A.U[t][fm], up = A.U[t][fp];
+    // flux
+    const double Fb = A.dt / A.dx[t] * (A.G[co + f] - A.G[co + fm]);
+    const double seb = Fb * u0;
+    double wl, wr;
+    weights(u0, A.rel_eps, wl, wr);
+    ub = want_u ? Fb * A.SE[t][co + f] : 0.0;
+    slb = wl * seb;
+    srb = wr * seb;
+    // corner states on t-faces, through the edge terms of the cells below
+    // and above the face
+    const double b = 0.25 * A.dt / A.dx[t];
+    double ubc = 0.0, slbc = 0.0, srbc = 0.0;
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        if (o == t) continue;
+        const int q = pidx(t, o);
+        const double ebm = A.EB[q][co + fm];
+        const double eb0 = A.EB[q][co + f];
+        const double kb = ebm * b * (u0 + um) - eb0 * b * (up + u0);
+        if (want_u) {
             const double* K = A.COR[q] + co;
-            const double ebm = A.EB[q][co + fm];  // term of the cell below face p
-            const double eb0 = A.EB[q][co + p];   // term of the cell above
-            const double kb = ebm * b * (u0 + um) - eb0 * b * (up + u0);
-            ub += ebm * b * (K[p] - K[fm]) + eb0 * b * (K[fp] - K[p]);
-            double wl, wr;
-            weights(u0, A.rel_eps, wl, wr);
-            const double lb = wl * kb, rb = wr * kb;
-            slb += lb;
-            srb += rb;
-            A.CLB[q][co + p] = lb;
-            A.CRB[q][co + p] = rb;
+            ubc += ebm * b * (K[f] - K[fm]) + eb0 * b * (K[fp] - K[f]);
         }
-        A.UB[t][co + p] += ub;
-        A.SLB[t][co + p] += slb;
-        A.SRB[t][co + p] += srb;
+        slbc += wl * kb;
+        srbc += wr * kb;
     }
+    ub += ubc;
+    slb += slbc;
+    srb += srbc;
+    // upwind state simh, through the transverse terms of the cells below and
+    // above the face
+    const double a = A.dt / 6.0 / A.dx[t];
+    const double tbm = A.TB[t][co + fm], tb0 = A.TB[t][co + f];
+    const double hb = tbm * a * (u0 + um) - tb0 * a * (up + u0);
+    if (want_u) {
+        const double* H = A.SIMH[t] + co;
+        ub = ub + (tbm * a * (H[f] - H[fm]) + tb0 * a * (H[fp] - H[f]));
+    }
+    double wl0, wr0;
+    weights(u0, 0.0, wl0, wr0);
+    slb = slb + wl0 * hb;
+    srb = srb + wr0 * hb;
 }
 
-// B4: per-cell transverse-term adjoints
+// adjoint of the corner state q on d-face (i,j,k) (through the edge terms of
+// the cells on both sides of it), with its upwinding weights: the left and
+// right predictor-state adjoints it passes on
+__device__ __forceinline__ void corner_bar(const AB& A, const Per& P, long long co, int i, int j,
+                                           int k, int d, int q, double& lb, double& rb) {
+    const double b = 0.25 * A.dt / A.dx[d];
+    const long long c = P.at(i, j, k);
+    const long long fm = P.nb(i, j, k, d, -1), fp = P.nb(i, j, k, d, 1);
+    const double u0 = A.U[d][c], um = A.U[d][fm], up = A.U[d][fp];
+    const double ebm = A.EB[q][co + fm];
+    const double eb0 = A.EB[q][co + c];
+    const double kb = ebm * b * (u0 + um) - eb0 * b * (up + u0);
+    double wl, wr;
+    weights(u0, A.rel_eps, wl, wr);
+    lb = wl * kb;
+    rb = wr * kb;
+}
+
+// B4: per-cell transverse-term adjoints: the transverse term of cell p in
+// direction o enters the corner states on the d-faces p (right state) and
+// p + d (left state), d != o
 __global__ void __launch_bounds__(NTB) k_adj_tterm(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
     const Per P{A.n};
     const long long co = (long long)blockIdx.y * A.cs;
+    double tb[3];
     for (int o = 0; o < 3; ++o) {
         double s = 0.0;
         for (int d = 0; d < 3; ++d) {
             if (d == o) continue;
             const int q = pidx(d, o);
-            s += A.CLB[q][co + P.nb(i, j, k, d, 1)] + A.CRB[q][co + p];
+            double lb1, rb1, lb0, rb0;
+            corner_bar(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, q, lb1, rb1);
+            corner_bar(A, P, co, i, j, k, d, q, lb0, rb0);
+            s += lb1 + rb0;
         }
-        A.TB[o][co + p] = -s;
+        tb[o] = -s;
     }
-}
-
-// B5: transverse terms -> upwind-state and velocity adjoints; predictor upwinding
-__global__ void __launch_bounds__(NTB) k_adj_simh(AB A) {
-    int i, j, k;
-    long long p;
-    if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
-    const long long co = (long long)blockIdx.y * A.cs;
-    const double dt6 = A.dt / 6.0;
-    for (int t = 0; t < 3; ++t) {
-        const double a = dt6 / A.dx[t];
-        const long long fm = P.nb(i, j, k, t, -1), fp = P.nb(i, j, k, t, 1);
-        const double u0 = A.U[t][p], um = A.U[t][fm], up = A.U[t][fp];
-        const double* H = A.SIMH[t] + co;
-        const double tbm = A.TB[t][co + fm], tb0 = A.TB[t][co + p];
-        const double hb = tbm * a * (u0 + um) - tb0 * a * (up + u0);
-        A.UB[t][co + p] += tbm * a * (H[p] - H[fm]) + tb0 * a * (H[fp] - H[p]);
-        double wl, wr;
-        weights(u0, 0.0, wl, wr);
-        A.SLB[t][co + p] += wl * hb;
-        A.SRB[t][co + p] += wr * hb;
-    }
+    for (int o = 0; o < 3; ++o) A.TB[o][co + p] = tb[o];
 }
 
 // B6: characteristic trace and CW limiter of cell p in direction d
@@ -318,6 +360,7 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
     const Per P{A.n};
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
+    double r0[3], r1[3], r2[3], r3[3], r4[3];
     for (int d = 0; d < 3; ++d) {
         double v[9];
         for (int o = -4; o <= 4; ++o) v[4 + o] = (o >= -2 && o <= 2) ? S[P.nb(i, j, k, d, o)] : 0.0;
@@ -351,8 +394,10 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
         const double uhi = A.U[d][fhi], ulo = A.U[d][p];
         const double h = A.dx[d];
         const double s6 = 6.0 * s0 - 3.0 * (sm + sp);
-        const double Ipb = A.SLB[d][co + fhi];
-        const double Imb = A.SRB[d][co + p];
+        double Ipb, Imb, unused0, unused1, unused2;
+        face_totals(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb, unused0,
+                    unused1);
+        face_totals(A, P, co, i, j, k, d, false, unused2, Imb, unused1);
         double smb = 0.0, spb = 0.0, s0b = 0.0, uhib = 0.0, ulob = 0.0;
         if (uhi > A.rel_eps) {
             const double sg = dabs(uhi) * A.dt / h;
@@ -392,15 +437,91 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
             sm0b = smb;
             sp0b = spb;
         }
-        A.S0B[d][co + p] = s0b;
-        A.FLB[d][co + p] = sm0b;  // face p (below the cell)
-        A.FRB[d][co + p] = sp0b;  // face above the cell
-        A.UHI[d][co + p] = uhib;
-        A.ULO[d][co + p] = ulob;
+        r0[d] = s0b;
+        r1[d] = sm0b;
+        r2[d] = sp0b;
+        r3[d] = uhib;
+        r4[d] = ulob;
+    }
+    for (int d = 0; d < 3; ++d) {
+        A.S0B[d][co + p] = r0[d];
+        A.FLB[d][co + p] = r1[d];  // face p (below the cell)
+        A.FRB[d][co + p] = r2[d];  // face above the cell
+        A.UHI[d][co + p] = r3[d];
+        A.ULO[d][co + p] = r4[d];
     }
 }
 
-// B7: PPM type 1 face values (face p: cells p-1 | p)
+// B7: adjoints through the PPM type 1 face value of d-face (i,j,k) (cells
+// below | above it): of the two cells' values (bsa, bsb) and of their van
+// Leer slopes (bla, blb)
+__device__ __forceinline__ void face_bar(const AB& A, const Per& P, long long co,
+                                         const double* S, int i, int j, int k, int d, double& bsa,
+                                         double& bsb, double& bla, double& blb) {
+    const long long c = P.at(i, j, k);
+    const long long cm = P.nb(i, j, k, d, -1);
+    const double eb = A.FLB[d][co + c] + A.FRB[d][co + cm];
+    const double sa = S[cm], sb = S[c];
+    const double sa2 = S[P.nb(i, j, k, d, -2)], sb2 = S[P.nb(i, j, k, d, 1)];
+    const double dla = vl_slope(sa2, sa, sb), dlb = vl_slope(sa, sb, sb2);
+    const double e0 = 0.5 * (sb + sa) - (dlb - dla) / 6.0;
+    const double m1 = dmin(sb, sa);
+    const bool m1a = sa < sb;  // dmin(sb, sa) returns sa
+    const double e1 = dmax(e0, m1);
+    const bool e1m = e0 < m1;
+    const double M = dmax(sb, sa);
+    const bool Ma = sb < sa;  // dmax(sb, sa) returns sa
+    bsa = 0.0;
+    bsb = 0.0;
+    bla = 0.0;
+    blb = 0.0;
+    if (M < e1) {
+        if (Ma) bsa = eb;
+        else bsb = eb;
+    } else if (e1m) {
+        if (m1a) bsa = eb;
+        else bsb = eb;
+    } else {
+        bsa = 0.5 * eb;
+        bsb = 0.5 * eb;
+        bla = eb / 6.0;
+        blb = -eb / 6.0;
+    }
+}
+
+// B8: adjoints of the van Leer slope of cell (i,j,k) along d, whose adjoint
+// is rb, to its three stencil values
+__device__ __forceinline__ void slope_bar(const Per& P, const double* S, int i, int j, int k,
+                                          int d, double rb, double& qm, double& q0, double& qp) {
+    const double a = S[P.nb(i, j, k, d, -1)], b = S[P.at(i, j, k)], c = S[P.nb(i, j, k, d, 1)];
+    const double dsc = 0.5 * (c - a);
+    const double dsl = 2.0 * (b - a);
+    const double dsr = 2.0 * (c - b);
+    // the slope is sign(dsc) * min(|dsc|, min(|dsl|, |dsr|)) when dsl dsr > 0:
+    // its derivative is w times the (cm, c0, cp) of the selected difference
+    double cm = 0.0, c0 = 0.0, cp = 0.0, w = 0.0;
+    if (dsl * dsr > 0.0) {
+        const double adsl = dabs(dsl), adsr = dabs(dsr), adsc = dabs(dsc);
+        const bool inner_r = adsr < adsl;
+        const double m = inner_r ? adsr : adsl;
+        const double g = dsign1(dsc) * rb;
+        if (m < adsc) {
+            w = g * dabs_d(inner_r ? dsr : dsl);
+            cm = inner_r ? 0.0 : -2.0;
+            c0 = inner_r ? -2.0 : 2.0;
+            cp = inner_r ? 2.0 : 0.0;
+        } else {
+            w = g * dabs_d(dsc);
+            cm = -0.5;
+            cp = 0.5;
+        }
+    }
+    qm = cm != 0.0 ? cm * w : 0.0;
+    q0 = c0 != 0.0 ? c0 * w : 0.0;
+    qp = cp != 0.0 ? cp * w : 0.0;
+}
+
+// B7: face-value adjoints of face p (per direction)
 __global__ void __launch_bounds__(NTB) k_adj_face(AB A) {
     int i, j, k;
     long long p;
@@ -408,40 +529,19 @@ __global__ void __launch_bounds__(NTB) k_adj_face(AB A) {
     const Per P{A.n};
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
+    double bsa[3], bsb[3], bla[3], blb[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) face_bar(A, P, co, S, i, j, k, d, bsa[d], bsb[d], bla[d], blb[d]);
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const long long cm = P.nb(i, j, k, d, -1);
-        const double eb = A.FLB[d][co + p] + A.FRB[d][co + cm];
-        const double sa = S[cm], sb = S[p];
-        const double sa2 = S[P.nb(i, j, k, d, -2)], sb2 = S[P.nb(i, j, k, d, 1)];
-        const double dla = vl_slope(sa2, sa, sb), dlb = vl_slope(sa, sb, sb2);
-        const double e0 = 0.5 * (sb + sa) - (dlb - dla) / 6.0;
-        const double m1 = dmin(sb, sa);
-        const bool m1a = sa < sb;  // dmin(sb, sa) returns sa
-        const double e1 = dmax(e0, m1);
-        const bool e1m = e0 < m1;
-        const double M = dmax(sb, sa);
-        const bool Ma = sb < sa;  // dmax(sb, sa) returns sa
-        double bsa = 0.0, bsb = 0.0, bla = 0.0, blb = 0.0;
-        if (M < e1) {
-            if (Ma) bsa = eb;
-            else bsb = eb;
-        } else if (e1m) {
-            if (m1a) bsa = eb;
-            else bsb = eb;
-        } else {
-            bsa = 0.5 * eb;
-            bsb = 0.5 * eb;
-            bla = eb / 6.0;
-            blb = -eb / 6.0;
-        }
-        A.DSA[d][co + p] = bsa;
-        A.DSB[d][co + p] = bsb;
-        A.DLA[d][co + p] = bla;
-        A.DLB[d][co + p] = blb;
+        A.DSA[d][co + p] = bsa[d];
+        A.DSB[d][co + p] = bsb[d];
+        A.DLA[d][co + p] = bla[d];
+        A.DLB[d][co + p] = blb[d];
     }
 }
 
-// B8: van Leer slope of cell p: adjoints of its three stencil values
+// B8: van Leer slope adjoints of cell p; its slope feeds faces p and p + d
 __global__ void __launch_bounds__(NTB) k_adj_slope(AB A) {
     int i, j, k;
     long long p;
@@ -449,39 +549,17 @@ __global__ void __launch_bounds__(NTB) k_adj_slope(AB A) {
     const Per P{A.n};
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
+    double qm[3], q0[3], qp[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const long long cp = P.nb(i, j, k, d, 1);
-        const double rb = A.DLB[d][co + p] + A.DLA[d][co + cp];
-        const double a = S[P.nb(i, j, k, d, -1)], b = S[p], c = S[cp];
-        const double dsc = 0.5 * (c - a);
-        const double dsl = 2.0 * (b - a);
-        const double dsr = 2.0 * (c - b);
-        double qm = 0.0, q0 = 0.0, qp = 0.0;
-        if (dsl * dsr > 0.0) {
-            // r = sign(dsc) * min(|dsc|, min(|dsl|, |dsr|))
-            const double adsl = dabs(dsl), adsr = dabs(dsr), adsc = dabs(dsc);
-            const bool inner_r = adsr < adsl;
-            const double m = inner_r ? adsr : adsl;
-            const double g = dsign1(dsc) * rb;
-            if (m < adsc) {
-                if (inner_r) {
-                    const double w = g * dabs_d(dsr);
-                    q0 = -2.0 * w;
-                    qp = 2.0 * w;
-                } else {
-                    const double w = g * dabs_d(dsl);
-                    qm = -2.0 * w;
-                    q0 = 2.0 * w;
-                }
-            } else {
-                const double w = g * dabs_d(dsc);
-                qm = -0.5 * w;
-                qp = 0.5 * w;
-            }
-        }
-        A.QM[d][co + p] = qm;
-        A.Q0[d][co + p] = q0;
-        A.QP[d][co + p] = qp;
+        const double rb = A.DLB[d][co + p] + A.DLA[d][co + P.nb(i, j, k, d, 1)];
+        slope_bar(P, S, i, j, k, d, rb, qm[d], q0[d], qp[d]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        A.QM[d][co + p] = qm[d];
+        A.Q0[d][co + p] = q0[d];
+        A.QP[d][co + p] = qp[d];
     }
 }
 
@@ -507,16 +585,21 @@ __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, double* ub0, double* ub1,
     long long p;
     if (!point(A, i, j, k, p)) return;
     const Per P{A.n};
-    double* out[3] = {ub0, ub1, ub2};
+    double r[3];
     for (int d = 0; d < 3; ++d) {
         const long long cm = P.nb(i, j, k, d, -1);
         double s = 0.0;
         for (int c = 0; c < A.ncomp; ++c) {
             const long long co = (long long)c * A.cs;
-            s += A.UB[d][co + p] + A.UHI[d][co + cm] + A.ULO[d][co + p];
+            double slb, srb, ub;
+            face_totals(A, P, co, i, j, k, d, true, slb, srb, ub);
+            s += ub + A.UHI[d][co + cm] + A.ULO[d][co + p];
         }
-        out[d][p] = s;
+        r[d] = s;
     }
+    ub0[p] = r[0];
+    ub1[p] = r[1];
+    ub2[p] = r[2];
 }
 
 // ---------------------------------------------------------------- glue
@@ -547,19 +630,25 @@ __global__ void __launch_bounds__(NTB) k_periodic_fill(const FillTask* __restric
 
 // dense copies between FABs and N^3 arrays
 __global__ void __launch_bounds__(NTB) k_to_dense(DA a, int comp0, int ncomp, int n, double* out) {
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long n3 = (long long)n * n * n;
-    if (p >= n3) return;
-    const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((long long)n * n));
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)n3) return;
+    const long long p = q;
+    const unsigned r = q / (unsigned)n;
+    const int i = (int)(q - r * (unsigned)n), k = (int)(r / (unsigned)n);
+    const int j = (int)(r - (unsigned)k * (unsigned)n);
     for (int c = 0; c < ncomp; ++c) out[(long long)c * n3 + p] = a(i, j, k, comp0 + c);
 }
 
 __global__ void __launch_bounds__(NTB) k_from_dense(const double* in, int ncomp, int n, DA a,
                                                     int nodal) {
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long n3 = (long long)n * n * n;
-    if (p >= n3) return;
-    const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((long long)n * n));
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)n3) return;
+    const long long p = q;
+    const unsigned r = q / (unsigned)n;
+    const int i = (int)(q - r * (unsigned)n), k = (int)(r / (unsigned)n);
+    const int j = (int)(r - (unsigned)k * (unsigned)n);
     for (int c = 0; c < ncomp; ++c) {
         a(i, j, k, c) = in[(long long)c * n3 + p];
         // the duplicate face n of a periodic face FAB carries no separate value
@@ -573,10 +662,13 @@ __global__ void __launch_bounds__(NTB) k_from_dense(const double* in, int ncomp,
 __global__ void __launch_bounds__(NTB) k_step_update(DA s, DA snew, DA u0, DA u1, DA u2, DA e0,
                                                      DA e1, DA e2, int n, int ncomp, double dt,
                                                      double dx0, double dx1, double dx2) {
-    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long n3 = (long long)n * n * n;
-    if (p >= n3) return;
-    const int i = (int)(p % n), j = (int)((p / n) % n), k = (int)(p / ((long long)n * n));
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)n3) return;
+    const long long p = q;
+    const unsigned r = q / (unsigned)n;
+    const int i = (int)(q - r * (unsigned)n), k = (int)(r / (unsigned)n);
+    const int j = (int)(r - (unsigned)k * (unsigned)n);
     for (int c = 0; c < ncomp; ++c) {
         const double fx1 = u0(i + 1, j, k) * e0(i + 1, j, k, c);
         const double fx0 = u0(i, j, k) * e0(i, j, k, c);
@@ -609,6 +701,7 @@ int step_check(const mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp,
         if (p.domain_lo[0][d] != 0 || p.domain_hi[0][d] != n - 1) return MX_ERR_UNSUPPORTED;
     }
     if (n < 4 || ncomp < 1 || ncomp > MX_MAX_SCAL) return MX_ERR_ARG;
+    if ((long long)n * n * n * ncomp > 0x7fffffffLL) return MX_ERR_UNSUPPORTED;
     if (!s || !u) return MX_ERR_ARG;
     HBox v = hbox(b);
     if (!covers(*s, grow(v, 3, 3), ncomp)) return MX_ERR_BOX;
@@ -719,10 +812,10 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
         if (!covers(ubar[d], nodal(v, d), 1)) return MX_ERR_BOX;
     if ((rc = periodic_fill(c, s, umac, ncomp, n))) return rc;
     const long long n3 = (long long)n * n * n;
-    // dense arrays: S, G, SB (ncomp each), 90 per-component intermediates
+    // dense arrays: S, G, SB (ncomp each), 63 per-component intermediates
     // and adjoints, U and the summed velocity adjoint (3 each)
     const size_t per = (size_t)n3 * sizeof(double) + 256;
-    const size_t count = (size_t)ncomp * 93 + 6;
+    const size_t count = (size_t)ncomp * 66 + 6;
     if ((rc = ws_reserve(c, per * count + 4096))) return rc;
     AB A;
     std::memset(&A, 0, sizeof(A));
@@ -736,11 +829,7 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
         A.IM[d] = take(ncomp);
         A.SIMH[d] = take(ncomp);
         A.SE[d] = take(ncomp);
-        A.LB[d] = take(ncomp);
-        A.RB[d] = take(ncomp);
         A.TB[d] = take(ncomp);
-        A.SLB[d] = take(ncomp);
-        A.SRB[d] = take(ncomp);
         A.S0B[d] = take(ncomp);
         A.FLB[d] = take(ncomp);
         A.FRB[d] = take(ncomp);
@@ -753,13 +842,10 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
         A.QM[d] = take(ncomp);
         A.Q0[d] = take(ncomp);
         A.QP[d] = take(ncomp);
-        A.UB[d] = take(ncomp);
     }
     for (int q = 0; q < 6; ++q) {
         A.COR[q] = take(ncomp);
         A.EB[q] = take(ncomp);
-        A.CLB[q] = take(ncomp);
-        A.CRB[q] = take(ncomp);
     }
     A.S = S;
     A.G = G;
@@ -782,10 +868,7 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
     MX_LAUNCH(c, "k_tape_corner", k_tape_corner, gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_tape_edge", k_tape_edge, gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_adj_flux", k_adj_flux, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_eterm", k_adj_eterm, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_corner", k_adj_corner, gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_adj_tterm", k_adj_tterm, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_simh", k_adj_simh, gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_adj_trace", k_adj_trace, gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_adj_face", k_adj_face, gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_adj_slope", k_adj_slope, gc, dim3(NTB), A);
