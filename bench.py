@@ -240,6 +240,9 @@ def adjoint_main(args):
     s = torch.from_numpy(s0).to("cuda:0")
     tgt = torch.from_numpy(rng.standard_normal(s0.shape)).to("cuda:0")
     st = ScalStepper(n, nc, 0.5 / n, 1e-8)
+    for kv in args.tune:
+        key, val = kv.split("=")
+        st.m.set_tuning(key, int(val))
     fwd_ms = time_ms(lambda: st.step(s, u), args.steps, args.warmup)
     g = torch.ones_like(s)
     adj_ms = time_ms(lambda: st.vjp(s, u, g), args.steps, args.warmup)

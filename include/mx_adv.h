@@ -400,10 +400,13 @@ int mx_adv_scal_step(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, mx_fab* snew, int
 
 /* Hand-written adjoint of mx_adv_scal_step (no counterpart in the
    reference): given gbar = dL/dsnew on the valid cells, writes sbar = dL/ds
-   on the valid cells and ubar[d] = dL/dumac[d] on faces 0..n-1 (face n is
-   set to 0: it is face 0).  Piecewise differentiation: every limiter and
-   upwind branch is taken as in the forward step.  Deterministic (no
-   atomics); the workspace holds about 93 ncomp + 6 arrays of n^3 doubles. */
+   on the valid cells and ubar[d] = dL/dumac[d] on faces 0..n-1 (face n, if
+   the FAB holds it, is set to 0: it is face 0).  It reads only the valid
+   cells of s and faces 0..n-1 of umac (no ghosts needed); inputs that are
+   dense (ncomp, n, n, n) arrays starting at index 0 are read in place.
+   Piecewise differentiation: every limiter and upwind branch is taken as in
+   the forward step.  Deterministic (no atomics); the workspace holds about
+   65 ncomp + 3 arrays of n^3 doubles. */
 int mx_adv_scal_step_vjp(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, const mx_fab* gbar,
                          mx_fab* sbar, mx_fab* ubar, int ncomp);
 

@@ -78,24 +78,14 @@ class ScalStepper:
         e = n + 2 * ng
         f64 = torch.float64
         self.s = torch.zeros((ncomp, e, e, e), dtype=f64, device=dev)
-        self.snew = torch.zeros((ncomp, n, n, n), dtype=f64, device=dev)
-        self.gbar = torch.zeros((ncomp, n, n, n), dtype=f64, device=dev)
-        self.sbar = torch.zeros((ncomp, n, n, n), dtype=f64, device=dev)
+        self.snew = torch.zeros((ncomp, n, n, n), dtype=f64, device=dev)  # shape template
         self.u = []
-        self.ubar = []
         for d in range(3):
             shp = [n + 2, n + 2, n + 2]
             shp[2 - d] = n + 3  # nodal in d, one ghost face each side
             self.u.append(torch.zeros(shp, dtype=f64, device=dev))
-            shpb = [n, n, n]
-            shpb[2 - d] = n + 1
-            self.ubar.append(torch.zeros(shpb, dtype=f64, device=dev))
         self._fab_s = self._fab(self.s, [-ng] * 3, ncomp)
-        self._fab_snew = self._fab(self.snew, [0] * 3, ncomp)
-        self._fab_gbar = self._fab(self.gbar, [0] * 3, ncomp)
-        self._fab_sbar = self._fab(self.sbar, [0] * 3, ncomp)
         self._fab_u = (_abi.Fab * 3)(*[self._fab(self.u[d], [-1] * 3, 1) for d in range(3)])
-        self._fab_ubar = (_abi.Fab * 3)(*[self._fab(self.ubar[d], [0] * 3, 1) for d in range(3)])
 
     @staticmethod
     def _fab(t, begin, nc):
@@ -123,22 +113,32 @@ class ScalStepper:
             raise _abi.MxError(rc, where)
 
     def step(self, s, u):
+        """snew for the state s (ncomp, n, n, n) and velocity u; the result is
+        a new tensor, ordered on the library's stream (torch's default)."""
         self._load(s, u)
+        snew = self.torch.empty_like(self.snew)
         self._check(self.fns["scal_step"](self.m.ctx, C.byref(self._fab_s), self._fab_u,
-                                          C.byref(self._fab_snew), self.ncomp), "scal_step")
-        self.m.sync()
-        return self.snew.clone()
+                                          C.byref(self._fab(snew, [0] * 3, self.ncomp)),
+                                          self.ncomp), "scal_step")
+        return snew
 
     def vjp(self, s, u, gbar):
-        self._load(s, u)
-        self.gbar.copy_(gbar)
-        self._check(self.fns["scal_step_vjp"](self.m.ctx, C.byref(self._fab_s), self._fab_u,
-                                              C.byref(self._fab_gbar), C.byref(self._fab_sbar),
-                                              self._fab_ubar, self.ncomp), "scal_step_vjp")
-        self.m.sync()
-        n = self.n
-        ub = [self.ubar[d][:n, :n, :n].clone() for d in range(3)]
-        return self.sbar.clone(), ub
+        """(sbar, [ubar_x, ubar_y, ubar_z]) for the output adjoint gbar: the
+        adjoint reads the dense inputs in place (it needs no ghost cells) and
+        writes new tensors."""
+        s = s.contiguous()
+        gbar = gbar.contiguous()
+        u = [x.contiguous() for x in u]
+        sb = self.torch.empty_like(s)
+        ub = [self.torch.empty_like(x) for x in u]
+        nc = self.ncomp
+        fu = (_abi.Fab * 3)(*[self._fab(u[d], [0] * 3, 1) for d in range(3)])
+        fub = (_abi.Fab * 3)(*[self._fab(ub[d], [0] * 3, 1) for d in range(3)])
+        self._check(self.fns["scal_step_vjp"](self.m.ctx, C.byref(self._fab(s, [0] * 3, nc)), fu,
+                                              C.byref(self._fab(gbar, [0] * 3, nc)),
+                                              C.byref(self._fab(sb, [0] * 3, nc)), fub, nc),
+                    "scal_step_vjp")
+        return sb, ub
 
 
 def _autograd():

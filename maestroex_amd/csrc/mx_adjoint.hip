@@ -43,11 +43,17 @@ constexpr double C23 = 2.0 / 3.0;
 __host__ __device__ constexpr int pidx(int d, int t) { return d * 2 + (t > d ? t - 1 : t); }
 
 // periodic indexing of a dense N^3 array (N^3 fits 31 bits; checked on the
-// host, so all index arithmetic is 32-bit)
-struct Per {
-    int n;
-    __device__ __forceinline__ int w(int x) const { return x < 0 ? x + n : (x >= n ? x - n : x); }
+// host, so all index arithmetic is 32-bit).  P2: N is a power of two and a
+// coordinate within one period of [0, N) wraps with a mask.
+template <bool P2>
+struct PerT {
+    int n, m, lg;
+    __device__ __forceinline__ int w(int x) const {
+        if (P2) return x & m;
+        return x < 0 ? x + n : (x >= n ? x - n : x);
+    }
     __device__ __forceinline__ int at(int i, int j, int k) const {
+        if (P2) return w(i) | (w(j) << lg) | (w(k) << (2 * lg));
         return w(i) + n * (w(j) + n * w(k));
     }
     // neighbour of (i,j,k) at offset o along direction d
@@ -56,18 +62,31 @@ struct Per {
     }
 };
 
+template <bool P2>
+__device__ __forceinline__ PerT<P2> make_per(int n) {
+    PerT<P2> P;
+    P.n = n;
+    P.m = n - 1;
+    P.lg = 31 - __clz(n);
+    return P;
+}
+
 struct AB {
     const double* S;   // state, ncomp x N^3
     const double* U[3];
     const double* G;   // adjoint of snew
     double *IP[3], *IM[3], *SIMH[3], *COR[6], *SE[3];
     double *EB[6], *TB[3];
+    double *SLB[3], *SRB[3];  // face sums (k_adj_fsum), when used
     double *S0B[3], *FLB[3], *FRB[3], *UHI[3], *ULO[3];
     double *DSA[3], *DSB[3], *DLA[3], *DLB[3], *QM[3], *Q0[3], *QP[3];
-    double* SB;        // state adjoint, ncomp x N^3
+    DA SBF;            // state adjoint: the caller's FAB
     long long cs;      // N^3
     int n, ncomp;
     double dt, rel_eps, dx[3];
+    // per-direction factors, evaluated once on the host by the same
+    // expressions: dt/h, (dt/4)/h (edge terms), (dt/6)/h (transverse terms)
+    double fdt[3], f4[3], f6[3];
 };
 
 __device__ __forceinline__ bool point(const AB& A, int& i, int& j, int& k, long long& p) {
@@ -104,63 +123,69 @@ __device__ __forceinline__ double upwind(double l, double r, double u, double th
 __device__ __forceinline__ double dabs_d(double x) { return signbit(x) ? -1.0 : 1.0; }
 
 // ---------------------------------------------------------------- forward tape
-// A1: traced PPM states of every cell and direction
+// A1 + A2: traced PPM states of every cell and direction, and the upwind
+// state simh on the face below the cell (from the traced state of the cell
+// below it, recomputed here)
+template <int D, class PT>
+__device__ __forceinline__ void ppm_cell(const AB& A, const PT& P, const double* S, int i, int j,
+                                         int k, int pos, double& Ip, double& Im) {
+    double v[9];
+#pragma unroll
+    for (int o = -4; o <= 4; ++o) v[4 + o] = (o >= -2 && o <= 2) ? S[P.nb(i, j, k, D, o)] : 0.0;
+    double sm, sp;
+    ppm_edges<1, D>(v, pos, 0, A.n - 1, MX_BC_INT_DIR, MX_BC_INT_DIR, sm, sp);
+    ppm_trace(v[4], sm, sp, A.U[D][P.nb(i, j, k, D, 1)], A.U[D][P.at(i, j, k)], A.dt, A.dx[D],
+              A.rel_eps, Ip, Im);
+}
+
+template <int D, class PT>
+__device__ __forceinline__ void tape_dir(const AB& A, const PT& P, const double* S, int i, int j,
+                                         int k, long long p, double& ip, double& im, double& h) {
+    const int pos = D == 0 ? i : (D == 1 ? j : k);
+    ppm_cell<D>(A, P, S, i, j, k, pos, ip, im);
+    double ipm, imm;
+    ppm_cell<D>(A, P, S, i - (D == 0), j - (D == 1), k - (D == 2), pos - 1, ipm, imm);
+    h = upwind(ipm, im, A.U[D][p], 0.0);
+}
+
+template <bool P2>
 __global__ void __launch_bounds__(NTB) k_tape_ppm(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
-    double ip[3], imv[3];
-    for (int d = 0; d < 3; ++d) {
-        double v[9];
-        for (int o = -4; o <= 4; ++o) v[4 + o] = (o >= -2 && o <= 2) ? S[P.nb(i, j, k, d, o)] : 0.0;
-        double sm, sp, Ip, Im;
-        if (d == 0) ppm_edges<1, 0>(v, i, 0, A.n - 1, MX_BC_INT_DIR, MX_BC_INT_DIR, sm, sp);
-        else if (d == 1) ppm_edges<1, 1>(v, j, 0, A.n - 1, MX_BC_INT_DIR, MX_BC_INT_DIR, sm, sp);
-        else ppm_edges<1, 2>(v, k, 0, A.n - 1, MX_BC_INT_DIR, MX_BC_INT_DIR, sm, sp);
-        ppm_trace(v[4], sm, sp, A.U[d][P.nb(i, j, k, d, 1)], A.U[d][p], A.dt, A.dx[d], A.rel_eps,
-                  Ip, Im);
-        ip[d] = Ip;
-        imv[d] = Im;
-    }
+    double ip[3], imv[3], h[3];
+    tape_dir<0>(A, P, S, i, j, k, p, ip[0], imv[0], h[0]);
+    tape_dir<1>(A, P, S, i, j, k, p, ip[1], imv[1], h[1]);
+    tape_dir<2>(A, P, S, i, j, k, p, ip[2], imv[2], h[2]);
     // every result is stored after the last load (stores and loads share
     // one in-order counter, so a load issued after a store waits for it)
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
         A.IP[d][co + p] = ip[d];
         A.IM[d][co + p] = imv[d];
+        A.SIMH[d][co + p] = h[d];
     }
 }
 
-// A2: upwind states simh on faces (face index = the cell above it)
-__global__ void __launch_bounds__(NTB) k_tape_simh(AB A) {
-    int i, j, k;
-    long long p;
-    if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
-    const long long co = (long long)blockIdx.y * A.cs;
-    double h[3];
-    for (int d = 0; d < 3; ++d)
-        h[d] = upwind(A.IP[d][co + P.nb(i, j, k, d, -1)], A.IM[d][co + p], A.U[d][p], 0.0);
-    for (int d = 0; d < 3; ++d) A.SIMH[d][co + p] = h[d];
-}
-
 // transverse term of cell c in direction t (dt/6h_t (u+ + u-) (simh+ - simh-))
-__device__ __forceinline__ double tt(const AB& A, const Per& P, long long co, int i, int j, int k,
+template <class PT>
+__device__ __forceinline__ double tt(const AB& A, const PT& P, long long co, int i, int j, int k,
                                      int t) {
     const long long c = P.at(i, j, k);
     const long long c1 = P.nb(i, j, k, t, 1);
-    const double dt6 = A.dt / 6.0;
-    return (dt6 / A.dx[t]) * (A.U[t][c1] + A.U[t][c]) * (A.SIMH[t][co + c1] - A.SIMH[t][co + c]);
+    return A.f6[t] * (A.U[t][c1] + A.U[t][c]) * (A.SIMH[t][co + c1] - A.SIMH[t][co + c]);
 }
 
 // A3: corner-coupled states on d-faces, corrected by t
+template <bool P2>
 __global__ void __launch_bounds__(NTB) k_tape_corner(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     double cor[6];
     for (int d = 0; d < 3; ++d) {
@@ -178,21 +203,22 @@ __global__ void __launch_bounds__(NTB) k_tape_corner(AB A) {
 }
 
 // edge term of cell c for the corner state on t-faces corrected by o
-__device__ __forceinline__ double et(const AB& A, const Per& P, long long co, int i, int j, int k,
+template <class PT>
+__device__ __forceinline__ double et(const AB& A, const PT& P, long long co, int i, int j, int k,
                                      int t, int o) {
     const long long c = P.at(i, j, k);
     const long long c1 = P.nb(i, j, k, t, 1);
-    const double dt4 = 0.25 * A.dt;
     const double* K = A.COR[pidx(t, o)] + co;
-    return (dt4 / A.dx[t]) * (A.U[t][c1] + A.U[t][c]) * (K[c1] - K[c]);
+    return A.f4[t] * (A.U[t][c1] + A.U[t][c]) * (K[c1] - K[c]);
 }
 
 // A4: edge states
+template <bool P2>
 __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     const double dt2 = 0.5 * A.dt;
     double se[3];
@@ -216,10 +242,11 @@ __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
 // ---------------------------------------------------------------- reverse
 // adjoint of the edge state on d-face c through the flux (Fb u) and its
 // upwinding weights
-__device__ __forceinline__ double edge_bar(const AB& A, const Per& P, long long co, int i, int j,
+template <class PT>
+__device__ __forceinline__ double edge_bar(const AB& A, const PT& P, long long co, int i, int j,
                                            int k, int d, double& wl, double& wr) {
     const long long c = P.at(i, j, k);
-    const double Fb = A.dt / A.dx[d] * (A.G[co + c] - A.G[co + P.nb(i, j, k, d, -1)]);
+    const double Fb = A.fdt[d] * (A.G[co + c] - A.G[co + P.nb(i, j, k, d, -1)]);
     const double u = A.U[d][c];
     weights(u, A.rel_eps, wl, wr);
     return Fb * u;
@@ -228,11 +255,12 @@ __device__ __forceinline__ double edge_bar(const AB& A, const Per& P, long long 
 // B1: flux and update, edge-state upwinding -> the per-cell edge-term
 // adjoints (each edge term of cell p enters the left state of d-face p + d
 // and the right state of d-face p, d = 3-t-o)
+template <bool P2>
 __global__ void __launch_bounds__(NTB) k_adj_flux(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     double lb1[3], rb0[3];
 #pragma unroll
@@ -259,14 +287,15 @@ __global__ void __launch_bounds__(NTB) k_adj_flux(AB A) {
 // (ub): each the sum, in this order, of the flux term (edge-state
 // upwinding), the corner-state terms (edge terms, corner upwinding) and the
 // transverse term (predictor upwinding)
-__device__ __forceinline__ void face_totals(const AB& A, const Per& P, long long co, int i, int j,
+template <class PT>
+__device__ __forceinline__ void face_totals(const AB& A, const PT& P, long long co, int i, int j,
                                             int k, int t, bool want_u, double& slb, double& srb,
                                             double& ub) {
     const long long f = P.at(i, j, k);
     const long long fm = P.nb(i, j, k, t, -1), fp = P.nb(i, j, k, t, 1);
     const double u0 = A.U[t][f], um = A.U[t][fm], up = A.U[t][fp];
     // flux
-    const double Fb = A.dt / A.dx[t] * (A.G[co + f] - A.G[co + fm]);
+    const double Fb = A.fdt[t] * (A.G[co + f] - A.G[co + fm]);
     const double seb = Fb * u0;
     double wl, wr;
     weights(u0, A.rel_eps, wl, wr);
@@ -275,7 +304,7 @@ __device__ __forceinline__ void face_totals(const AB& A, const Per& P, long long
     srb = wr * seb;
     // corner states on t-faces, through the edge terms of the cells below
     // and above the face
-    const double b = 0.25 * A.dt / A.dx[t];
+    const double b = A.f4[t];
     double ubc = 0.0, slbc = 0.0, srbc = 0.0;
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
@@ -296,7 +325,7 @@ __device__ __forceinline__ void face_totals(const AB& A, const Per& P, long long
     srb += srbc;
     // upwind state simh, through the transverse terms of the cells below and
     // above the face
-    const double a = A.dt / 6.0 / A.dx[t];
+    const double a = A.f6[t];
     const double tbm = A.TB[t][co + fm], tb0 = A.TB[t][co + f];
     const double hb = tbm * a * (u0 + um) - tb0 * a * (up + u0);
     if (want_u) {
@@ -312,9 +341,10 @@ __device__ __forceinline__ void face_totals(const AB& A, const Per& P, long long
 // adjoint of the corner state q on d-face (i,j,k) (through the edge terms of
 // the cells on both sides of it), with its upwinding weights: the left and
 // right predictor-state adjoints it passes on
-__device__ __forceinline__ void corner_bar(const AB& A, const Per& P, long long co, int i, int j,
+template <class PT>
+__device__ __forceinline__ void corner_bar(const AB& A, const PT& P, long long co, int i, int j,
                                            int k, int d, int q, double& lb, double& rb) {
-    const double b = 0.25 * A.dt / A.dx[d];
+    const double b = A.f4[d];
     const long long c = P.at(i, j, k);
     const long long fm = P.nb(i, j, k, d, -1), fp = P.nb(i, j, k, d, 1);
     const double u0 = A.U[d][c], um = A.U[d][fm], up = A.U[d][fp];
@@ -330,11 +360,12 @@ __device__ __forceinline__ void corner_bar(const AB& A, const Per& P, long long 
 // B4: per-cell transverse-term adjoints: the transverse term of cell p in
 // direction o enters the corner states on the d-faces p (right state) and
 // p + d (left state), d != o
+template <bool P2>
 __global__ void __launch_bounds__(NTB) k_adj_tterm(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     double tb[3];
     for (int o = 0; o < 3; ++o) {
@@ -352,12 +383,33 @@ __global__ void __launch_bounds__(NTB) k_adj_tterm(AB A) {
     for (int o = 0; o < 3; ++o) A.TB[o][co + p] = tb[o];
 }
 
-// B6: characteristic trace and CW limiter of cell p in direction d
+// B3 + B5 stored once per face: the left and right predictor-state adjoints
+// of every d-face (for the trace variant that reads them)
+template <bool P2>
+__global__ void __launch_bounds__(NTB) k_adj_fsum(AB A) {
+    int i, j, k;
+    long long p;
+    if (!point(A, i, j, k, p)) return;
+    const PerT<P2> P = make_per<P2>(A.n);
+    const long long co = (long long)blockIdx.y * A.cs;
+    double l[3], r[3], unused;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) face_totals(A, P, co, i, j, k, d, false, l[d], r[d], unused);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        A.SLB[d][co + p] = l[d];
+        A.SRB[d][co + p] = r[d];
+    }
+}
+
+// B6: characteristic trace and CW limiter of cell p in direction d (FS: the
+// face adjoints are read from k_adj_fsum's arrays instead of recomputed)
+template <bool P2, bool FS>
 __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
     double r0[3], r1[3], r2[3], r3[3], r4[3];
@@ -394,10 +446,16 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
         const double uhi = A.U[d][fhi], ulo = A.U[d][p];
         const double h = A.dx[d];
         const double s6 = 6.0 * s0 - 3.0 * (sm + sp);
-        double Ipb, Imb, unused0, unused1, unused2;
-        face_totals(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb, unused0,
-                    unused1);
-        face_totals(A, P, co, i, j, k, d, false, unused2, Imb, unused1);
+        double Ipb, Imb;
+        if (FS) {
+            Ipb = A.SLB[d][co + fhi];
+            Imb = A.SRB[d][co + p];
+        } else {
+            double unused0, unused1, unused2;
+            face_totals(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb,
+                        unused0, unused1);
+            face_totals(A, P, co, i, j, k, d, false, unused2, Imb, unused1);
+        }
         double smb = 0.0, spb = 0.0, s0b = 0.0, uhib = 0.0, ulob = 0.0;
         if (uhi > A.rel_eps) {
             const double sg = dabs(uhi) * A.dt / h;
@@ -455,7 +513,8 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
 // B7: adjoints through the PPM type 1 face value of d-face (i,j,k) (cells
 // below | above it): of the two cells' values (bsa, bsb) and of their van
 // Leer slopes (bla, blb)
-__device__ __forceinline__ void face_bar(const AB& A, const Per& P, long long co,
+template <class PT>
+__device__ __forceinline__ void face_bar(const AB& A, const PT& P, long long co,
                                          const double* S, int i, int j, int k, int d, double& bsa,
                                          double& bsb, double& bla, double& blb) {
     const long long c = P.at(i, j, k);
@@ -491,7 +550,8 @@ __device__ __forceinline__ void face_bar(const AB& A, const Per& P, long long co
 
 // B8: adjoints of the van Leer slope of cell (i,j,k) along d, whose adjoint
 // is rb, to its three stencil values
-__device__ __forceinline__ void slope_bar(const Per& P, const double* S, int i, int j, int k,
+template <class PT>
+__device__ __forceinline__ void slope_bar(const PT& P, const double* S, int i, int j, int k,
                                           int d, double rb, double& qm, double& q0, double& qp) {
     const double a = S[P.nb(i, j, k, d, -1)], b = S[P.at(i, j, k)], c = S[P.nb(i, j, k, d, 1)];
     const double dsc = 0.5 * (c - a);
@@ -522,11 +582,12 @@ __device__ __forceinline__ void slope_bar(const Per& P, const double* S, int i, 
 }
 
 // B7: face-value adjoints of face p (per direction)
+template <bool P2>
 __global__ void __launch_bounds__(NTB) k_adj_face(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
     double bsa[3], bsb[3], bla[3], blb[3];
@@ -542,11 +603,12 @@ __global__ void __launch_bounds__(NTB) k_adj_face(AB A) {
 }
 
 // B8: van Leer slope adjoints of cell p; its slope feeds faces p and p + d
+template <bool P2>
 __global__ void __launch_bounds__(NTB) k_adj_slope(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
     double qm[3], q0[3], qp[3];
@@ -563,12 +625,43 @@ __global__ void __launch_bounds__(NTB) k_adj_slope(AB A) {
     }
 }
 
+// B7 + B8 + B9 in one gather: the state adjoint of cell p with the face
+// adjoints of faces p - d .. p + 2d and the slope adjoints of cells
+// p - d .. p + d recomputed per direction
+template <bool P2>
+__global__ void __launch_bounds__(NTB) k_adj_state_fused(AB A) {
+    int i, j, k;
+    long long p;
+    if (!point(A, i, j, k, p)) return;
+    const PerT<P2> P = make_per<P2>(A.n);
+    const long long co = (long long)blockIdx.y * A.cs;
+    const double* S = A.S + co;
+    double s = A.G[co + p];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int di = d == 0, dj = d == 1, dk = d == 2;
+        double fsa[4], fsb[4], fla[4], flb[4];  // faces p + (m - 1) d
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            face_bar(A, P, co, S, i + (m - 1) * di, j + (m - 1) * dj, k + (m - 1) * dk, d, fsa[m],
+                     fsb[m], fla[m], flb[m]);
+        double qm[3], q0[3], qp[3];  // cells p + (m - 1) d
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            slope_bar(P, S, i + (m - 1) * di, j + (m - 1) * dj, k + (m - 1) * dk, d,
+                      flb[m] + fla[m + 1], qm[m], q0[m], qp[m]);
+        s += A.S0B[d][co + p] + fsb[1] + fsa[2] + qp[0] + q0[1] + qm[2];
+    }
+    A.SBF(i, j, k, (int)blockIdx.y) = s;
+}
+
 // B9: state adjoint
+template <bool P2>
 __global__ void __launch_bounds__(NTB) k_adj_state(AB A) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     double s = A.G[co + p];
     for (int d = 0; d < 3; ++d) {
@@ -576,16 +669,20 @@ __global__ void __launch_bounds__(NTB) k_adj_state(AB A) {
         s += A.S0B[d][co + p] + A.DSB[d][co + p] + A.DSA[d][co + cp] + A.QP[d][co + cm] +
              A.Q0[d][co + p] + A.QM[d][co + cp];
     }
-    A.SB[co + p] = s;
+    A.SBF(i, j, k, (int)blockIdx.y) = s;
 }
 
 // B10: velocity adjoint per face, summed over components in order
-__global__ void __launch_bounds__(NTB) k_adj_vel(AB A, double* ub0, double* ub1, double* ub2) {
+// into the caller's face FABs; the duplicate face n of a periodic face FAB
+// carries no separate value and is set to zero
+template <bool P2>
+__global__ void __launch_bounds__(NTB) k_adj_vel(AB A, DA ub0, DA ub1, DA ub2, int has_n) {
     int i, j, k;
     long long p;
     if (!point(A, i, j, k, p)) return;
-    const Per P{A.n};
+    const PerT<P2> P = make_per<P2>(A.n);
     double r[3];
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
         const long long cm = P.nb(i, j, k, d, -1);
         double s = 0.0;
@@ -597,9 +694,13 @@ __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, double* ub0, double* ub1,
         }
         r[d] = s;
     }
-    ub0[p] = r[0];
-    ub1[p] = r[1];
-    ub2[p] = r[2];
+    ub0(i, j, k) = r[0];
+    ub1(i, j, k) = r[1];
+    ub2(i, j, k) = r[2];
+    const int n = A.n;
+    if (i == 0 && (has_n & 1)) ub0(n, j, k) = 0.0;
+    if (j == 0 && (has_n & 2)) ub1(i, n, k) = 0.0;
+    if (k == 0 && (has_n & 4)) ub2(i, j, n) = 0.0;
 }
 
 // ---------------------------------------------------------------- glue
@@ -628,7 +729,7 @@ __global__ void __launch_bounds__(NTB) k_periodic_fill(const FillTask* __restric
     for (int c = 0; c < t.ncomp; ++c) t.a(i, j, k, c) = t.a(s[0], s[1], s[2], c);
 }
 
-// dense copies between FABs and N^3 arrays
+// dense copy of a FAB's valid cells (or faces 0..N-1) into an N^3 array
 __global__ void __launch_bounds__(NTB) k_to_dense(DA a, int comp0, int ncomp, int n, double* out) {
     const long long n3 = (long long)n * n * n;
     const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -638,24 +739,6 @@ __global__ void __launch_bounds__(NTB) k_to_dense(DA a, int comp0, int ncomp, in
     const int i = (int)(q - r * (unsigned)n), k = (int)(r / (unsigned)n);
     const int j = (int)(r - (unsigned)k * (unsigned)n);
     for (int c = 0; c < ncomp; ++c) out[(long long)c * n3 + p] = a(i, j, k, comp0 + c);
-}
-
-__global__ void __launch_bounds__(NTB) k_from_dense(const double* in, int ncomp, int n, DA a,
-                                                    int nodal) {
-    const long long n3 = (long long)n * n * n;
-    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (unsigned)n3) return;
-    const long long p = q;
-    const unsigned r = q / (unsigned)n;
-    const int i = (int)(q - r * (unsigned)n), k = (int)(r / (unsigned)n);
-    const int j = (int)(r - (unsigned)k * (unsigned)n);
-    for (int c = 0; c < ncomp; ++c) {
-        a(i, j, k, c) = in[(long long)c * n3 + p];
-        // the duplicate face n of a periodic face FAB carries no separate value
-        if (nodal == 0 && i == 0) a(n, j, k, c) = 0.0;
-        if (nodal == 1 && j == 0) a(i, n, k, c) = 0.0;
-        if (nodal == 2 && k == 0) a(i, j, n, c) = 0.0;
-    }
 }
 
 // the update snew = s + dt (-div(u sedge) + 0)
@@ -689,7 +772,10 @@ __global__ void __launch_bounds__(NTB) k_fill_zero(double* a, long long n) {
 }
 
 // the step's preconditions: one periodic box [0, n)^3 at level 0
-int step_check(const mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp, int& n) {
+// ghosts: the step reads s and umac ghosts (>= 3 cells, >= 1 face); the
+// adjoint reads only valid cells and faces 0..n-1
+int step_check(const mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp, int& n,
+               bool ghosts = true) {
     const mx_adv_params& p = c->p;
     if (p.dim != 3 || p.ppm_type != 1 || p.ppm_trace_forces != 0) return MX_ERR_UNSUPPORTED;
     if (!p.is_periodic[0] || !p.is_periodic[1] || !p.is_periodic[2]) return MX_ERR_UNSUPPORTED;
@@ -704,10 +790,17 @@ int step_check(const mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp,
     if ((long long)n * n * n * ncomp > 0x7fffffffLL) return MX_ERR_UNSUPPORTED;
     if (!s || !u) return MX_ERR_ARG;
     HBox v = hbox(b);
-    if (!covers(*s, grow(v, 3, 3), ncomp)) return MX_ERR_BOX;
+    if (!covers(*s, ghosts ? grow(v, 3, 3) : v, ncomp)) return MX_ERR_BOX;
     for (int d = 0; d < 3; ++d)
-        if (!covers(u[d], grow(nodal(v, d), 3, 1), 1)) return MX_ERR_BOX;
+        if (!covers(u[d], ghosts ? grow(nodal(v, d), 3, 1) : v, 1)) return MX_ERR_BOX;
     return MX_OK;
+}
+
+// is the FAB exactly a dense (ncomp, n, n, n) array starting at cell 0?
+bool dense_fab(const mx_fab& f, int n, int ncomp) {
+    const long long n2 = (long long)n * n;
+    return f.begin[0] == 0 && f.begin[1] == 0 && f.begin[2] == 0 && f.jstride == n &&
+           f.kstride == n2 && (ncomp == 1 || f.nstride == n2 * n);
 }
 
 dim3 grid1(long long npts) { return dim3((unsigned)((npts + NTB - 1) / NTB)); }
@@ -737,6 +830,38 @@ int periodic_fill(mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp, in
     if (rc) return rc;
     dim3 gr = grid_for(maxpts, (int)tasks.size(), 1, NTB);
     MX_LAUNCH(c, "k_periodic_fill", k_periodic_fill, gr, dim3(NTB), tab.dev<FillTask>(off), n);
+    return MX_OK;
+}
+
+// the tape and reverse kernels of the step adjoint
+template <bool P2>
+int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, mx_fab* ubar) {
+    MX_LAUNCH(c, "k_tape_ppm", k_tape_ppm<P2>, gc, dim3(NTB), A);
+    MX_LAUNCH(c, "k_tape_corner", k_tape_corner<P2>, gc, dim3(NTB), A);
+    MX_LAUNCH(c, "k_tape_edge", k_tape_edge<P2>, gc, dim3(NTB), A);
+    MX_LAUNCH(c, "k_adj_flux", k_adj_flux<P2>, gc, dim3(NTB), A);
+    // the transverse-term and trace gathers run faster with compare
+    // indexing on MI355X (measured), whatever n is
+    MX_LAUNCH(c, "k_adj_tterm", k_adj_tterm<false>, gc, dim3(NTB), A);
+    if (c->adj_face_sums) {
+        MX_LAUNCH(c, "k_adj_fsum", k_adj_fsum<P2>, gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, true>), gc, dim3(NTB), A);
+    } else {
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, false>), gc, dim3(NTB), A);
+    }
+    if (c->adj_state_fused) {
+        MX_LAUNCH(c, "k_adj_state_fused", k_adj_state_fused<P2>, gc, dim3(NTB), A);
+    } else {
+        MX_LAUNCH(c, "k_adj_face", k_adj_face<P2>, gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_slope", k_adj_slope<P2>, gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_state", k_adj_state<P2>, gc, dim3(NTB), A);
+    }
+    // velocity adjoint, into the caller's face FABs
+    int has_n = 0;
+    for (int d = 0; d < 3; ++d)
+        if (ubar[d].end[d] > A.n) has_n |= 1 << d;
+    MX_LAUNCH(c, "k_adj_vel", k_adj_vel<P2>, g1, dim3(NTB), A, to_da(ubar[0]), to_da(ubar[1]),
+              to_da(ubar[2]), has_n);
     return MX_OK;
 }
 
@@ -804,32 +929,38 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
                          mx_fab* sbar, mx_fab* ubar, int ncomp) {
     if (!c || !s || !umac || !gbar || !sbar || !ubar) return MX_ERR_ARG;
     int n = 0;
-    int rc = step_check(c, s, umac, ncomp, n);
+    int rc = step_check(c, s, umac, ncomp, n, false);
     if (rc) return rc;
     HBox v = hbox(c->grids[0][0]);
     if (!covers(*gbar, v, ncomp) || !covers(*sbar, v, ncomp)) return MX_ERR_BOX;
     for (int d = 0; d < 3; ++d)
-        if (!covers(ubar[d], nodal(v, d), 1)) return MX_ERR_BOX;
-    if ((rc = periodic_fill(c, s, umac, ncomp, n))) return rc;
+        if (!covers(ubar[d], v, 1)) return MX_ERR_BOX;
     const long long n3 = (long long)n * n * n;
-    // dense arrays: S, G, SB (ncomp each), 63 per-component intermediates
-    // and adjoints, U and the summed velocity adjoint (3 each)
+    // dense arrays: S and G (ncomp each), 63 per-component intermediates and
+    // adjoints, U (3)
     const size_t per = (size_t)n3 * sizeof(double) + 256;
-    const size_t count = (size_t)ncomp * 66 + 6;
+    const size_t count = (size_t)ncomp * (65 + (c->adj_face_sums ? 6 : 0)) + 3;
     if ((rc = ws_reserve(c, per * count + 4096))) return rc;
     AB A;
     std::memset(&A, 0, sizeof(A));
     auto take = [&](int m) { return ws_take(c, (size_t)n3 * m); };
-    double* S = take(ncomp);
-    double* G = take(ncomp);
-    A.SB = take(ncomp);
-    for (int d = 0; d < 3; ++d) A.U[d] = take(1);
+    // inputs that are already dense (ncomp, n, n, n) arrays are read in place
+    const bool sd = dense_fab(*s, n, ncomp), gd = dense_fab(*gbar, n, ncomp);
+    bool ud[3];
+    for (int d = 0; d < 3; ++d) ud[d] = dense_fab(umac[d], n, 1);
+    double* S = sd ? s->p : take(ncomp);
+    double* G = gd ? gbar->p : take(ncomp);
+    for (int d = 0; d < 3; ++d) A.U[d] = ud[d] ? umac[d].p : take(1);
     for (int d = 0; d < 3; ++d) {
         A.IP[d] = take(ncomp);
         A.IM[d] = take(ncomp);
         A.SIMH[d] = take(ncomp);
         A.SE[d] = take(ncomp);
         A.TB[d] = take(ncomp);
+        if (c->adj_face_sums) {
+            A.SLB[d] = take(ncomp);
+            A.SRB[d] = take(ncomp);
+        }
         A.S0B[d] = take(ncomp);
         A.FLB[d] = take(ncomp);
         A.FRB[d] = take(ncomp);
@@ -849,38 +980,31 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
     }
     A.S = S;
     A.G = G;
+    A.SBF = to_da(*sbar);
     A.cs = n3;
     A.n = n;
     A.ncomp = ncomp;
     A.dt = c->p.dt;
     A.rel_eps = c->p.rel_eps;
-    for (int d = 0; d < 3; ++d) A.dx[d] = c->p.dx[0][d];
+    for (int d = 0; d < 3; ++d) {
+        A.dx[d] = c->p.dx[0][d];
+        A.fdt[d] = A.dt / A.dx[d];
+        const double dt4 = 0.25 * A.dt, dt6 = A.dt / 6.0;
+        A.f4[d] = dt4 / A.dx[d];
+        A.f6[d] = dt6 / A.dx[d];
+    }
 
     const dim3 g1 = grid1(n3);
     const dim3 gc(g1.x, (unsigned)ncomp);
-    MX_LAUNCH(c, "k_to_dense", k_to_dense, g1, dim3(NTB), to_da(*s), 0, ncomp, n, S);
-    MX_LAUNCH(c, "k_to_dense", k_to_dense, g1, dim3(NTB), to_da(*gbar), 0, ncomp, n, G);
+    if (!sd) MX_LAUNCH(c, "k_to_dense", k_to_dense, g1, dim3(NTB), to_da(*s), 0, ncomp, n, S);
+    if (!gd)
+        MX_LAUNCH(c, "k_to_dense", k_to_dense, g1, dim3(NTB), to_da(*gbar), 0, ncomp, n, G);
     for (int d = 0; d < 3; ++d)
-        MX_LAUNCH(c, "k_to_dense", k_to_dense, g1, dim3(NTB), to_da(umac[d]), 0, 1, n,
-                  const_cast<double*>(A.U[d]));
-    MX_LAUNCH(c, "k_tape_ppm", k_tape_ppm, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_tape_simh", k_tape_simh, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_tape_corner", k_tape_corner, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_tape_edge", k_tape_edge, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_flux", k_adj_flux, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_tterm", k_adj_tterm, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_trace", k_adj_trace, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_face", k_adj_face, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_slope", k_adj_slope, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_state", k_adj_state, gc, dim3(NTB), A);
-    // velocity adjoint (dense, then into the caller's face FABs)
-    double* UBD = take(3);
-    MX_LAUNCH(c, "k_adj_vel", k_adj_vel, g1, dim3(NTB), A, UBD, UBD + n3, UBD + 2 * n3);
-    MX_LAUNCH(c, "k_from_dense", k_from_dense, g1, dim3(NTB), A.SB, ncomp, n, to_da(*sbar), -1);
-    for (int d = 0; d < 3; ++d)
-        MX_LAUNCH(c, "k_from_dense", k_from_dense, g1, dim3(NTB), UBD + d * n3, 1, n,
-                  to_da(ubar[d]), d);
-    return MX_OK;
+        if (!ud[d])
+            MX_LAUNCH(c, "k_to_dense", k_to_dense, g1, dim3(NTB), to_da(umac[d]), 0, 1, n,
+                      const_cast<double*>(A.U[d]));
+    const bool p2 = c->adj_pow2 && (n & (n - 1)) == 0;
+    return p2 ? vjp_kernels<true>(c, A, g1, gc, ubar) : vjp_kernels<false>(c, A, g1, gc, ubar);
 }
 
 }  // extern "C"

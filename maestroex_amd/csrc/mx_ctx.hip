@@ -811,6 +811,15 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
     } else if (k == "edge_scal_per_box") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->fused_per_box = (int)value;
+    } else if (k == "adjoint_state_fused") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->adj_state_fused = (int)value;
+    } else if (k == "adjoint_face_sums") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->adj_face_sums = (int)value;
+    } else if (k == "adjoint_pow2_index") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->adj_pow2 = (int)value;
     } else {
         return MX_ERR_ARG;
     }

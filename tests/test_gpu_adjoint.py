@@ -54,8 +54,9 @@ def test_step_forward_parity():
     assert np.array_equal(g, o), f"{int((g != o).sum())} values differ"
 
 
-def test_vjp_dot_product():
-    s, u, dt, eps = setup_case(seed=3)
+@pytest.mark.parametrize("n", [16, 12])
+def test_vjp_dot_product(n):
+    s, u, dt, eps = setup_case(n=n, seed=3)
     n, nc = s.shape[1], s.shape[0]
     print("[parity] scal_step vjp dot product", flush=True)
     rng = np.random.default_rng(11)
@@ -74,6 +75,39 @@ def test_vjp_dot_product():
     # the state part alone (w in the velocity zero)
     _, jws = pyoracle.scal_step(s, u, dt, [1.0 / n] * 3, eps, ws, None)
     assert abs(float(np.sum(v * jws)) - float(np.sum(sb * ws))) <= 1e-12 * abs(float(np.sum(v * jws)))
+
+
+VJP_VARIANTS = [
+    {"adjoint_state_fused": 0},
+    {"adjoint_face_sums": 1},
+    {"adjoint_pow2_index": 0},
+    {"adjoint_state_fused": 0, "adjoint_face_sums": 1, "adjoint_pow2_index": 0},
+]
+VJP_DEFAULTS = {"adjoint_state_fused": 1, "adjoint_face_sums": 0, "adjoint_pow2_index": 1}
+
+
+@pytest.mark.parametrize("variant", VJP_VARIANTS)
+def test_vjp_kernel_variants_are_identical(variant):
+    """The adjoint's kernel arrangements (one- or three-kernel state gather,
+    face adjoints stored or recomputed, mask or compare indexing) evaluate
+    the same expressions: equal bit for bit."""
+    s, u, dt, eps = setup_case(seed=5)
+    n, nc = s.shape[1], s.shape[0]
+    print(f"[parity] scal_step vjp variant {variant}", flush=True)
+    v = np.random.default_rng(12).standard_normal(s.shape)
+    st = stepper_for(n, nc, dt, eps)
+    args = (to_dev(s), tuple(to_dev(x) for x in u), to_dev(v))
+    sb0, ub0 = st.vjp(*args)
+    for key, val in variant.items():
+        st.m.set_tuning(key, val)
+    try:
+        sb1, ub1 = st.vjp(*args)
+    finally:
+        for key, val in VJP_DEFAULTS.items():
+            st.m.set_tuning(key, val)
+    assert np.array_equal(sb0.cpu().numpy(), sb1.cpu().numpy())
+    for d in range(3):
+        assert np.array_equal(ub0[d].cpu().numpy(), ub1[d].cpu().numpy())
 
 
 def test_autograd_function_matches_vjp():
