@@ -78,6 +78,7 @@ struct AB {
     double *IP[3], *IM[3], *SIMH[3], *COR[6], *SE[3];
     double *EB[6], *TB[3];
     double *SLB[3], *SRB[3];  // face sums (k_adj_fsum), when used
+    double* UBC[3];           // per-component face velocity adjoints (k_adj_trace)
     double *S0B[3], *FLB[3], *FRB[3], *UHI[3], *ULO[3];
     double *DSA[3], *DSB[3], *DLA[3], *DLB[3], *QM[3], *Q0[3], *QP[3];
     DA SBF;            // state adjoint: the caller's FAB
@@ -412,7 +413,7 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
     const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
-    double r0[3], r1[3], r2[3], r3[3], r4[3];
+    double r0[3], r1[3], r2[3], r3[3], r4[3], ubd[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < 3; ++d) {
         double v[9];
         for (int o = -4; o <= 4; ++o) v[4 + o] = (o >= -2 && o <= 2) ? S[P.nb(i, j, k, d, o)] : 0.0;
@@ -454,7 +455,8 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
             double unused0, unused1, unused2;
             face_totals(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb,
                         unused0, unused1);
-            face_totals(A, P, co, i, j, k, d, false, unused2, Imb, unused1);
+            // face p's totals also give its velocity adjoint for this component
+            face_totals(A, P, co, i, j, k, d, true, unused2, Imb, ubd[d]);
         }
         double smb = 0.0, spb = 0.0, s0b = 0.0, uhib = 0.0, ulob = 0.0;
         if (uhi > A.rel_eps) {
@@ -507,6 +509,7 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
         A.FRB[d][co + p] = r2[d];  // face above the cell
         A.UHI[d][co + p] = r3[d];
         A.ULO[d][co + p] = r4[d];
+        if (!FS) A.UBC[d][co + p] = ubd[d];
     }
 }
 
@@ -675,7 +678,7 @@ __global__ void __launch_bounds__(NTB) k_adj_state(AB A) {
 // B10: velocity adjoint per face, summed over components in order
 // into the caller's face FABs; the duplicate face n of a periodic face FAB
 // carries no separate value and is set to zero
-template <bool P2>
+template <bool P2, bool STORED>
 __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, DA ub0, DA ub1, DA ub2, int has_n) {
     int i, j, k;
     long long p;
@@ -688,8 +691,13 @@ __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, DA ub0, DA ub1, DA ub2, i
         double s = 0.0;
         for (int c = 0; c < A.ncomp; ++c) {
             const long long co = (long long)c * A.cs;
-            double slb, srb, ub;
-            face_totals(A, P, co, i, j, k, d, true, slb, srb, ub);
+            double ub;
+            if (STORED) {
+                ub = A.UBC[d][co + p];
+            } else {
+                double slb, srb;
+                face_totals(A, P, co, i, j, k, d, true, slb, srb, ub);
+            }
             s += ub + A.UHI[d][co + cm] + A.ULO[d][co + p];
         }
         r[d] = s;
@@ -860,8 +868,12 @@ int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, mx_fab* ubar) {
     int has_n = 0;
     for (int d = 0; d < 3; ++d)
         if (ubar[d].end[d] > A.n) has_n |= 1 << d;
-    MX_LAUNCH(c, "k_adj_vel", k_adj_vel<P2>, g1, dim3(NTB), A, to_da(ubar[0]), to_da(ubar[1]),
-              to_da(ubar[2]), has_n);
+    if (c->adj_face_sums)
+        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<P2, false>), g1, dim3(NTB), A, to_da(ubar[0]),
+                  to_da(ubar[1]), to_da(ubar[2]), has_n);
+    else
+        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<P2, true>), g1, dim3(NTB), A, to_da(ubar[0]),
+                  to_da(ubar[1]), to_da(ubar[2]), has_n);
     return MX_OK;
 }
 
@@ -939,7 +951,7 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
     // dense arrays: S and G (ncomp each), 63 per-component intermediates and
     // adjoints, U (3)
     const size_t per = (size_t)n3 * sizeof(double) + 256;
-    const size_t count = (size_t)ncomp * (65 + (c->adj_face_sums ? 6 : 0)) + 3;
+    const size_t count = (size_t)ncomp * (65 + 3 + (c->adj_face_sums ? 6 : 0)) + 3;
     if ((rc = ws_reserve(c, per * count + 4096))) return rc;
     AB A;
     std::memset(&A, 0, sizeof(A));
@@ -957,6 +969,7 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
         A.SIMH[d] = take(ncomp);
         A.SE[d] = take(ncomp);
         A.TB[d] = take(ncomp);
+        A.UBC[d] = take(ncomp);
         if (c->adj_face_sums) {
             A.SLB[d] = take(ncomp);
             A.SRB[d] = take(ncomp);
