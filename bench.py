@@ -27,6 +27,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Gcell-updates/s per FP64 advect step, % HBM peak; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
+# MI355X FP64 vector rate without FMA (1024 SIMDs x 16 lanes x 2.4 GHz): the
+# op count below counts an FMA-able pair as two operations
+FP64_PEAK_TOPS = 39.3
+# FP64 operations per cell and component of MakeEdgeScal (3-D, advective,
+# ppm_type 1) in the formulation with neighbour-shared slopes and face values:
+# the oracle's op-counting instantiation of the step on smooth data gives 439
+# per cell in the per-point form (PPM and trace 277, predictor 9, corner 78,
+# edge 75), less 2 van Leer slopes (13.5 on average) and 1 face value (9) per
+# cell and direction shared with the neighbours; tests/test_oracle.py
+# (test_step_op_count) recomputes it
+EDGE_OPS_PER_CELL_COMP = 331
 
 # Algorithmic HBM bytes per valid cell of one pass (FP64, valid region only,
 # compulsory traffic across the reference's stage boundaries): 3-D, nc = 5.
@@ -123,6 +134,29 @@ def cpu_baseline(args):
                   f"median of {args.cpu_reps} passes after 1 warm-up, OpenMP {cores} threads, "
                   f"g++ -O3 -fopenmp -ffp-contract=off; {med:.2f} s per pass",
     }
+
+
+def copy_bandwidth_gbs(mib=2048, reps=5):
+    """Measured device-to-device copy bandwidth (read + write bytes / time)
+    of a plain torch copy of `mib` MiB, the best of `reps` copies: the
+    achievable HBM rate the kernels are also compared with."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        torch.cuda.synchronize()
+        best = min(best, a.elapsed_time(b))
+    del src, dst
+    return 2.0 * n * 8 / (best * 1e-3) / 1e9
 
 
 def time_ms(fn, steps, warmup):
@@ -400,6 +434,20 @@ def main():
     breakdown = {k: round(v[0] / args.steps, 4) for k, v in sorted(kernels.items(),
                                                                     key=lambda kv: -kv[1][0])}
     print(f"[bench] per-pass kernel ms: {breakdown}", file=sys.stderr, flush=True)
+
+    if dname == "k_es_fused":
+        # the kernel's FP64 operation rate against the VALU rate (the
+        # co-binding roofline)
+        ops = EDGE_OPS_PER_CELL_COMP * nc * cells
+        tops = ops / (avg_ms * 1e-3) / 1e12
+        roofline["fp64_ops_per_launch"] = int(ops)
+        roofline["fp64_tops"] = round(tops, 3)
+        roofline["fp64_peak_tops"] = FP64_PEAK_TOPS
+        roofline["fp64_frac"] = round(tops / FP64_PEAK_TOPS, 4)
+    copy_gbs = copy_bandwidth_gbs()
+    roofline["measured_copy_gbs"] = round(copy_gbs, 1)
+    roofline["frac_of_measured_copy"] = round(achieved / copy_gbs, 4) if achieved else None
+    print(f"[bench] device copy {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

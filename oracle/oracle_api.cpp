@@ -64,6 +64,8 @@ int MakeEdgeState1d(const Ctx* c, const mx_radial_geom* rg, const mx_basestate* 
 void ScalStep(int n, int ncomp, double dt, const double dx[3], double rel_eps, const double* s,
               const double* const u[3], double* snew, const double* ds,
               const double* const du[3], double* dsnew);
+void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const double* s,
+                 const double* const u[3], double out[7]);
 
 }  // namespace mxo
 
@@ -332,6 +334,16 @@ int mxo_scal_step_jvp(int n, int ncomp, double dt, const double* dx, double rel_
     const double* u[3] = {u0, u1, u2};
     const double* du[3] = {du0, du1, du2};
     ScalStep(n, ncomp, dt, dx, rel_eps, s, u, snew, ds, du, dsnew);
+    return MX_OK;
+}
+
+// FP64 operation counts per cell of the step of one component (see
+// StepOpCount in oracle_step.cpp).
+int mxo_step_op_count(int n, double dt, const double* dx, double rel_eps, const double* s,
+                      const double* u0, const double* u1, const double* u2, double* out) {
+    if (n < 4 || !dx || !s || !u0 || !u1 || !u2 || !out) return MX_ERR_ARG;
+    const double* u[3] = {u0, u1, u2};
+    StepOpCount(n, dt, dx, rel_eps, s, u, out);
     return MX_OK;
 }
 

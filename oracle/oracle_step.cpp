@@ -40,8 +40,39 @@ inline Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
 inline Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
 inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
 
+// Op-counting number (for the algorithmic FP64 operation count of the step):
+// every +, -, *, / on it, every comparison of its value, |x| and copysign
+// counts one operation; negation counts none.  val() of it is a proxy whose
+// comparisons count one each.
+struct Cnt {
+    double v;
+};
+long long g_ops = 0;
+struct CmpD {
+    double v;
+    operator double() const { return v; }
+};
+inline bool operator<(CmpD a, CmpD b) { ++g_ops; return a.v < b.v; }
+inline bool operator<(CmpD a, double b) { ++g_ops; return a.v < b; }
+inline bool operator>(CmpD a, double b) { ++g_ops; return a.v > b; }
+inline bool operator<=(CmpD a, double b) { ++g_ops; return a.v <= b; }
+inline bool operator>=(CmpD a, CmpD b) { ++g_ops; return a.v >= b.v; }
+inline Cnt operator+(Cnt a, Cnt b) { ++g_ops; return {a.v + b.v}; }
+inline Cnt operator-(Cnt a, Cnt b) { ++g_ops; return {a.v - b.v}; }
+inline Cnt operator*(Cnt a, Cnt b) { ++g_ops; return {a.v * b.v}; }
+inline Cnt operator/(Cnt a, Cnt b) { ++g_ops; return {a.v / b.v}; }
+inline Cnt operator-(Cnt a) { return {-a.v}; }
+inline Cnt operator*(double a, Cnt b) { ++g_ops; return {a * b.v}; }
+inline Cnt operator*(Cnt a, double b) { ++g_ops; return {a.v * b}; }
+inline Cnt operator+(Cnt a, double b) { ++g_ops; return {a.v + b}; }
+inline Cnt operator+(double a, Cnt b) { ++g_ops; return {a + b.v}; }
+inline Cnt operator-(double a, Cnt b) { ++g_ops; return {a - b.v}; }
+inline Cnt operator-(Cnt a, double b) { ++g_ops; return {a.v - b}; }
+inline Cnt operator/(Cnt a, double b) { ++g_ops; return {a.v / b}; }
+
 inline double val(double x) { return x; }
 inline double val(Dual x) { return x.v; }
+inline CmpD val(Cnt x) { return {x.v}; }
 inline double mk(double x, double) { return x; }
 template <class T>
 inline T cst(double x);
@@ -49,6 +80,8 @@ template <>
 inline double cst<double>(double x) { return x; }
 template <>
 inline Dual cst<Dual>(double x) { return {x, 0.0}; }
+template <>
+inline Cnt cst<Cnt>(double x) { return {x}; }
 
 // amrex::min / max (the first argument wins ties), |x|, copysign(1, x)
 template <class T>
@@ -57,8 +90,16 @@ template <class T>
 inline T tmax(T a, T b) { return (val(a) < val(b)) ? b : a; }
 inline double tabs(double a) { return std::fabs(a); }
 inline Dual tabs(Dual a) { return std::signbit(a.v) ? Dual{-a.v, -a.d} : a; }
+inline Cnt tabs(Cnt a) {
+    ++g_ops;
+    return {std::fabs(a.v)};
+}
 template <class T>
 inline double sgn1(T a) { return std::copysign(1.0, val(a)); }
+inline double sgn1(Cnt a) {
+    ++g_ops;
+    return std::copysign(1.0, a.v);
+}
 
 template <class T>
 T vl(T a, T b, T c) {  // van Leer slope (MaestroPPM.cpp:46-57)
@@ -130,9 +171,10 @@ inline void step_e(int d, int& a, int& b, int& c) {
 }
 
 // one step of one component; s, u[3] and the output are N^3 arrays
+// marks (op counting only): the counter after each of the five phases
 template <class T>
 void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double dt,
-               const double dx[3], double rel_eps) {
+               const double dx[3], double rel_eps, long long* marks = nullptr) {
     const int n = g.n;
     const long long N3 = (long long)n * n * n;
     std::vector<T> Ip[3], Im[3], simh[3], corner[6], se[3];
@@ -159,6 +201,7 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
                     trace(v[2], sm, sp, u[d][g.at(i + a, j + b, k + c)], u[d][g.at(i, j, k)], dt,
                           dx[d], rel_eps, Ip[d][g.at(i, j, k)], Im[d][g.at(i, j, k)]);
                 }
+    if (marks) marks[0] = g_ops;
     // predictor and upwind states on faces (3-D threshold 0)
     for (int k = 0; k < n; ++k)
         for (int j = 0; j < n; ++j)
@@ -169,6 +212,7 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
                     const long long f = g.at(i, j, k);
                     simh[d][f] = upwind(Ip[d][g.at(i - a, j - b, k - c)], Im[d][f], u[d][f], 0.0);
                 }
+    if (marks) marks[1] = g_ops;
     // corner-coupled states simh_dt on d-faces, corrected by t
     const double dt6 = dt / 6.0;
     for (int k = 0; k < n; ++k)
@@ -191,6 +235,7 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
                         rr = rr - (dt6 / ht) * (u[t][R1] + u[t][f]) * (simh[t][R1] - simh[t][f]);
                         corner[pair_index(d, t)][f] = upwind(l, rr, u[d][f], rel_eps);
                     }
+    if (marks) marks[2] = g_ops;
     // edge states: d-face terms in the order of the reference (x: y, z;
     // y: x, z; z: x, y), force zero
     const double dt4 = 0.25 * dt;
@@ -221,6 +266,7 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
                     T sedger = rr + dt2 * 0.0;
                     se[d][f] = upwind(sedgel, sedger, u[d][f], rel_eps);
                 }
+    if (marks) marks[3] = g_ops;
     // fluxes and the update
     for (int k = 0; k < n; ++k)
         for (int j = 0; j < n; ++j)
@@ -237,9 +283,60 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
                 divterm = divterm + (fz1 - fz0) / dx[2];
                 snew[cc] = s[cc] + dt * (-divterm + 0.0);
             }
+    if (marks) marks[4] = g_ops;
 }
 
 }  // namespace
+
+// FP64 operations per cell of one component of the step, in the per-point
+// formulation of the reference (every cell evaluates its own three van Leer
+// slopes and two face values per direction), on the data given:
+// out[0..4] the phases PPM and trace, predictor upwinding, corner-coupled
+// states, edge states, flux and update; out[5] one van Leer slope and out[6]
+// one PPM type 1 face value (limits included), averaged over the calls.
+void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const double* s,
+                 const double* const u[3], double out[7]) {
+    const Grid g{n};
+    const long long N3 = (long long)n * n * n;
+    std::vector<Cnt> sc(N3), out_c(N3), uc[3];
+    const Cnt* up[3];
+    for (long long x = 0; x < N3; ++x) sc[x] = {s[x]};
+    for (int d = 0; d < 3; ++d) {
+        uc[d].resize(N3);
+        for (long long x = 0; x < N3; ++x) uc[d][x] = {u[d][x]};
+        up[d] = uc[d].data();
+    }
+    long long marks[5];
+    g_ops = 0;
+    step_comp<Cnt>(g, sc.data(), up, out_c.data(), dt, dx, rel_eps, marks);
+    long long prev = 0;
+    for (int q = 0; q < 5; ++q) {
+        out[q] = (double)(marks[q] - prev) / (double)N3;
+        prev = marks[q];
+    }
+    // one slope and one face value, over every cell and direction
+    long long slopes = 0, faces = 0;
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i)
+                for (int d = 0; d < 3; ++d) {
+                    int a, b, c;
+                    step_e(d, a, b, c);
+                    Cnt v[4];
+                    for (int o = -2; o <= 1; ++o) v[o + 2] = sc[g.at(i + o * a, j + o * b, k + o * c)];
+                    g_ops = 0;
+                    const Cnt dl = vl(v[0], v[1], v[2]);
+                    slopes += g_ops;
+                    const Cnt dr = vl(v[1], v[2], v[3]);
+                    g_ops = 0;
+                    Cnt e = 0.5 * (v[2] + v[1]) - (dr - dl) / 6.0;
+                    e = tmax(e, tmin(v[2], v[1]));
+                    e = tmin(e, tmax(v[2], v[1]));
+                    faces += g_ops;
+                }
+    out[5] = (double)slopes / (double)(3 * N3);
+    out[6] = (double)faces / (double)(3 * N3);
+}
 
 // the step (T = double) or its directional derivative (ds, du != null):
 // s, snew: ncomp x N^3; u[d]: N^3 (face N is face 0)

@@ -56,6 +56,26 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def step_op_count(s, u, dt, dx, rel_eps):
+    """FP64 operations per cell of the differentiable step for one component
+    s (N, N, N), counted by the oracle's op-counting instantiation: a dict of
+    the phases (ppm, predictor, corner, edge, flux_update) and the cost of one
+    van Leer slope and one PPM type 1 face value."""
+    import numpy as np
+    lib, _ = load(False)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    u = [np.ascontiguousarray(x, dtype=np.float64) for x in u]
+    out = np.zeros(7)
+    f = lib.mxo_step_op_count
+    f.restype = C.c_int
+    rc = f(s.shape[0], C.c_double(dt), (C.c_double * 3)(*dx), C.c_double(rel_eps), _dptr(s),
+           *[_dptr(x) for x in u], _dptr(out))
+    if rc:
+        raise _abi.MxError(rc, "oracle.step_op_count")
+    keys = ["ppm", "predictor", "corner", "edge", "flux_update", "slope", "face"]
+    return dict(zip(keys, out.tolist()))
+
+
 def scal_step(s, u, dt, dx, rel_eps, ds=None, du=None):
     """The differentiable scalar step on a periodic N^3 box (oracle_step.cpp).
     s: (ncomp, N, N, N) indexed [c, k, j, i]; u: three (N, N, N) face arrays

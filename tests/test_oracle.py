@@ -574,3 +574,36 @@ def test_scal_step_conserves_and_keeps_constants():
     c = np.full((1, n, n, n), 0.7)
     outc = pyoracle.scal_step(c, u, 0.5 / n, [1.0 / n] * 3, 1e-8)
     assert np.max(np.abs(outc - 0.7)) <= 1e-14
+
+
+def test_step_op_count():
+    """The op-counting instantiation of the oracle's step: the phase counts
+    are fixed by the formulas (predictor 9, corner 78, edge 75, flux and
+    update 17 per cell; one face value 13), a constant field takes the
+    shortest slope branch everywhere, and the non-redundant MakeEdgeScal
+    figure that bench.py reports against the VALU rate follows from the
+    counts on a smooth field."""
+    import bench
+    from maestroex_amd.adjoint import periodic_mac_velocity
+    n = 16
+    u = periodic_mac_velocity(n, seed=1)
+    x = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    smooth = np.exp(-((X - 0.5) ** 2 + (Y - 0.3) ** 2 + (Z - 0.5) ** 2) / 0.02)
+    r = pyoracle.step_op_count(smooth, u, 0.5 / n, [1.0 / n] * 3, 1e-8)
+    # per cell and direction: simh is one upwind (u > 0, |u|, |u| > 0: 3);
+    # a corner state 2 x (sum, product, difference, product, difference) + 3;
+    # an edge state 2 x 2 x 5 + 2 + 3; the flux update 6 products, 3
+    # differences, 3 quotients, 2 sums and 3 for the update; a face value
+    # 5 + 2 min + 2 max (the dt/h factors are constants, not counted)
+    assert (r["predictor"], r["corner"], r["edge"], r["flux_update"], r["face"]) == (
+        9.0, 78.0, 75.0, 17.0, 9.0)
+    # a van Leer slope costs 8 (differences, product, test) or 15 (with the
+    # sign, three |.| and two min, and the product)
+    assert 8.0 <= r["slope"] <= 15.0
+    const = pyoracle.step_op_count(np.full((n, n, n), 2.5), u, 0.5 / n, [1.0 / n] * 3, 1e-8)
+    assert const["slope"] == 8.0
+    per_point = r["ppm"] + r["predictor"] + r["corner"] + r["edge"]
+    shared = per_point - 3 * (2 * r["slope"] + r["face"])
+    print(f"[oracle] step ops per cell: {r}, MakeEdgeScal non-redundant {shared:.1f}")
+    assert abs(shared - bench.EDGE_OPS_PER_CELL_COMP) <= 0.01 * bench.EDGE_OPS_PER_CELL_COMP
