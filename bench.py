@@ -42,6 +42,8 @@ EDGE_OPS_PER_CELL_COMP = 331
 # Algorithmic HBM bytes per valid cell of one pass (FP64, valid region only,
 # compulsory traffic across the reference's stage boundaries): 3-D, nc = 5.
 PASS_BYTES_PER_CELL = {3: 488, 2: 352}
+# spherical: w0mac read in MakeUtrans and VelPred as well (+6 doubles)
+PASS_BYTES_PER_CELL_SPHERICAL = 536
 
 
 def kernel_bytes_per_cell(name, nc, dim):
@@ -88,6 +90,9 @@ def parse():
                     help="the batched AMR hierarchy (3 levels, uneven boxes) instead of 256^3")
     ap.add_argument("--adjoint", action="store_true",
                     help="the differentiable step: 128^3 periodic, 4 scalars, adjoint/forward cost")
+    ap.add_argument("--spherical", action="store_true",
+                    help="the spherical star: 384^3 in 216 uneven boxes (edges 48-96), "
+                         "ppm_type 2, radial w0, one graph launch per pass")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     return ap.parse_args()
@@ -345,7 +350,19 @@ def main():
 
     dev = mx.device_backend("cuda:0")
     t0 = time.perf_counter()
-    w = Workload(dim=3, n=args.n, max_grid=args.max_grid, ppm_type=args.ppm, backend=dev)
+    if args.spherical:
+        from maestroex_amd.workloads import uneven_boxes
+        args.n, args.ppm, args.no_cpu = 384, 2, True
+        w = Workload(dim=3, n=args.n, ppm_type=args.ppm, spherical=True, backend=dev,
+                     boxes=uneven_boxes(args.n, 6, 48, 96, seed=4))
+        workload = (f"3-D spherical star 384^3 FP64 in {len(w.boxes)} uneven boxes (edges 48-96), "
+                    f"4 ghosts, 5 scalars + 3 velocities, radial w0 = 0.05 r (1 - r), ppm_type 2")
+        bytes_per_cell = PASS_BYTES_PER_CELL_SPHERICAL
+    else:
+        w = Workload(dim=3, n=args.n, max_grid=args.max_grid, ppm_type=args.ppm, backend=dev)
+        workload = (f"3-D {args.n}^3 FP64 as {len(w.boxes)} boxes of {args.max_grid}^3, "
+                    f"4 ghosts, 5 scalars + 3 velocities, ppm_type {args.ppm}")
+        bytes_per_cell = PASS_BYTES_PER_CELL[3]
     print(f"[bench] workload {args.n}^3 in {len(w.boxes)} boxes built in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     m = mx.gpu_maestro(w.params, [w.boxes])
@@ -402,7 +419,7 @@ def main():
     cells = w.valid_cells()
     ms_per_step = total_ms / args.steps
     value = cells / (ms_per_step * 1e-3) / 1e9
-    pass_gbs = PASS_BYTES_PER_CELL[3] * cells / (ms_per_step * 1e-3) / 1e9
+    pass_gbs = bytes_per_cell * cells / (ms_per_step * 1e-3) / 1e9
 
     dom = max(kernels.items(), key=lambda kv: kv[1][0])
     dname, (dms, dcnt) = dom
@@ -435,7 +452,7 @@ def main():
                                                                     key=lambda kv: -kv[1][0])}
     print(f"[bench] per-pass kernel ms: {breakdown}", file=sys.stderr, flush=True)
 
-    if dname == "k_es_fused":
+    if dname == "k_es_fused" and args.ppm == 1 and not args.spherical:
         # the kernel's FP64 operation rate against the VALU rate (the
         # co-binding roofline)
         ops = EDGE_OPS_PER_CELL_COMP * nc * cells
@@ -466,8 +483,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"3-D {args.n}^3 FP64 as {len(w.boxes)} boxes of {args.max_grid}^3, "
-                               f"4 ghosts, 5 scalars + 3 velocities, ppm_type {args.ppm}",
+        "config": {"workload": workload,
                    "global_batch": cells, "parallelism": "single GPU",
                    "pass_launch": "one HIP graph launch per pass",
                    "eager_ms_per_step": round(eager_ms, 4),

@@ -60,7 +60,8 @@ class Workload:
     def __init__(self, dim=3, n=32, max_grid=16, ppm_type=1, nspec=3, seed=1234, nmodes=None,
                  ng=4, all_periodic=False, backend=None, ppm_trace_forces=0,
                  phys_bc=None, zero_velocity=False, shift=(0, 0, 0), spherical=False,
-                 use_exact_base_state=0, species_pred_type=1, enthalpy_pred_type=1):
+                 use_exact_base_state=0, species_pred_type=1, enthalpy_pred_type=1,
+                 boxes=None):
         from .fabs import Backend
         self.backend = backend or Backend("numpy")
         self.dim = dim
@@ -87,7 +88,9 @@ class Workload:
                 lo_bc[d], hi_bc[d] = bl, bh
                 periodic[d] = 1 if bl == _abi.PHYS_INTERIOR else 0
         self.geom = Geometry(dim, [n] * dim, [1.0 / n] * dim, periodic, lo_bc, hi_bc)
-        self.boxes = split_domain(self.geom, max_grid)
+        # boxes: an explicit box list [((lo), (hi)), ...] covering the domain,
+        # else boxes of max_grid cells
+        self.boxes = list(boxes) if boxes is not None else split_domain(self.geom, max_grid)
         rng = np.random.default_rng(seed)
 
         nscal = 4 + nspec
@@ -314,6 +317,26 @@ def _hash_uniform(seed, lev, comp, i, j, k):
         x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
         x = x ^ (x >> np.uint64(31))
     return (x >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+
+
+def uneven_boxes(n, parts, lo, hi, seed=0):
+    """The domain [0, n)^3 as parts^3 boxes whose edges, per direction, are a
+    seeded random split of n into `parts` lengths in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    cuts = []
+    for _ in range(3):
+        lens = np.full(parts, n // parts)
+        lens[: n - lens.sum()] += 1
+        for _ in range(8 * parts):
+            a, b = rng.integers(0, parts, size=2)
+            m = int(rng.integers(1, 9))
+            if a != b and lens[a] - m >= lo and lens[b] + m <= hi:
+                lens[a] -= m
+                lens[b] += m
+        edges = np.concatenate([[0], np.cumsum(lens)])
+        cuts.append([(int(edges[q]), int(edges[q + 1]) - 1) for q in range(parts)])
+    return [((x0, y0, z0), (x1, y1, z1)) for (z0, z1) in cuts[2] for (y0, y1) in cuts[1]
+            for (x0, x1) in cuts[0]]
 
 
 def _partition(rng, n, lo, hi):

@@ -11,6 +11,7 @@ import pytest
 
 from harness import (assert_bitexact, compare_face, compare_mf, gpu_for, oracle_for, pair)
 from maestroex_amd import _abi
+from maestroex_amd.workloads import uneven_boxes
 
 pytestmark = pytest.mark.gpu
 
@@ -46,6 +47,8 @@ CASES = [
     ("3d_spherical_ppm1", dict(dim=3, n=16, max_grid=8, ppm_type=1, spherical=True)),
     ("3d_spherical_ppm2_uneven", dict(dim=3, n=24, max_grid=(16, 8, 12), ppm_type=2,
                                       spherical=True)),
+    ("3d_spherical_ppm2_random_boxes", dict(dim=3, n=32, ppm_type=2, spherical=True,
+                                            boxes=uneven_boxes(32, 3, 8, 14, seed=4))),
     ("3d_spherical_exact", dict(dim=3, n=16, max_grid=8, ppm_type=2, spherical=True,
                                 use_exact_base_state=1)),
 ]
