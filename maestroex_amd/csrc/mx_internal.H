@@ -278,6 +278,23 @@ int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mult
 int velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
             const mx_face_multifab* utrans, const mx_face_multifab* umac,
             const mx_face_multifab* w0mac, const mx_multifab* force, const mx_multifab* w0_cart);
+// VelPred in two parts around MakeUtrans (the pass): setup uploads the box
+// table, reserves the temporaries and runs the PPM kernel; MakeUtrans then
+// reads the normal-direction traced states; finish runs the transverse and
+// velocity kernels once utrans (with w0) is final
+struct VpPlan {
+    const void* dp = nullptr;  // device box table (VpBox)
+    int nb = 0;
+    long long maxpts = 0;
+    int lev = 0;
+    const mx_face_multifab* utrans = nullptr;
+};
+int velpred_setup(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
+                  const mx_face_multifab* utrans, const mx_face_multifab* umac,
+                  const mx_face_multifab* w0mac, const mx_multifab* force,
+                  const mx_multifab* w0_cart, VpPlan& plan);
+int utrans_from_plan(mx_adv_ctx* c, const VpPlan& plan);
+int velpred_finish(mx_adv_ctx* c, const VpPlan& plan);
 int make_edge_scal(mx_adv_ctx* c, int lev, const mx_multifab* state,
                    const mx_face_multifab* sedge, const mx_face_multifab* umac,
                    const mx_multifab* force, int is_vel, const mx_bcrec* bcs, int nbccomp,

@@ -425,35 +425,41 @@ __device__ __forceinline__ void rhox_face(const FluxBox& B, const KGeom& g, cons
     const DA& sf = B.sf[D];
     const DA& se = B.se[D];
     const double um = B.um[D](i, j, k);
-    sf(i, j, k, A.Rho) = 0.0;
+    // F_Rho and etarhoflux accumulate in registers in the reference's order
+    // (0 + F_1 + F_2 + ..., MaestroMakeFlux.cpp:139) and are stored once
+    const bool eta_on = vertical && !sph && A.evolve && !A.use_exact;
+    double eta = eta_on ? B.eta(i, j, k) : 0.0;
+    double frho = 0.0;
+    double rho0_edge;
+    if (sph) {
+        rho0_edge = 0.5 * B.r0o[D](i, j, k) + 0.5 * B.r0n[D](i, j, k);
+    } else if (vertical) {
+        rho0_edge = 0.5 * (A.rho0_edge_old(r) + A.rho0_edge_new(r));
+    } else {
+        rho0_edge = 0.5 * (A.rho0_old(r) + A.rho0_new(r));
+    }
+    const double se_rho = se(i, j, k, A.Rho);
     for (int comp = A.start_comp; comp < A.num_comp + A.start_comp; ++comp) {
-        double rho0_edge;
-        if (sph) {
-            rho0_edge = 0.5 * B.r0o[D](i, j, k) + 0.5 * B.r0n[D](i, j, k);
-        } else if (vertical) {
-            rho0_edge = 0.5 * (A.rho0_edge_old(r) + A.rho0_edge_new(r));
-        } else {
-            rho0_edge = 0.5 * (A.rho0_old(r) + A.rho0_new(r));
-        }
-        double f = sf(i, j, k, comp);
+        double f;
         if (A.species_pred_type == 1) {
-            f = um * (rho0_edge + se(i, j, k, A.Rho)) * se(i, j, k, comp);
+            f = um * (rho0_edge + se_rho) * se(i, j, k, comp);
         } else if (A.species_pred_type == 2) {
             f = um * se(i, j, k, comp);
         } else if (A.species_pred_type == 3) {
-            f = um * se(i, j, k, A.Rho) * se(i, j, k, comp);
+            f = um * se_rho * se(i, j, k, comp);
+        } else {
+            f = sf(i, j, k, comp);  // no flux form: the value is kept
         }
         sf(i, j, k, comp) = f;
-        if (vertical && !sph && A.evolve && !A.use_exact) {
-            if (comp >= A.FirstSpec && comp <= A.FirstSpec + A.NumSpec - 1) {
-                B.eta(i, j, k) += f;
-            }
-            if (comp == A.FirstSpec + A.NumSpec - 1) {
-                B.eta(i, j, k) -= A.w0(r) * A.rho0_pe(r);
-            }
+        if (eta_on) {
+            if (comp >= A.FirstSpec && comp <= A.FirstSpec + A.NumSpec - 1) eta += f;
+            if (comp == A.FirstSpec + A.NumSpec - 1) eta -= A.w0(r) * A.rho0_pe(r);
         }
-        sf(i, j, k, A.Rho) += f;
+        if (comp == A.Rho) frho = f;  // the component store would have replaced it
+        frho += f;
     }
+    sf(i, j, k, A.Rho) = frho;
+    if (eta_on) B.eta(i, j, k) = eta;
 }
 
 __global__ void __launch_bounds__(256) k_rhox_flux(const FluxBox* __restrict__ boxes, KGeom g,
@@ -935,12 +941,15 @@ int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_arg
     for (int i = 0; i < nlev; ++i) {
         const int lev = lev0 + i;
         const mx_adv_pass_args* A = &a[i];
-        if ((rc = make_utrans(c, lev, &A->utilde, &A->ufull, &A->utrans, &A->w0mac, &A->w0_cart)))
+        // VelPred's reconstruction runs first and MakeUtrans reads its traced
+        // normal-direction states (the numbers MakeUtrans would compute)
+        VpPlan vp;
+        if ((rc = velpred_setup(c, lev, &A->utilde, &A->ufull, &A->utrans, &A->umac, &A->w0mac,
+                                &A->vel_force, &A->w0_cart, vp)))
             return rc;
+        if ((rc = utrans_from_plan(c, vp))) return rc;
         if ((rc = addw0(c, lev, &A->utrans, &A->w0mac, &A->w0, 1.0))) return rc;
-        if ((rc = velpred(c, lev, &A->utilde, &A->ufull, &A->utrans, &A->umac, &A->w0mac,
-                          &A->vel_force, &A->w0_cart)))
-            return rc;
+        if ((rc = velpred_finish(c, vp))) return rc;
         if ((rc = addw0(c, lev, &A->umac, &A->w0mac, &A->w0, 1.0))) return rc;
     }
     const int groups[3][2] = {{p.first_spec, p.num_spec}, {p.comp_rho, 1}, {p.comp_rhoh, 1}};

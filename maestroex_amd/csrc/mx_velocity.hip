@@ -42,6 +42,10 @@ struct UtArgs {
     int bclo[3][3], bchi[3][3];  // bcs_u[comp].lo/hi[dir]
 };
 
+template <int D, class BOX>
+__device__ __forceinline__ void utrans_riemann(const BOX& B, const KGeom& g, int i, int j, int k,
+                                               double ul, double ur);
+
 template <int PPM, int D>
 __device__ __forceinline__ void utrans_face(const UtBox& B, const KGeom& g, const UtArgs& A,
                                             int i, int j, int k) {
@@ -75,6 +79,17 @@ __device__ __forceinline__ void utrans_face(const UtBox& B, const KGeom& g, cons
         ur = Im;
     }
 
+    utrans_riemann<D>(B, g, i, j, k, ul, ur);
+}
+
+// the PhysBC overrides and the Riemann solve of MakeUtrans on D-face
+// (i, j, k) from its left and right states (MaestroMakeUtrans.cpp:109-150,
+// 253-266); BOX: any box descriptor with ut, w0c, w0m and tr views
+template <int D, class BOX>
+__device__ __forceinline__ void utrans_riemann(const BOX& B, const KGeom& g, int i, int j, int k,
+                                               double ul, double ur) {
+    const int im = i - e(D, 0), jm = j - e(D, 1), km = k - e(D, 2);
+    const int pos = D == 0 ? i : (D == 1 ? j : k);
     if (pos == g.domlo[D]) {
         switch (g.phys_lo[D]) {
             case MX_PHYS_INFLOW:
@@ -504,6 +519,48 @@ __global__ void __launch_bounds__(256) k_vp_vel(const VpBox* __restrict__ boxes,
         vp_vel<2>(B, g, i, j, k);
 }
 
+// MakeUtrans inside the pass: the left and right states of D-face (i,j,k)
+// are VelPred's traced states of the normal component (PPM: Ip of the cell
+// below, Im of the cell above; ppm_type 0: the slopes), which are the same
+// numbers MakeUtrans's own reconstruction gives (same stencil, limiter,
+// boundary stencils and tracing velocity ufull)
+template <int PPM, int D>
+__device__ __forceinline__ void utrans_pp_face(const VpBox& B, const KGeom& g, int i, int j,
+                                               int k) {
+    const int im = i - e(D, 0), jm = j - e(D, 1), km = k - e(D, 2);
+    double ul, ur;
+    if (PPM == 0) {
+        const double dt2 = 0.5 * g.dt;
+        const double hd = g.dx[D];
+        const double slL = B.Ip[D](im, jm, km, D);
+        const double slR = B.Ip[D](i, j, k, D);
+        ul = B.ut(im, jm, km, D) + (0.5 - (dt2 / hd) * dmax(0.0, B.uf(im, jm, km, D))) * slL;
+        ur = B.ut(i, j, k, D) - (0.5 + (dt2 / hd) * dmin(0.0, B.uf(i, j, k, D))) * slR;
+    } else {
+        ul = B.Ip[D](im, jm, km, D);
+        ur = B.Im[D](i, j, k, D);
+    }
+    utrans_riemann<D>(B, g, i, j, k, ul, ur);
+}
+
+template <int PPM>
+__global__ void __launch_bounds__(256) k_utrans_pp(const VpBox* __restrict__ boxes, KGeom g) {
+    const VpBox& B = boxes[blockIdx.y];
+    const int d = blockIdx.z;
+    int lo[3] = {B.lo[0], B.lo[1], B.lo[2]};
+    int hi[3] = {B.hi[0], B.hi[1], B.hi[2]};
+    hi[d] += 1;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!region_point(idx, lo, hi, i, j, k)) return;
+    if (d == 0)
+        utrans_pp_face<PPM, 0>(B, g, i, j, k);
+    else if (d == 1)
+        utrans_pp_face<PPM, 1>(B, g, i, j, k);
+    else
+        utrans_pp_face<PPM, 2>(B, g, i, j, k);
+}
+
 int check_phys(const mx_adv_ctx* c) {
     for (int d = 0; d < c->p.dim; ++d) {
         if (c->p.phys_bc_lo[d] < MX_PHYS_INTERIOR || c->p.phys_bc_lo[d] > MX_PHYS_NOSLIPWALL)
@@ -568,9 +625,10 @@ int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mult
     return fill_face_ghosts(c, lev, utrans);
 }
 
-int velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
-            const mx_face_multifab* utrans, const mx_face_multifab* umac,
-            const mx_face_multifab* w0mac, const mx_multifab* force, const mx_multifab* w0_cart) {
+int velpred_setup(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
+                  const mx_face_multifab* utrans, const mx_face_multifab* umac,
+                  const mx_face_multifab* w0mac, const mx_multifab* force,
+                  const mx_multifab* w0_cart, VpPlan& plan) {
     const int dim = c->p.dim;
     const auto& grids = c->grids[lev];
     const bool sph = dim == 3 && c->p.spherical;
@@ -648,17 +706,46 @@ int velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab
         MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm<1>,  gr, dim3(block), dp, kg, A);
     else
         MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm<2>,  gr, dim3(block), dp, kg, A);
+    plan.dp = dp;
+    plan.nb = nb;
+    plan.maxpts = maxpts;
+    plan.lev = lev;
+    plan.utrans = utrans;
+    return MX_OK;
+}
 
+// MakeUtrans from the traced states of velpred_setup, then the face ghosts
+int utrans_from_plan(mx_adv_ctx* c, const VpPlan& plan) {
+    const KGeom kg = kgeom(c, plan.lev);
+    const int block = 256;
+    const dim3 gr = grid_for(plan.maxpts, plan.nb, c->p.dim, block);
+    if (c->p.ppm_type == 0)
+        MX_LAUNCH(c, "k_utrans", k_utrans_pp<0>, gr, dim3(block), (const VpBox*)plan.dp, kg);
+    else
+        MX_LAUNCH(c, "k_utrans", k_utrans_pp<1>, gr, dim3(block), (const VpBox*)plan.dp, kg);
+    return fill_face_ghosts(c, plan.lev, plan.utrans);
+}
 
-    if (dim == 3) {
-        MX_LAUNCH(c, "k_vp_trans", k_vp_trans,  grid_for(maxpts, nb, 6, block), dim3(block), dp, kg);
-
-    }
-    MX_LAUNCH(c, "k_vp_vel", k_vp_vel,  grid_for(maxpts, nb, dim, block), dim3(block), dp,
-                       kg);
-
+// VelPred after its PPM kernel: transverse states and MAC velocities
+int velpred_finish(mx_adv_ctx* c, const VpPlan& plan) {
+    const KGeom kg = kgeom(c, plan.lev);
+    const int block = 256;
+    if (c->p.dim == 3)
+        MX_LAUNCH(c, "k_vp_trans", k_vp_trans, grid_for(plan.maxpts, plan.nb, 6, block),
+                  dim3(block), (const VpBox*)plan.dp, kg);
+    MX_LAUNCH(c, "k_vp_vel", k_vp_vel, grid_for(plan.maxpts, plan.nb, c->p.dim, block),
+              dim3(block), (const VpBox*)plan.dp, kg);
     // AverageDownFaces(umac) is a no-op on a single level
     return MX_OK;
+}
+
+int velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
+            const mx_face_multifab* utrans, const mx_face_multifab* umac,
+            const mx_face_multifab* w0mac, const mx_multifab* force, const mx_multifab* w0_cart) {
+    VpPlan plan;
+    int rc = velpred_setup(c, lev, utilde, ufull, utrans, umac, w0mac, force, w0_cart, plan);
+    if (rc) return rc;
+    return velpred_finish(c, plan);
 }
 
 }  // namespace mxh
