@@ -80,8 +80,10 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--max-grid", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default launch shape of the fused edge kernel")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the autotuner pick the fused edge kernel's launch shape (the "
+                         "hierarchy mode always does)")
+    ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -369,8 +371,8 @@ def main():
     lib, fns = mx.load()
     fields = w.pass_fields()
     tuned = None
-    shape = {"tile": "32x8", "tz": 32}  # the library's defaults
-    if not args.no_autotune:
+    shape = {"tile": "32x8", "tz": 128}  # the library's defaults
+    if args.autotune and not args.no_autotune:
         # the autotuner picks the fused edge kernel's tile and z-chunk on
         # this workload (untimed, before the warm-up)
         cands, best = m.autotune([fields])

@@ -38,6 +38,7 @@ using namespace mxh;
 namespace {
 
 constexpr int NTB = 256;
+
 constexpr double C23 = 2.0 / 3.0;
 
 __host__ __device__ constexpr int pidx(int d, int t) { return d * 2 + (t > d ? t - 1 : t); }
@@ -79,7 +80,7 @@ struct AB {
     double *EB[6], *TB[3];
     double *SLB[3], *SRB[3];  // face sums (k_adj_fsum), when used
     double* UBC[3];           // per-component face velocity adjoints (k_adj_trace)
-    double *S0B[3], *FLB[3], *FRB[3], *UHI[3], *ULO[3];
+    double *S0S, *FLB[3], *FRB[3], *UHI[3], *ULO[3];  // S0S: trace terms of s, summed over d
     double *DSA[3], *DSB[3], *DLA[3], *DLB[3], *QM[3], *Q0[3], *QP[3];
     DA SBF;            // state adjoint: the caller's FAB
     long long cs;      // N^3
@@ -405,7 +406,7 @@ __global__ void __launch_bounds__(NTB) k_adj_fsum(AB A) {
 
 // B6: characteristic trace and CW limiter of cell p in direction d (FS: the
 // face adjoints are read from k_adj_fsum's arrays instead of recomputed)
-template <bool P2, bool FS>
+template <bool P2, bool FS, bool ROLL>
 __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
     int i, j, k;
     long long p;
@@ -413,7 +414,11 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
     const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
+    // ROLL: one direction per loop trip with its stores at the end of the
+    // trip (fewer registers, more waves); else all loads before any store
     double r0[3], r1[3], r2[3], r3[3], r4[3], ubd[3] = {0.0, 0.0, 0.0};
+    double s0s = 0.0;  // the trace's direct terms of s, summed over d in order
+#pragma unroll(ROLL ? 1 : 3)
     for (int d = 0; d < 3; ++d) {
         double v[9];
         for (int o = -4; o <= 4; ++o) v[4 + o] = (o >= -2 && o <= 2) ? S[P.nb(i, j, k, d, o)] : 0.0;
@@ -456,7 +461,7 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
             face_totals(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb,
                         unused0, unused1);
             // face p's totals also give its velocity adjoint for this component
-            face_totals(A, P, co, i, j, k, d, true, unused2, Imb, ubd[d]);
+            face_totals(A, P, co, i, j, k, d, true, unused2, Imb, ubd[ROLL ? 0 : d]);
         }
         double smb = 0.0, spb = 0.0, s0b = 0.0, uhib = 0.0, ulob = 0.0;
         if (uhi > A.rel_eps) {
@@ -497,19 +502,38 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
             sm0b = smb;
             sp0b = spb;
         }
+        if (ROLL) {
+            s0s = d == 0 ? s0b : s0s + s0b;  // (t0 + t1) + t2, as unrolled
+            A.FLB[d][co + p] = sm0b;
+            A.FRB[d][co + p] = sp0b;
+            A.UHI[d][co + p] = uhib;
+            if (FS)
+                A.ULO[d][co + p] = ulob;
+            else
+                A.UBC[d][co + p] = ubd[0] + ulob;  // both belong to face p
+            continue;
+        }
         r0[d] = s0b;
         r1[d] = sm0b;
         r2[d] = sp0b;
         r3[d] = uhib;
         r4[d] = ulob;
     }
+    if (ROLL) {
+        A.S0S[co + p] = s0s;
+        return;
+    }
+    A.S0S[co + p] = (r0[0] + r0[1]) + r0[2];
     for (int d = 0; d < 3; ++d) {
-        A.S0B[d][co + p] = r0[d];
         A.FLB[d][co + p] = r1[d];  // face p (below the cell)
         A.FRB[d][co + p] = r2[d];  // face above the cell
         A.UHI[d][co + p] = r3[d];
-        A.ULO[d][co + p] = r4[d];
-        if (!FS) A.UBC[d][co + p] = ubd[d];
+        // the face-p velocity adjoints of this component: through the face
+        // totals and through this cell's trace (its lower face)
+        if (FS)
+            A.ULO[d][co + p] = r4[d];
+        else
+            A.UBC[d][co + p] = ubd[d] + r4[d];
     }
 }
 
@@ -639,7 +663,7 @@ __global__ void __launch_bounds__(NTB) k_adj_state_fused(AB A) {
     const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
-    double s = A.G[co + p];
+    double s = A.G[co + p] + A.S0S[co + p];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const int di = d == 0, dj = d == 1, dk = d == 2;
@@ -653,7 +677,7 @@ __global__ void __launch_bounds__(NTB) k_adj_state_fused(AB A) {
         for (int m = 0; m < 3; ++m)
             slope_bar(P, S, i + (m - 1) * di, j + (m - 1) * dj, k + (m - 1) * dk, d,
                       flb[m] + fla[m + 1], qm[m], q0[m], qp[m]);
-        s += A.S0B[d][co + p] + fsb[1] + fsa[2] + qp[0] + q0[1] + qm[2];
+        s += fsb[1] + fsa[2] + qp[0] + q0[1] + qm[2];
     }
     A.SBF(i, j, k, (int)blockIdx.y) = s;
 }
@@ -666,11 +690,11 @@ __global__ void __launch_bounds__(NTB) k_adj_state(AB A) {
     if (!point(A, i, j, k, p)) return;
     const PerT<P2> P = make_per<P2>(A.n);
     const long long co = (long long)blockIdx.y * A.cs;
-    double s = A.G[co + p];
+    double s = A.G[co + p] + A.S0S[co + p];
     for (int d = 0; d < 3; ++d) {
         const long long cm = P.nb(i, j, k, d, -1), cp = P.nb(i, j, k, d, 1);
-        s += A.S0B[d][co + p] + A.DSB[d][co + p] + A.DSA[d][co + cp] + A.QP[d][co + cm] +
-             A.Q0[d][co + p] + A.QM[d][co + cp];
+        s += A.DSB[d][co + p] + A.DSA[d][co + cp] + A.QP[d][co + cm] + A.Q0[d][co + p] +
+             A.QM[d][co + cp];
     }
     A.SBF(i, j, k, (int)blockIdx.y) = s;
 }
@@ -691,14 +715,17 @@ __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, DA ub0, DA ub1, DA ub2, i
         double s = 0.0;
         for (int c = 0; c < A.ncomp; ++c) {
             const long long co = (long long)c * A.cs;
+            // (face totals + lower-face trace term) + upper-face trace term of
+            // the cell below, in every variant
             double ub;
             if (STORED) {
                 ub = A.UBC[d][co + p];
             } else {
-                double slb, srb;
-                face_totals(A, P, co, i, j, k, d, true, slb, srb, ub);
+                double slb, srb, ubt;
+                face_totals(A, P, co, i, j, k, d, true, slb, srb, ubt);
+                ub = ubt + A.ULO[d][co + p];
             }
-            s += ub + A.UHI[d][co + cm] + A.ULO[d][co + p];
+            s += ub + A.UHI[d][co + cm];
         }
         r[d] = s;
     }
@@ -853,9 +880,11 @@ int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, mx_fab* ubar) {
     MX_LAUNCH(c, "k_adj_tterm", k_adj_tterm<false>, gc, dim3(NTB), A);
     if (c->adj_face_sums) {
         MX_LAUNCH(c, "k_adj_fsum", k_adj_fsum<P2>, gc, dim3(NTB), A);
-        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, true>), gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, true, false>), gc, dim3(NTB), A);
+    } else if (c->adj_trace_rolled) {
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, false, true>), gc, dim3(NTB), A);
     } else {
-        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, false>), gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, false, false>), gc, dim3(NTB), A);
     }
     if (c->adj_state_fused) {
         MX_LAUNCH(c, "k_adj_state_fused", k_adj_state_fused<P2>, gc, dim3(NTB), A);
@@ -951,7 +980,7 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
     // dense arrays: S and G (ncomp each), 63 per-component intermediates and
     // adjoints, U (3)
     const size_t per = (size_t)n3 * sizeof(double) + 256;
-    const size_t count = (size_t)ncomp * (65 + 3 + (c->adj_face_sums ? 6 : 0)) + 3;
+    const size_t count = (size_t)ncomp * (63 + (c->adj_face_sums ? 9 : 0)) + 3;
     if ((rc = ws_reserve(c, per * count + 4096))) return rc;
     AB A;
     std::memset(&A, 0, sizeof(A));
@@ -974,11 +1003,10 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
             A.SLB[d] = take(ncomp);
             A.SRB[d] = take(ncomp);
         }
-        A.S0B[d] = take(ncomp);
         A.FLB[d] = take(ncomp);
         A.FRB[d] = take(ncomp);
         A.UHI[d] = take(ncomp);
-        A.ULO[d] = take(ncomp);
+        if (c->adj_face_sums) A.ULO[d] = take(ncomp);
         A.DSA[d] = take(ncomp);
         A.DSB[d] = take(ncomp);
         A.DLA[d] = take(ncomp);
@@ -993,6 +1021,7 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
     }
     A.S = S;
     A.G = G;
+    A.S0S = take(ncomp);
     A.SBF = to_da(*sbar);
     A.cs = n3;
     A.n = n;

@@ -907,7 +907,7 @@ int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& lev
                                 const mx_bcrec* bcs, int start_scomp, int start_bccomp,
                                 int num_comp, int is_conservative) {
     if (levs.empty()) return MX_OK;
-    const int tz = c->fused_tz > 0 ? c->fused_tz : 32;
+    const int tz = c->fused_tz > 0 ? c->fused_tz : 128;
     if (c->fused_per_box) {
         // comparison mode: the same work as one launch per box
         std::vector<EdgeLevel> one(1);

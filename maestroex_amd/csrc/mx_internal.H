@@ -86,12 +86,13 @@ struct mx_adv_ctx {
 
     // kernel selection (mx_adv_set_tuning)
     int fused_edge = 1;
-    int fused_tz = 32;
+    int fused_tz = 128;
     int fused_tile = 0;     // 0: 32 x 8 columns, 1: 16 x 16
     int fused_per_box = 0;  // 1: one fused launch per box (comparison mode)
     int adj_state_fused = 1;  // adjoint: face, slope and state adjoints in one gather
     int adj_face_sums = 0;    // adjoint: predictor-state face adjoints stored once per face
     int adj_pow2 = 1;         // adjoint: mask indexing when n is a power of two
+    int adj_trace_rolled = 0; // adjoint: trace kernel with one direction per loop trip
 
     // box tables under graph capture: 0 eager (one device table, copied at
     // every call); 1 record (eager, and keep each table's bytes in order);

@@ -188,30 +188,29 @@ struct VpArgs {
     int bclo[3][3], bchi[3][3];  // bcs_u[comp].lo/hi[dir]
 };
 
+// the PPM traced states (or ppm_type 0: the slope) of velocity component n
+// along D at cell (i,j,k); with traced forces also theirs (f0, f1)
 template <int PPM, int D>
 __device__ __forceinline__ void vp_ppm_dir(const VpBox& B, const KGeom& g, const VpArgs& A, int n,
-                                           int i, int j, int k) {
+                                           int i, int j, int k, double& r0, double& r1,
+                                           double& f0, double& f1) {
     const int pos = D == 0 ? i : (D == 1 ? j : k);
     double v[9];
     if (PPM == 0) {
         load_stencil<1>(B.ut.p + B.ut.off(i, j, k, n), B.ut.stride(D), v);
-        B.Ip[D](i, j, k, n) = slope_point<D>(v, g.slope_order, pos, g.domlo[D], g.domhi[D],
-                                             A.bclo[n][D], A.bchi[n][D]);
+        r0 = slope_point<D>(v, g.slope_order, pos, g.domlo[D], g.domhi[D], A.bclo[n][D],
+                            A.bchi[n][D]);
         return;
     }
-    double sm, sp, Ip, Im;
+    double sm, sp;
     load_stencil<PPM>(B.ut.p + B.ut.off(i, j, k, n), B.ut.stride(D), v);
     ppm_edges<PPM, D>(v, pos, g.domlo[D], g.domhi[D], A.bclo[n][D], A.bchi[n][D], sm, sp);
     const double uc = B.uf(i, j, k, D);
-    ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[D], g.rel_eps, Ip, Im);
-    B.Ip[n](i, j, k, D) = Ip;
-    B.Im[n](i, j, k, D) = Im;
+    ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[D], g.rel_eps, r0, r1);
     if (g.ppm_trace_forces == 1) {
         load_stencil<PPM>(B.fo.p + B.fo.off(i, j, k, n), B.fo.stride(D), v);
         ppm_edges<PPM, D>(v, pos, g.domlo[D], g.domhi[D], A.bclo[n][D], A.bchi[n][D], sm, sp);
-        ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[D], g.rel_eps, Ip, Im);
-        B.Ipf[n](i, j, k, D) = Ip;
-        B.Imf[n](i, j, k, D) = Im;
+        ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[D], g.rel_eps, f0, f1);
     }
 }
 
@@ -227,9 +226,27 @@ __global__ void __launch_bounds__(256) k_vp_ppm(const VpBox* __restrict__ boxes,
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
     if (!region_point(idx, lo, hi, i, j, k)) return;
-    vp_ppm_dir<PPM, 0>(B, g, A, n, i, j, k);
-    vp_ppm_dir<PPM, 1>(B, g, A, n, i, j, k);
-    if (g.dim == 3) vp_ppm_dir<PPM, 2>(B, g, A, n, i, j, k);
+    // every direction is computed before any store (loads and stores share
+    // one in-order counter, so a load issued after a store waits for it)
+    double r0[3] = {0.0, 0.0, 0.0}, r1[3] = {0.0, 0.0, 0.0};
+    double f0[3] = {0.0, 0.0, 0.0}, f1[3] = {0.0, 0.0, 0.0};
+    vp_ppm_dir<PPM, 0>(B, g, A, n, i, j, k, r0[0], r1[0], f0[0], f1[0]);
+    vp_ppm_dir<PPM, 1>(B, g, A, n, i, j, k, r0[1], r1[1], f0[1], f1[1]);
+    if (g.dim == 3) vp_ppm_dir<PPM, 2>(B, g, A, n, i, j, k, r0[2], r1[2], f0[2], f1[2]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d >= g.dim) break;
+        if (PPM == 0) {
+            B.Ip[d](i, j, k, n) = r0[d];
+            continue;
+        }
+        B.Ip[n](i, j, k, d) = r0[d];
+        B.Im[n](i, j, k, d) = r1[d];
+        if (g.ppm_trace_forces == 1) {
+            B.Ipf[n](i, j, k, d) = f0[d];
+            B.Imf[n](i, j, k, d) = f1[d];
+        }
+    }
 }
 
 // interface states ul/ur of velocity component n on D-faces (VelPredInterface)
