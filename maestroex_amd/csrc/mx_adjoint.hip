@@ -21,6 +21,7 @@
 // Adjoints that are cheap to recompute from their neighbours' inputs are
 // recomputed where they are gathered instead of stored:
 //   B1 flux, update and edge-state upwinding -> per-cell edge-term adjoints
+//      (in the edge-state tape kernel: it needs only gbar and the velocity)
 //   B4 corner terms (recomputed)    -> per-cell transverse-term adjoints
 //   B3 + B5 the face adjoints of the predictor states and velocity: flux,
 //      corner and transverse terms, summed where B6 and B10 read them
@@ -214,6 +215,18 @@ __device__ __forceinline__ double et(const AB& A, const PT& P, long long co, int
     return A.f4[t] * (A.U[t][c1] + A.U[t][c]) * (K[c1] - K[c]);
 }
 
+// adjoint of the edge state on d-face c through the flux (Fb u) and its
+// upwinding weights
+template <class PT>
+__device__ __forceinline__ double edge_bar(const AB& A, const PT& P, long long co, int i, int j,
+                                           int k, int d, double& wl, double& wr) {
+    const long long c = P.at(i, j, k);
+    const double Fb = A.fdt[d] * (A.G[co + c] - A.G[co + P.nb(i, j, k, d, -1)]);
+    const double u = A.U[d][c];
+    weights(u, A.rel_eps, wl, wr);
+    return Fb * u;
+}
+
 // A4: edge states
 template <bool P2>
 __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
@@ -238,32 +251,10 @@ __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
         const double sedger = r + dt2 * 0.0;
         se[d] = upwind(sedgel, sedger, A.U[d][p], A.rel_eps);
     }
-    for (int d = 0; d < 3; ++d) A.SE[d][co + p] = se[d];
-}
-
-// ---------------------------------------------------------------- reverse
-// adjoint of the edge state on d-face c through the flux (Fb u) and its
-// upwinding weights
-template <class PT>
-__device__ __forceinline__ double edge_bar(const AB& A, const PT& P, long long co, int i, int j,
-                                           int k, int d, double& wl, double& wr) {
-    const long long c = P.at(i, j, k);
-    const double Fb = A.fdt[d] * (A.G[co + c] - A.G[co + P.nb(i, j, k, d, -1)]);
-    const double u = A.U[d][c];
-    weights(u, A.rel_eps, wl, wr);
-    return Fb * u;
-}
-
-// B1: flux and update, edge-state upwinding -> the per-cell edge-term
-// adjoints (each edge term of cell p enters the left state of d-face p + d
-// and the right state of d-face p, d = 3-t-o)
-template <bool P2>
-__global__ void __launch_bounds__(NTB) k_adj_flux(AB A) {
-    int i, j, k;
-    long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
-    const long long co = (long long)blockIdx.y * A.cs;
+    // B1 (it needs only the output adjoint and the velocity): the flux,
+    // update and edge-state upwinding give the per-cell edge-term adjoints;
+    // each edge term of cell p enters the left state of d-face p + d and the
+    // right state of d-face p, d = 3-t-o
     double lb1[3], rb0[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -274,6 +265,7 @@ __global__ void __launch_bounds__(NTB) k_adj_flux(AB A) {
         const double seb1 = edge_bar(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, wl1, wr1);
         lb1[d] = wl1 * seb1;
     }
+    for (int d = 0; d < 3; ++d) A.SE[d][co + p] = se[d];
 #pragma unroll
     for (int t = 0; t < 3; ++t)
 #pragma unroll
@@ -284,6 +276,7 @@ __global__ void __launch_bounds__(NTB) k_adj_flux(AB A) {
         }
 }
 
+// ---------------------------------------------------------------- reverse
 // B3 + B5: the adjoints on t-face (i,j,k) of its left and right predictor
 // states (slb, srb) and, with want_u, of its velocity for this component
 // (ub): each the sum, in this order, of the flux term (edge-state
@@ -874,7 +867,6 @@ int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, mx_fab* ubar) {
     MX_LAUNCH(c, "k_tape_ppm", k_tape_ppm<P2>, gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_tape_corner", k_tape_corner<P2>, gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_tape_edge", k_tape_edge<P2>, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_flux", k_adj_flux<P2>, gc, dim3(NTB), A);
     // the transverse-term and trace gathers run faster with compare
     // indexing on MI355X (measured), whatever n is
     MX_LAUNCH(c, "k_adj_tterm", k_adj_tterm<false>, gc, dim3(NTB), A);
