@@ -607,3 +607,62 @@ def test_step_op_count():
     shared = per_point - 3 * (2 * r["slope"] + r["face"])
     print(f"[oracle] step ops per cell: {r}, MakeEdgeScal non-redundant {shared:.1f}")
     assert abs(shared - bench.EDGE_OPS_PER_CELL_COMP) <= 0.01 * bench.EDGE_OPS_PER_CELL_COMP
+
+
+def test_uneven_boxes_partition_the_domain():
+    """uneven_boxes: parts^3 boxes, edges within [lo, hi], covering [0, n)^3
+    exactly once."""
+    from maestroex_amd.workloads import uneven_boxes
+    n, parts, lo, hi = 48, 4, 8, 18
+    boxes = uneven_boxes(n, parts, lo, hi, seed=7)
+    assert len(boxes) == parts ** 3
+    cover = np.zeros((n, n, n), dtype=np.int32)
+    for blo, bhi in boxes:
+        for d in range(3):
+            assert lo <= bhi[d] - blo[d] + 1 <= hi
+        cover[blo[2]:bhi[2] + 1, blo[1]:bhi[1] + 1, blo[0]:bhi[0] + 1] += 1
+    assert np.all(cover == 1)
+
+
+def _global_faces(fmf, d, n, nc):
+    """The valid faces of direction d of every box, scattered into one global
+    (nc, n(+1 in d), ...) array; faces shared by two boxes must agree."""
+    shape = [nc, n, n, n]
+    shape[3 - d] += 1
+    out = np.full(shape, np.nan)
+    mf = fmf[d]
+    for b, (lo, hi) in enumerate(mf.boxes):
+        a = mf.to_numpy()[b][(slice(None),) + mf.valid_region(b)]
+        sl = [slice(None)] + [slice(lo[q], hi[q] + 1 + (1 if q == d else 0)) for q in (2, 1, 0)]
+        prev = out[tuple(sl)]
+        seen = ~np.isnan(prev)
+        assert np.array_equal(prev[seen], a[seen]), "a shared face differs between boxes"
+        out[tuple(sl)] = a
+    return out
+
+
+@pytest.mark.parametrize("spherical", [False, True])
+def test_pass_is_independent_of_box_decomposition(spherical):
+    """One level's pass gives the same value on every face whether the domain
+    is cut into regular boxes or into uneven ones: the ghost cells carry the
+    neighbours' values, so the decomposition only changes where each face is
+    computed (spherical: ppm_type 2, all sides outflow)."""
+    from maestroex_amd.fabs import Backend
+    from maestroex_amd.workloads import Workload, uneven_boxes
+    n = 24
+    cfg = dict(dim=3, n=n, ppm_type=2 if spherical else 1, spherical=spherical,
+               backend=Backend("numpy"))
+    results = []
+    for boxes in (None, uneven_boxes(n, 3, 6, 10, seed=4)):
+        w = Workload(max_grid=12, boxes=boxes, **cfg)
+        m = pyoracle.oracle_maestro(w.params, [w.boxes])
+        m.advance_pass(0, w.pass_fields())
+        results.append([(_global_faces(w.umac, d, n, 1), _global_faces(w.sedge, d, n, w.nscal),
+                         _global_faces(w.sflux, d, n, w.nscal)) for d in range(3)])
+    comps = [0, 1] + list(range(4, 4 + 3))
+    for d in range(3):
+        ua, ea, fa = results[0][d]
+        ub, eb, fb = results[1][d]
+        assert np.array_equal(ua, ub)
+        assert np.array_equal(ea[comps], eb[comps])
+        assert np.array_equal(fa[comps], fb[comps])
