@@ -389,10 +389,11 @@ int mx_adv_autotune(mx_adv_ctx* ctx, int nlev, const mx_adv_pass_args* args,
    composition of MakeEdgeScal, MakeRhoXFlux and UpdateScal for a species
    (MaestroDensityAdvance.cpp:115-122, MaestroMakeFlux.cpp:118-140,
    MaestroAdvection.cpp:165-183) with a fixed MAC velocity and no forcing):
-     sedge = MakeEdgeScal(s, umac, force 0, advective form), ppm_type 1
+     sedge = MakeEdgeScal(s, umac, force 0, advective form), ppm_type 1 or 2
      snew  = s + dt (-div(umac sedge) + 0)
    on a context whose level 0 is one fully periodic box [0, n)^3.  s holds
-   ncomp components with >= 3 ghost cells, umac[0..2] one face FAB per
+   ncomp components with >= 3 ghost cells (>= 4 for ppm_type 2; fewer is
+   MX_ERR_BOX), umac[0..2] one face FAB per
    direction with >= 1 ghost face.  The call fills the ghost cells of s and
    umac by periodic copy and sets face n of umac to face 0 (one periodic face
    carries one value). */

@@ -1,8 +1,8 @@
 """Differentiable scalar advection on a periodic box (mx_adv_scal_step and its
 hand-written adjoint mx_adv_scal_step_vjp), as torch.autograd.Functions.
 
-* `ScalStepper(n, ncomp, dt, rel_eps)`: a C-ABI context for one fully
-  periodic box [0, n)^3 (dx = 1/n, ppm_type 1) with device buffers of the
+* `ScalStepper(n, ncomp, dt, rel_eps, ppm_type=1)`: a C-ABI context for one
+  fully periodic box [0, n)^3 (dx = 1/n, ppm_type 1 or 2) with device buffers of the
   step; `step(s, u)` and `vjp(s, u, gbar)` take dense torch tensors:
   s (ncomp, n, n, n) indexed [c, k, j, i] and u = (ux, uy, uz), each
   (n, n, n) holding the faces 0..n-1 of its direction (face n is face 0).
@@ -62,19 +62,22 @@ def periodic_divergence(u):
 class ScalStepper:
     """Device buffers and a C-ABI context for the step on [0, n)^3."""
 
-    NG = 3
+    NG = {1: 3, 2: 4}  # ghost cells of the state the edge stencils read, per ppm_type
 
-    def __init__(self, n, ncomp, dt, rel_eps, device="cuda:0"):
+    def __init__(self, n, ncomp, dt, rel_eps, device="cuda:0", ppm_type=1):
         import torch
         import maestroex_amd as mx
         self.torch = torch
         self.n, self.ncomp, self.dt, self.rel_eps = n, ncomp, dt, rel_eps
+        self.ppm_type = ppm_type
         g = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 1], [0, 0, 0], [0, 0, 0])
-        p = make_params(g, ppm_type=1, nspec=ncomp, dt=dt, rel_eps=rel_eps)
+        p = make_params(g, ppm_type=ppm_type, nspec=ncomp, dt=dt, rel_eps=rel_eps)
         self.m = mx.gpu_maestro(p, [[((0, 0, 0), (n - 1, n - 1, n - 1))]])
         self.lib, self.fns = mx.load()
         dev = torch.device(device)
-        ng = self.NG
+        if ppm_type not in self.NG:
+            raise ValueError(f"ppm_type {ppm_type}: the step supports 1 and 2")
+        self.ng = ng = self.NG[ppm_type]
         e = n + 2 * ng
         f64 = torch.float64
         self.s = torch.zeros((ncomp, e, e, e), dtype=f64, device=dev)
@@ -102,7 +105,7 @@ class ScalStepper:
         return f
 
     def _load(self, s, u):
-        n, ng = self.n, self.NG
+        n, ng = self.n, self.ng
         self.s[:, ng:ng + n, ng:ng + n, ng:ng + n].copy_(s)
         for d in range(3):
             sl = [slice(1, 1 + n)] * 3

@@ -82,6 +82,7 @@ struct AB {
     double *SLB[3], *SRB[3];  // face sums (k_adj_fsum), when used
     double* UBC[3];           // per-component face velocity adjoints (k_adj_trace)
     double *S0S, *FLB[3], *FRB[3], *UHI[3], *ULO[3];  // S0S: trace terms of s, summed over d
+    double* DV[3][4];  // ppm_type 2: limiter adjoints of s(p - 2d, p - d, p + d, p + 2d)
     double *DSA[3], *DSB[3], *DLA[3], *DLB[3], *QM[3], *Q0[3], *QP[3];
     DA SBF;            // state adjoint: the caller's FAB
     long long cs;      // N^3
@@ -129,29 +130,30 @@ __device__ __forceinline__ double dabs_d(double x) { return signbit(x) ? -1.0 : 
 // A1 + A2: traced PPM states of every cell and direction, and the upwind
 // state simh on the face below the cell (from the traced state of the cell
 // below it, recomputed here)
-template <int D, class PT>
+template <int PPM, int D, class PT>
 __device__ __forceinline__ void ppm_cell(const AB& A, const PT& P, const double* S, int i, int j,
                                          int k, int pos, double& Ip, double& Im) {
+    constexpr int G = PPM == 2 ? 3 : 2;
     double v[9];
 #pragma unroll
-    for (int o = -4; o <= 4; ++o) v[4 + o] = (o >= -2 && o <= 2) ? S[P.nb(i, j, k, D, o)] : 0.0;
+    for (int o = -4; o <= 4; ++o) v[4 + o] = (o >= -G && o <= G) ? S[P.nb(i, j, k, D, o)] : 0.0;
     double sm, sp;
-    ppm_edges<1, D>(v, pos, 0, A.n - 1, MX_BC_INT_DIR, MX_BC_INT_DIR, sm, sp);
+    ppm_edges<PPM, D>(v, pos, 0, A.n - 1, MX_BC_INT_DIR, MX_BC_INT_DIR, sm, sp);
     ppm_trace(v[4], sm, sp, A.U[D][P.nb(i, j, k, D, 1)], A.U[D][P.at(i, j, k)], A.dt, A.dx[D],
               A.rel_eps, Ip, Im);
 }
 
-template <int D, class PT>
+template <int PPM, int D, class PT>
 __device__ __forceinline__ void tape_dir(const AB& A, const PT& P, const double* S, int i, int j,
                                          int k, long long p, double& ip, double& im, double& h) {
     const int pos = D == 0 ? i : (D == 1 ? j : k);
-    ppm_cell<D>(A, P, S, i, j, k, pos, ip, im);
+    ppm_cell<PPM, D>(A, P, S, i, j, k, pos, ip, im);
     double ipm, imm;
-    ppm_cell<D>(A, P, S, i - (D == 0), j - (D == 1), k - (D == 2), pos - 1, ipm, imm);
+    ppm_cell<PPM, D>(A, P, S, i - (D == 0), j - (D == 1), k - (D == 2), pos - 1, ipm, imm);
     h = upwind(ipm, im, A.U[D][p], 0.0);
 }
 
-template <bool P2>
+template <int PPM, bool P2>
 __global__ void __launch_bounds__(NTB) k_tape_ppm(AB A) {
     int i, j, k;
     long long p;
@@ -160,9 +162,9 @@ __global__ void __launch_bounds__(NTB) k_tape_ppm(AB A) {
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
     double ip[3], imv[3], h[3];
-    tape_dir<0>(A, P, S, i, j, k, p, ip[0], imv[0], h[0]);
-    tape_dir<1>(A, P, S, i, j, k, p, ip[1], imv[1], h[1]);
-    tape_dir<2>(A, P, S, i, j, k, p, ip[2], imv[2], h[2]);
+    tape_dir<PPM, 0>(A, P, S, i, j, k, p, ip[0], imv[0], h[0]);
+    tape_dir<PPM, 1>(A, P, S, i, j, k, p, ip[1], imv[1], h[1]);
+    tape_dir<PPM, 2>(A, P, S, i, j, k, p, ip[2], imv[2], h[2]);
     // every result is stored after the last load (stores and loads share
     // one in-order counter, so a load issued after a store waits for it)
 #pragma unroll
@@ -397,6 +399,44 @@ __global__ void __launch_bounds__(NTB) k_adj_fsum(AB A) {
     }
 }
 
+// adjoint of the characteristic trace (ppm_trace) of a cell: from the
+// adjoints Ipb, Imb of its traced states to its sm, sp, s0 and the velocities
+// of its upper and lower faces
+__device__ __forceinline__ void trace_bar(const AB& A, double s0, double sm, double sp,
+                                          double s6, double uhi, double ulo, double h,
+                                          double Ipb, double Imb, double& smb, double& spb,
+                                          double& s0b, double& uhib, double& ulob) {
+    smb = 0.0;
+    spb = 0.0;
+    s0b = 0.0;
+    uhib = 0.0;
+    ulob = 0.0;
+    if (uhi > A.rel_eps) {
+        const double sg = dabs(uhi) * A.dt / h;
+        const double f = 1.0 - C23 * sg;
+        const double Aq = sp - sm - f * s6;
+        spb += Ipb * (1.0 - 0.5 * sg * (1.0 + 3.0 * f));
+        smb += Ipb * (-0.5 * sg * (-1.0 + 3.0 * f));
+        s0b += Ipb * (3.0 * sg * f);
+        const double sgb = Ipb * (-0.5 * Aq - 0.5 * sg * (C23 * s6));
+        uhib += sgb * dabs_d(uhi) * A.dt / h;
+    } else {
+        s0b += Ipb;
+    }
+    if (ulo < -A.rel_eps) {
+        const double sg = dabs(ulo) * A.dt / h;
+        const double f = 1.0 - C23 * sg;
+        const double Bq = sp - sm + f * s6;
+        smb += Imb * (1.0 + 0.5 * sg * (-1.0 - 3.0 * f));
+        spb += Imb * (0.5 * sg * (1.0 - 3.0 * f));
+        s0b += Imb * (3.0 * sg * f);
+        const double sgb = Imb * (0.5 * Bq - 0.5 * sg * (C23 * s6));
+        ulob += sgb * dabs_d(ulo) * A.dt / h;
+    } else {
+        s0b += Imb;
+    }
+}
+
 // B6: characteristic trace and CW limiter of cell p in direction d (FS: the
 // face adjoints are read from k_adj_fsum's arrays instead of recomputed)
 template <bool P2, bool FS, bool ROLL>
@@ -456,31 +496,8 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
             // face p's totals also give its velocity adjoint for this component
             face_totals(A, P, co, i, j, k, d, true, unused2, Imb, ubd[ROLL ? 0 : d]);
         }
-        double smb = 0.0, spb = 0.0, s0b = 0.0, uhib = 0.0, ulob = 0.0;
-        if (uhi > A.rel_eps) {
-            const double sg = dabs(uhi) * A.dt / h;
-            const double f = 1.0 - C23 * sg;
-            const double Aq = sp - sm - f * s6;
-            spb += Ipb * (1.0 - 0.5 * sg * (1.0 + 3.0 * f));
-            smb += Ipb * (-0.5 * sg * (-1.0 + 3.0 * f));
-            s0b += Ipb * (3.0 * sg * f);
-            const double sgb = Ipb * (-0.5 * Aq - 0.5 * sg * (C23 * s6));
-            uhib += sgb * dabs_d(uhi) * A.dt / h;
-        } else {
-            s0b += Ipb;
-        }
-        if (ulo < -A.rel_eps) {
-            const double sg = dabs(ulo) * A.dt / h;
-            const double f = 1.0 - C23 * sg;
-            const double Bq = sp - sm + f * s6;
-            smb += Imb * (1.0 + 0.5 * sg * (-1.0 - 3.0 * f));
-            spb += Imb * (0.5 * sg * (1.0 - 3.0 * f));
-            s0b += Imb * (3.0 * sg * f);
-            const double sgb = Imb * (0.5 * Bq - 0.5 * sg * (C23 * s6));
-            ulob += sgb * dabs_d(ulo) * A.dt / h;
-        } else {
-            s0b += Imb;
-        }
+        double smb, spb, s0b, uhib, ulob;
+        trace_bar(A, s0, sm, sp, s6, uhi, ulo, h, Ipb, Imb, smb, spb, s0b, uhib, ulob);
         // CW limiter
         double sm0b = 0.0, sp0b = 0.0;
         if (br == 0) {
@@ -528,6 +545,264 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
         else
             A.UBC[d][co + p] = ubd[d] + r4[d];
     }
+}
+
+// ---------------------------------------------------------------- ppm_type 2
+// gradient of the Colella-Sekora face value cs_face(sa, sb, sc, sd) with
+// respect to its four cells, on the branch the forward takes
+__device__ __forceinline__ void cs_face_grad(double sa, double sb, double sc, double sd,
+                                             double g[4]) {
+    const double C = 1.25;
+    const double e = (7.0 / 12.0) * (sb + sc) - (1.0 / 12.0) * (sa + sd);
+    g[0] = -1.0 / 12.0;
+    g[1] = 7.0 / 12.0;
+    g[2] = 7.0 / 12.0;
+    g[3] = -1.0 / 12.0;
+    if ((e - sb) * (sc - e) < 0.0) {
+        // e = (sb + sc) / 2 - D2LIM / 6 with D2LIM = sgn * max(min(C sgn D2L,
+        // min(C sgn D2R, sgn D2)), 0); sgn is piecewise constant
+        const double D2 = 3.0 * (sb - 2.0 * e + sc);
+        const double D2L = sa - 2.0 * sb + sc;
+        const double D2R = sb - 2.0 * sc + sd;
+        const double sgn = dsign1(D2);
+        const double a1 = C * sgn * D2L, a2 = C * sgn * D2R, a3 = sgn * D2;
+        const double inner = dmin(a2, a3);
+        const bool in3 = a3 < a2;          // dmin(a2, a3) returns a3
+        const double outer = dmin(a1, inner);
+        const bool out_in = inner < a1;    // dmin(a1, inner) returns inner
+        const bool zero = outer < 0.0;     // dmax(outer, 0) returns 0
+        // d D2LIM / d(sa, sb, sc, sd); sgn^2 = 1
+        double l[4] = {0.0, 0.0, 0.0, 0.0};
+        if (!zero) {
+            if (!out_in) {
+                l[0] = C; l[1] = -2.0 * C; l[2] = C;                       // C D2L
+            } else if (!in3) {
+                l[1] = C; l[2] = -2.0 * C; l[3] = C;                       // C D2R
+            } else {
+                l[0] = 0.5; l[1] = -0.5; l[2] = -0.5; l[3] = 0.5;          // D2
+            }
+        }
+        g[0] = -l[0] / 6.0;
+        g[1] = 0.5 - l[1] / 6.0;
+        g[2] = 0.5 - l[2] / 6.0;
+        g[3] = -l[3] / 6.0;
+    }
+}
+
+// adjoint of the Colella-Sekora cell limiter (interior: threshold 1e-10) of
+// the cell with v[-2..2] = s(cell-2 .. cell+2) and faces fl, f0, fr, frr
+// (cell-1 .. cell+2): from smb, spb to the cells (vb[-2..2]) and to the
+// faces f0, fr (fl and frr enter only branch decisions)
+__device__ __forceinline__ void cs_cell_bar(const double* v, double fl, double f0, double fr,
+                                            double frr, double smb, double spb, double* vb,
+                                            double& f0b, double& frb) {
+    const double C = 1.25;
+    const double s0 = v[0];
+    const double ap0 = fr - s0;
+    const double am0 = f0 - s0;
+    const bool bigp = dabs(ap0) > 2.0 * dabs(am0);
+    const bool bigm = dabs(am0) > 2.0 * dabs(ap0);
+    bool extremum = false;
+    if (am0 * ap0 >= 0.0) {
+        extremum = true;
+    } else if (bigp || bigm) {
+        const double dafacem = f0 - fl, dafacep = frr - fr;
+        const double dabarm = s0 - v[-1], dabarp = v[1] - s0;
+        const double dafacemin = dmin(dabs(dafacem), dabs(dafacep));
+        const double dabarmin = dmin(dabs(dabarm), dabs(dabarp));
+        const bool face = dafacemin >= dabarmin;
+        const double dachkm = face ? dafacem : dabarm;
+        const double dachkp = face ? dafacep : dabarp;
+        extremum = dachkm * dachkp <= 0.0;
+    }
+    for (int o = -2; o <= 2; ++o) vb[o] = 0.0;
+    // sm = s0 + alpham, sp = s0 + alphap
+    vb[0] = smb + spb;
+    double amb = 0.0, apb = 0.0;  // adjoints of am0, ap0
+    if (extremum) {
+        const double D2 = 6.0 * (am0 + ap0);
+        const double D2L = v[-2] - 2.0 * v[-1] + s0;
+        const double D2R = s0 - 2.0 * v[1] + v[2];
+        const double D2C = v[-1] - 2.0 * s0 + v[1];
+        const double sgn = dsign1(D2);
+        const double t0 = sgn * D2, t1 = C * sgn * D2L, t2 = C * sgn * D2R, t3 = C * sgn * D2C;
+        // D2LIM = dmax(dmin(t0, dmin(t1, dmin(t2, t3))), 0)
+        const double m23 = dmin(t2, t3);
+        const int i23 = t3 < t2 ? 3 : 2;
+        const double m123 = dmin(t1, m23);
+        const int i123 = m23 < t1 ? i23 : 1;
+        const double m0123 = dmin(t0, m123);
+        const int imin = m123 < t0 ? i123 : 0;
+        const bool lzero = m0123 < 0.0;
+        const double D2LIM = lzero ? 0.0 : m0123;
+        const double D2ABS = dmax(dabs(D2), 1.e-10);
+        const bool absd = !(dabs(D2) < 1.e-10);  // dmax returns |D2|
+        // alpham = am0 * D2LIM / D2ABS, alphap = ap0 * D2LIM / D2ABS
+        amb = smb * D2LIM / D2ABS;
+        apb = spb * D2LIM / D2ABS;
+        const double wsum = smb * am0 + spb * ap0;
+        const double limb = wsum / D2ABS;
+        const double absb = -wsum * D2LIM / (D2ABS * D2ABS);
+        double D2b = absd ? absb * sgn : 0.0;  // d|D2|/dD2 = sgn
+        if (!lzero) {
+            if (imin == 0) {
+                D2b += limb * sgn;
+            } else {
+                const double w = limb * C * sgn;
+                if (imin == 1) {
+                    vb[-2] += w;
+                    vb[-1] -= 2.0 * w;
+                    vb[0] += w;
+                } else if (imin == 2) {
+                    vb[0] += w;
+                    vb[1] -= 2.0 * w;
+                    vb[2] += w;
+                } else {
+                    vb[-1] += w;
+                    vb[0] -= 2.0 * w;
+                    vb[1] += w;
+                }
+            }
+        }
+        amb += 6.0 * D2b;
+        apb += 6.0 * D2b;
+    } else {
+        amb = smb;
+        apb = spb;
+        if (bigp) {
+            const double sgn = dsign1(am0);
+            const double amax = -ap0 * ap0 / (4.0 * (am0 + ap0));
+            const double delam = v[-1] - s0;
+            if (sgn * amax >= sgn * delam) {
+                // alphap is replaced
+                apb = 0.0;
+                if (sgn * (delam - am0) >= 1.e-10) {
+                    const double q = delam * delam - delam * am0;
+                    const double r = sqrt(q);
+                    double delb = -2.0 * spb;
+                    if (r > 0.0) {
+                        delb += -spb * sgn * (2.0 * delam - am0) / r;
+                        amb += spb * sgn * delam / r;
+                    }
+                    vb[-1] += delb;
+                    vb[0] -= delb;
+                } else {
+                    amb += -2.0 * spb;
+                }
+            }
+        }
+        if (bigm) {
+            const double sgn = dsign1(ap0);
+            const double amax = -am0 * am0 / (4.0 * (am0 + ap0));
+            const double delap = v[1] - s0;
+            if (sgn * amax >= sgn * delap) {
+                amb = 0.0;
+                if (sgn * (delap - ap0) >= 1.e-10) {
+                    const double q = delap * delap - delap * ap0;
+                    const double r = sqrt(q);
+                    double delb = -2.0 * smb;
+                    if (r > 0.0) {
+                        delb += -smb * sgn * (2.0 * delap - ap0) / r;
+                        apb += smb * sgn * delap / r;
+                    }
+                    vb[1] += delb;
+                    vb[0] -= delb;
+                } else {
+                    apb += -2.0 * smb;
+                }
+            }
+        }
+    }
+    // am0 = f0 - s0, ap0 = fr - s0
+    f0b = amb;
+    frb = apb;
+    vb[0] -= amb + apb;
+}
+
+// B6 for ppm_type 2: trace and Colella-Sekora limiter of cell p in every
+// direction.  Per direction it stores the adjoints of its faces p (FLB) and
+// p + d (FRB), the direct adjoints of the cells p - 2d .. p + 2d other than p
+// (DV[d][0..3] for offsets -2, -1, 1, 2), the velocity terms, and adds its
+// direct term of s(p) to S0S.
+template <bool P2>
+__global__ void __launch_bounds__(NTB) k_adj_trace2(AB A) {
+    int i, j, k;
+    long long p;
+    if (!point(A, i, j, k, p)) return;
+    const PerT<P2> P = make_per<P2>(A.n);
+    const long long co = (long long)blockIdx.y * A.cs;
+    const double* S = A.S + co;
+    double s0s = 0.0;
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+        double w[7];
+        for (int o = -3; o <= 3; ++o) w[3 + o] = S[P.nb(i, j, k, d, o)];
+        const double fl = cs_face(w[0], w[1], w[2], w[3]);
+        const double f0 = cs_face(w[1], w[2], w[3], w[4]);
+        const double fr = cs_face(w[2], w[3], w[4], w[5]);
+        const double frr = cs_face(w[3], w[4], w[5], w[6]);
+        double v9[9];
+        for (int o = -4; o <= 4; ++o) v9[4 + o] = (o >= -3 && o <= 3) ? w[3 + o] : 0.0;
+        double sm, sp;
+        cs_cell(v9, fl, f0, fr, frr, 1.e-10, sm, sp);
+        const double s0 = w[3];
+        const long long fhi = P.nb(i, j, k, d, 1);
+        const double uhi = A.U[d][fhi], ulo = A.U[d][p];
+        const double s6 = 6.0 * s0 - 3.0 * (sm + sp);
+        double Ipb, Imb, ubd, unused0, unused1, unused2;
+        face_totals(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb, unused0,
+                    unused1);
+        face_totals(A, P, co, i, j, k, d, true, unused2, Imb, ubd);
+        double smb, spb, s0b, uhib, ulob;
+        trace_bar(A, s0, sm, sp, s6, uhi, ulo, A.dx[d], Ipb, Imb, smb, spb, s0b, uhib, ulob);
+        double vb5[5], f0b, frb;
+        cs_cell_bar(w + 3, fl, f0, fr, frr, smb, spb, vb5 + 2, f0b, frb);
+        s0s = d == 0 ? s0b + vb5[2] : s0s + (s0b + vb5[2]);
+        A.FLB[d][co + p] = f0b;
+        A.FRB[d][co + p] = frb;
+        A.DV[d][0][co + p] = vb5[0];
+        A.DV[d][1][co + p] = vb5[1];
+        A.DV[d][2][co + p] = vb5[3];
+        A.DV[d][3][co + p] = vb5[4];
+        A.UHI[d][co + p] = uhib;
+        A.UBC[d][co + p] = ubd + ulob;
+    }
+    A.S0S[co + p] = s0s;
+}
+
+// B7-B9 for ppm_type 2: the state adjoint of cell p gathers, per direction,
+// the direct limiter terms of the cells p - 2d .. p + 2d and the face
+// adjoints of the faces p - d .. p + 2d through their Colella-Sekora values
+template <bool P2>
+__global__ void __launch_bounds__(NTB) k_adj_state2(AB A) {
+    int i, j, k;
+    long long p;
+    if (!point(A, i, j, k, p)) return;
+    const PerT<P2> P = make_per<P2>(A.n);
+    const long long co = (long long)blockIdx.y * A.cs;
+    const double* S = A.S + co;
+    double s = A.G[co + p] + A.S0S[co + p];
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+        // direct limiter terms: cell c gives DV[o](c) to s(c + o)
+        s += A.DV[d][0][co + P.nb(i, j, k, d, 2)] + A.DV[d][1][co + P.nb(i, j, k, d, 1)] +
+             A.DV[d][2][co + P.nb(i, j, k, d, -1)] + A.DV[d][3][co + P.nb(i, j, k, d, -2)];
+        // faces f = p - d .. p + 2d (f: cells f - d | f, value from f - 2d .. f + d)
+        double w[7];
+        for (int o = -3; o <= 3; ++o) w[3 + o] = S[P.nb(i, j, k, d, o)];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int fo = m - 1;  // face offset from p
+            const long long f = P.nb(i, j, k, d, fo), fm = P.nb(i, j, k, d, fo - 1);
+            const double fb = A.FLB[d][co + f] + A.FRB[d][co + fm];
+            double g[4];
+            // inputs s(f - 2d .. f + d) = w[3 + fo - 2 .. 3 + fo + 1]
+            cs_face_grad(w[1 + fo], w[2 + fo], w[3 + fo], w[4 + fo], g);
+            // p is input 2 - fo of face p + fo
+            s += fb * g[2 - fo];
+        }
+    }
+    A.SBF(i, j, k, (int)blockIdx.y) = s;
 }
 
 // B7: adjoints through the PPM type 1 face value of d-face (i,j,k) (cells
@@ -800,12 +1075,14 @@ __global__ void __launch_bounds__(NTB) k_fill_zero(double* a, long long n) {
 }
 
 // the step's preconditions: one periodic box [0, n)^3 at level 0
-// ghosts: the step reads s and umac ghosts (>= 3 cells, >= 1 face); the
+// ghosts: the step reads s and umac ghosts (>= 3 cells for ppm_type 1, >= 4
+// for ppm_type 2, >= 1 face); the
 // adjoint reads only valid cells and faces 0..n-1
 int step_check(const mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp, int& n,
                bool ghosts = true) {
     const mx_adv_params& p = c->p;
-    if (p.dim != 3 || p.ppm_type != 1 || p.ppm_trace_forces != 0) return MX_ERR_UNSUPPORTED;
+    if (p.dim != 3 || (p.ppm_type != 1 && p.ppm_type != 2) || p.ppm_trace_forces != 0)
+        return MX_ERR_UNSUPPORTED;
     if (!p.is_periodic[0] || !p.is_periodic[1] || !p.is_periodic[2]) return MX_ERR_UNSUPPORTED;
     if (c->grids[0].size() != 1) return MX_ERR_UNSUPPORTED;
     const mx_box& b = c->grids[0][0];
@@ -818,7 +1095,10 @@ int step_check(const mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp,
     if ((long long)n * n * n * ncomp > 0x7fffffffLL) return MX_ERR_UNSUPPORTED;
     if (!s || !u) return MX_ERR_ARG;
     HBox v = hbox(b);
-    if (!covers(*s, ghosts ? grow(v, 3, 3) : v, ncomp)) return MX_ERR_BOX;
+    // the edge-state stencils of the step reach 3 (ppm_type 1) or 4
+    // (ppm_type 2) cells past the box
+    const int g = p.ppm_type == 2 ? 4 : 3;
+    if (!covers(*s, ghosts ? grow(v, 3, g) : v, ncomp)) return MX_ERR_BOX;
     for (int d = 0; d < 3; ++d)
         if (!covers(u[d], ghosts ? grow(nodal(v, d), 3, 1) : v, 1)) return MX_ERR_BOX;
     return MX_OK;
@@ -864,13 +1144,22 @@ int periodic_fill(mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp, in
 // the tape and reverse kernels of the step adjoint
 template <bool P2>
 int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, mx_fab* ubar) {
-    MX_LAUNCH(c, "k_tape_ppm", k_tape_ppm<P2>, gc, dim3(NTB), A);
+    const bool ppm2 = c->p.ppm_type == 2;
+    if (ppm2)
+        MX_LAUNCH(c, "k_tape_ppm", (k_tape_ppm<2, P2>), gc, dim3(NTB), A);
+    else
+        MX_LAUNCH(c, "k_tape_ppm", (k_tape_ppm<1, P2>), gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_tape_corner", k_tape_corner<P2>, gc, dim3(NTB), A);
     MX_LAUNCH(c, "k_tape_edge", k_tape_edge<P2>, gc, dim3(NTB), A);
     // the transverse-term and trace gathers run faster with compare
     // indexing on MI355X (measured), whatever n is
     MX_LAUNCH(c, "k_adj_tterm", k_adj_tterm<false>, gc, dim3(NTB), A);
-    if (c->adj_face_sums) {
+    if (ppm2) {
+        // ppm_type 2: the Colella-Sekora trace and state sweeps (the kernel
+        // arrangement options apply to ppm_type 1)
+        MX_LAUNCH(c, "k_adj_trace", k_adj_trace2<false>, gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_state", k_adj_state2<P2>, gc, dim3(NTB), A);
+    } else if (c->adj_face_sums) {
         MX_LAUNCH(c, "k_adj_fsum", k_adj_fsum<P2>, gc, dim3(NTB), A);
         MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, true, false>), gc, dim3(NTB), A);
     } else if (c->adj_trace_rolled) {
@@ -878,7 +1167,9 @@ int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, mx_fab* ubar) {
     } else {
         MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, false, false>), gc, dim3(NTB), A);
     }
-    if (c->adj_state_fused) {
+    if (ppm2) {
+        // the state sweep ran above
+    } else if (c->adj_state_fused) {
         MX_LAUNCH(c, "k_adj_state_fused", k_adj_state_fused<P2>, gc, dim3(NTB), A);
     } else {
         MX_LAUNCH(c, "k_adj_face", k_adj_face<P2>, gc, dim3(NTB), A);
@@ -889,7 +1180,7 @@ int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, mx_fab* ubar) {
     int has_n = 0;
     for (int d = 0; d < 3; ++d)
         if (ubar[d].end[d] > A.n) has_n |= 1 << d;
-    if (c->adj_face_sums)
+    if (c->adj_face_sums && !ppm2)
         MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<P2, false>), g1, dim3(NTB), A, to_da(ubar[0]),
                   to_da(ubar[1]), to_da(ubar[2]), has_n);
     else
@@ -972,7 +1263,8 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
     // dense arrays: S and G (ncomp each), 63 per-component intermediates and
     // adjoints, U (3)
     const size_t per = (size_t)n3 * sizeof(double) + 256;
-    const size_t count = (size_t)ncomp * (63 + (c->adj_face_sums ? 9 : 0)) + 3;
+    const bool fs = c->adj_face_sums && c->p.ppm_type == 1;
+    const size_t count = (size_t)ncomp * (63 + (fs ? 9 : 0) + (c->p.ppm_type == 2 ? 12 : 0)) + 3;
     if ((rc = ws_reserve(c, per * count + 4096))) return rc;
     AB A;
     std::memset(&A, 0, sizeof(A));
@@ -991,14 +1283,16 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
         A.SE[d] = take(ncomp);
         A.TB[d] = take(ncomp);
         A.UBC[d] = take(ncomp);
-        if (c->adj_face_sums) {
+        if (c->p.ppm_type == 2)
+            for (int o = 0; o < 4; ++o) A.DV[d][o] = take(ncomp);
+        if (fs) {
             A.SLB[d] = take(ncomp);
             A.SRB[d] = take(ncomp);
         }
         A.FLB[d] = take(ncomp);
         A.FRB[d] = take(ncomp);
         A.UHI[d] = take(ncomp);
-        if (c->adj_face_sums) A.ULO[d] = take(ncomp);
+        if (fs) A.ULO[d] = take(ncomp);
         A.DSA[d] = take(ncomp);
         A.DSB[d] = take(ncomp);
         A.DLA[d] = take(ncomp);

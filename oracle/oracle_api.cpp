@@ -61,8 +61,8 @@ int MakeRhoHFlux(const Ctx& c, int lev, const std::vector<Arr> sflux[3],
 void CelltoEdge(int lev, int nr, const mx_basestate* cell, mx_basestate* edge);
 int MakeEdgeState1d(const Ctx* c, const mx_radial_geom* rg, const mx_basestate* s,
                     mx_basestate* sedge, const mx_basestate* force, const mx_basestate* w0);
-void ScalStep(int n, int ncomp, double dt, const double dx[3], double rel_eps, const double* s,
-              const double* const u[3], double* snew, const double* ds,
+void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double rel_eps,
+              const double* s, const double* const u[3], double* snew, const double* ds,
               const double* const du[3], double* dsnew);
 void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const double* s,
                  const double* const u[3], double out[7]);
@@ -315,25 +315,27 @@ int mxo_make_edge_state_1d(Ctx* c, const mx_radial_geom* rg, const mx_basestate*
 
 // The differentiable scalar step on a periodic box (dense N^3 arrays, face N
 // is face 0) and its Jacobian-vector product (du[d] may be null).
-int mxo_scal_step(int n, int ncomp, double dt, const double* dx, double rel_eps,
+int mxo_scal_step(int n, int ncomp, int ppm, double dt, const double* dx, double rel_eps,
                   const double* s, const double* u0, const double* u1, const double* u2,
                   double* snew) {
-    if (n < 4 || ncomp < 1 || !dx || !s || !u0 || !u1 || !u2 || !snew) return MX_ERR_ARG;
+    if (n < 4 || ncomp < 1 || (ppm != 1 && ppm != 2) || !dx || !s || !u0 || !u1 || !u2 || !snew)
+        return MX_ERR_ARG;
     const double* u[3] = {u0, u1, u2};
     const double* du[3] = {nullptr, nullptr, nullptr};
-    ScalStep(n, ncomp, dt, dx, rel_eps, s, u, snew, nullptr, du, nullptr);
+    ScalStep(n, ncomp, ppm, dt, dx, rel_eps, s, u, snew, nullptr, du, nullptr);
     return MX_OK;
 }
 
-int mxo_scal_step_jvp(int n, int ncomp, double dt, const double* dx, double rel_eps,
+int mxo_scal_step_jvp(int n, int ncomp, int ppm, double dt, const double* dx, double rel_eps,
                       const double* s, const double* u0, const double* u1, const double* u2,
                       const double* ds, const double* du0, const double* du1,
                       const double* du2, double* snew, double* dsnew) {
-    if (n < 4 || ncomp < 1 || !dx || !s || !u0 || !u1 || !u2 || !ds || !snew || !dsnew)
+    if (n < 4 || ncomp < 1 || (ppm != 1 && ppm != 2) || !dx || !s || !u0 || !u1 || !u2 || !ds ||
+        !snew || !dsnew)
         return MX_ERR_ARG;
     const double* u[3] = {u0, u1, u2};
     const double* du[3] = {du0, du1, du2};
-    ScalStep(n, ncomp, dt, dx, rel_eps, s, u, snew, ds, du, dsnew);
+    ScalStep(n, ncomp, ppm, dt, dx, rel_eps, s, u, snew, ds, du, dsnew);
     return MX_OK;
 }
 

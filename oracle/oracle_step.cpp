@@ -4,7 +4,7 @@
 //
 // The step, on one fully periodic box [0, N)^3 with a fixed MAC velocity
 // (face N is face 0), for each scalar component:
-//   sedge = MakeEdgeScal(s, umac, force = 0, advective form, ppm_type 1)
+//   sedge = MakeEdgeScal(s, umac, force = 0, advective form, ppm_type 1 or 2)
 //           (MaestroMakeEdgeScalars.cpp: PPM, Predictor :662-863, Transverse
 //            :865-1359, Edges :1361-1667; interior formulas only, periodic)
 //   F     = umac * sedge                     (MakeRhoXFlux, predict_rhoX,
@@ -55,6 +55,8 @@ struct CmpD {
 inline bool operator<(CmpD a, CmpD b) { ++g_ops; return a.v < b.v; }
 inline bool operator<(CmpD a, double b) { ++g_ops; return a.v < b; }
 inline bool operator>(CmpD a, double b) { ++g_ops; return a.v > b; }
+inline bool operator>(CmpD a, CmpD b) { ++g_ops; return a.v > b.v; }
+inline bool operator>=(CmpD a, double b) { ++g_ops; return a.v >= b; }
 inline bool operator<=(CmpD a, double b) { ++g_ops; return a.v <= b; }
 inline bool operator>=(CmpD a, CmpD b) { ++g_ops; return a.v >= b.v; }
 inline Cnt operator+(Cnt a, Cnt b) { ++g_ops; return {a.v + b.v}; }
@@ -134,6 +136,104 @@ void ppm1(const T* v, T& sm, T& sp) {
     }
 }
 
+inline double tsqrt(double a) { return std::sqrt(a); }
+inline Dual tsqrt(Dual a) {
+    const double r = std::sqrt(a.v);
+    return {r, r > 0.0 ? a.d / (2.0 * r) : 0.0};
+}
+inline Cnt tsqrt(Cnt a) {
+    ++g_ops;
+    return {std::sqrt(a.v)};
+}
+
+// PPM type 2 (Colella-Sekora) face value between cells sb and sc, from the
+// cells sa, sb, sc, sd (MaestroPPM.cpp:356-381, interior formula)
+template <class T>
+T cs_face(T sa, T sb, T sc, T sd) {
+    const double C = 1.25;
+    T e = (7.0 / 12.0) * (sb + sc) - (1.0 / 12.0) * (sa + sd);
+    if (val((e - sb) * (sc - e)) < 0.0) {
+        T D2 = 3.0 * (sb - 2.0 * e + sc);
+        T D2L = sa - 2.0 * sb + sc;
+        T D2R = sb - 2.0 * sc + sd;
+        double sgn = sgn1(D2);
+        T D2LIM = sgn * tmax(tmin(C * sgn * D2L, tmin(C * sgn * D2R, sgn * D2)), cst<T>(0.0));
+        e = 0.5 * (sb + sc) - D2LIM / 6.0;
+    }
+    return e;
+}
+
+// PPM type 2 cell limiter (MaestroPPM.cpp:383-441, interior: threshold 1e-10);
+// v[-2..2] = s(cell-2 .. cell+2), faces fl, f0, fr, frr = cell-1 .. cell+2
+template <class T>
+void cs_cell(const T* v, T fl, T f0, T fr, T frr, T& sm, T& sp) {
+    const double C = 1.25;
+    const T s0 = v[0];
+    T alphap = fr - s0;
+    T alpham = f0 - s0;
+    const bool bigp = val(tabs(alphap)) > val(2.0 * tabs(alpham));
+    const bool bigm = val(tabs(alpham)) > val(2.0 * tabs(alphap));
+    bool extremum = false;
+    if (val(alpham * alphap) >= 0.0) {
+        extremum = true;
+    } else if (bigp || bigm) {
+        T dafacem = f0 - fl;
+        T dafacep = frr - fr;
+        T dabarm = s0 - v[-1];
+        T dabarp = v[1] - s0;
+        T dafacemin = tmin(tabs(dafacem), tabs(dafacep));
+        T dabarmin = tmin(tabs(dabarm), tabs(dabarp));
+        T dachkm, dachkp;
+        if (val(dafacemin) >= val(dabarmin)) {
+            dachkm = dafacem;
+            dachkp = dafacep;
+        } else {
+            dachkm = dabarm;
+            dachkp = dabarp;
+        }
+        extremum = val(dachkm * dachkp) <= 0.0;
+    }
+    if (extremum) {
+        T D2 = 6.0 * (alpham + alphap);
+        T D2L = v[-2] - 2.0 * v[-1] + s0;
+        T D2R = s0 - 2.0 * v[1] + v[2];
+        T D2C = v[-1] - 2.0 * s0 + v[1];
+        double sgn = sgn1(D2);
+        T D2LIM = tmax(tmin(sgn * D2, tmin(C * sgn * D2L, tmin(C * sgn * D2R, C * sgn * D2C))),
+                       cst<T>(0.0));
+        T D2ABS = tmax(tabs(D2), cst<T>(1.e-10));
+        alpham = alpham * D2LIM / D2ABS;
+        alphap = alphap * D2LIM / D2ABS;
+    } else {
+        if (bigp) {
+            double sgn = sgn1(alpham);
+            T amax = -alphap * alphap / (4.0 * (alpham + alphap));
+            T delam = v[-1] - s0;
+            if (val(sgn * amax) >= val(sgn * delam)) {
+                if (val(sgn * (delam - alpham)) >= 1.e-10) {
+                    alphap = -2.0 * delam - 2.0 * sgn * tsqrt(delam * delam - delam * alpham);
+                } else {
+                    alphap = -2.0 * alpham;
+                }
+            }
+        }
+        if (bigm) {
+            double sgn = sgn1(alphap);
+            T amax = -alpham * alpham / (4.0 * (alpham + alphap));
+            T delap = v[1] - s0;
+            if (val(sgn * amax) >= val(sgn * delap)) {
+                if (val(sgn * (delap - alphap)) >= 1.e-10) {
+                    alpham = -2.0 * delap - 2.0 * sgn * tsqrt(delap * delap - delap * alphap);
+                } else {
+                    alpham = -2.0 * alphap;
+                }
+            }
+        }
+    }
+    sm = s0 + alpham;
+    sp = s0 + alphap;
+}
+
 template <class T>
 void trace(T s0, T sm, T sp, T uhi, T ulo, double dt, double h, double rel_eps, T& Ip, T& Im) {
     T s6 = 6.0 * s0 - 3.0 * (sm + sp);
@@ -174,7 +274,7 @@ inline void step_e(int d, int& a, int& b, int& c) {
 // marks (op counting only): the counter after each of the five phases
 template <class T>
 void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double dt,
-               const double dx[3], double rel_eps, long long* marks = nullptr) {
+               const double dx[3], double rel_eps, long long* marks = nullptr, int ppm = 1) {
     const int n = g.n;
     const long long N3 = (long long)n * n * n;
     std::vector<T> Ip[3], Im[3], simh[3], corner[6], se[3];
@@ -194,11 +294,20 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
                 for (int d = 0; d < 3; ++d) {
                     int a, b, c;
                     step_e(d, a, b, c);
-                    T v[5];
-                    for (int o = -2; o <= 2; ++o) v[o + 2] = s[g.at(i + o * a, j + o * b, k + o * c)];
+                    T v[7];
+                    for (int o = -3; o <= 3; ++o) v[o + 3] = s[g.at(i + o * a, j + o * b, k + o * c)];
                     T sm, sp;
-                    ppm1(v + 2, sm, sp);
-                    trace(v[2], sm, sp, u[d][g.at(i + a, j + b, k + c)], u[d][g.at(i, j, k)], dt,
+                    if (ppm == 1) {
+                        ppm1(v + 3, sm, sp);
+                    } else {
+                        // faces cell-1 .. cell+2
+                        const T fl = cs_face(v[0], v[1], v[2], v[3]);
+                        const T f0 = cs_face(v[1], v[2], v[3], v[4]);
+                        const T fr = cs_face(v[2], v[3], v[4], v[5]);
+                        const T frr = cs_face(v[3], v[4], v[5], v[6]);
+                        cs_cell(v + 3, fl, f0, fr, frr, sm, sp);
+                    }
+                    trace(v[3], sm, sp, u[d][g.at(i + a, j + b, k + c)], u[d][g.at(i, j, k)], dt,
                           dx[d], rel_eps, Ip[d][g.at(i, j, k)], Im[d][g.at(i, j, k)]);
                 }
     if (marks) marks[0] = g_ops;
@@ -340,13 +449,14 @@ void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const dou
 
 // the step (T = double) or its directional derivative (ds, du != null):
 // s, snew: ncomp x N^3; u[d]: N^3 (face N is face 0)
-void ScalStep(int n, int ncomp, double dt, const double dx[3], double rel_eps, const double* s,
-              const double* const u[3], double* snew, const double* ds,
+void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double rel_eps,
+              const double* s, const double* const u[3], double* snew, const double* ds,
               const double* const du[3], double* dsnew) {
     const Grid g{n};
     const long long N3 = (long long)n * n * n;
     if (!ds) {
-        for (int c = 0; c < ncomp; ++c) step_comp<double>(g, s + c * N3, u, snew + c * N3, dt, dx, rel_eps);
+        for (int c = 0; c < ncomp; ++c)
+            step_comp<double>(g, s + c * N3, u, snew + c * N3, dt, dx, rel_eps, nullptr, ppm);
         return;
     }
     std::vector<Dual> ud[3];
@@ -359,7 +469,7 @@ void ScalStep(int n, int ncomp, double dt, const double dx[3], double rel_eps, c
     std::vector<Dual> sd(N3), out(N3);
     for (int c = 0; c < ncomp; ++c) {
         for (long long x = 0; x < N3; ++x) sd[x] = {s[c * N3 + x], ds[c * N3 + x]};
-        step_comp<Dual>(g, sd.data(), up, out.data(), dt, dx, rel_eps);
+        step_comp<Dual>(g, sd.data(), up, out.data(), dt, dx, rel_eps, nullptr, ppm);
         for (long long x = 0; x < N3; ++x) {
             snew[c * N3 + x] = out[x].v;
             dsnew[c * N3 + x] = out[x].d;

@@ -76,8 +76,9 @@ def step_op_count(s, u, dt, dx, rel_eps):
     return dict(zip(keys, out.tolist()))
 
 
-def scal_step(s, u, dt, dx, rel_eps, ds=None, du=None):
-    """The differentiable scalar step on a periodic N^3 box (oracle_step.cpp).
+def scal_step(s, u, dt, dx, rel_eps, ds=None, du=None, ppm_type=1):
+    """The differentiable scalar step on a periodic N^3 box (oracle_step.cpp),
+    ppm_type 1 or 2.
     s: (ncomp, N, N, N) indexed [c, k, j, i]; u: three (N, N, N) face arrays
     (face index = the cell above it; face N is face 0).  Returns snew, or
     (snew, dsnew) when the direction ds (and optionally du) is given."""
@@ -91,8 +92,8 @@ def scal_step(s, u, dt, dx, rel_eps, ds=None, du=None):
     if ds is None:
         f = lib.mxo_scal_step
         f.restype = C.c_int
-        rc = f(n, ncomp, C.c_double(dt), dxa, C.c_double(rel_eps), _dptr(s), *[_dptr(x) for x in u],
-               _dptr(snew))
+        rc = f(n, ncomp, ppm_type, C.c_double(dt), dxa, C.c_double(rel_eps), _dptr(s),
+               *[_dptr(x) for x in u], _dptr(snew))
         if rc:
             raise _abi.MxError(rc, "oracle.scal_step")
         return snew
@@ -103,9 +104,9 @@ def scal_step(s, u, dt, dx, rel_eps, ds=None, du=None):
     f = lib.mxo_scal_step_jvp
     f.restype = C.c_int
     null = C.POINTER(C.c_double)()
-    rc = f(n, ncomp, C.c_double(dt), dxa, C.c_double(rel_eps), _dptr(s), *[_dptr(x) for x in u],
-           _dptr(ds), *[(_dptr(x) if x is not None else null) for x in dun], _dptr(snew),
-           _dptr(dsnew))
+    rc = f(n, ncomp, ppm_type, C.c_double(dt), dxa, C.c_double(rel_eps), _dptr(s),
+           *[_dptr(x) for x in u], _dptr(ds), *[(_dptr(x) if x is not None else null) for x in dun],
+           _dptr(snew), _dptr(dsnew))
     if rc:
         raise _abi.MxError(rc, "oracle.scal_step_jvp")
     return snew, dsnew

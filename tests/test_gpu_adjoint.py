@@ -5,7 +5,8 @@ divergence-free MAC velocity:
 * the GPU step equals the oracle's step bit for bit;
 * dot-product test: <v, J w> from the oracle's forward-mode derivative
   (dual numbers, oracle/oracle_step.cpp) equals <J^T v, w> from the GPU
-  adjoint, for random w in the state and the velocity, to 1e-12 relative;
+  adjoint, for random w in the state and the velocity, to 1e-12 relative
+  (ppm_type 1 and 2);
 * the torch.autograd.Function of one step returns the adjoint's gradients;
 * checkpointing every 5 steps gives the same gradient as no checkpoint;
 * central finite differences (eps = 1e-6) of a 20-step rollout's loss at
@@ -34,9 +35,9 @@ def setup_case(n=16, ncomp=2, seed=0):
     return s, u, dt, 1e-8
 
 
-def stepper_for(n, ncomp, dt, rel_eps):
+def stepper_for(n, ncomp, dt, rel_eps, ppm_type=1):
     from maestroex_amd.adjoint import ScalStepper
-    return ScalStepper(n, ncomp, dt, rel_eps)
+    return ScalStepper(n, ncomp, dt, rel_eps, ppm_type=ppm_type)
 
 
 def to_dev(a):
@@ -44,27 +45,28 @@ def to_dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
 
 
-def test_step_forward_parity():
+@pytest.mark.parametrize("ppm", [1, 2])
+def test_step_forward_parity(ppm):
     s, u, dt, eps = setup_case()
     n, nc = s.shape[1], s.shape[0]
-    print(f"[parity] scal_step n={n} ncomp={nc}", flush=True)
-    st = stepper_for(n, nc, dt, eps)
+    print(f"[parity] scal_step n={n} ncomp={nc} ppm_type={ppm}", flush=True)
+    st = stepper_for(n, nc, dt, eps, ppm)
     g = st.step(to_dev(s), tuple(to_dev(x) for x in u)).cpu().numpy()
-    o = pyoracle.scal_step(s, u, dt, [1.0 / n] * 3, eps)
+    o = pyoracle.scal_step(s, u, dt, [1.0 / n] * 3, eps, ppm_type=ppm)
     assert np.array_equal(g, o), f"{int((g != o).sum())} values differ"
 
 
-@pytest.mark.parametrize("n", [16, 12])
-def test_vjp_dot_product(n):
+@pytest.mark.parametrize("n,ppm", [(16, 1), (12, 1), (16, 2), (12, 2)])
+def test_vjp_dot_product(n, ppm):
     s, u, dt, eps = setup_case(n=n, seed=3)
     n, nc = s.shape[1], s.shape[0]
-    print("[parity] scal_step vjp dot product", flush=True)
+    print(f"[parity] scal_step vjp dot product n={n} ppm_type={ppm}", flush=True)
     rng = np.random.default_rng(11)
     ws = rng.standard_normal(s.shape)
     wu = [rng.standard_normal(u[0].shape) for _ in range(3)]
     v = rng.standard_normal(s.shape)
-    _, jw = pyoracle.scal_step(s, u, dt, [1.0 / n] * 3, eps, ws, wu)
-    st = stepper_for(n, nc, dt, eps)
+    _, jw = pyoracle.scal_step(s, u, dt, [1.0 / n] * 3, eps, ws, wu, ppm_type=ppm)
+    st = stepper_for(n, nc, dt, eps, ppm)
     sb, ub = st.vjp(to_dev(s), tuple(to_dev(x) for x in u), to_dev(v))
     sb = sb.cpu().numpy()
     ub = [x.cpu().numpy() for x in ub]
@@ -73,7 +75,7 @@ def test_vjp_dot_product(n):
     print(f"[adjoint] <v, J w> = {lhs!r}  <J^T v, w> = {rhs!r}", flush=True)
     assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
     # the state part alone (w in the velocity zero)
-    _, jws = pyoracle.scal_step(s, u, dt, [1.0 / n] * 3, eps, ws, None)
+    _, jws = pyoracle.scal_step(s, u, dt, [1.0 / n] * 3, eps, ws, None, ppm_type=ppm)
     assert abs(float(np.sum(v * jws)) - float(np.sum(sb * ws))) <= 1e-12 * abs(float(np.sum(v * jws)))
 
 
