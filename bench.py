@@ -280,7 +280,7 @@ def adjoint_main(args):
                    + 0.05 * rng.standard_normal((n, n, n)) for c in range(nc)])
     s = torch.from_numpy(s0).to("cuda:0")
     tgt = torch.from_numpy(rng.standard_normal(s0.shape)).to("cuda:0")
-    st = ScalStepper(n, nc, 0.5 / n, 1e-8)
+    st = ScalStepper(n, nc, 0.5 / n, 1e-8, ppm_type=args.ppm)
     for kv in args.tune:
         key, val = kv.split("=")
         st.m.set_tuning(key, int(val))
@@ -323,7 +323,7 @@ def adjoint_main(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "differentiable scalar step: 128^3 periodic, 4 scalars, fixed "
-                               "divergence-free MAC velocity, ppm_type 1",
+                               f"divergence-free MAC velocity, ppm_type {args.ppm}",
                    "global_batch": cells, "parallelism": "single GPU"},
         "adjoint": {"forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4),
                     "adjoint_over_forward": round(adj_ms / fwd_ms, 3),

@@ -402,39 +402,38 @@ __global__ void __launch_bounds__(NTB) k_adj_fsum(AB A) {
 // adjoint of the characteristic trace (ppm_trace) of a cell: from the
 // adjoints Ipb, Imb of its traced states to its sm, sp, s0 and the velocities
 // of its upper and lower faces
-__device__ __forceinline__ void trace_bar(const AB& A, double s0, double sm, double sp,
-                                          double s6, double uhi, double ulo, double h,
-                                          double Ipb, double Imb, double& smb, double& spb,
-                                          double& s0b, double& uhib, double& ulob) {
-    smb = 0.0;
-    spb = 0.0;
-    s0b = 0.0;
-    uhib = 0.0;
-    ulob = 0.0;
+struct TraceBar {
+    double smb, spb, s0b, uhib, ulob;
+};
+__device__ __forceinline__ TraceBar trace_bar(const AB& A, double s0, double sm, double sp,
+                                              double s6, double uhi, double ulo, double h,
+                                              double Ipb, double Imb) {
+    TraceBar r = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (uhi > A.rel_eps) {
         const double sg = dabs(uhi) * A.dt / h;
         const double f = 1.0 - C23 * sg;
         const double Aq = sp - sm - f * s6;
-        spb += Ipb * (1.0 - 0.5 * sg * (1.0 + 3.0 * f));
-        smb += Ipb * (-0.5 * sg * (-1.0 + 3.0 * f));
-        s0b += Ipb * (3.0 * sg * f);
+        r.spb += Ipb * (1.0 - 0.5 * sg * (1.0 + 3.0 * f));
+        r.smb += Ipb * (-0.5 * sg * (-1.0 + 3.0 * f));
+        r.s0b += Ipb * (3.0 * sg * f);
         const double sgb = Ipb * (-0.5 * Aq - 0.5 * sg * (C23 * s6));
-        uhib += sgb * dabs_d(uhi) * A.dt / h;
+        r.uhib += sgb * dabs_d(uhi) * A.dt / h;
     } else {
-        s0b += Ipb;
+        r.s0b += Ipb;
     }
     if (ulo < -A.rel_eps) {
         const double sg = dabs(ulo) * A.dt / h;
         const double f = 1.0 - C23 * sg;
         const double Bq = sp - sm + f * s6;
-        smb += Imb * (1.0 + 0.5 * sg * (-1.0 - 3.0 * f));
-        spb += Imb * (0.5 * sg * (1.0 - 3.0 * f));
-        s0b += Imb * (3.0 * sg * f);
+        r.smb += Imb * (1.0 + 0.5 * sg * (-1.0 - 3.0 * f));
+        r.spb += Imb * (0.5 * sg * (1.0 - 3.0 * f));
+        r.s0b += Imb * (3.0 * sg * f);
         const double sgb = Imb * (0.5 * Bq - 0.5 * sg * (C23 * s6));
-        ulob += sgb * dabs_d(ulo) * A.dt / h;
+        r.ulob += sgb * dabs_d(ulo) * A.dt / h;
     } else {
-        s0b += Imb;
+        r.s0b += Imb;
     }
+    return r;
 }
 
 // B6: characteristic trace and CW limiter of cell p in direction d (FS: the
@@ -496,8 +495,9 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
             // face p's totals also give its velocity adjoint for this component
             face_totals(A, P, co, i, j, k, d, true, unused2, Imb, ubd[ROLL ? 0 : d]);
         }
-        double smb, spb, s0b, uhib, ulob;
-        trace_bar(A, s0, sm, sp, s6, uhi, ulo, h, Ipb, Imb, smb, spb, s0b, uhib, ulob);
+        const TraceBar tr = trace_bar(A, s0, sm, sp, s6, uhi, ulo, h, Ipb, Imb);
+        const double smb = tr.smb, spb = tr.spb, uhib = tr.uhib, ulob = tr.ulob;
+        double s0b = tr.s0b;
         // CW limiter
         double sm0b = 0.0, sp0b = 0.0;
         if (br == 0) {
@@ -753,8 +753,8 @@ __global__ void __launch_bounds__(NTB) k_adj_trace2(AB A) {
         face_totals(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb, unused0,
                     unused1);
         face_totals(A, P, co, i, j, k, d, true, unused2, Imb, ubd);
-        double smb, spb, s0b, uhib, ulob;
-        trace_bar(A, s0, sm, sp, s6, uhi, ulo, A.dx[d], Ipb, Imb, smb, spb, s0b, uhib, ulob);
+        const TraceBar tr = trace_bar(A, s0, sm, sp, s6, uhi, ulo, A.dx[d], Ipb, Imb);
+        const double smb = tr.smb, spb = tr.spb, s0b = tr.s0b, uhib = tr.uhib, ulob = tr.ulob;
         double vb5[5], f0b, frb;
         cs_cell_bar(w + 3, fl, f0, fr, frr, smb, spb, vb5 + 2, f0b, frb);
         s0s = d == 0 ? s0b + vb5[2] : s0s + (s0b + vb5[2]);
