@@ -46,6 +46,10 @@ PASS_BYTES_PER_CELL = {3: 488, 2: 352}
 PASS_BYTES_PER_CELL_SPHERICAL = 536
 
 
+# edge_scal_tile values: tile shape (cell columns) and threads per block
+TILE_NAMES = ["32x8", "16x16", "16x12", "16x8"]
+TILE_THREADS = [384, 384, 256, 192]
+
 def kernel_bytes_per_cell(name, nc, dim):
     """Algorithmic bytes per valid cell of one launch of each kernel: its
     compulsory inputs read once and its outputs written once (FP64)."""
@@ -202,15 +206,11 @@ def amr_main(args):
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     m = mx.gpu_maestro(h.params, h.boxes)
     fields = [h.pass_fields(l) for l in range(h.nlev)]
-    for kv in args.tune:
+    for kv in args.tune:  # keys other than the edge kernel's shape, which the autotuner sets
         key, val = kv.split("=")
         m.set_tuning(key, int(val))
-        if key == "edge_scal_tile":
-            shape["tile"] = ["32x8", "16x16"][int(val)]
-        elif key == "edge_scal_tz":
-            shape["tz"] = int(val)
     cands, best = m.autotune(fields)
-    tuned = {"tile": ["32x8", "16x16"][cands[best][0]], "tz": cands[best][1]}
+    tuned = {"tile": TILE_NAMES[cands[best][0]], "tz": cands[best][1]}
     print(f"[bench] autotune {cands} -> {tuned}", file=sys.stderr, flush=True)
     eager_ms = time_ms(lambda: m.advance_pass_levels(fields), args.steps, args.warmup)
     g = m.pass_graph(fields)
@@ -371,12 +371,12 @@ def main():
     lib, fns = mx.load()
     fields = w.pass_fields()
     tuned = None
-    shape = {"tile": "32x8", "tz": 128}  # the library's defaults
+    shape = {"tile": "16x12", "tz": 32}  # the library's defaults
     if args.autotune and not args.no_autotune:
         # the autotuner picks the fused edge kernel's tile and z-chunk on
         # this workload (untimed, before the warm-up)
         cands, best = m.autotune([fields])
-        tuned = {"tile": ["32x8", "16x16"][cands[best][0]], "tz": cands[best][1]}
+        tuned = {"tile": TILE_NAMES[cands[best][0]], "tz": cands[best][1]}
         shape = dict(tuned)
         print(f"[bench] autotune {[(c[0], c[1], round(c[2], 3)) for c in cands]} -> {tuned}",
               file=sys.stderr, flush=True)
@@ -384,7 +384,7 @@ def main():
         key, val = kv.split("=")
         m.set_tuning(key, int(val))
         if key == "edge_scal_tile":
-            shape["tile"] = ["32x8", "16x16"][int(val)]
+            shape["tile"] = TILE_NAMES[int(val)]
         elif key == "edge_scal_tz":
             shape["tz"] = int(val)
 
@@ -489,7 +489,8 @@ def main():
                    "global_batch": cells, "parallelism": "single GPU",
                    "pass_launch": "one HIP graph launch per pass",
                    "eager_ms_per_step": round(eager_ms, 4),
-                   "edge_kernel_launch": shape, "autotuned": tuned is not None},
+                   "edge_kernel_launch": dict(shape, threads=TILE_THREADS[TILE_NAMES.index(shape["tile"])]),
+                   "autotuned": tuned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -187,9 +187,10 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
                         no traced forces runs as one z-marching kernel;
                         0: the three-kernel form (PPM, transverse, edges),
                         which also serves ppm_type 0, traced forces and 2-D.
-     "edge_scal_tz"     planes per z-chunk of the fused kernel (default 128).
-     "edge_scal_tile"   cell columns per block of the fused kernel:
-                        0 (default) 32 x 8, 1: 16 x 16.
+     "edge_scal_tz"     planes per z-chunk of the fused kernel (default 32).
+     "edge_scal_tile"   cell columns per block of the fused kernel, and the
+                        block size: 0: 32 x 8 (384 threads), 1: 16 x 16 (384),
+                        2 (default): 16 x 12 (256), 3: 16 x 8 (192).
    Unknown keys and out-of-range values return MX_ERR_ARG. */
 int mx_adv_set_tuning(mx_adv_ctx* ctx, const char* key, long long value);
 
@@ -370,8 +371,8 @@ int mx_adv_pass_graph_destroy(mx_adv_graph* g);
 
 /* Autotuner of the fused edge-state kernel (no counterpart in the
    reference): runs the hierarchy pass once, then times the species edge-state
-   launch for every tile shape (32x8, 16x16 columns) and z-chunk (16, 32, 64,
-   128 planes), keeps the fastest in the context ("edge_scal_tile",
+   launch for every tile shape (16x12, 16x8, 32x8 columns) and z-chunk (16,
+   32, 64, 128 planes), keeps the fastest in the context ("edge_scal_tile",
    "edge_scal_tz") and reports every candidate. */
 #define MX_TUNE_MAX 16
 typedef struct {

@@ -48,7 +48,11 @@ namespace {
 typedef __attribute__((address_space(1))) const double gcd;
 typedef __attribute__((address_space(1))) double gd;
 
-constexpr int NT = 384;
+// threads of a block: the tile's columns grown by one cell on each side,
+// rounded up to whole waves
+__host__ __device__ constexpr int fused_nt(int tx, int ty) {
+    return ((tx + 2) * (ty + 2) + 63) / 64 * 64;
+}
 
 struct FBox {
     int lo[3], hi[3];
@@ -189,11 +193,12 @@ __device__ __forceinline__ double apply3(double x, double a, double b, double c)
 // in this class when its stencils keep clear of every physical x/y domain
 // side, or when those sides carry no boundary override), so the per-lane
 // boundary branches of those directions are left out.
-template <int PPM, bool CONS, int TX, int TY, bool XYF>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) k_es_fused(const FBox* __restrict__ boxes,
-                                                 const FTile* __restrict__ tiles,
-                                                 const FComp* __restrict__ comps, KGeom g,
-                                                 FArgs A) {
+// WPE: the register budget, as waves per SIMD
+template <int PPM, bool CONS, int TX, int TY, bool XYF, int WPE>
+__global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_per_eu(WPE)))
+    k_es_fused(const FBox* __restrict__ boxes, const FTile* __restrict__ tiles,
+               const FComp* __restrict__ comps, KGeom g, FArgs A) {
+    constexpr int NT = fused_nt(TX, TY);
     constexpr int G = PPM == 2 ? 3 : 2;
     constexpr int W = TX + 2;
     constexpr int H = TY + 2;
@@ -795,27 +800,27 @@ bool tile_xy_free(const int lo0, int n, int dlo, int dhi, bool act_lo, bool act_
     return free_lo && free_hi;
 }
 
-template <int TX, int TY, bool XYF>
+template <int TX, int TY, int WPE, bool XYF>
 int launch_one(mx_adv_ctx* c, const FBox* db, const FTile* dtl, size_t ntile, const FComp* dc,
                const KGeom& kg, const FArgs& A, int num_comp, bool cons) {
     if (ntile == 0) return MX_OK;
     dim3 grid((unsigned)ntile, 1, (unsigned)num_comp);
-    dim3 block(NT);
+    dim3 block(fused_nt(TX, TY));
     if (c->p.ppm_type == 1) {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true, TX, TY, XYF>), grid, block, db, dtl, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false, TX, TY, XYF>), grid, block, db, dtl, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A);
     } else {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true, TX, TY, XYF>), grid, block, db, dtl, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false, TX, TY, XYF>), grid, block, db, dtl, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A);
     }
     return MX_OK;
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int WPE>
 int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int is_vel,
                  const mx_bcrec* bcs, int start_scomp, int start_bccomp, int num_comp,
                  int is_conservative) {
@@ -895,10 +900,33 @@ int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int 
     const FComp* dc = tab.dev<FComp>(oc);
     KGeom kg = kgeom(c, levs[0].lev);
     const bool cons = is_conservative != 0;
-    rc = launch_one<TX, TY, true>(c, db, dtl, nfree, dc, kg, A, num_comp, cons);
+    rc = launch_one<TX, TY, WPE, true>(c, db, dtl, nfree, dc, kg, A, num_comp, cons);
     if (rc) return rc;
-    return launch_one<TX, TY, false>(c, db, dtl + nfree, tiles.size() - nfree, dc, kg, A,
+    return launch_one<TX, TY, WPE, false>(c, db, dtl + nfree, tiles.size() - nfree, dc, kg, A,
                                      num_comp, cons);
+}
+
+// the tile shape of `fused_tile`, with the block size (columns rounded up to
+// whole waves) and the register budget (waves per SIMD):
+//   0: 32 x 8 (384 threads), 1: 16 x 16 (384), 2: 16 x 12 (256), 3: 16 x 8 (192)
+// all at 3 waves per SIMD (168 VGPRs), so a CU holds 2, 2, 3 or 4 blocks
+int launch_tile(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int is_vel,
+                const mx_bcrec* bcs, int start_scomp, int start_bccomp, int num_comp,
+                int is_conservative) {
+    switch (c->fused_tile) {
+        case 1:
+            return launch_fused<16, 16, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
+                                           num_comp, is_conservative);
+        case 2:
+            return launch_fused<16, 12, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
+                                           num_comp, is_conservative);
+        case 3:
+            return launch_fused<16, 8, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
+                                          num_comp, is_conservative);
+        default:
+            return launch_fused<32, 8, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
+                                          num_comp, is_conservative);
+    }
 }
 
 }  // namespace
@@ -907,7 +935,7 @@ int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& lev
                                 const mx_bcrec* bcs, int start_scomp, int start_bccomp,
                                 int num_comp, int is_conservative) {
     if (levs.empty()) return MX_OK;
-    const int tz = c->fused_tz > 0 ? c->fused_tz : 128;
+    const int tz = c->fused_tz > 0 ? c->fused_tz : 32;
     if (c->fused_per_box) {
         // comparison mode: the same work as one launch per box
         std::vector<EdgeLevel> one(1);
@@ -915,20 +943,14 @@ int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& lev
             for (size_t b = 0; b < c->grids[L.lev].size(); ++b) {
                 one[0] = L;
                 one[0].box = (int)b;
-                int rc = c->fused_tile == 1
-                             ? launch_fused<16, 16>(c, one, tz, is_vel, bcs, start_scomp,
-                                                    start_bccomp, num_comp, is_conservative)
-                             : launch_fused<32, 8>(c, one, tz, is_vel, bcs, start_scomp,
-                                                   start_bccomp, num_comp, is_conservative);
+                int rc = launch_tile(c, one, tz, is_vel, bcs, start_scomp, start_bccomp,
+                                     num_comp, is_conservative);
                 if (rc) return rc;
             }
         return MX_OK;
     }
-    if (c->fused_tile == 1)
-        return launch_fused<16, 16>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
-                                    num_comp, is_conservative);
-    return launch_fused<32, 8>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp, num_comp,
-                               is_conservative);
+    return launch_tile(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp, num_comp,
+                       is_conservative);
 }
 
 int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,

@@ -86,8 +86,8 @@ struct mx_adv_ctx {
 
     // kernel selection (mx_adv_set_tuning)
     int fused_edge = 1;
-    int fused_tz = 128;
-    int fused_tile = 0;     // 0: 32 x 8 columns, 1: 16 x 16
+    int fused_tz = 32;
+    int fused_tile = 2;     // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8
     int fused_per_box = 0;  // 1: one fused launch per box (comparison mode)
     int adj_state_fused = 1;  // adjoint: face, slope and state adjoints in one gather
     int adj_face_sums = 0;    // adjoint: predictor-state face adjoints stored once per face

@@ -102,10 +102,29 @@ def test_pass_kernel_choices(name, cfg, fused, tz):
     assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
 
 
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("name,cfg", KERNEL_CHOICE_CASES, ids=[c[0] for c in KERNEL_CHOICE_CASES])
+def test_pass_fused_tile_shapes(name, cfg, tile):
+    """The fused kernel's tile shapes and block sizes (32 x 8 and 16 x 16 on
+    384 threads, 16 x 12 on 256, 16 x 8 on 192), with 5-plane z-chunks:
+    bit-exact, partial tiles included."""
+    print(f"[parity] fused tile {tile} {name}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    m.set_tuning("edge_scal_tile", tile)
+    m.set_tuning("edge_scal_tz", 5)
+    _run_pass(m, wg)
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} sedge")
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
+
+
 @pytest.mark.parametrize("is_vel,is_cons,ppm", [(1, 0, 1), (1, 0, 2), (0, 1, 1), (0, 1, 2), (1, 0, 0),
                                                 (0, 1, 0)])
-@pytest.mark.parametrize("dim,fused", [(2, 0), (3, 0), (3, 1)])
-def test_edge_scal_variants(dim, fused, is_vel, is_cons, ppm):
+@pytest.mark.parametrize("dim,fused,tile", [(2, 0, 0), (3, 0, 0), (3, 1, 0), (3, 1, 1), (3, 1, 2),
+                                            (3, 1, 3)])
+def test_edge_scal_variants(dim, fused, tile, is_vel, is_cons, ppm):
     """MakeEdgeScal on the velocity (VelocityAdvance: is_vel, bcs_u) and in
     conservative form, with walls on every side; in 3-D with the three-kernel
     and with the fused kernel (ppm_type 0 always takes the three kernels)."""
@@ -114,12 +133,14 @@ def test_edge_scal_variants(dim, fused, is_vel, is_cons, ppm):
     if dim == 3:
         bc[2] = (P.PHYS_SYMMETRY, P.PHYS_OUTFLOW)
     cfg = dict(dim=dim, n=n, max_grid=n // 2, ppm_type=ppm, phys_bc=bc)
-    print(f"[parity] edge_scal dim={dim} fused={fused} is_vel={is_vel} cons={is_cons} ppm={ppm}",
+    print(f"[parity] edge_scal dim={dim} fused={fused} tile={tile} is_vel={is_vel} cons={is_cons} "
+          f"ppm={ppm}",
           flush=True)
     wc, wg = pair(**cfg)
     res = []
     mg = gpu_for(wg)
     mg.set_tuning("edge_scal_fused", fused)
+    mg.set_tuning("edge_scal_tile", tile)
     for w, m in ((wc, oracle_for(wc)), (wg, mg)):
         # a MAC velocity with filled ghosts from the first two stages
         m.MakeUtrans(0, w.utilde, w.ufull, w.utrans, None, w.w0_cart)
