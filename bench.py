@@ -220,12 +220,15 @@ def amr_main(args):
     m.set_tuning("edge_scal_per_box", 0)
     g.close()
 
+    # the single-box reference: one 128^3 box, its own autotuned edge-kernel
+    # shape, one graph launch per pass (as the hierarchy)
     w = Workload(dim=3, n=128, max_grid=128, ppm_type=args.ppm, backend=dev)
     ms1 = mx.gpu_maestro(w.params, [w.boxes])
-    ms1.set_tuning("edge_scal_tile", cands[best][0])
-    ms1.set_tuning("edge_scal_tz", cands[best][1])
     f1 = w.pass_fields()
-    single_ms = time_ms(lambda: ms1.advance_pass(0, f1), args.steps, args.warmup)
+    cands1, best1 = ms1.autotune([f1])
+    g1 = ms1.pass_graph([f1])
+    single_ms = time_ms(g1.launch, args.steps, args.warmup)
+    g1.close()
     per_cell = graph_ms * 1e6 / cells
     per_cell_1 = single_ms * 1e6 / w.valid_cells()
     value = cells / (graph_ms * 1e-3) / 1e9
@@ -252,6 +255,8 @@ def amr_main(args):
                 "ns_per_cell": round(per_cell, 4),
                 "single_box_ns_per_cell": round(per_cell_1, 4),
                 "per_cell_ratio_vs_single_box": round(per_cell / per_cell_1, 4),
+                "single_box_autotuned": {"tile": TILE_NAMES[cands1[best1][0]],
+                                         "tz": cands1[best1][1]},
                 "autotuned": tuned,
                 "autotune_candidates_ms": [[c[0], c[1], round(c[2], 4)] for c in cands]},
         "roofline": None,

@@ -311,7 +311,8 @@ __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_
         e1Col += e1.off(I, J, z0 - 1, comp);
         e2Col += e2.off(I, J, z0 - 1, comp);
     }
-    const long long fks = fv.ks, wks = wv.ks, e0ks = e0.ks, e1ks = e1.ks, e2ks = e2.ks;
+    const int fks = (int)fv.ks, wks = (int)wv.ks;
+    const int e0ks = (int)e0.ks, e1ks = (int)e1.ks, e2ks = (int)e2.ks;
 
     // column state
     double v[9];
@@ -853,7 +854,8 @@ int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int 
                 B.se[d] = to_da(L.sedge->dir[d].fab[b]);
             }
             // the kernel keeps the plane strides in 32 bits
-            const DA* views[] = {&B.s, &B.fo, &B.um[0], &B.um[1], &B.um[2]};
+            const DA* views[] = {&B.s,     &B.fo,    &B.um[0], &B.um[1],
+                                 &B.um[2], &B.se[0], &B.se[1], &B.se[2]};
             for (const DA* d : views)
                 if (d->js > lim || d->ks > lim) return MX_ERR_UNSUPPORTED;
             const int ntx = (g.hi[0] - g.lo[0] + TX) / TX;

@@ -26,25 +26,40 @@ def short_name(name):
 
 
 def per_dispatch(csv_path, counter):
-    """{kernel short name: [value per dispatch]} for one counter."""
+    """{kernel short name: [value per dispatch, in dispatch order]} for one
+    counter."""
     out = {}
     acc = {}
     for r in csv.DictReader(open(csv_path)):
         if r["Counter_Name"] != counter:
             continue
-        key = (r["Dispatch_Id"], r["Kernel_Name"])
+        key = (int(r["Dispatch_Id"]), r["Kernel_Name"])
         acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
-    for (_, name), v in acc.items():
+    for (_, name), v in sorted(acc.items()):
         out.setdefault(short_name(name), []).append(v)
     return out
+
+
+def calibration_copy(counts, copy_kib):
+    """The counter value of the calibration copy: the first copy dispatch
+    that moves at least an eighth of the calibration size (the calibration
+    copy runs before any other large copy; bench.py's later copy-bandwidth
+    copies are larger and must not be taken for it)."""
+    for name, vs in counts.items():
+        if "copy" not in name.lower():
+            continue
+        for v in vs:
+            if v >= copy_kib / 8:
+                return v
+    raise SystemExit("no calibration copy in the counter file")
 
 
 def summarize(fetch_csv, write_csv, copy_mib, kernel):
     f = per_dispatch(fetch_csv, "FETCH_SIZE")
     w = per_dispatch(write_csv, "WRITE_SIZE")
     copy_kib = copy_mib * 1024.0
-    cf = max(v for k, vs in f.items() if "copy" in k.lower() for v in vs)
-    cw = max(v for k, vs in w.items() if "copy" in k.lower() for v in vs)
+    cf = calibration_copy(f, copy_kib)
+    cw = calibration_copy(w, copy_kib)
     fs = copy_kib / cf
     ws = copy_kib / cw
     kf = statistics.mean(f[kernel])
