@@ -96,6 +96,8 @@ def parse():
                     help="the batched AMR hierarchy (3 levels, uneven boxes) instead of 256^3")
     ap.add_argument("--adjoint", action="store_true",
                     help="the differentiable step: 128^3 periodic, 4 scalars, adjoint/forward cost")
+    ap.add_argument("--update", action="store_true",
+                    help="UpdateScal + UpdateVel (the update that closes the step) on the 3-D level")
     ap.add_argument("--spherical", action="store_true",
                     help="the spherical star: 384^3 in 216 uneven boxes (edges 48-96), "
                          "ppm_type 2, radial w0, one graph launch per pass")
@@ -340,6 +342,176 @@ def adjoint_main(args):
     print(json.dumps(out), flush=True)
 
 
+# algorithmic bytes per cell of the update stages (3-D planar, NumSpec = 3,
+# each face counted once): UpdateScal species 2 + 6 NumSpec doubles (old and
+# new density and species, three fluxes and the force of each species),
+# enthalpy 6, UpdateVel 22 (old and new velocity, force, MAC velocity, nine
+# edge velocities, w0)
+UPDATE_BYTES_PER_CELL = {"k_update_rhox": 8 * 20, "k_update_rhoh": 8 * 6, "k_update_vel": 8 * 22}
+
+
+def update_fields(boxes, nspec, backend, fill):
+    """Inputs and outputs of UpdateScal and UpdateVel on one level (3-D);
+    fill(array, lo, hi) puts uniform random values into an array."""
+    from maestroex_amd.fabs import FaceMultiFab, MultiFab
+    nc = 4 + nspec
+    f = {
+        "sold": MultiFab(boxes, nc, 0, 3, backend), "snew": MultiFab(boxes, nc, 0, 3, backend),
+        "force": MultiFab(boxes, nc, 0, 3, backend), "sflux": FaceMultiFab(boxes, nc, 0, 3, backend),
+        "uold": MultiFab(boxes, 3, 0, 3, backend), "unew": MultiFab(boxes, 3, 0, 3, backend),
+        "vforce": MultiFab(boxes, 3, 0, 3, backend), "umac": FaceMultiFab(boxes, 1, 0, 3, backend),
+        "uedge": FaceMultiFab(boxes, 3, 0, 3, backend), "w0c": MultiFab(boxes, 3, 1, 3, backend),
+    }
+    for key in ("force", "uold", "vforce", "w0c"):
+        for a in f[key].arrays:
+            fill(a, -1.0, 1.0)
+    for key in ("sflux", "umac", "uedge"):
+        for d in range(3):
+            for a in f[key][d].arrays:
+                fill(a, -0.01, 0.01)
+    for a in f["sold"].arrays:
+        fill(a, 0.2, 1.0)
+        a[0] = a[4:4 + nspec].sum(0)  # rho: the sum of the species
+    return f
+
+
+def run_update(m, f, nspec, bcd):
+    m.UpdateScal(0, f["sold"], f["snew"], f["sflux"], f["force"], 4, nspec, None, bcd)
+    m.UpdateScal(0, f["sold"], f["snew"], f["sflux"], f["force"], 1, 1, None, bcd)
+    m.UpdateVel(0, f["uold"], f["unew"], f["umac"], f["uedge"], f["vforce"], None, None,
+                f["w0c"], False)
+
+
+def update_caller(m, f, nspec, bcd):
+    """run_update with every descriptor built once, as a C++ caller holds its
+    MultiFab views: each call is then the three C-ABI calls alone."""
+    from maestroex_amd.fabs import empty_face_desc, empty_mf_desc
+    fns, ctx = m.fns, m.ctx
+    so, sn, fo = f["sold"].desc(), f["snew"].desc(), f["force"].desc()
+    sf = f["sflux"].desc()
+    uo, un, vf, w0 = f["uold"].desc(), f["unew"].desc(), f["vforce"].desc(), f["w0c"].desc()
+    um, ue = f["umac"].desc(), f["uedge"].desc()
+    none_mf, none_face = empty_mf_desc(), empty_face_desc()
+    r = [C.byref(x) for x in (so, sn, sf, fo, none_mf, uo, un, um, ue, vf, none_face, w0)]
+    keep = (so, sn, fo, sf, uo, un, vf, w0, um, ue, none_mf, none_face)
+
+    def call():
+        rc = fns["update_scal"](ctx, 0, r[0], r[1], r[2], r[3], 4, nspec, r[4], bcd, 0)
+        rc = rc or fns["update_scal"](ctx, 0, r[0], r[1], r[2], r[3], 1, 1, r[4], bcd, 0)
+        rc = rc or fns["update_vel"](ctx, 0, r[5], r[6], r[7], r[8], r[9], r[4], r[10], r[11], 0)
+        if rc:
+            raise RuntimeError(f"update: error {rc}")
+    call.keep = keep
+    return call
+
+
+def update_main(args):
+    """UpdateScal (species, enthalpy) and UpdateVel, the update that closes
+    the step, on the 3-D 256^3 level of 8 boxes of 128^3 with 3 species:
+    HBM-bound stages measured against the HBM roofline."""
+    import torch
+    import maestroex_amd as mx
+    from maestroex_amd.fabs import Backend, Geometry, make_params, split_domain
+
+    dev = mx.device_backend("cuda:0")
+    n, nspec, bcd = args.n, 3, 1e-3
+    geom = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 1], [0] * 3, [0] * 3)
+    params = make_params(geom, nspec=nspec, dt=1e-3)
+    boxes = split_domain(geom, args.max_grid)
+    gen = torch.Generator(device="cuda:0").manual_seed(5)
+
+    def fill(a, lo, hi):
+        a.uniform_(lo, hi, generator=gen)
+    f = update_fields(boxes, nspec, dev, fill)
+    m = mx.gpu_maestro(params, [boxes])
+    call = update_caller(m, f, nspec, bcd)
+    ms = time_ms(call, args.steps, args.warmup)
+    fns = m.fns
+    fns["prof_enable"](m.ctx, 1)
+    for _ in range(args.steps):
+        call()
+    nk = fns["prof_read"](m.ctx, -1, None, None, None)
+    kern = {}
+    for idx in range(nk):
+        nm, kms, cnt = C.c_char_p(), C.c_double(), C.c_longlong()
+        fns["prof_read"](m.ctx, idx, C.byref(nm), C.byref(kms), C.byref(cnt))
+        kern[nm.value.decode()] = kms.value / max(cnt.value, 1)
+    fns["prof_enable"](m.ctx, 0)
+    cells = n ** 3
+    per_kernel = {}
+    for k, avg in sorted(kern.items()):
+        bpc = UPDATE_BYTES_PER_CELL.get(k)
+        gbs = bpc * cells / (avg * 1e-3) / 1e9 if bpc else None
+        per_kernel[k] = {"avg_launch_ms": round(avg, 4), "bytes_per_cell": bpc,
+                         "achieved_gbs": round(gbs, 1) if gbs else None,
+                         "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    print(f"[bench] update kernels: {per_kernel}", file=sys.stderr, flush=True)
+    dom = "k_update_vel"
+    copy_gbs = copy_bandwidth_gbs()
+    d = per_kernel[dom]
+    cpu = None
+    if not args.no_cpu:
+        cpu = update_cpu_baseline(args, nspec, bcd)
+    out = {
+        "metric": METRIC,
+        "value": round(cells / (ms * 1e-3) / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"UpdateScal (3 species, then enthalpy) + UpdateVel on 3-D {n}^3 "
+                               f"FP64 as {len(boxes)} boxes of {args.max_grid}^3",
+                   "global_batch": cells, "parallelism": "single GPU"},
+        "update": per_kernel,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["achieved_gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"], "traffic": None,
+                     "avg_launch_ms": d["avg_launch_ms"], "bytes_per_cell": d["bytes_per_cell"],
+                     "measured_copy_gbs": round(copy_gbs, 1),
+                     "frac_of_measured_copy": round(d["achieved_gbs"] / copy_gbs, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def update_cpu_baseline(args, nspec, bcd):
+    """The oracle's UpdateScal + UpdateVel on one 128^3 box, timed on the
+    host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    from maestroex_amd.fabs import Backend, Geometry, make_params, split_domain
+    cores = min(int(os.environ.get("MX_CPU_THREADS", "0")) or len(os.sched_getaffinity(0)), 16)
+    pyoracle.set_threads(cores, fast=True)
+    n = args.cpu_n
+    geom = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 1], [0] * 3, [0] * 3)
+    params = make_params(geom, nspec=nspec, dt=1e-3)
+    boxes = split_domain(geom, n)
+    rng = np.random.default_rng(5)
+
+    def fill(a, lo, hi):
+        a[...] = rng.uniform(lo, hi, size=a.shape)
+    f = update_fields(boxes, nspec, Backend("numpy"), fill)
+    m = pyoracle.oracle_maestro(params, [boxes], fast=True)
+    run_update(m, f, nspec, bcd)
+    times = []
+    for _ in range(args.cpu_reps):
+        t0 = time.perf_counter()
+        run_update(m, f, nspec, bcd)
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    return {"value": round(n ** 3 / med / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"one {n}^3 box, 3 species, median of {args.cpu_reps} updates after 1 "
+                      f"warm-up, OpenMP {cores} threads; {med * 1e3:.1f} ms per update"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -350,6 +522,8 @@ def main():
         return amr_main(args)
     if args.adjoint:
         return adjoint_main(args)
+    if args.update:
+        return update_main(args)
     import torch
 
     import maestroex_amd as mx

@@ -385,6 +385,38 @@ typedef struct {
 int mx_adv_autotune(mx_adv_ctx* ctx, int nlev, const mx_adv_pass_args* args,
                     mx_adv_tune_report* report);
 
+/* ---- the update that closes the step ------------------------------------ */
+/* Maestro::UpdateScal (Maestro.H:306-311, MaestroAdvection.cpp:86-297) on
+   level lev: statenew = stateold + dt (-div sflux + force) on the valid
+   cells, for start_comp == RhoH (num_comp 1) or FirstSpec (num_comp NumSpec;
+   then also Rho as the sum of the species updates, the auxiliary components
+   FirstSpec + NumSpec .. nscal - 1 rescaled by the density, the density floor
+   0.5 base_cutoff_density and the redistribution of negative species).  The
+   reference reads base_cutoff_density and do_eos_h_above_cutoff as members;
+   here they are arguments.  do_eos_h_above_cutoff != 0 returns
+   MX_ERR_UNSUPPORTED: its branch calls the equation of state, which is not
+   part of this library; p0_cart is read only by that branch (NULL is fine).
+   The reference's reflux, AverageDown and FillPatch after the update stay
+   with the caller.  All MultiFabs hold nscal components; no ghosts needed. */
+int mx_adv_update_scal(mx_adv_ctx* ctx, int lev, const mx_multifab* stateold,
+                       mx_multifab* statenew, const mx_face_multifab* sflux,
+                       const mx_multifab* force, int start_comp, int num_comp,
+                       const mx_multifab* p0_cart, double base_cutoff_density,
+                       int do_eos_h_above_cutoff);
+
+/* Maestro::UpdateVel (Maestro.H:320-326, MaestroAdvection.cpp:299-526) on
+   level lev: unew = uold - dt (Utilde . grad) Utilde + dt force - the
+   (w0 . grad) Utilde term, times the sponge when do_sponge != 0, on the valid
+   cells.  uold and unew are the Maestro members of the same names; w0_cart
+   (planar: dim components, 1 ghost cell, read in the cell above each valid
+   cell as the reference does) and do_sponge are members there too.  w0mac
+   is read in spherical contexts only, w0_cart in planar ones only.
+   AverageDown and FillPatch stay with the caller. */
+int mx_adv_update_vel(mx_adv_ctx* ctx, int lev, const mx_multifab* uold, mx_multifab* unew,
+                      const mx_face_multifab* umac, const mx_face_multifab* uedge,
+                      const mx_multifab* force, const mx_multifab* sponge,
+                      const mx_face_multifab* w0mac, const mx_multifab* w0_cart, int do_sponge);
+
 /* ---- differentiation ---------------------------------------------------- */
 /* A differentiable scalar advection step (no single counterpart: the
    composition of MakeEdgeScal, MakeRhoXFlux and UpdateScal for a species

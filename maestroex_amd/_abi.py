@@ -195,6 +195,11 @@ SIGNATURES = {
     "make_edge_state_1d": [_vp, P(RadialGeom), P(BaseStateDesc), P(BaseStateDesc),
                            P(BaseStateDesc), P(BaseStateDesc)],
     "advance_pass": [_vp, _i, P(PassArgs)],
+    "update_scal": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc),
+                    _i, _i, P(MultiFabDesc), _d, _i],
+    "update_vel": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc),
+                   P(FaceMultiFabDesc), P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc),
+                   P(MultiFabDesc), _i],
 }
 
 MX_TUNE_MAX = 16

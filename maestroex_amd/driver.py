@@ -122,6 +122,24 @@ class Maestro:
                                                start_bccomp, num_comp, int(is_conservative)),
                     "MakeEdgeScal")
 
+    def UpdateScal(self, lev, stateold, statenew, sflux, force, start_comp, num_comp,
+                   p0_cart=None, base_cutoff_density=0.0, do_eos_h_above_cutoff=False):
+        """Maestro::UpdateScal on one level (the valid cells; reflux, AverageDown
+        and FillPatch stay with the caller)."""
+        a = [_mf(stateold), _mf(statenew), _face(sflux), _mf(force)]
+        p0 = _mf(p0_cart)
+        self._check(self.fns["update_scal"](self.ctx, lev, *[C.byref(x) for x in a], start_comp,
+                                            num_comp, C.byref(p0), float(base_cutoff_density),
+                                            int(do_eos_h_above_cutoff)), "UpdateScal")
+
+    def UpdateVel(self, lev, uold, unew, umac, uedge, force, sponge, w0mac, w0_cart,
+                  do_sponge=False):
+        """Maestro::UpdateVel on one level (the valid cells)."""
+        a = [_mf(uold), _mf(unew), _face(umac), _face(uedge), _mf(force), _mf(sponge),
+             _face(w0mac), _mf(w0_cart)]
+        self._check(self.fns["update_vel"](self.ctx, lev, *[C.byref(x) for x in a],
+                                           int(do_sponge)), "UpdateVel")
+
     def MakeRhoXFlux(self, lev, sflux, etarhoflux, sedge, umac, rho0_old, rho0_edge_old,
                      r0mac_old, rho0_new, rho0_edge_new, r0mac_new, rho0_predicted_edge, w0,
                      start_comp, num_comp):

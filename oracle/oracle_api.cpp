@@ -66,6 +66,13 @@ void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double r
               const double* const du[3], double* dsnew);
 void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const double* s,
                  const double* const u[3], double out[7]);
+int UpdateScal(const Ctx& c, int lev, const std::vector<Arr>& sold, const std::vector<Arr>& snew,
+               const std::vector<Arr> sflux[3], const std::vector<Arr>& force, int start_comp,
+               double base_cutoff_density);
+int UpdateVel(const Ctx& c, int lev, const std::vector<Arr>& uold, const std::vector<Arr>& unew,
+              const std::vector<Arr> umac[3], const std::vector<Arr> uedge[3],
+              const std::vector<Arr>& force, const std::vector<Arr>& sponge,
+              const std::vector<Arr> w0mac[3], const std::vector<Arr>& w0_cart, bool do_sponge);
 
 }  // namespace mxo
 
@@ -406,6 +413,51 @@ int mxo_advance_pass(Ctx* c, int lev, const mx_adv_pass_args* a) {
                       r0mac.d, base(&a->rho0), base(&a->rho0_edge), r0mac.d, base(&a->rhoh0),
                       base(&a->rhoh0_edge), rh0mac.d, base(&a->rhoh0), base(&a->rhoh0_edge),
                       rh0mac.d, h0mac.d, h0mac.d);
+    return finish(rc);
+}
+
+int mxo_update_scal(Ctx* c, int lev, const mx_multifab* stateold, mx_multifab* statenew,
+                    const mx_face_multifab* sflux, const mx_multifab* force, int start_comp,
+                    int num_comp, const mx_multifab* p0_cart, double base_cutoff_density,
+                    int do_eos_h_above_cutoff) {
+    (void)p0_cart;
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!stateold || !statenew || !sflux || !force) return MX_ERR_ARG;
+    if (do_eos_h_above_cutoff) return MX_ERR_UNSUPPORTED;
+    if (start_comp == c->p.comp_rhoh) {
+        if (num_comp != 1) return MX_ERR_ARG;
+    } else if (start_comp == c->p.first_spec) {
+        if (num_comp != c->p.num_spec) return MX_ERR_ARG;
+    } else {
+        return MX_ERR_ARG;
+    }
+    Face sf = faces(sflux);
+    rc = UpdateScal(*c, lev, arrs(stateold), arrs(statenew), sf.d, arrs(force), start_comp,
+                    base_cutoff_density);
+    return finish(rc);
+}
+
+int mxo_update_vel(Ctx* c, int lev, const mx_multifab* uold, mx_multifab* unew,
+                   const mx_face_multifab* umac, const mx_face_multifab* uedge,
+                   const mx_multifab* force, const mx_multifab* sponge,
+                   const mx_face_multifab* w0mac, const mx_multifab* w0_cart, int do_sponge) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!uold || !unew || !umac || !uedge || !force) return MX_ERR_ARG;
+    if (do_sponge && (!sponge || sponge->nfab != (int)c->grids[lev].size())) return MX_ERR_ARG;
+    if (c->p.spherical) {
+        if (!w0mac) return MX_ERR_ARG;
+        for (int d = 0; d < 3; ++d)
+            if (w0mac->dir[d].nfab != (int)c->grids[lev].size()) return MX_ERR_ARG;
+    } else if (!w0_cart || w0_cart->nfab != (int)c->grids[lev].size()) {
+        return MX_ERR_ARG;
+    }
+    Face um = faces(umac), ue = faces(uedge), wm = faces(w0mac);
+    std::vector<Arr> sp;
+    if (do_sponge) sp = arrs(sponge);
+    rc = UpdateVel(*c, lev, arrs(uold), arrs(unew), um.d, ue.d, arrs(force), sp, wm.d,
+                   arrs(w0_cart), do_sponge != 0);
     return finish(rc);
 }
 
