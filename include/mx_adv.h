@@ -385,6 +385,70 @@ typedef struct {
 int mx_adv_autotune(mx_adv_ctx* ctx, int nlev, const mx_adv_pass_args* args,
                     mx_adv_tune_report* report);
 
+/* ---- base state <-> Cartesian producers --------------------------------- */
+/* The spherical geometry these read (Maestro members center, geom.ProbLo,
+   base_geom and the *_interp_type runtime parameters).  r_cc_loc and
+   r_edge_loc are row 0 of base_geom.r_cc_loc / r_edge_loc (nr_fine and
+   nr_fine + 1 values) in memory the library's kernels can read (device
+   memory).  Only the non-exact base state (use_exact_base_state = 0, radial
+   index int(radius / dr_fine)) is supported: the exact one maps cells to
+   radii through the cell_cc_to_r integer MultiFab. */
+typedef struct {
+    double center[3];
+    double prob_lo[3];
+    double dr_fine;
+    int nr_fine;
+    const double* r_cc_loc;
+    const double* r_edge_loc;
+    int s0_interp_type;    /* 1-3 (piecewise constant, linear, quadratic) */
+    int w0_interp_type;    /* 1-3 */
+    int s0mac_interp_type; /* 1-3 */
+    int w0mac_interp_type; /* 1-4 */
+} mx_sph_geom;
+
+/* Maestro::Put1dArrayOnCart, one level (Maestro.H, MaestroFill3dData.cpp:49-339):
+   the base state s0 (bin- or edge-centred) put on the valid cells of s0_cart:
+   component dim - 1 of a vector or component 0 of a scalar (planar), or all
+   three components of a vector along the radius (spherical, sg required).
+   The reference's multi-level wrapper (:9-38) then calls AverageDown and
+   FillPatch; those stay with the caller. */
+int mx_adv_put_1d_array_on_cart(mx_adv_ctx* ctx, int lev, const mx_basestate* s0,
+                                mx_multifab* s0_cart, int is_input_edge_centered,
+                                int is_output_a_vector, const mx_sph_geom* sg);
+
+/* Maestro::MakeW0mac, one level (MaestroFill3dData.cpp:464-710), spherical:
+   w0mac on every face of each box grown by one cell (w0mac needs 1 ghost).
+   w0mac_interp_type 1 averages w0_cart (3 components, 2 ghost cells filled
+   by the caller); 2 and 3 interpolate the edge-centred w0 to the face; 4
+   averages w0 interpolated to the face's four nodes. */
+int mx_adv_make_w0mac(mx_adv_ctx* ctx, int lev, const mx_basestate* w0,
+                      const mx_multifab* w0_cart, mx_face_multifab* w0mac,
+                      const mx_sph_geom* sg);
+
+/* Maestro::MakeS0mac, one level (MaestroFill3dData.cpp:712-1188), spherical:
+   s0mac on every face of each box grown by one cell.  s0mac_interp_type 1
+   averages s0_cart, which the reference builds with Put1dArrayOnCart and
+   fills with FillPatch: the caller passes it (1 component, 2 ghost cells);
+   2 and 3 interpolate the bin-centred s0 to the face (s0_cart unused). */
+int mx_adv_make_s0mac(mx_adv_ctx* ctx, int lev, const mx_basestate* s0,
+                      const mx_multifab* s0_cart, mx_face_multifab* s0mac,
+                      const mx_sph_geom* sg);
+
+/* Maestro::PutInPertForm, one level (MaestroConvert.cpp:36-62): scal[comp]
+   -= s0 on the cells (flag != 0) or += s0 (flag == 0), valid cells; s0 is
+   put on the cells as Put1dArrayOnCart does (bin-centred, scalar).  The
+   reference's average_down and FillPatch stay with the caller. */
+int mx_adv_put_in_pert_form(mx_adv_ctx* ctx, int lev, mx_multifab* scal,
+                            const mx_basestate* s0, int comp, int flag,
+                            const mx_sph_geom* sg);
+
+/* Maestro::ConvertRhoXToX (MaestroConvert.cpp:64-91) and ConvertRhoHToH
+   (:93-114), one level: the species (or RhoH) divided by Rho (flag != 0) or
+   multiplied by it (flag == 0) on the valid cells.  AverageDown and
+   FillPatch stay with the caller. */
+int mx_adv_convert_rhox_to_x(mx_adv_ctx* ctx, int lev, mx_multifab* scal, int flag);
+int mx_adv_convert_rhoh_to_h(mx_adv_ctx* ctx, int lev, mx_multifab* scal, int flag);
+
 /* ---- the update that closes the step ------------------------------------ */
 /* Maestro::UpdateScal (Maestro.H:306-311, MaestroAdvection.cpp:86-297) on
    level lev: statenew = stateold + dt (-div sflux + force) on the valid

@@ -105,6 +105,22 @@ class Params(C.Structure):
 MX_MAX_CHUNK = 16
 
 
+class SphGeom(C.Structure):
+    """mx_sph_geom: spherical geometry of the base-state producers."""
+    _fields_ = [
+        ("center", C.c_double * 3),
+        ("prob_lo", C.c_double * 3),
+        ("dr_fine", C.c_double),
+        ("nr_fine", C.c_int),
+        ("r_cc_loc", C.c_void_p),
+        ("r_edge_loc", C.c_void_p),
+        ("s0_interp_type", C.c_int),
+        ("w0_interp_type", C.c_int),
+        ("s0mac_interp_type", C.c_int),
+        ("w0mac_interp_type", C.c_int),
+    ]
+
+
 class RadialGeom(C.Structure):
     """mx_radial_geom: radial levels, cell counts, widths and chunks."""
     _fields_ = [
@@ -195,6 +211,12 @@ SIGNATURES = {
     "make_edge_state_1d": [_vp, P(RadialGeom), P(BaseStateDesc), P(BaseStateDesc),
                            P(BaseStateDesc), P(BaseStateDesc)],
     "advance_pass": [_vp, _i, P(PassArgs)],
+    "put_1d_array_on_cart": [_vp, _i, P(BaseStateDesc), P(MultiFabDesc), _i, _i, P(SphGeom)],
+    "make_w0mac": [_vp, _i, P(BaseStateDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(SphGeom)],
+    "make_s0mac": [_vp, _i, P(BaseStateDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(SphGeom)],
+    "put_in_pert_form": [_vp, _i, P(MultiFabDesc), P(BaseStateDesc), _i, _i, P(SphGeom)],
+    "convert_rhox_to_x": [_vp, _i, P(MultiFabDesc), _i],
+    "convert_rhoh_to_h": [_vp, _i, P(MultiFabDesc), _i],
     "update_scal": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc),
                     _i, _i, P(MultiFabDesc), _d, _i],
     "update_vel": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc),

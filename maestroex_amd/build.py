@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = ["mx_ctx.hip", "mx_velocity.hip", "mx_scalar.hip", "mx_edge_fused.hip", "mx_edge1d.hip",
-           "mx_graph.hip", "mx_adjoint.hip", "mx_update.hip"]
+           "mx_graph.hip", "mx_adjoint.hip", "mx_update.hip", "mx_base.hip"]
 LIB = os.path.join(HERE, "lib", "libmaestroex_amd.so")
 ARCH = os.environ.get("MX_OFFLOAD_ARCH", "gfx950")
 
@@ -27,7 +27,7 @@ FLAGS = [
     "-ffp-contract=off",
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function",
+    "-Wall", "-Wno-unused-function", "-Werror=return-type",
 ]
 
 

@@ -122,6 +122,42 @@ class Maestro:
                                                start_bccomp, num_comp, int(is_conservative)),
                     "MakeEdgeScal")
 
+    # ---- base state <-> Cartesian ---------------------------------------
+    def Put1dArrayOnCart(self, lev, s0, s0_cart, is_input_edge_centered, is_output_a_vector,
+                         sg=None):
+        """Maestro::Put1dArrayOnCart on one level (sg: SphGeom, spherical only)."""
+        a = [_bs(s0), _mf(s0_cart)]
+        self._check(self.fns["put_1d_array_on_cart"](
+            self.ctx, lev, *[C.byref(x) for x in a], int(is_input_edge_centered),
+            int(is_output_a_vector), C.byref(sg) if sg is not None else None), "Put1dArrayOnCart")
+
+    def MakeW0mac(self, lev, w0, w0_cart, w0mac, sg):
+        a = [_bs(w0), _mf(w0_cart), _face(w0mac)]
+        self._check(self.fns["make_w0mac"](self.ctx, lev, *[C.byref(x) for x in a],
+                                           C.byref(sg)), "MakeW0mac")
+
+    def MakeS0mac(self, lev, s0, s0_cart, s0mac, sg):
+        a = [_bs(s0), _mf(s0_cart), _face(s0mac)]
+        self._check(self.fns["make_s0mac"](self.ctx, lev, *[C.byref(x) for x in a],
+                                           C.byref(sg)), "MakeS0mac")
+
+    def PutInPertForm(self, lev, scal, s0, comp, flag, sg=None):
+        a = [_mf(scal), _bs(s0)]
+        self._check(self.fns["put_in_pert_form"](self.ctx, lev, *[C.byref(x) for x in a], comp,
+                                                 int(flag),
+                                                 C.byref(sg) if sg is not None else None),
+                    "PutInPertForm")
+
+    def ConvertRhoXToX(self, lev, scal, flag):
+        m = _mf(scal)
+        self._check(self.fns["convert_rhox_to_x"](self.ctx, lev, C.byref(m), int(flag)),
+                    "ConvertRhoXToX")
+
+    def ConvertRhoHToH(self, lev, scal, flag):
+        m = _mf(scal)
+        self._check(self.fns["convert_rhoh_to_h"](self.ctx, lev, C.byref(m), int(flag)),
+                    "ConvertRhoHToH")
+
     def UpdateScal(self, lev, stateold, statenew, sflux, force, start_comp, num_comp,
                    p0_cart=None, base_cutoff_density=0.0, do_eos_h_above_cutoff=False):
         """Maestro::UpdateScal on one level (the valid cells; reflux, AverageDown

@@ -66,6 +66,16 @@ void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double r
               const double* const du[3], double* dsnew);
 void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const double* s,
                  const double* const u[3], double out[7]);
+int Put1dArrayOnCart(const Ctx& c, int lev, const BaseArr& s0, const std::vector<Arr>& s0_cart,
+                     bool edge, bool vec, const mx_sph_geom* sg);
+int MakeW0mac(const Ctx& c, int lev, const BaseArr& w0, const std::vector<Arr>& w0_cart,
+              const std::vector<Arr> w0mac[3], const mx_sph_geom* sg);
+int MakeS0mac(const Ctx& c, int lev, const BaseArr& s0, const std::vector<Arr>& s0_cart,
+              const std::vector<Arr> s0mac[3], const mx_sph_geom* sg);
+int PutInPertForm(const Ctx& c, int lev, const std::vector<Arr>& scal, const BaseArr& s0,
+                  int comp, bool flag, const mx_sph_geom* sg);
+int ConvertByRho(const Ctx& c, int lev, const std::vector<Arr>& scal, int comp0, int ncomp,
+                 bool flag);
 int UpdateScal(const Ctx& c, int lev, const std::vector<Arr>& sold, const std::vector<Arr>& snew,
                const std::vector<Arr> sflux[3], const std::vector<Arr>& force, int start_comp,
                double base_cutoff_density);
@@ -459,6 +469,70 @@ int mxo_update_vel(Ctx* c, int lev, const mx_multifab* uold, mx_multifab* unew,
     rc = UpdateVel(*c, lev, arrs(uold), arrs(unew), um.d, ue.d, arrs(force), sp, wm.d,
                    arrs(w0_cart), do_sponge != 0);
     return finish(rc);
+}
+
+int mxo_put_1d_array_on_cart(Ctx* c, int lev, const mx_basestate* s0, mx_multifab* s0_cart,
+                             int is_input_edge_centered, int is_output_a_vector,
+                             const mx_sph_geom* sg) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!s0 || !s0_cart || s0_cart->nfab != (int)c->grids[lev].size()) return MX_ERR_ARG;
+    rc = Put1dArrayOnCart(*c, lev, base(s0), arrs(s0_cart), is_input_edge_centered != 0,
+                          is_output_a_vector != 0, sg);
+    return finish(rc);
+}
+
+int mxo_make_w0mac(Ctx* c, int lev, const mx_basestate* w0, const mx_multifab* w0_cart,
+                   mx_face_multifab* w0mac, const mx_sph_geom* sg) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!w0 || !w0mac) return MX_ERR_ARG;
+    std::vector<Arr> wc;
+    if (sg && sg->w0mac_interp_type == 1) {
+        if (!w0_cart || w0_cart->nfab != (int)c->grids[lev].size()) return MX_ERR_ARG;
+        wc = arrs(w0_cart);
+    }
+    Face wm = faces(w0mac);
+    rc = MakeW0mac(*c, lev, base(w0), wc, wm.d, sg);
+    return finish(rc);
+}
+
+int mxo_make_s0mac(Ctx* c, int lev, const mx_basestate* s0, const mx_multifab* s0_cart,
+                   mx_face_multifab* s0mac, const mx_sph_geom* sg) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!s0 || !s0mac) return MX_ERR_ARG;
+    std::vector<Arr> sc;
+    if (sg && sg->s0mac_interp_type == 1) {
+        if (!s0_cart || s0_cart->nfab != (int)c->grids[lev].size()) return MX_ERR_ARG;
+        sc = arrs(s0_cart);
+    }
+    Face sm = faces(s0mac);
+    rc = MakeS0mac(*c, lev, base(s0), sc, sm.d, sg);
+    return finish(rc);
+}
+
+int mxo_put_in_pert_form(Ctx* c, int lev, mx_multifab* scal, const mx_basestate* s0, int comp,
+                         int flag, const mx_sph_geom* sg) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!scal || !s0 || comp < 0 || comp >= c->p.nscal) return MX_ERR_ARG;
+    rc = PutInPertForm(*c, lev, arrs(scal), base(s0), comp, flag != 0, sg);
+    return finish(rc);
+}
+
+int mxo_convert_rhox_to_x(Ctx* c, int lev, mx_multifab* scal, int flag) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!scal) return MX_ERR_ARG;
+    return finish(ConvertByRho(*c, lev, arrs(scal), c->p.first_spec, c->p.num_spec, flag != 0));
+}
+
+int mxo_convert_rhoh_to_h(Ctx* c, int lev, mx_multifab* scal, int flag) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (!scal) return MX_ERR_ARG;
+    return finish(ConvertByRho(*c, lev, arrs(scal), c->p.comp_rhoh, 1, flag != 0));
 }
 
 int mxo_omp_threads(int n) {
