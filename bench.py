@@ -96,6 +96,8 @@ def parse():
                     help="the batched AMR hierarchy (3 levels, uneven boxes) instead of 256^3")
     ap.add_argument("--adjoint", action="store_true",
                     help="the differentiable step: 128^3 periodic, 4 scalars, adjoint/forward cost")
+    ap.add_argument("--base", action="store_true",
+                    help="the base-state <-> Cartesian producers on the 3-D level")
     ap.add_argument("--update", action="store_true",
                     help="UpdateScal + UpdateVel (the update that closes the step) on the 3-D level")
     ap.add_argument("--spherical", action="store_true",
@@ -512,6 +514,109 @@ def update_cpu_baseline(args, nspec, bcd):
                       f"warm-up, OpenMP {cores} threads; {med * 1e3:.1f} ms per update"}
 
 
+def base_main(args):
+    """The base-state <-> Cartesian producers on the 3-D 256^3 level of 8
+    boxes of 128^3: planar Put1dArrayOnCart, PutInPertForm and
+    ConvertRhoXToX, then on a spherical domain Put1dArrayOnCart (quadratic),
+    MakeW0mac and MakeS0mac (linear).  Each writes its output once and reads
+    little besides (the base state stays in cache): HBM-bound on the writes."""
+    import numpy as np
+    import maestroex_amd as mx
+    from maestroex_amd import _abi
+    from maestroex_amd.fabs import BaseState, FaceMultiFab, Geometry, MultiFab, make_params
+    from maestroex_amd.fabs import split_domain
+
+    dev = mx.device_backend("cuda:0")
+    n = args.n
+    cells = n ** 3
+    faces = 3 * n * n * (n + 1)
+    # algorithmic bytes per point: the outputs written and the cell fields read
+    point_bytes = {"k_put1d": 8, "k_pert_form": 16, "k_convert": 8 * 7, "k_put1d_sph": 24,
+                   "k_w0mac": 8, "k_s0mac": 8}
+    per_kernel = {}
+
+    def timed(name, fn, npts):
+        ms = time_ms(fn, args.steps, args.warmup)
+        gbs = point_bytes[name] * npts / (ms * 1e-3) / 1e9
+        per_kernel[name] = {"ms": round(ms, 4), "bytes_per_point": point_bytes[name],
+                            "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+    rng = np.random.default_rng(2)
+    for sph in (False, True):
+        periodic = [0 if sph else 1] * 3
+        geom = Geometry(3, [n] * 3, [1.0 / n] * 3, periodic, [0] * 3, [0] * 3)
+        params = make_params(geom, nspec=3, spherical=int(sph))
+        boxes = split_domain(geom, args.max_grid)
+        m = mx.gpu_maestro(params, [boxes])
+        if not sph:
+            s0 = BaseState(rng.uniform(0.5, 2.0, (1, n + 1)), dev)
+            out = MultiFab(boxes, 1, 0, 3, dev)
+            scal = MultiFab(boxes, params.nscal, 0, 3, dev)
+            for a in scal.arrays:
+                a.uniform_(0.5, 2.0)
+            d_s0, d_out, d_scal = s0.desc(), out.desc(), scal.desc()
+            f = m.fns
+            timed("k_put1d", lambda: f["put_1d_array_on_cart"](m.ctx, 0, C.byref(d_s0), C.byref(d_out),
+                                                              0, 0, None), cells)
+            timed("k_pert_form", lambda: f["put_in_pert_form"](m.ctx, 0, C.byref(d_scal), C.byref(d_s0),
+                                                               0, 1, None), cells)
+            timed("k_convert", lambda: f["convert_rhox_to_x"](m.ctx, 0, C.byref(d_scal), 1), cells)
+        else:
+            dx = 1.0 / n
+            drf = dx / 5
+            nr = int(np.sqrt(3.0) * (0.5 + 3 * dx) / drf) + 6
+            rcc = dev.from_numpy((np.arange(nr) + 0.5) * drf)
+            redge = dev.from_numpy(np.arange(nr + 1) * drf)
+            g = _abi.SphGeom()
+            for d in range(3):
+                g.center[d], g.prob_lo[d] = 0.5, 0.0
+            g.dr_fine, g.nr_fine = drf, nr
+            g.r_cc_loc, g.r_edge_loc = dev.ptr(rcc), dev.ptr(redge)
+            g.s0_interp_type, g.w0_interp_type, g.s0mac_interp_type, g.w0mac_interp_type = 3, 2, 2, 2
+            s0 = BaseState(rng.uniform(0.5, 2.0, (1, nr)), dev)
+            w0 = BaseState(rng.uniform(-1.0, 1.0, (1, nr + 1)), dev)
+            vec = MultiFab(boxes, 3, 0, 3, dev)
+            mac = FaceMultiFab(boxes, 1, 1, 3, dev)
+            d_s0, d_w0, d_vec, d_mac = s0.desc(), w0.desc(), vec.desc(), mac.desc()
+            none = _abi.MultiFabDesc()
+            f = m.fns
+            timed("k_put1d_sph", lambda: f["put_1d_array_on_cart"](
+                m.ctx, 0, C.byref(d_w0), C.byref(d_vec), 1, 1, C.byref(g)), cells)
+            timed("k_w0mac", lambda: f["make_w0mac"](m.ctx, 0, C.byref(d_w0), C.byref(none),
+                                                     C.byref(d_mac), C.byref(g)), faces)
+            timed("k_s0mac", lambda: f["make_s0mac"](m.ctx, 0, C.byref(d_s0), C.byref(none),
+                                                     C.byref(d_mac), C.byref(g)), faces)
+        del m
+    print(f"[bench] base-state producers: {per_kernel}", file=sys.stderr, flush=True)
+    total_ms = sum(v["ms"] for v in per_kernel.values())
+    dom = "k_w0mac"
+    d = per_kernel[dom]
+    out = {
+        "metric": METRIC,
+        "value": round(6 * cells / (total_ms * 1e-3) / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"base-state producers on 3-D {n}^3 FP64 as 8 boxes of "
+                               f"{args.max_grid}^3: planar Put1dArrayOnCart, PutInPertForm, "
+                               "ConvertRhoXToX; spherical Put1dArrayOnCart (w0, vector), "
+                               "MakeW0mac and MakeS0mac (linear); value counts the 6 calls",
+                   "global_batch": cells, "parallelism": "single GPU"},
+        "base": per_kernel,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["achieved_gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["frac"], "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -524,6 +629,8 @@ def main():
         return adjoint_main(args)
     if args.update:
         return update_main(args)
+    if args.base:
+        return base_main(args)
     import torch
 
     import maestroex_amd as mx
