@@ -442,6 +442,45 @@ int mx_adv_put_in_pert_form(mx_adv_ctx* ctx, int lev, mx_multifab* scal,
                             const mx_basestate* s0, int comp, int flag,
                             const mx_sph_geom* sg);
 
+/* Maestro::MakeVelForce, one level (Maestro.H, MaestroForce.cpp:6-258),
+   without ROTATION (a compile-time option of the reference, off by default):
+   vel_force on the valid cells from the buoyancy of rho - rho0, -gpi / rho,
+   the w0 force and, with do_add_utilde_force, the Utilde . grad w0 term.
+   rho is component 0 of the given MultiFab (the state); gpi,
+   w0_force_cart: dim components; w0_cart: dim components with 1 ghost cell
+   (planar, read with do_add_utilde_force); normal: 3 components
+   (spherical).  rho0 and grav_cell are put on the cells as Put1dArrayOnCart
+   does; gradw0 = (w0(r+1) - w0(r)) / dr_fine is zero with
+   average_base_state or the exact base state, as in the reference.
+   buoyancy_cutoff_factor, base_cutoff_density, do_add_utilde_force and
+   average_base_state are Maestro members there; do_add_utilde_force must be
+   0 or 1 (the reference reads it as a truth value in the planar branch and as
+   == 1 in the spherical one).  The ghosts of vel_force
+   (zeroed, then FillPatch in the reference) stay with the caller. */
+int mx_adv_make_vel_force(mx_adv_ctx* ctx, int lev, mx_multifab* vel_force,
+                          const mx_face_multifab* uedge, const mx_multifab* rho,
+                          const mx_multifab* gpi, const mx_multifab* w0_force_cart,
+                          const mx_multifab* w0_cart, const mx_multifab* normal,
+                          const mx_basestate* rho0, const mx_basestate* grav_cell,
+                          const mx_basestate* w0, double buoyancy_cutoff_factor,
+                          double base_cutoff_density, int do_add_utilde_force,
+                          int average_base_state, const mx_sph_geom* sg);
+
+/* Maestro::ModifyScalForce, one level (MaestroForce.cpp:261-466): component
+   comp of scal_force gains -s div(U) (fullform) or -(s - s0) div(U) -
+   div(s0 U) on the valid cells, with div(U) including w0 (planar: w0_cart,
+   dim components, 1 ghost cell; spherical: divw0 of w0 put on the cells).
+   s0_cart: 1 component (spherical, not fullform: 1 ghost cell).  Planar
+   s0_edge is put on the cells as the edge below each cell; the cell above a
+   box's top cell takes that value inside the domain and, past the domain
+   top, the top cell's value (non-periodic: the first-order extrapolation of
+   the reference's FillPatch at level 0) or the periodic image. */
+int mx_adv_modify_scal_force(mx_adv_ctx* ctx, int lev, mx_multifab* scal_force,
+                             const mx_multifab* state, const mx_face_multifab* umac,
+                             const mx_basestate* s0_edge, const mx_multifab* s0_cart,
+                             const mx_multifab* w0_cart, const mx_basestate* w0, int comp,
+                             int fullform, const mx_sph_geom* sg);
+
 /* Maestro::ConvertRhoXToX (MaestroConvert.cpp:64-91) and ConvertRhoHToH
    (:93-114), one level: the species (or RhoH) divided by Rho (flag != 0) or
    multiplied by it (flag == 0) on the valid cells.  AverageDown and

@@ -215,6 +215,12 @@ SIGNATURES = {
     "make_w0mac": [_vp, _i, P(BaseStateDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(SphGeom)],
     "make_s0mac": [_vp, _i, P(BaseStateDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(SphGeom)],
     "put_in_pert_form": [_vp, _i, P(MultiFabDesc), P(BaseStateDesc), _i, _i, P(SphGeom)],
+    "make_vel_force": [_vp, _i, P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc), P(MultiFabDesc),
+                       P(MultiFabDesc), P(MultiFabDesc), P(MultiFabDesc), P(BaseStateDesc),
+                       P(BaseStateDesc), P(BaseStateDesc), _d, _d, _i, _i, P(SphGeom)],
+    "modify_scal_force": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc),
+                          P(BaseStateDesc), P(MultiFabDesc), P(MultiFabDesc), P(BaseStateDesc), _i,
+                          _i, P(SphGeom)],
     "convert_rhox_to_x": [_vp, _i, P(MultiFabDesc), _i],
     "convert_rhoh_to_h": [_vp, _i, P(MultiFabDesc), _i],
     "update_scal": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc),

@@ -243,6 +243,193 @@ __global__ void __launch_bounds__(NTBS) k_face_base(const BBox* __restrict__ box
     B.out(i, j, k) = v;
 }
 
+// ---------------------------------------------------------------- forces
+// MakeVelForce (MaestroForce.cpp:6-258) and ModifyScalForce (:261-466).  The
+// Cartesian base-state arrays the reference builds with Put1dArrayOnCart
+// (rho0, grav_cell, gradw0, divw0, s0_edge) are evaluated per cell by the
+// same formulas.
+
+struct VfBox {
+    int lo[3], hi[3];
+    DA vf, rho, gpi, wf, w0c, nrm;
+    DA ue[3];
+};
+
+struct VfArgs {
+    const double* rho0;    // row lev (planar) or 0 (spherical)
+    const double* grav;
+    const double* gradw0;  // spherical: nr_fine values
+    double cutoff;         // buoyancy_cutoff_factor * base_cutoff_density
+    double dx[3];
+    int dim, domhi, add_utilde, s0_type;
+};
+
+// the cell value of a bin-centred base state (Put1dArrayOnCart, scalar)
+template <bool SPH>
+__device__ __forceinline__ double cell_base(const SphG& G, const double* s0, int type, int dim,
+                                            int i, int j, int k, double radius) {
+    if (!SPH) return s0[dim == 2 ? j : k];
+    return cell_value(G, s0, radius, type);
+}
+
+template <bool SPH>
+__global__ void __launch_bounds__(NTBS) k_vel_force(const VfBox* __restrict__ boxes, SphG G,
+                                                    VfArgs A) {
+    const VfBox& B = boxes[blockIdx.y];
+    int i, j, k;
+    {
+        const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (!region_point(idx, B.lo, B.hi, i, j, k)) return;
+    }
+    const int dim = A.dim;
+    double x = 0.0, y = 0.0, z = 0.0, radius = 0.0;
+    if (SPH) {
+        pos(G, i, j, k, 0.5, 0.5, 0.5, x, y, z);
+        radius = sqrt(x * x + y * y + z * z);
+    }
+    const double ro = B.rho(i, j, k);
+    double rhopert = ro - cell_base<SPH>(G, A.rho0, A.s0_type, dim, i, j, k, radius);
+    if (ro < A.cutoff) rhopert = 0.0;
+    const double g = cell_base<SPH>(G, A.grav, A.s0_type, dim, i, j, k, radius);
+    if (!SPH) {
+        for (int d = 0; d < dim - 1; ++d) B.vf(i, j, k, d) = -B.gpi(i, j, k, d) / ro;
+        double v = (rhopert * g - B.gpi(i, j, k, dim - 1)) / ro - B.wf(i, j, k, dim - 1);
+        if (A.add_utilde) {
+            if (dim == 2) {
+                if (!(j <= -1 || j >= A.domhi))
+                    v -= (B.ue[1](i, j + 1, k) + B.ue[1](i, j, k)) *
+                         (B.w0c(i, j + 1, k, 1) - B.w0c(i, j, k, 1)) / (2.0 * A.dx[1]);
+            } else {
+                if (!(k <= -1 || k >= A.domhi))
+                    v -= (B.ue[2](i, j, k + 1) + B.ue[2](i, j, k)) *
+                         (B.w0c(i, j, k + 1, 2) - B.w0c(i, j, k, 2)) / (2.0 * A.dx[2]);
+            }
+        }
+        B.vf(i, j, k, dim - 1) = v;
+    } else {
+        const double gc[3] = {g * x / radius, g * y / radius, g * z / radius};
+        double v[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            v[d] = (rhopert * gc[d] - B.gpi(i, j, k, d)) / ro - B.wf(i, j, k, d);
+        if (A.add_utilde == 1) {
+            const double gw = cell_value(G, A.gradw0, radius, A.s0_type);
+            const double ut = 0.5 * (B.ue[0](i, j, k) + B.ue[0](i + 1, j, k)) * B.nrm(i, j, k, 0) +
+                              0.5 * (B.ue[1](i, j, k) + B.ue[1](i, j + 1, k)) * B.nrm(i, j, k, 1) +
+                              0.5 * (B.ue[2](i, j, k) + B.ue[2](i, j, k + 1)) * B.nrm(i, j, k, 2);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) v[d] -= ut * gw * B.nrm(i, j, k, d);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) B.vf(i, j, k, d) = v[d];
+    }
+}
+
+struct SfBox {
+    int lo[3], hi[3];
+    DA f, s, s0c, w0c;
+    DA um[3];
+};
+
+struct SfArgs {
+    const double* s0e;    // planar: row lev of s0_edge
+    const double* divw0;  // spherical: nr_fine values
+    double dx[3];
+    int comp, fullform, dim, s0_type;
+    int vlo, vhi, vper;
+    int domlo[3], domhi[3];
+};
+
+template <bool SPH>
+__global__ void __launch_bounds__(NTBS) k_scal_force(const SfBox* __restrict__ boxes, SphG G,
+                                                     SfArgs A) {
+    const SfBox& B = boxes[blockIdx.y];
+    int i, j, k;
+    {
+        const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (!region_point(idx, B.lo, B.hi, i, j, k)) return;
+    }
+    const int comp = A.comp;
+    double& f = B.f(i, j, k, comp);
+    const double sc = B.s(i, j, k, comp);
+    const double* dx = A.dx;
+    if (SPH) {
+        double x, y, z;
+        pos(G, i, j, k, 0.5, 0.5, 0.5, x, y, z);
+        const double radius = sqrt(x * x + y * y + z * z);
+        const double dv = cell_value(G, A.divw0, radius, A.s0_type);
+        const double divumac = (B.um[0](i + 1, j, k) - B.um[0](i, j, k)) / dx[0] +
+                               (B.um[1](i, j + 1, k) - B.um[1](i, j, k)) / dx[1] +
+                               (B.um[2](i, j, k + 1) - B.um[2](i, j, k)) / dx[2];
+        if (A.fullform) {
+            f -= sc * (divumac + dv);
+        } else {
+            const double c0 = B.s0c(i, j, k);
+            const double xhi = i < A.domhi[0] ? 0.5 * (c0 + B.s0c(i + 1, j, k)) : c0;
+            const double xlo = i > A.domlo[0] ? 0.5 * (c0 + B.s0c(i - 1, j, k)) : c0;
+            const double yhi = j < A.domhi[1] ? 0.5 * (c0 + B.s0c(i, j + 1, k)) : c0;
+            const double ylo = j > A.domlo[1] ? 0.5 * (c0 + B.s0c(i, j - 1, k)) : c0;
+            const double zhi = k < A.domhi[2] ? 0.5 * (c0 + B.s0c(i, j, k + 1)) : c0;
+            const double zlo = k > A.domlo[2] ? 0.5 * (c0 + B.s0c(i, j, k - 1)) : c0;
+            const double divs0u =
+                (B.um[0](i + 1, j, k) * xhi - B.um[0](i, j, k) * xlo) / dx[0] +
+                (B.um[1](i, j + 1, k) * yhi - B.um[1](i, j, k) * ylo) / dx[1] +
+                (B.um[2](i, j, k + 1) * zhi - B.um[2](i, j, k) * zlo) / dx[2];
+            f += -divs0u - (sc - c0) * (divumac + dv);
+        }
+    } else {
+        const int dim = A.dim;
+        double divu = (B.um[0](i + 1, j, k) - B.um[0](i, j, k)) / dx[0] +
+                      (B.um[1](i, j + 1, k) - B.um[1](i, j, k)) / dx[1];
+        if (dim == 3) divu = divu + (B.um[2](i, j, k + 1) - B.um[2](i, j, k)) / dx[2];
+        if (dim == 2)
+            divu += (B.w0c(i, j + 1, k, 1) - B.w0c(i, j, k, 1)) / dx[1];
+        else
+            divu += (B.w0c(i, j, k + 1, 2) - B.w0c(i, j, k, 2)) / dx[2];
+        if (A.fullform) {
+            f -= sc * divu;
+        } else {
+            // s0_edge put on the cells: the edge below each cell; above the
+            // domain top, the top cell's value (first-order extrapolation) or
+            // the periodic image
+            const int r = dim == 2 ? j : k;
+            int r1 = r + 1;
+            if (r1 > A.vhi) r1 = A.vper ? r1 - (A.vhi - A.vlo + 1) : A.vhi;
+            const double e0 = A.s0e[r], e1 = A.s0e[r1];
+            const double c0 = B.s0c(i, j, k);
+            double divs0u;
+            if (dim == 2)
+                divs0u = c0 * (B.um[0](i + 1, j, k) - B.um[0](i, j, k)) / dx[0] +
+                         (B.um[1](i, j + 1, k) * e1 - B.um[1](i, j, k) * e0) / dx[1];
+            else
+                divs0u = c0 * ((B.um[0](i + 1, j, k) - B.um[0](i, j, k)) / dx[0] +
+                               (B.um[1](i, j + 1, k) - B.um[1](i, j, k)) / dx[1]) +
+                         (B.um[2](i, j, k + 1) * e1 - B.um[2](i, j, k) * e0) / dx[2];
+            f += -(sc - c0) * divu - divs0u;
+        }
+    }
+}
+
+// the radial arrays of the spherical forces: gradw0 (MaestroForce.cpp:34-44)
+// or divw0 (:277-291), nr_fine values
+__global__ void __launch_bounds__(NTBS) k_radial_w0(double* out, const double* w0,
+                                                    const double* redge, const double* rcc,
+                                                    int nr, double drf, int what, int zero) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    double v = 0.0;
+    if (!zero) {
+        if (what == 0) {
+            v = (w0[r + 1] - w0[r]) / drf;
+        } else if (r < nr - 1) {
+            const double dr_loc = redge[r + 1] - redge[r];
+            v = (redge[r + 1] * redge[r + 1] * w0[r + 1] - redge[r] * redge[r] * w0[r]) /
+                (dr_loc * rcc[r] * rcc[r]);
+        }
+    }
+    out[r] = v;
+}
+
 SphG sph_g(const mx_adv_ctx* c, int lev, const mx_sph_geom* sg) {
     SphG G;
     for (int d = 0; d < 3; ++d) {
@@ -489,6 +676,181 @@ static int convert(mx_adv_ctx* c, int lev, mx_multifab* scal, int comp0, int nco
     A.ncomp = ncomp;
     A.flag = flag != 0;
     return cell_pass(c, lev, scal, A, false, false, sph_g(c, lev, nullptr));
+}
+
+// the spherical radial array `what` (0 gradw0, 1 divw0) in workspace
+static int radial_w0(mx_adv_ctx* c, const mx_basestate* w0, const mx_sph_geom* sg, int what,
+                     int zero, double** out) {
+    const int nr = sg->nr_fine;
+    int rc = ws_reserve(c, (size_t)nr * sizeof(double) + 4096);
+    if (rc) return rc;
+    *out = ws_take(c, (size_t)nr);
+    MX_LAUNCH(c, "k_radial_w0", k_radial_w0, dim3((nr + NTBS - 1) / NTBS), dim3(NTBS), *out, w0->p,
+              sg->r_edge_loc, sg->r_cc_loc, nr, sg->dr_fine, what, zero);
+    return MX_OK;
+}
+
+int mx_adv_make_vel_force(mx_adv_ctx* c, int lev, mx_multifab* vel_force,
+                          const mx_face_multifab* uedge, const mx_multifab* rho,
+                          const mx_multifab* gpi, const mx_multifab* w0_force_cart,
+                          const mx_multifab* w0_cart, const mx_multifab* normal,
+                          const mx_basestate* rho0, const mx_basestate* grav_cell,
+                          const mx_basestate* w0, double buoyancy_cutoff_factor,
+                          double base_cutoff_density, int do_add_utilde_force,
+                          int average_base_state, const mx_sph_geom* sg) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    const mx_adv_params& p = c->p;
+    const int dim = p.dim;
+    const bool sph = dim == 3 && p.spherical;
+    if (!rho0 || !rho0->p || !grav_cell || !grav_cell->p) return MX_ERR_ARG;
+    // the reference reads the flag as a truth value (planar) and as == 1
+    // (spherical): only 0 and 1 mean the same to both
+    if (do_add_utilde_force != 0 && do_add_utilde_force != 1) return MX_ERR_ARG;
+    if ((rc = check_mf(c, lev, vel_force, dim, 0, -1))) return rc;
+    if ((rc = check_mf(c, lev, rho, 1, 0, -1))) return rc;
+    if ((rc = check_mf(c, lev, gpi, dim, 0, -1))) return rc;
+    if ((rc = check_mf(c, lev, w0_force_cart, dim, 0, -1))) return rc;
+    if (do_add_utilde_force && (rc = check_face(c, lev, uedge, 1, 0))) return rc;
+    VfArgs A;
+    A.cutoff = buoyancy_cutoff_factor * base_cutoff_density;
+    for (int d = 0; d < 3; ++d) A.dx[d] = p.dx[lev][d];
+    A.dim = dim;
+    A.domhi = p.domain_hi[lev][dim - 1];
+    A.add_utilde = do_add_utilde_force;
+    A.s0_type = 0;
+    A.gradw0 = nullptr;
+    if (sph) {
+        if (!w0 || !w0->p) return MX_ERR_ARG;
+        if ((rc = sph_check(c, lev, sg, rho0, 2, 0))) return rc;
+        if ((rc = sph_check(c, lev, sg, grav_cell, 2, 0))) return rc;
+        if (w0->len < sg->nr_fine + 1) return MX_ERR_ARG;
+        if ((rc = check_mf(c, lev, normal, 3, 0, -1))) return rc;
+        A.s0_type = sg->s0_interp_type;
+        if (A.s0_type < 1 || A.s0_type > 3) return MX_ERR_ARG;
+        A.rho0 = rho0->p;
+        A.grav = grav_cell->p;
+        double* gw = nullptr;
+        if (do_add_utilde_force == 1) {
+            rc = radial_w0(c, w0, sg, 0, p.use_exact_base_state || average_base_state, &gw);
+            if (rc) return rc;
+        }
+        A.gradw0 = gw;
+    } else {
+        const int need = p.domain_hi[lev][dim - 1] + 1;
+        if (lev >= rho0->nlev || lev >= grav_cell->nlev || rho0->len < need ||
+            grav_cell->len < need)
+            return MX_ERR_ARG;
+        A.rho0 = rho0->p + (long long)lev * rho0->len;
+        A.grav = grav_cell->p + (long long)lev * grav_cell->len;
+        if (do_add_utilde_force && (rc = check_mf(c, lev, w0_cart, dim, 1, -1))) return rc;
+    }
+    const SphG G = sph_g(c, lev, sg);
+    std::vector<VfBox> boxes;
+    long long maxpts = 0;
+    for (size_t b = 0; b < c->grids[lev].size(); ++b) {
+        VfBox B;
+        HBox r = hbox(c->grids[lev][b]);
+        for (int d = 0; d < 3; ++d) {
+            B.lo[d] = r.lo[d];
+            B.hi[d] = r.hi[d];
+        }
+        B.vf = to_da(vel_force->fab[b]);
+        B.rho = to_da(rho->fab[b]);
+        B.gpi = to_da(gpi->fab[b]);
+        B.wf = to_da(w0_force_cart->fab[b]);
+        B.w0c = (!sph && do_add_utilde_force) ? to_da(w0_cart->fab[b]) : null_da();
+        B.nrm = sph ? to_da(normal->fab[b]) : null_da();
+        for (int d = 0; d < 3; ++d)
+            B.ue[d] = (do_add_utilde_force && d < dim) ? to_da(uedge->dir[d].fab[b]) : null_da();
+        maxpts = std::max(maxpts, points(r));
+        boxes.push_back(B);
+    }
+    Table tab(c);
+    size_t ob = tab.add(boxes);
+    if ((rc = tab.upload())) return rc;
+    const dim3 gr = grid_for(maxpts, (int)boxes.size(), 1, NTBS);
+    if (sph)
+        MX_LAUNCH(c, "k_vel_force", k_vel_force<true>, gr, dim3(NTBS), tab.dev<VfBox>(ob), G, A);
+    else
+        MX_LAUNCH(c, "k_vel_force", k_vel_force<false>, gr, dim3(NTBS), tab.dev<VfBox>(ob), G, A);
+    return MX_OK;
+}
+
+int mx_adv_modify_scal_force(mx_adv_ctx* c, int lev, mx_multifab* scal_force,
+                             const mx_multifab* state, const mx_face_multifab* umac,
+                             const mx_basestate* s0_edge, const mx_multifab* s0_cart,
+                             const mx_multifab* w0_cart, const mx_basestate* w0, int comp,
+                             int fullform, const mx_sph_geom* sg) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    const mx_adv_params& p = c->p;
+    const int dim = p.dim;
+    const bool sph = dim == 3 && p.spherical;
+    if (comp < 0 || comp >= p.nscal) return MX_ERR_ARG;
+    if ((rc = check_mf(c, lev, scal_force, comp + 1, 0, -1))) return rc;
+    if ((rc = check_mf(c, lev, state, comp + 1, 0, -1))) return rc;
+    if ((rc = check_face(c, lev, umac, 1, 0))) return rc;
+    SfArgs A;
+    for (int d = 0; d < 3; ++d) {
+        A.dx[d] = p.dx[lev][d];
+        A.domlo[d] = p.domain_lo[lev][d];
+        A.domhi[d] = p.domain_hi[lev][d];
+    }
+    A.comp = comp;
+    A.fullform = fullform != 0;
+    A.dim = dim;
+    A.s0_type = 0;
+    A.vlo = p.domain_lo[lev][dim - 1];
+    A.vhi = p.domain_hi[lev][dim - 1];
+    A.vper = p.is_periodic[dim - 1] != 0;
+    A.s0e = nullptr;
+    A.divw0 = nullptr;
+    if (sph) {
+        if (!w0 || !w0->p || !sg) return MX_ERR_ARG;
+        if ((rc = sph_check(c, lev, sg, w0, 2, 0))) return rc;
+        if (!A.fullform && (rc = check_mf(c, lev, s0_cart, 1, 1, -1))) return rc;
+        A.s0_type = sg->s0_interp_type;
+        if (A.s0_type < 1 || A.s0_type > 3) return MX_ERR_ARG;
+        double* dw = nullptr;
+        if ((rc = radial_w0(c, w0, sg, 1, p.use_exact_base_state, &dw))) return rc;
+        A.divw0 = dw;
+    } else {
+        if ((rc = check_mf(c, lev, w0_cart, dim, 1, -1))) return rc;
+        if (!A.fullform) {
+            if ((rc = check_mf(c, lev, s0_cart, 1, 0, -1))) return rc;
+            if (!s0_edge || !s0_edge->p || lev >= s0_edge->nlev || s0_edge->len < A.vhi + 1)
+                return MX_ERR_ARG;
+            A.s0e = s0_edge->p + (long long)lev * s0_edge->len;
+        }
+    }
+    const SphG G = sph_g(c, lev, sg);
+    std::vector<SfBox> boxes;
+    long long maxpts = 0;
+    for (size_t b = 0; b < c->grids[lev].size(); ++b) {
+        SfBox B;
+        HBox r = hbox(c->grids[lev][b]);
+        for (int d = 0; d < 3; ++d) {
+            B.lo[d] = r.lo[d];
+            B.hi[d] = r.hi[d];
+        }
+        B.f = to_da(scal_force->fab[b]);
+        B.s = to_da(state->fab[b]);
+        B.s0c = A.fullform ? null_da() : to_da(s0_cart->fab[b]);
+        B.w0c = sph ? null_da() : to_da(w0_cart->fab[b]);
+        for (int d = 0; d < 3; ++d) B.um[d] = d < dim ? to_da(umac->dir[d].fab[b]) : null_da();
+        maxpts = std::max(maxpts, points(r));
+        boxes.push_back(B);
+    }
+    Table tab(c);
+    size_t ob = tab.add(boxes);
+    if ((rc = tab.upload())) return rc;
+    const dim3 gr = grid_for(maxpts, (int)boxes.size(), 1, NTBS);
+    if (sph)
+        MX_LAUNCH(c, "k_scal_force", k_scal_force<true>, gr, dim3(NTBS), tab.dev<SfBox>(ob), G, A);
+    else
+        MX_LAUNCH(c, "k_scal_force", k_scal_force<false>, gr, dim3(NTBS), tab.dev<SfBox>(ob), G, A);
+    return MX_OK;
 }
 
 int mx_adv_convert_rhox_to_x(mx_adv_ctx* c, int lev, mx_multifab* scal, int flag) {

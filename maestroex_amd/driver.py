@@ -148,6 +148,26 @@ class Maestro:
                                                  C.byref(sg) if sg is not None else None),
                     "PutInPertForm")
 
+    def MakeVelForce(self, lev, vel_force, uedge, rho, gpi, w0_force_cart, w0_cart, normal,
+                     rho0, grav_cell, w0, buoyancy_cutoff_factor, base_cutoff_density,
+                     do_add_utilde_force, average_base_state=False, sg=None):
+        """Maestro::MakeVelForce on one level (without ROTATION)."""
+        a = [_mf(vel_force), _face(uedge), _mf(rho), _mf(gpi), _mf(w0_force_cart), _mf(w0_cart),
+             _mf(normal), _bs(rho0), _bs(grav_cell), _bs(w0)]
+        self._check(self.fns["make_vel_force"](
+            self.ctx, lev, *[C.byref(x) for x in a], float(buoyancy_cutoff_factor),
+            float(base_cutoff_density), int(do_add_utilde_force), int(average_base_state),
+            C.byref(sg) if sg is not None else None), "MakeVelForce")
+
+    def ModifyScalForce(self, lev, scal_force, state, umac, s0_edge, s0_cart, w0_cart, w0, comp,
+                        fullform, sg=None):
+        """Maestro::ModifyScalForce on one level."""
+        a = [_mf(scal_force), _mf(state), _face(umac), _bs(s0_edge), _mf(s0_cart), _mf(w0_cart),
+             _bs(w0)]
+        self._check(self.fns["modify_scal_force"](
+            self.ctx, lev, *[C.byref(x) for x in a], comp, int(fullform),
+            C.byref(sg) if sg is not None else None), "ModifyScalForce")
+
     def ConvertRhoXToX(self, lev, scal, flag):
         m = _mf(scal)
         self._check(self.fns["convert_rhox_to_x"](self.ctx, lev, C.byref(m), int(flag)),

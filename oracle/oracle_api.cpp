@@ -76,6 +76,18 @@ int PutInPertForm(const Ctx& c, int lev, const std::vector<Arr>& scal, const Bas
                   int comp, bool flag, const mx_sph_geom* sg);
 int ConvertByRho(const Ctx& c, int lev, const std::vector<Arr>& scal, int comp0, int ncomp,
                  bool flag);
+int MakeVelForce(const Ctx& c, int lev, const std::vector<Arr>& vel_force,
+                 const std::vector<Arr> uedge[3], const std::vector<Arr>& rho,
+                 const std::vector<Arr>& gpi, const std::vector<Arr>& w0_force_cart,
+                 const std::vector<Arr>& w0_cart, const std::vector<Arr>& normal,
+                 const BaseArr& rho0, const BaseArr& grav_cell, const BaseArr& w0,
+                 double buoyancy_cutoff_factor, double base_cutoff_density,
+                 bool do_add_utilde_force, bool average_base_state, const mx_sph_geom* sg);
+int ModifyScalForce(const Ctx& c, int lev, const std::vector<Arr>& scal_force,
+                    const std::vector<Arr>& state, const std::vector<Arr> umac[3],
+                    const BaseArr& s0_edge, const std::vector<Arr>& s0_cart,
+                    const std::vector<Arr>& w0_cart, const BaseArr& w0, int comp, bool fullform,
+                    const mx_sph_geom* sg);
 int UpdateScal(const Ctx& c, int lev, const std::vector<Arr>& sold, const std::vector<Arr>& snew,
                const std::vector<Arr> sflux[3], const std::vector<Arr>& force, int start_comp,
                double base_cutoff_density);
@@ -518,6 +530,55 @@ int mxo_put_in_pert_form(Ctx* c, int lev, mx_multifab* scal, const mx_basestate*
     if (rc) return rc;
     if (!scal || !s0 || comp < 0 || comp >= c->p.nscal) return MX_ERR_ARG;
     rc = PutInPertForm(*c, lev, arrs(scal), base(s0), comp, flag != 0, sg);
+    return finish(rc);
+}
+
+int mxo_make_vel_force(Ctx* c, int lev, mx_multifab* vel_force, const mx_face_multifab* uedge,
+                       const mx_multifab* rho, const mx_multifab* gpi,
+                       const mx_multifab* w0_force_cart, const mx_multifab* w0_cart,
+                       const mx_multifab* normal, const mx_basestate* rho0,
+                       const mx_basestate* grav_cell, const mx_basestate* w0,
+                       double buoyancy_cutoff_factor, double base_cutoff_density,
+                       int do_add_utilde_force, int average_base_state, const mx_sph_geom* sg) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    const size_t nb = c->grids[lev].size();
+    auto ok = [&](const mx_multifab* m) { return m && m->nfab == (int)nb; };
+    if (!ok(vel_force) || !ok(rho) || !ok(gpi) || !ok(w0_force_cart) || !rho0 || !grav_cell)
+        return MX_ERR_ARG;
+    const bool sph = c->p.spherical != 0;
+    if (sph && (!ok(normal) || !w0)) return MX_ERR_ARG;
+    if (!sph && do_add_utilde_force && !ok(w0_cart)) return MX_ERR_ARG;
+    if (do_add_utilde_force && !uedge) return MX_ERR_ARG;
+    if (do_add_utilde_force != 0 && do_add_utilde_force != 1) return MX_ERR_ARG;
+    Face ue = faces(uedge);
+    std::vector<Arr> w0c = (!sph && do_add_utilde_force) ? arrs(w0_cart) : std::vector<Arr>();
+    std::vector<Arr> nrm = sph ? arrs(normal) : std::vector<Arr>();
+    rc = MakeVelForce(*c, lev, arrs(vel_force), ue.d, arrs(rho), arrs(gpi), arrs(w0_force_cart),
+                      w0c, nrm, base(rho0), base(grav_cell), base(w0), buoyancy_cutoff_factor,
+                      base_cutoff_density, do_add_utilde_force != 0, average_base_state != 0, sg);
+    return finish(rc);
+}
+
+int mxo_modify_scal_force(Ctx* c, int lev, mx_multifab* scal_force, const mx_multifab* state,
+                          const mx_face_multifab* umac, const mx_basestate* s0_edge,
+                          const mx_multifab* s0_cart, const mx_multifab* w0_cart,
+                          const mx_basestate* w0, int comp, int fullform, const mx_sph_geom* sg) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    const size_t nb = c->grids[lev].size();
+    auto ok = [&](const mx_multifab* m) { return m && m->nfab == (int)nb; };
+    if (!ok(scal_force) || !ok(state) || !umac || comp < 0 || comp >= c->p.nscal)
+        return MX_ERR_ARG;
+    const bool sph = c->p.spherical != 0;
+    if (sph && !w0) return MX_ERR_ARG;
+    if (!sph && !ok(w0_cart)) return MX_ERR_ARG;
+    if (!fullform && (!ok(s0_cart) || (!sph && !s0_edge))) return MX_ERR_ARG;
+    Face um = faces(umac);
+    std::vector<Arr> s0c = fullform ? std::vector<Arr>() : arrs(s0_cart);
+    std::vector<Arr> w0c = sph ? std::vector<Arr>() : arrs(w0_cart);
+    rc = ModifyScalForce(*c, lev, arrs(scal_force), arrs(state), um.d, base(s0_edge), s0c, w0c,
+                         base(w0), comp, fullform != 0, sg);
     return finish(rc);
 }
 
