@@ -532,7 +532,13 @@ def base_main(args):
     faces = 3 * n * n * (n + 1)
     # algorithmic bytes per point: the outputs written and the cell fields read
     point_bytes = {"k_put1d": 8, "k_pert_form": 16, "k_convert": 8 * 7, "k_put1d_sph": 24,
-                   "k_w0mac": 8, "k_s0mac": 8}
+                   "k_w0mac": 8, "k_s0mac": 8,
+                   # rho, gpi, the w0 force, w, w0_cart in; 3 components out
+                   "k_vel_force": 8 * 10,
+                   # rho, gpi, w0 force, Utilde . n (3 + 3 normal) in; 3 out
+                   "k_vel_force_sph": 8 * 16,
+                   # force in and out, state, umac, w0_cart, s0_cart
+                   "k_scal_force": 8 * 8, "k_scal_force_sph": 8 * 8}
     per_kernel = {}
 
     def timed(name, fn, npts):
@@ -561,6 +567,22 @@ def base_main(args):
             timed("k_pert_form", lambda: f["put_in_pert_form"](m.ctx, 0, C.byref(d_scal), C.byref(d_s0),
                                                                0, 1, None), cells)
             timed("k_convert", lambda: f["convert_rhox_to_x"](m.ctx, 0, C.byref(d_scal), 1), cells)
+            vf, gpi, w0c = (MultiFab(boxes, 3, 0, 3, dev), MultiFab(boxes, 3, 0, 3, dev),
+                            MultiFab(boxes, 3, 1, 3, dev))
+            ue = FaceMultiFab(boxes, 3, 0, 3, dev)
+            s0c = MultiFab(boxes, 1, 1, 3, dev)
+            for a in gpi.arrays + w0c.arrays + s0c.arrays:
+                a.uniform_(-1.0, 1.0)
+            d_vf, d_gpi, d_w0c, d_ue, d_s0c = vf.desc(), gpi.desc(), w0c.desc(), ue.desc(), s0c.desc()
+            grav = BaseState(rng.uniform(-2.0, -0.5, (1, n)), dev)
+            d_grav, none_mf = grav.desc(), _abi.MultiFabDesc()
+            timed("k_vel_force", lambda: f["make_vel_force"](
+                m.ctx, 0, C.byref(d_vf), C.byref(d_ue), C.byref(d_scal), C.byref(d_gpi),
+                C.byref(d_gpi), C.byref(d_w0c), C.byref(none_mf), C.byref(d_s0), C.byref(d_grav),
+                C.byref(d_s0), 0.5, 1.0, 1, 0, None), cells)
+            timed("k_scal_force", lambda: f["modify_scal_force"](
+                m.ctx, 0, C.byref(d_scal), C.byref(d_scal), C.byref(d_ue), C.byref(d_s0),
+                C.byref(d_s0c), C.byref(d_w0c), C.byref(d_s0), 4, 0, None), cells)
         else:
             dx = 1.0 / n
             drf = dx / 5
@@ -586,6 +608,25 @@ def base_main(args):
                                                      C.byref(d_mac), C.byref(g)), faces)
             timed("k_s0mac", lambda: f["make_s0mac"](m.ctx, 0, C.byref(d_s0), C.byref(none),
                                                      C.byref(d_mac), C.byref(g)), faces)
+            scal = MultiFab(boxes, params.nscal, 0, 3, dev)
+            gpi, nrm, s0c = (MultiFab(boxes, 3, 0, 3, dev), MultiFab(boxes, 3, 0, 3, dev),
+                             MultiFab(boxes, 1, 1, 3, dev))
+            ue = FaceMultiFab(boxes, 1, 0, 3, dev)
+            for a in scal.arrays:
+                a.uniform_(0.5, 2.0)
+            for a in gpi.arrays + nrm.arrays + s0c.arrays:
+                a.uniform_(-1.0, 1.0)
+            vf = MultiFab(boxes, 3, 0, 3, dev)
+            grav = BaseState(rng.uniform(-2.0, -0.5, (1, nr)), dev)
+            d_scal, d_gpi, d_nrm, d_s0c, d_ue, d_vf, d_grav = (
+                scal.desc(), gpi.desc(), nrm.desc(), s0c.desc(), ue.desc(), vf.desc(), grav.desc())
+            timed("k_vel_force_sph", lambda: f["make_vel_force"](
+                m.ctx, 0, C.byref(d_vf), C.byref(d_ue), C.byref(d_scal), C.byref(d_gpi),
+                C.byref(d_gpi), C.byref(none), C.byref(d_nrm), C.byref(d_s0), C.byref(d_grav),
+                C.byref(d_w0), 0.5, 1.0, 1, 0, C.byref(g)), cells)
+            timed("k_scal_force_sph", lambda: f["modify_scal_force"](
+                m.ctx, 0, C.byref(d_scal), C.byref(d_scal), C.byref(d_ue), C.byref(d_w0),
+                C.byref(d_s0c), C.byref(none), C.byref(d_w0), 4, 0, C.byref(g)), cells)
         del m
     print(f"[bench] base-state producers: {per_kernel}", file=sys.stderr, flush=True)
     total_ms = sum(v["ms"] for v in per_kernel.values())
@@ -593,7 +634,7 @@ def base_main(args):
     d = per_kernel[dom]
     out = {
         "metric": METRIC,
-        "value": round(6 * cells / (total_ms * 1e-3) / 1e9, 4),
+        "value": round(len(per_kernel) * cells / (total_ms * 1e-3) / 1e9, 4),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -606,8 +647,9 @@ def base_main(args):
         "data": "synthetic",
         "config": {"workload": f"base-state producers on 3-D {n}^3 FP64 as 8 boxes of "
                                f"{args.max_grid}^3: planar Put1dArrayOnCart, PutInPertForm, "
-                               "ConvertRhoXToX; spherical Put1dArrayOnCart (w0, vector), "
-                               "MakeW0mac and MakeS0mac (linear); value counts the 6 calls",
+                               "ConvertRhoXToX, MakeVelForce, ModifyScalForce; spherical "
+                               "Put1dArrayOnCart (w0, vector), MakeW0mac and MakeS0mac (linear), "
+                               "MakeVelForce, ModifyScalForce; value counts the 10 calls",
                    "global_batch": cells, "parallelism": "single GPU"},
         "base": per_kernel,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["achieved_gbs"],
