@@ -488,6 +488,25 @@ int mx_adv_modify_scal_force(mx_adv_ctx* ctx, int lev, mx_multifab* scal_force,
 int mx_adv_convert_rhox_to_x(mx_adv_ctx* ctx, int lev, mx_multifab* scal, int flag);
 int mx_adv_convert_rhoh_to_h(mx_adv_ctx* ctx, int lev, mx_multifab* scal, int flag);
 
+/* ---- AMR restriction ---------------------------------------------------- */
+/* amrex::average_down(fine, crse, ..., comp, ncomp, ratio) between levels
+   lev + 1 (fine) and lev (crse), the step of Maestro::AverageDown
+   (MaestroFillData.cpp:145-154, which loops lev = finest_level - 1 .. 0):
+   every coarse valid cell covered by the coarsened fine BoxArray becomes
+   the mean of its r^dim fine cells (summed k outermost, i innermost, times
+   1 / r^dim); other coarse cells and the coarse ghosts keep their values.
+   The ratio is the ratio of the two levels' domains. */
+int mx_adv_average_down(mx_adv_ctx* ctx, int lev, const mx_multifab* fine, mx_multifab* crse,
+                        int comp, int ncomp);
+
+/* amrex::average_down_faces(fine, crse, ratio) between levels lev + 1 and
+   lev, the step of Maestro::AverageDownFaces (MaestroFillData.cpp:156-172):
+   every coarse face covered by the coarsened fine faces becomes the mean of
+   the r^(dim-1) fine faces on it, every component.  A face shared by two
+   fine boxes must hold one value in both (as after FillBoundary). */
+int mx_adv_average_down_faces(mx_adv_ctx* ctx, int lev, const mx_face_multifab* fine,
+                              mx_face_multifab* crse);
+
 /* ---- the update that closes the step ------------------------------------ */
 /* Maestro::UpdateScal (Maestro.H:306-311, MaestroAdvection.cpp:86-297) on
    level lev: statenew = stateold + dt (-div sflux + force) on the valid

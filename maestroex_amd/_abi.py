@@ -221,6 +221,8 @@ SIGNATURES = {
     "modify_scal_force": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc),
                           P(BaseStateDesc), P(MultiFabDesc), P(MultiFabDesc), P(BaseStateDesc), _i,
                           _i, P(SphGeom)],
+    "average_down": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i],
+    "average_down_faces": [_vp, _i, P(FaceMultiFabDesc), P(FaceMultiFabDesc)],
     "convert_rhox_to_x": [_vp, _i, P(MultiFabDesc), _i],
     "convert_rhoh_to_h": [_vp, _i, P(MultiFabDesc), _i],
     "update_scal": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc),

@@ -168,6 +168,29 @@ class Maestro:
             self.ctx, lev, *[C.byref(x) for x in a], comp, int(fullform),
             C.byref(sg) if sg is not None else None), "ModifyScalForce")
 
+    def average_down(self, lev, fine, crse, comp, ncomp):
+        """amrex::average_down between levels lev + 1 and lev (one step of
+        Maestro::AverageDown)."""
+        a = [_mf(fine), _mf(crse)]
+        self._check(self.fns["average_down"](self.ctx, lev, *[C.byref(x) for x in a], comp, ncomp),
+                    "average_down")
+
+    def average_down_faces(self, lev, fine, crse):
+        """amrex::average_down_faces between levels lev + 1 and lev."""
+        a = [_face(fine), _face(crse)]
+        self._check(self.fns["average_down_faces"](self.ctx, lev, *[C.byref(x) for x in a]),
+                    "average_down_faces")
+
+    def AverageDown(self, mfs, comp, ncomp):
+        """Maestro::AverageDown (MaestroFillData.cpp:145-154): mfs[lev] per level."""
+        for lev in range(len(mfs) - 2, -1, -1):
+            self.average_down(lev, mfs[lev + 1], mfs[lev], comp, ncomp)
+
+    def AverageDownFaces(self, edges):
+        """Maestro::AverageDownFaces (MaestroFillData.cpp:156-172)."""
+        for lev in range(len(edges) - 2, -1, -1):
+            self.average_down_faces(lev, edges[lev + 1], edges[lev])
+
     def ConvertRhoXToX(self, lev, scal, flag):
         m = _mf(scal)
         self._check(self.fns["convert_rhox_to_x"](self.ctx, lev, C.byref(m), int(flag)),

@@ -76,6 +76,10 @@ int PutInPertForm(const Ctx& c, int lev, const std::vector<Arr>& scal, const Bas
                   int comp, bool flag, const mx_sph_geom* sg);
 int ConvertByRho(const Ctx& c, int lev, const std::vector<Arr>& scal, int comp0, int ncomp,
                  bool flag);
+int AverageDown(const Ctx& c, int lev, const std::vector<Arr>& fine,
+                const std::vector<Arr>& crse, int comp, int ncomp);
+int AverageDownFaces(const Ctx& c, int lev, const std::vector<Arr> fine[3],
+                     const std::vector<Arr> crse[3]);
 int MakeVelForce(const Ctx& c, int lev, const std::vector<Arr>& vel_force,
                  const std::vector<Arr> uedge[3], const std::vector<Arr>& rho,
                  const std::vector<Arr>& gpi, const std::vector<Arr>& w0_force_cart,
@@ -580,6 +584,29 @@ int mxo_modify_scal_force(Ctx* c, int lev, mx_multifab* scal_force, const mx_mul
     rc = ModifyScalForce(*c, lev, arrs(scal_force), arrs(state), um.d, base(s0_edge), s0c, w0c,
                          base(w0), comp, fullform != 0, sg);
     return finish(rc);
+}
+
+int mxo_average_down(Ctx* c, int lev, const mx_multifab* fine, mx_multifab* crse, int comp,
+                     int ncomp) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (lev + 1 > c->p.finest_level || c->grids[lev + 1].empty() || !fine || !crse ||
+        fine->nfab != (int)c->grids[lev + 1].size() || crse->nfab != (int)c->grids[lev].size())
+        return MX_ERR_ARG;
+    return finish(AverageDown(*c, lev, arrs(fine), arrs(crse), comp, ncomp));
+}
+
+int mxo_average_down_faces(Ctx* c, int lev, const mx_face_multifab* fine,
+                           mx_face_multifab* crse) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (lev + 1 > c->p.finest_level || c->grids[lev + 1].empty() || !fine || !crse)
+        return MX_ERR_ARG;
+    Face f = faces(fine), cr = faces(crse);
+    for (int d = 0; d < c->p.dim; ++d)
+        if (f.d[d].size() != c->grids[lev + 1].size() || cr.d[d].size() != c->grids[lev].size())
+            return MX_ERR_ARG;
+    return finish(AverageDownFaces(*c, lev, f.d, cr.d));
 }
 
 int mxo_convert_rhox_to_x(Ctx* c, int lev, mx_multifab* scal, int flag) {
