@@ -196,7 +196,10 @@ def amr_main(args):
     per edge, level 1 (ratio 2) 150 boxes of 16-48, level 2 200 boxes of
     8-32, all 5 scalars + 3 velocities.  The hierarchy pass is timed as one
     graph launch, as an eager multi-level call, and with one fused launch per
-    box; the per-cell cost is compared with a single 128^3 box."""
+    box; the per-cell cost is compared with a single 128^3 box.  The AMR
+    restriction that follows an update (AverageDown of the scalars,
+    AverageDownFaces of umac) is timed on the same hierarchy."""
+    import numpy as np
     import maestroex_amd as mx
     from maestroex_amd.workloads import Hierarchy, Workload
 
@@ -223,6 +226,38 @@ def amr_main(args):
     per_box_ms = time_ms(lambda: m.advance_pass_levels(fields), args.steps, args.warmup)
     m.set_tuning("edge_scal_per_box", 0)
     g.close()
+
+    # AMR restriction after the update: AverageDown of every scalar and
+    # AverageDownFaces of umac, finest level first (HIP events per kernel);
+    # descriptors built once, as a C++ caller holds its MultiFab views
+    nsc = h.params.nscal
+    descs = [(lev, fields[lev + 1]["scal"].desc(), fields[lev]["scal"].desc(),
+              fields[lev + 1]["umac"].desc(), fields[lev]["umac"].desc())
+             for lev in range(h.nlev - 2, -1, -1)]
+
+    def restrict():
+        for lev, fs, cs, fu, cu in descs:
+            rc = m.fns["average_down"](m.ctx, lev, C.byref(fs), C.byref(cs), 0, nsc)
+            rc = rc or m.fns["average_down_faces"](m.ctx, lev, C.byref(fu), C.byref(cu))
+            if rc:
+                raise RuntimeError(f"restriction: error {rc}")
+    sync_ms = time_ms(restrict, args.steps, args.warmup)
+    m.fns["prof_enable"](m.ctx, 1)
+    restrict()
+    nk = m.fns["prof_read"](m.ctx, -1, None, None, None)
+    sync_kernel_ms = 0.0
+    for idx in range(nk):
+        nm, kms, cnt = C.c_char_p(), C.c_double(), C.c_longlong()
+        m.fns["prof_read"](m.ctx, idx, C.byref(nm), C.byref(kms), C.byref(cnt))
+        if nm.value.decode() == "k_average_down":
+            sync_kernel_ms = kms.value
+    m.fns["prof_enable"](m.ctx, 0)
+    fine_cells = sum(int(np.prod([hi[d] - lo[d] + 1 for d in range(3)]))
+                     for bl in h.boxes[1:] for (lo, hi) in bl)
+    # bytes: every fine cell read and an eighth as many coarse cells written
+    # (nscal components); half the fine faces of each direction read and a
+    # quarter of that written (1 component)
+    sync_bytes = 8 * fine_cells * (nsc * (1 + 1 / 8) + 3 * 0.5 * (1 + 1 / 4))
 
     # the single-box reference: one 128^3 box, its own autotuned edge-kernel
     # shape, one graph launch per pass (as the hierarchy)
@@ -262,7 +297,11 @@ def amr_main(args):
                 "single_box_autotuned": {"tile": TILE_NAMES[cands1[best1][0]],
                                          "tz": cands1[best1][1]},
                 "autotuned": tuned,
-                "autotune_candidates_ms": [[c[0], c[1], round(c[2], 4)] for c in cands]},
+                "autotune_candidates_ms": [[c[0], c[1], round(c[2], 4)] for c in cands],
+                "restriction": {"ms": round(sync_ms, 4), "kernel_ms": round(sync_kernel_ms, 4),
+                                "fine_cells": fine_cells,
+                                "kernel_gbs": round(sync_bytes / (sync_kernel_ms * 1e-3) / 1e9, 1)
+                                if sync_kernel_ms > 0 else None}},
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -4,7 +4,8 @@
 //
 // The host intersects every coarse box with every coarsened fine box (cells,
 // or the faces of one direction) and uploads the non-empty intersections as
-// tasks; one launch covers them all, one thread per coarse point.  A coarse
+// tasks; one launch covers them all, one thread per coarse point and
+// component.  A coarse
 // cell gets (1 / r^dim) times the sum of its r^dim fine cells, summed k
 // outermost and i innermost; a coarse face the mean of the fine faces on
 // it.  Coarsened fine cells never overlap; a coarse face shared by two
@@ -22,35 +23,37 @@ constexpr int NTS = 256;
 
 struct SyncTask {
     int lo[3], hi[3];  // coarse points
+    int t[3];          // fine points summed per coarse point in each direction
+    int blk0;          // the task's first block
+    double scale;      // 1 / (t0 t1 t2)
     DA crse, fine;
 };
 
 struct SyncArgs {
     int r[3];   // refinement ratio
-    int t[3];   // fine points summed per coarse point in each direction
     int comp, ncomp;
-    double scale;  // 1 / (t0 t1 t2)
-    int cells;     // 1: cell average (volfrac * sum), 0: face average (sum * facinv)
-    int pad_ = 0;
+    int cells;  // 1: cell average (volfrac * sum), 0: face average (sum * facinv)
 };
 
+// grid: x over the blocks of all tasks (blk[x] names the task; no block is
+// empty), z over components (one component per thread keeps every thread's
+// loads independent of the store before them)
 __global__ void __launch_bounds__(NTS) k_average_down(const SyncTask* __restrict__ tasks,
-                                                      SyncArgs A) {
-    const SyncTask& T = tasks[blockIdx.y];
+                                                      const int* __restrict__ blk, SyncArgs A) {
+    const SyncTask T = tasks[blk[blockIdx.x]];
     int i, j, k;
     {
-        const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+        const long long idx = (long long)(blockIdx.x - T.blk0) * NTS + threadIdx.x;
         if (!region_point(idx, T.lo, T.hi, i, j, k)) return;
     }
     const int ii = i * A.r[0], jj = j * A.r[1], kk = k * A.r[2];
-    for (int n = A.comp; n < A.comp + A.ncomp; ++n) {
-        double s = 0.0;
-        for (int kref = 0; kref < A.t[2]; ++kref)
-            for (int jref = 0; jref < A.t[1]; ++jref)
-                for (int iref = 0; iref < A.t[0]; ++iref)
-                    s += T.fine(ii + iref, jj + jref, kk + kref, n);
-        T.crse(i, j, k, n) = A.cells ? A.scale * s : s * A.scale;
-    }
+    const int n = A.comp + (int)blockIdx.z;
+    double s = 0.0;
+    for (int kref = 0; kref < T.t[2]; ++kref)
+        for (int jref = 0; jref < T.t[1]; ++jref)
+            for (int iref = 0; iref < T.t[0]; ++iref)
+                s += T.fine(ii + iref, jj + jref, kk + kref, n);
+    T.crse(i, j, k, n) = A.cells ? T.scale * s : s * T.scale;
 }
 
 int floordiv(int a, int r) { return a >= 0 ? a / r : -((-a + r - 1) / r); }
@@ -92,7 +95,7 @@ int ratio(const mx_adv_ctx* c, int lev, int r[3]) {
 // a task reads is checked to lie in the fine FAB
 int tasks_for(const mx_adv_ctx* c, int lev, int d, const mx_multifab* fine,
               const mx_multifab* crse, const int r[3], int ncomp_needed,
-              std::vector<SyncTask>& out, long long& maxpts) {
+              std::vector<SyncTask>& out) {
     const auto& fg = c->grids[lev + 1];
     const auto& cg = c->grids[lev];
     for (size_t f = 0; f < fg.size(); ++f) {
@@ -116,28 +119,39 @@ int tasks_for(const mx_adv_ctx* c, int lev, int d, const mx_multifab* fine,
             for (int q = 0; q < 3; ++q) {
                 T.lo[q] = reg.lo[q];
                 T.hi[q] = reg.hi[q];
+                T.t[q] = q == d ? 1 : r[q];
             }
+            T.scale = 1.0 / double(T.t[0] * T.t[1] * T.t[2]);
             T.crse = to_da(crse->fab[b]);
             T.fine = to_da(fine->fab[f]);
-            long long np = 1;
-            for (int q = 0; q < 3; ++q) np *= (long long)(reg.hi[q] - reg.lo[q] + 1);
-            maxpts = std::max(maxpts, np);
+            T.blk0 = 0;
             out.push_back(T);
         }
     }
     return MX_OK;
 }
 
-int launch_tasks(mx_adv_ctx* c, const std::vector<SyncTask>& tasks, long long maxpts,
-                 const SyncArgs& A) {
+int launch_tasks(mx_adv_ctx* c, std::vector<SyncTask>& tasks, const SyncArgs& A) {
     if (tasks.empty()) return MX_OK;
-    if (tasks.size() > 65535) return MX_ERR_UNSUPPORTED;
+    if (A.ncomp > 65535) return MX_ERR_UNSUPPORTED;
+    std::vector<int> blk;
+    for (size_t q = 0; q < tasks.size(); ++q) {
+        const SyncTask& T = tasks[q];
+        long long np = 1;
+        for (int d = 0; d < 3; ++d) np *= (long long)(T.hi[d] - T.lo[d] + 1);
+        const long long nb = (np + NTS - 1) / NTS;
+        if ((long long)blk.size() + nb > 0x7fffffffLL) return MX_ERR_UNSUPPORTED;
+        tasks[q].blk0 = (int)blk.size();
+        blk.insert(blk.end(), (size_t)nb, (int)q);
+    }
     Table tab(c);
-    size_t ot = tab.add(tasks);
+    const size_t ot = tab.add(tasks);
+    const size_t ob = tab.add(blk);
     int rc = tab.upload();
     if (rc) return rc;
-    MX_LAUNCH(c, "k_average_down", k_average_down, grid_for(maxpts, (int)tasks.size(), 1, NTS),
-              dim3(NTS), tab.dev<SyncTask>(ot), A);
+    MX_LAUNCH(c, "k_average_down", k_average_down,
+              dim3((unsigned)blk.size(), 1, (unsigned)A.ncomp), dim3(NTS), tab.dev<SyncTask>(ot),
+              tab.dev<int>(ob), A);
     return MX_OK;
 }
 
@@ -156,18 +170,13 @@ int mx_adv_average_down(mx_adv_ctx* c, int lev, const mx_multifab* fine, mx_mult
     int r[3];
     if (!ratio(c, lev, r)) return MX_ERR_ARG;
     std::vector<SyncTask> tasks;
-    long long maxpts = 0;
-    if ((rc = tasks_for(c, lev, -1, fine, crse, r, comp + ncomp, tasks, maxpts))) return rc;
+    if ((rc = tasks_for(c, lev, -1, fine, crse, r, comp + ncomp, tasks))) return rc;
     SyncArgs A;
-    for (int d = 0; d < 3; ++d) {
-        A.r[d] = r[d];
-        A.t[d] = r[d];
-    }
+    for (int d = 0; d < 3; ++d) A.r[d] = r[d];
     A.comp = comp;
     A.ncomp = ncomp;
-    A.scale = 1.0 / double(r[0] * r[1] * r[2]);
     A.cells = 1;
-    return launch_tasks(c, tasks, maxpts, A);
+    return launch_tasks(c, tasks, A);
 }
 
 int mx_adv_average_down_faces(mx_adv_ctx* c, int lev, const mx_face_multifab* fine,
@@ -178,28 +187,25 @@ int mx_adv_average_down_faces(mx_adv_ctx* c, int lev, const mx_face_multifab* fi
     if (!fine || !crse) return MX_ERR_ARG;
     int r[3];
     if (!ratio(c, lev, r)) return MX_ERR_ARG;
+    // every direction's faces in one launch (one component count for all)
+    std::vector<SyncTask> tasks;
+    int nc = -1;
     for (int d = 0; d < c->p.dim; ++d) {
         const mx_multifab* f = &fine->dir[d];
         mx_multifab* cm = &crse->dir[d];
         if ((int)c->grids[lev + 1].size() != f->nfab || (int)c->grids[lev].size() != cm->nfab ||
             !f->fab || !cm->fab)
             return MX_ERR_ARG;
-        const int nc = cm->fab[0].ncomp;
-        std::vector<SyncTask> tasks;
-        long long maxpts = 0;
-        if ((rc = tasks_for(c, lev, d, f, cm, r, nc, tasks, maxpts))) return rc;
-        SyncArgs A;
-        for (int q = 0; q < 3; ++q) {
-            A.r[q] = r[q];
-            A.t[q] = q == d ? 1 : r[q];
-        }
-        A.comp = 0;
-        A.ncomp = nc;
-        A.scale = 1.0 / double(A.t[0] * A.t[1] * A.t[2]);
-        A.cells = 0;
-        if ((rc = launch_tasks(c, tasks, maxpts, A))) return rc;
+        if (nc < 0) nc = cm->fab[0].ncomp;
+        if (cm->fab[0].ncomp != nc) return MX_ERR_ARG;
+        if ((rc = tasks_for(c, lev, d, f, cm, r, nc, tasks))) return rc;
     }
-    return MX_OK;
+    SyncArgs A;
+    for (int q = 0; q < 3; ++q) A.r[q] = r[q];
+    A.comp = 0;
+    A.ncomp = nc;
+    A.cells = 0;
+    return launch_tasks(c, tasks, A);
 }
 
 }  // extern "C"

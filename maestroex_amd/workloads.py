@@ -361,17 +361,19 @@ def _partition(rng, n, lo, hi):
     return segs
 
 
-def _random_boxes(rng, n, count, lo, hi, region_frac=1.0, tries=200000):
+def _random_boxes(rng, n, count, lo, hi, region_frac=1.0, align=1, tries=200000):
     """`count` non-overlapping boxes with random edges lo..hi inside the
-    first region_frac of [0, n)^3 along every axis."""
+    first region_frac of [0, n)^3 along every axis; corners and edges are
+    multiples of `align` (2: every box coarsens by the refinement ratio onto
+    whole coarse cells, as AMReX's blocking factor guarantees)."""
     extent = max(hi, int(round(n * region_frac)))
     occ = []
     boxes = []
     t = 0
     while len(boxes) < count and t < tries:
         t += 1
-        e = [int(rng.integers(lo, hi + 1)) for _ in range(3)]
-        a = [int(rng.integers(0, extent - e[d] + 1)) for d in range(3)]
+        e = [align * int(rng.integers(-(-lo // align), hi // align + 1)) for _ in range(3)]
+        a = [align * int(rng.integers(0, (extent - e[d]) // align + 1)) for d in range(3)]
         b = [a[d] + e[d] - 1 for d in range(3)]
         if any(all(a[d] <= q[1][d] and q[0][d] <= b[d] for d in range(3)) for q in occ):
             continue
@@ -385,8 +387,10 @@ class Hierarchy:
     evaluated per box (no full-level arrays), refinement ratio 2.
 
     level 0: n0^3 split into boxes whose edges are lo0..hi0 along each axis;
-    level l > 0: `counts[l-1]` non-overlapping boxes with random edges
-    `edges[l-1]` inside the first `fracs[l-1]` of the level's index space.
+    level l > 0: `counts[l-1]` non-overlapping boxes with random even edges
+    `edges[l-1]` at even corners inside the first `fracs[l-1]` of the level's
+    index space (each coarsens onto whole cells of level l - 1; boxes are not
+    nested in the coarser level's boxes).
     Periodic in x and y, outflow in z; w0 = 0.01 z on every level; one dt
     (CFL 0.5 on the finest level) for all levels.  Every level follows the
     single-level semantics of the pass: ghost cells hold the analytic fields
@@ -411,7 +415,7 @@ class Hierarchy:
         self.boxes = [b0]
         for l in range(1, nlev):
             self.boxes.append(_random_boxes(rng, self.ns[l], counts[l - 1], *edges[l - 1],
-                                            region_frac=fracs[l - 1]))
+                                            region_frac=fracs[l - 1], align=2))
         geoms = [Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 0],
                           [_abi.PHYS_INTERIOR, _abi.PHYS_INTERIOR, _abi.PHYS_OUTFLOW],
                           [_abi.PHYS_INTERIOR, _abi.PHYS_INTERIOR, _abi.PHYS_OUTFLOW])
