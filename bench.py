@@ -100,6 +100,8 @@ def parse():
                     help="the base-state <-> Cartesian producers on the 3-D level")
     ap.add_argument("--update", action="store_true",
                     help="UpdateScal + UpdateVel (the update that closes the step) on the 3-D level")
+    ap.add_argument("--fill", action="store_true",
+                    help="FillPatch (level 0) of the scalars on the 3-D level")
     ap.add_argument("--spherical", action="store_true",
                     help="the spherical star: 384^3 in 216 uneven boxes (edges 48-96), "
                          "ppm_type 2, radial w0, one graph launch per pass")
@@ -520,6 +522,127 @@ def update_main(args):
     print(json.dumps(out), flush=True)
 
 
+def fill_main(args):
+    """Maestro::FillPatch at level 0 of every scalar (7 components) on the 3-D
+    256^3 level of 8 boxes of 128^3 with 4 ghost cells: periodic x and y,
+    outflow z (BCSetup's BCRecs).  HBM-bound: one read and one write per
+    ghost cell and component."""
+    import numpy as np
+    import torch
+    import maestroex_amd as mx
+    from maestroex_amd import _abi
+    from maestroex_amd.fabs import Geometry, MultiFab, make_params, split_domain
+
+    dev = mx.device_backend("cuda:0")
+    n, ng = args.n, 4
+    out_bc = [_abi.PHYS_INTERIOR, _abi.PHYS_INTERIOR, _abi.PHYS_OUTFLOW]
+    geom = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 0], out_bc, out_bc)
+    params = make_params(geom, nspec=3)
+    boxes = split_domain(geom, args.max_grid)
+    nc = params.nscal
+    m = mx.gpu_maestro(params, [boxes])
+    mf = MultiFab(boxes, nc, ng, 3, dev)
+    gen = torch.Generator(device="cuda:0").manual_seed(9)
+    for a in mf.arrays:
+        a.uniform_(-1.0, 1.0, generator=gen)
+    bs, _ = m.bc_setup()
+    desc = mf.desc()
+
+    def call():
+        rc = m.fns["fill_patch"](m.ctx, 0, C.byref(desc), C.byref(desc), 0, 0, nc, bs, 0)
+        if rc:
+            raise RuntimeError(f"fill_patch: error {rc}")
+    ms = time_ms(call, args.steps, args.warmup)
+    m.fns["prof_enable"](m.ctx, 1)
+    for _ in range(args.steps):
+        call()
+    nk = m.fns["prof_read"](m.ctx, -1, None, None, None)
+    kern = {}
+    for idx in range(nk):
+        nm, kms, cnt = C.c_char_p(), C.c_double(), C.c_longlong()
+        m.fns["prof_read"](m.ctx, idx, C.byref(nm), C.byref(kms), C.byref(cnt))
+        kern[nm.value.decode()] = (kms.value / args.steps, cnt.value // args.steps)
+    m.fns["prof_enable"](m.ctx, 0)
+    ghost = sum(int(np.prod([hi[d] - lo[d] + 1 + 2 * ng for d in range(3)]))
+                - int(np.prod([hi[d] - lo[d] + 1 for d in range(3)])) for lo, hi in boxes)
+    slab = sum(int(np.prod([hi[d] - lo[d] + 1 + 2 * ng for d in range(2)])) * ng
+               * ((lo[2] == 0) + (hi[2] == n - 1)) for lo, hi in boxes)
+    # one read and one write of 8 bytes per ghost cell and component
+    bytes_copy = 16 * nc * (ghost - slab)
+    bytes_phys = 16 * nc * slab
+    kms_total = sum(v[0] for v in kern.values())
+    achieved = 16 * nc * ghost / (kms_total * 1e-3) / 1e9
+    per_kernel = {k: {"ms_per_call": round(v[0], 4), "launches_per_call": v[1]}
+                  for k, v in sorted(kern.items())}
+    per_kernel.get("k_fill_copy", {})["achieved_gbs"] = (
+        round(bytes_copy / (kern["k_fill_copy"][0] * 1e-3) / 1e9, 1) if "k_fill_copy" in kern
+        else None)
+    per_kernel.get("k_fill_phys", {})["achieved_gbs"] = (
+        round(bytes_phys / (kern["k_fill_phys"][0] * 1e-3) / 1e9, 1) if "k_fill_phys" in kern
+        else None)
+    print(f"[bench] fill kernels: {per_kernel}", file=sys.stderr, flush=True)
+    cpu = None if args.no_cpu else fill_cpu_baseline(args, ng)
+    out = {
+        "metric": METRIC,
+        "value": round(n ** 3 / (ms * 1e-3) / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"FillPatch (level 0) of {nc} scalars, {ng} ghost cells, on 3-D "
+                               f"{n}^3 FP64 as {len(boxes)} boxes of {args.max_grid}^3, periodic "
+                               f"x/y, outflow z",
+                   "global_batch": n ** 3, "ghost_cells": ghost, "parallelism": "single GPU"},
+        "fill": per_kernel,
+        "roofline": {"bound": "hbm", "kernel": "k_fill_copy + k_fill_phys",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms_per_call": round(kms_total, 4),
+                     "bytes_per_ghost_cell": 16 * nc},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def fill_cpu_baseline(args, ng):
+    """The oracle's FillPatch of the same 7 scalars on one 128^3 box (periodic
+    x/y, outflow z), timed on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    from maestroex_amd import _abi
+    from maestroex_amd.fabs import Backend, Geometry, MultiFab, make_params, split_domain
+    cores = min(int(os.environ.get("MX_CPU_THREADS", "0")) or len(os.sched_getaffinity(0)), 16)
+    pyoracle.set_threads(cores, fast=True)
+    n = args.cpu_n
+    out_bc = [_abi.PHYS_INTERIOR, _abi.PHYS_INTERIOR, _abi.PHYS_OUTFLOW]
+    geom = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 0], out_bc, out_bc)
+    params = make_params(geom, nspec=3)
+    boxes = split_domain(geom, n)
+    m = pyoracle.oracle_maestro(params, [boxes], fast=True)
+    mf = MultiFab(boxes, params.nscal, ng, 3, Backend("numpy"))
+    rng = np.random.default_rng(9)
+    for a in mf.arrays:
+        a[...] = rng.uniform(-1.0, 1.0, size=a.shape)
+    bs, _ = m.bc_setup()
+    m.FillPatch(0, mf, mf, 0, 0, params.nscal, 0, bs, 0)
+    times = []
+    for _ in range(args.cpu_reps):
+        t0 = time.perf_counter()
+        m.FillPatch(0, mf, mf, 0, 0, params.nscal, 0, bs, 0)
+        times.append(time.perf_counter() - t0)
+    t = min(times)
+    return {"value": round(n ** 3 / t / 1e9, 5), "unit": "Gcell-updates/s", "cores": cores,
+            "kind": "port", "sample": f"the oracle's FillPatch of {params.nscal} scalars on one "
+                                      f"{n}^3 box, best of {args.cpu_reps}"}
+
+
 def update_cpu_baseline(args, nspec, bcd):
     """The oracle's UpdateScal + UpdateVel on one 128^3 box, timed on the
     host cores."""
@@ -712,6 +835,8 @@ def main():
         return update_main(args)
     if args.base:
         return base_main(args)
+    if args.fill:
+        return fill_main(args)
     import torch
 
     import maestroex_amd as mx

@@ -488,6 +488,33 @@ int mx_adv_modify_scal_force(mx_adv_ctx* ctx, int lev, mx_multifab* scal_force,
 int mx_adv_convert_rhox_to_x(mx_adv_ctx* ctx, int lev, mx_multifab* scal, int flag);
 int mx_adv_convert_rhoh_to_h(mx_adv_ctx* ctx, int lev, mx_multifab* scal, int flag);
 
+/* ---- Ghost fill ---------------------------------------------------------- */
+/* Maestro::FillPatch(lev, time, mf, mf_old, mf_new, srccomp, destcomp, ncomp,
+   startbccomp, bcs_in, variable_type) at level 0 (MaestroFillData.cpp:24-47):
+   amrex::FillPatchSingleLevel with ScalarFill / VelFill and FillExtBC
+   (MaestroBCFill.cpp:8-171).  `src` is the MultiFab GetData picks for `time`
+   (mf_new or mf_old; it may be `mf` itself); `bcs` points at the ncomp
+   records bcs_in[startbccomp ..]; variable_type 1 is velocity.
+     1. every cell of mf's FABs (valid and ghost) that lies in a valid box of
+        src, or in a periodic image of one, takes src's value (srccomp.. ->
+        destcomp..);
+     2. cells beyond a domain side take fab_filcc's rule for their
+        component's BCRec (foextrap, hoextrap, reflect_even, reflect_odd;
+        int_dir and ext_dir leave them), direction x, then y, then z, for
+        every FAB whose grown box leaves the domain grown in the periodic
+        directions;
+     3. beyond an ext_dir side Pi and velocity become 0; any other scalar
+        with an ext_dir side returns MX_ERR_BC (the reference aborts) before
+        anything is written.
+   Every FAB of mf is its valid box grown by one ghost width.  Level > 0
+   (FillPatchTwoLevels, coarse-fine interpolation) returns
+   MX_ERR_UNSUPPORTED.  There is no time argument: a time strictly between
+   t_old and t_new, for which GetData hands over both MultiFabs, has no
+   counterpart here. */
+int mx_adv_fill_patch(mx_adv_ctx* ctx, int lev, mx_multifab* mf, const mx_multifab* src,
+                      int srccomp, int destcomp, int ncomp, const mx_bcrec* bcs,
+                      int variable_type);
+
 /* ---- AMR restriction ---------------------------------------------------- */
 /* amrex::average_down(fine, crse, ..., comp, ncomp, ratio) between levels
    lev + 1 (fine) and lev (crse), the step of Maestro::AverageDown

@@ -221,6 +221,7 @@ SIGNATURES = {
     "modify_scal_force": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc),
                           P(BaseStateDesc), P(MultiFabDesc), P(MultiFabDesc), P(BaseStateDesc), _i,
                           _i, P(SphGeom)],
+    "fill_patch": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i, _i, P(BCRec), _i],
     "average_down": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i],
     "average_down_faces": [_vp, _i, P(FaceMultiFabDesc), P(FaceMultiFabDesc)],
     "convert_rhox_to_x": [_vp, _i, P(MultiFabDesc), _i],

@@ -168,6 +168,22 @@ class Maestro:
             self.ctx, lev, *[C.byref(x) for x in a], comp, int(fullform),
             C.byref(sg) if sg is not None else None), "ModifyScalForce")
 
+    def FillPatch(self, lev, mf, src, srccomp, destcomp, ncomp, startbccomp, bcs_in,
+                  variable_type):
+        """Maestro::FillPatch at level 0 (MaestroFillData.cpp:24-47): src is the
+        MultiFab GetData picks for the fill time (mf_new or mf_old, possibly
+        mf itself); bcs_in a BCRec array (bcs_s or bcs_u) or a list of
+        (lo, hi) tuples."""
+        if not isinstance(bcs_in, C.Array):
+            bcs_in = bcrec_array(bcs_in)
+        recs = bcrec_array(bcrec_to_list(bcs_in)[startbccomp:startbccomp + ncomp])
+        if len(recs) != ncomp:
+            raise _abi.MxError(_abi.MX_ERR_ARG, "FillPatch: bcs_in shorter than startbccomp + ncomp")
+        a = [_mf(mf), _mf(src)]
+        self._check(self.fns["fill_patch"](self.ctx, lev, C.byref(a[0]), C.byref(a[1]), srccomp,
+                                           destcomp, ncomp, recs, int(variable_type)),
+                    "FillPatch")
+
     def average_down(self, lev, fine, crse, comp, ncomp):
         """amrex::average_down between levels lev + 1 and lev (one step of
         Maestro::AverageDown)."""

@@ -78,6 +78,9 @@ int ConvertByRho(const Ctx& c, int lev, const std::vector<Arr>& scal, int comp0,
                  bool flag);
 int AverageDown(const Ctx& c, int lev, const std::vector<Arr>& fine,
                 const std::vector<Arr>& crse, int comp, int ncomp);
+int FillPatchSingleLevel(const Ctx& c, int lev, const std::vector<Arr>& mf,
+                         const std::vector<Arr>& src, int scomp, int dcomp, int ncomp,
+                         const mx_bcrec* bcs, bool is_vel);
 int AverageDownFaces(const Ctx& c, int lev, const std::vector<Arr> fine[3],
                      const std::vector<Arr> crse[3]);
 int MakeVelForce(const Ctx& c, int lev, const std::vector<Arr>& vel_force,
@@ -607,6 +610,33 @@ int mxo_average_down_faces(Ctx* c, int lev, const mx_face_multifab* fine,
         if (f.d[d].size() != c->grids[lev + 1].size() || cr.d[d].size() != c->grids[lev].size())
             return MX_ERR_ARG;
     return finish(AverageDownFaces(*c, lev, f.d, cr.d));
+}
+
+int mxo_fill_patch(Ctx* c, int lev, mx_multifab* mf, const mx_multifab* src, int srccomp,
+                   int destcomp, int ncomp, const mx_bcrec* bcs, int variable_type) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (lev != 0) return MX_ERR_UNSUPPORTED;  // FillPatchTwoLevels is not restated
+    const size_t nb = c->grids[lev].size();
+    if (!mf || !src || !bcs || mf->nfab != (int)nb || src->nfab != (int)nb || srccomp < 0 ||
+        destcomp < 0 || ncomp < 1)
+        return MX_ERR_ARG;
+    std::vector<Arr> m = arrs(mf), s = arrs(src);
+    for (size_t b = 0; b < nb; ++b) {
+        const IBox& g = c->grids[lev][b];
+        if (!m[b].valid() || !s[b].valid() || m[b].nc < destcomp + ncomp ||
+            s[b].nc < srccomp + ncomp)
+            return MX_ERR_ARG;
+        for (int d = 0; d < 3; ++d) {
+            const int ng = g.lo[d] - m[b].b[d];
+            if (ng < 0 || m[b].e[d] - 1 - g.hi[d] != ng || s[b].b[d] > g.lo[d] ||
+                s[b].e[d] - 1 < g.hi[d])
+                return MX_ERR_BOX;
+            if (b > 0 && ng != c->grids[lev][0].lo[d] - m[0].b[d]) return MX_ERR_BOX;
+        }
+    }
+    return finish(FillPatchSingleLevel(*c, lev, m, s, srccomp, destcomp, ncomp, bcs,
+                                       variable_type == 1));
 }
 
 int mxo_convert_rhox_to_x(Ctx* c, int lev, mx_multifab* scal, int flag) {

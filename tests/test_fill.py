@@ -1,0 +1,164 @@
+"""The oracle's level-0 FillPatch (oracle/oracle_fill.cpp) against closed
+forms: periodic images of a field of the global index; first-order
+extrapolation; the second-order face value of hoextrap, exact for a
+quadratic; even and odd reflection; zero beyond ext_dir for Pi and velocity
+and the scalar abort; corners between two bounded directions; the copy from
+another MultiFab with component offsets."""
+import numpy as np
+import pytest
+
+import pyoracle
+from fill_cases import (EXT_DIR, FOEXTRAP, HOEXTRAP, INT_DIR, REFLECT_EVEN, REFLECT_ODD,
+                        SENTINEL, bcs, layout, new_mf, random_field)
+from maestroex_amd import _abi
+from maestroex_amd.fabs import Backend
+
+NP = Backend("numpy")
+
+
+def cells(m, b):
+    a = m.arrays[b]
+    beg = m.begin[b]
+    _, nz, ny, nx = a.shape
+    for idx in np.ndindex(nz, ny, nx):
+        k, j, i = idx
+        yield idx, (i + beg[0], j + beg[1], k + beg[2])
+
+
+def field(I, J, K, c):
+    return 1.0 * I + 16.0 * J + 256.0 * K + 4096.0 * c
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_periodic_images(dim):
+    params, boxes = layout(dim, periodic=(1, 1, 1))
+    m = pyoracle.oracle_maestro(params, [boxes])
+    mf = new_mf(boxes, 3, 2, dim, NP, field)
+    m.FillPatch(0, mf, mf, 0, 0, 3, 0, bcs(3, (INT_DIR,) * 3, (INT_DIR,) * 3), 0)
+    for b in range(len(boxes)):
+        for idx, p in cells(mf, b):
+            q = [p[d] % 8 if d < dim else 0 for d in range(3)]
+            for c in range(3):
+                assert mf.arrays[b][(c,) + idx] == field(*q, c), (b, p, c)
+
+
+def test_foextrap_z_periodic_xy():
+    params, boxes = layout(3, periodic=(1, 1, 0))
+    m = pyoracle.oracle_maestro(params, [boxes])
+    mf = new_mf(boxes, 2, 3, 3, NP, field)
+    rec = bcs(2, (INT_DIR, INT_DIR, FOEXTRAP), (INT_DIR, INT_DIR, FOEXTRAP))
+    m.FillPatch(0, mf, mf, 0, 0, 2, 0, rec, 0)
+    for b in range(len(boxes)):
+        for idx, p in cells(mf, b):
+            q = (p[0] % 8, p[1] % 8, min(max(p[2], 0), 7))
+            for c in range(2):
+                assert mf.arrays[b][(c,) + idx] == field(*q, c), (b, p, c)
+
+
+def test_hoextrap_quadratic():
+    """q = 3 + 2 z + z^2 at cell centres z = k + 1/2: the first ghost holds
+    q at the domain face, the others the edge cell's value"""
+    params, boxes = layout(3, periodic=(1, 1, 0))
+    m = pyoracle.oracle_maestro(params, [boxes])
+    quad = lambda z: 3.0 + 2.0 * z + z * z
+    mf = new_mf(boxes, 1, 3, 3, NP, lambda I, J, K, c: quad(K + 0.5))
+    rec = bcs(1, (INT_DIR, INT_DIR, HOEXTRAP), (INT_DIR, INT_DIR, HOEXTRAP))
+    m.FillPatch(0, mf, mf, 0, 0, 1, 0, rec, 0)
+    for b in range(len(boxes)):
+        for idx, p in cells(mf, b):
+            k = p[2]
+            want = {-1: quad(0.0), 8: quad(8.0)}.get(k)
+            if want is None:
+                want = quad(min(max(k, 0), 7) + 0.5)
+            assert mf.arrays[b][(0,) + idx] == want, (b, p)
+
+
+def test_reflect_even_odd():
+    """x bounded with reflection (component 0 even, 1 odd), y periodic, 2-D"""
+    params, boxes = layout(2, periodic=(0, 1))
+    m = pyoracle.oracle_maestro(params, [boxes])
+    mf = new_mf(boxes, 2, 3, 2, NP, random_field(5))
+    rec = [((REFLECT_EVEN, INT_DIR, 0), (REFLECT_EVEN, INT_DIR, 0)),
+           ((REFLECT_ODD, INT_DIR, 0), (REFLECT_ODD, INT_DIR, 0))]
+    m.FillPatch(0, mf, mf, 0, 0, 2, 0, rec, 0)
+    ref = random_field(5)
+    for b in range(len(boxes)):
+        for idx, p in cells(mf, b):
+            i = p[0]
+            src = -1 - i if i < 0 else (15 - i if i > 7 else i)
+            sgn = [1.0, -1.0 if src != i else 1.0]
+            for c in range(2):
+                want = sgn[c] * ref(np.array(src), np.array(p[1] % 8), np.array(0), c)
+                assert mf.arrays[b][(c,) + idx] == want, (b, p, c)
+
+
+def test_ext_dir_zero_and_scalar_abort():
+    """velocity and Pi are zero beyond an ext_dir side; another scalar with
+    ext_dir is an error that writes nothing"""
+    params, boxes = layout(3, periodic=(1, 1, 0))
+    m = pyoracle.oracle_maestro(params, [boxes])
+    rec = bcs(3, (INT_DIR, INT_DIR, EXT_DIR), (INT_DIR, INT_DIR, FOEXTRAP))
+    u = new_mf(boxes, 3, 2, 3, NP, field)
+    m.FillPatch(0, u, u, 0, 0, 3, 0, rec, 1)
+    pi = params.comp_pi
+    s = new_mf(boxes, params.nscal, 2, 3, NP, field)
+    m.FillPatch(0, s, s, pi, pi, 1, 0, rec, 0)
+    for b in range(len(boxes)):
+        for idx, p in cells(u, b):
+            q = (p[0] % 8, p[1] % 8, min(max(p[2], 0), 7))
+            for c in range(3):
+                want = 0.0 if p[2] < 0 else field(*q, c)
+                assert u.arrays[b][(c,) + idx] == want, (b, p, c)
+            want = 0.0 if p[2] < 0 else field(*q, pi)
+            assert s.arrays[b][(pi,) + idx] == want, (b, p)
+    t = new_mf(boxes, params.nscal, 2, 3, NP, field)
+    before = [a.copy() for a in t.arrays]
+    with pytest.raises(_abi.MxError) as e:
+        m.FillPatch(0, t, t, 0, 0, 1, 0, rec, 0)
+    assert e.value.code == _abi.MX_ERR_BC
+    for a, b0 in zip(t.arrays, before):
+        assert np.array_equal(a, b0)
+
+
+def test_corner_two_bounded_directions():
+    """2-D, x and y bounded with foextrap: a corner ghost takes the corner
+    cell's value"""
+    params, boxes = layout(2, periodic=(0, 0))
+    m = pyoracle.oracle_maestro(params, [boxes])
+    mf = new_mf(boxes, 1, 2, 2, NP, field)
+    rec = bcs(1, (FOEXTRAP, FOEXTRAP, 0), (FOEXTRAP, FOEXTRAP, 0))
+    m.FillPatch(0, mf, mf, 0, 0, 1, 0, rec, 0)
+    for b in range(len(boxes)):
+        for idx, p in cells(mf, b):
+            q = (min(max(p[0], 0), 7), min(max(p[1], 0), 7), 0)
+            assert mf.arrays[b][(0,) + idx] == field(*q, 0), (b, p)
+
+
+def test_copy_from_other_multifab():
+    """src != mf with component offsets: valid and ghost cells of the
+    destination components come from src; other components keep theirs"""
+    params, boxes = layout(3, periodic=(1, 1, 1))
+    m = pyoracle.oracle_maestro(params, [boxes])
+    src = new_mf(boxes, 4, 0, 3, NP, field)
+    mf = new_mf(boxes, 3, 1, 3, NP, lambda I, J, K, c: -1.0 - c, sentinel=-5.0)
+    m.FillPatch(0, mf, src, 2, 1, 2, 0, bcs(2, (INT_DIR,) * 3, (INT_DIR,) * 3), 0)
+    for b in range(len(boxes)):
+        for idx, p in cells(mf, b):
+            q = (p[0] % 8, p[1] % 8, p[2] % 8)
+            lo, hi = boxes[b]
+            valid = all(lo[d] <= p[d] <= hi[d] for d in range(3))
+            assert mf.arrays[b][(0,) + idx] == (-1.0 if valid else -5.0)
+            assert mf.arrays[b][(1,) + idx] == field(*q, 2)
+            assert mf.arrays[b][(2,) + idx] == field(*q, 3)
+
+
+def test_errors():
+    params, boxes = layout(3, periodic=(1, 1, 0))
+    m = pyoracle.oracle_maestro(params, [boxes])
+    mf = new_mf(boxes, 1, 2, 3, NP, field)
+    with pytest.raises(_abi.MxError) as e:  # an unknown BCType
+        m.FillPatch(0, mf, mf, 0, 0, 1, 0, bcs(1, (0, 0, 7), (0, 0, 2)), 0)
+    assert e.value.code == _abi.MX_ERR_BC
+    with pytest.raises(_abi.MxError) as e:  # more components than the FABs hold
+        m.FillPatch(0, mf, mf, 0, 0, 2, 0, bcs(2, (0, 0, 2), (0, 0, 2)), 0)
+    assert e.value.code == _abi.MX_ERR_ARG
