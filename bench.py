@@ -100,6 +100,8 @@ def parse():
                     help="the base-state <-> Cartesian producers on the 3-D level")
     ap.add_argument("--update", action="store_true",
                     help="UpdateScal + UpdateVel (the update that closes the step) on the 3-D level")
+    ap.add_argument("--plotfile", action="store_true",
+                    help="write the 3-D level as an AMReX plotfile")
     ap.add_argument("--fill", action="store_true",
                     help="FillPatch (level 0) of the scalars on the 3-D level")
     ap.add_argument("--spherical", action="store_true",
@@ -610,6 +612,101 @@ def fill_main(args):
     print(json.dumps(out), flush=True)
 
 
+def plotfile_main(args):
+    """mx_adv_write_plotfile of 7 components of the 3-D 256^3 level (8 boxes
+    of 128^3, 4 ghost cells, valid cells written): device-to-host copies and
+    the file writes, into a scratch directory removed after every write."""
+    import shutil
+    import tempfile
+    import torch
+    import maestroex_amd as mx
+    from maestroex_amd.fabs import Geometry, MultiFab, make_params, split_domain
+
+    dev = mx.device_backend("cuda:0")
+    n = args.n
+    geom = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 1], [0] * 3, [0] * 3)
+    params = make_params(geom, nspec=3)
+    boxes = split_domain(geom, args.max_grid)
+    nc = params.nscal
+    m = mx.gpu_maestro(params, [boxes])
+    mf = MultiFab(boxes, nc, 4, 3, dev)
+    gen = torch.Generator(device="cuda:0").manual_seed(11)
+    for a in mf.arrays:
+        a.uniform_(-1.0, 1.0, generator=gen)
+    names = [f"var{c}" for c in range(nc)]
+    root = tempfile.mkdtemp(prefix="mx_plt_", dir=os.environ.get("TMPDIR", "/tmp"))
+    reps = max(1, min(args.steps, 3))
+    times = []
+    try:
+        for r in range(reps + 1):
+            path = os.path.join(root, f"plt{r:05d}")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            m.write_plotfile(path, [mf], names, 0.0, [r], [0.0] * 3, [1.0] * 3)
+            t = time.perf_counter() - t0
+            size = sum(os.path.getsize(os.path.join(d, f)) for d, _, fs in os.walk(path)
+                       for f in fs)
+            shutil.rmtree(path)
+            if r > 0:  # the first write is the warm-up
+                times.append(t)
+            print(f"[bench] plotfile write {r}: {t * 1e3:.1f} ms, {size / 1e6:.1f} MB",
+                  file=sys.stderr, flush=True)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    t = min(times)
+    out = {
+        "metric": METRIC,
+        "value": round(n ** 3 / t / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "warmup": 1,
+        "ms_per_step": round(t * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"AMReX plotfile of {nc} components of 3-D {n}^3 FP64 as "
+                               f"{len(boxes)} boxes of {args.max_grid}^3 (device to host, then "
+                               f"the files)", "global_batch": n ** 3,
+                   "parallelism": "single GPU"},
+        "plotfile": {"bytes": size, "write_gbs": round(size / t / 1e9, 3)},
+        "roofline": None,
+        "cpu_baseline": None if args.no_cpu else plotfile_cpu_baseline(args, nc),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def plotfile_cpu_baseline(args, nc):
+    """The plotfile restatement (numpy) writing the same components of one
+    128^3 box from host memory."""
+    import shutil
+    import tempfile
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from plotfile_oracle import write_plotfile
+    n = args.cpu_n
+    a = np.random.default_rng(11).uniform(-1.0, 1.0, (nc, n, n, n))
+    root = tempfile.mkdtemp(prefix="mx_plt_cpu_", dir=os.environ.get("TMPDIR", "/tmp"))
+    times = []
+    try:
+        for r in range(args.cpu_reps):
+            path = os.path.join(root, f"plt{r}")
+            t0 = time.perf_counter()
+            write_plotfile(path, 3, [f"var{c}" for c in range(nc)], 0.0, [0.0] * 3, [1.0] * 3,
+                           [((0, 0, 0), (n - 1,) * 3)], [[1.0 / n] * 3], [0],
+                           [[((0, 0, 0), (n - 1,) * 3)]], [[a]])
+            times.append(time.perf_counter() - t0)
+            shutil.rmtree(path)
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    return {"value": round(n ** 3 / min(times) / 1e9, 5), "unit": "Gcell-updates/s",
+            "cores": 1, "kind": "port",
+            "sample": f"the plotfile restatement writing {nc} components of one {n}^3 box from "
+                      f"host memory, best of {args.cpu_reps}"}
+
+
 def fill_cpu_baseline(args, ng):
     """The oracle's FillPatch of the same 7 scalars on one 128^3 box (periodic
     x/y, outflow z), timed on the host cores."""
@@ -837,6 +934,8 @@ def main():
         return base_main(args)
     if args.fill:
         return fill_main(args)
+    if args.plotfile:
+        return plotfile_main(args)
     import torch
 
     import maestroex_amd as mx

@@ -49,7 +49,8 @@ enum {
     MX_ERR_HIP = -4,         /* HIP runtime failure */
     MX_ERR_UNSUPPORTED = -5, /* configuration the path does not implement */
     MX_ERR_NOMEM = -6,       /* workspace allocation failed */
-    MX_ERR_NOLIB = -7        /* no usable GPU */
+    MX_ERR_NOLIB = -7,       /* no usable GPU */
+    MX_ERR_IO = -8           /* a file could not be created or written */
 };
 
 /* amrex::BCType values (mathematical boundary types, per component) */
@@ -514,6 +515,21 @@ int mx_adv_convert_rhoh_to_h(mx_adv_ctx* ctx, int lev, mx_multifab* scal, int fl
 int mx_adv_fill_patch(mx_adv_ctx* ctx, int lev, mx_multifab* mf, const mx_multifab* src,
                       int srccomp, int destcomp, int ncomp, const mx_bcrec* bcs,
                       int variable_type);
+
+/* ---- Plotfile -------------------------------------------------------------- */
+/* amrex::WriteMultiLevelPlotfile(plotfilename, nlevels, mf, varnames, geom,
+   time, level_steps, ref_ratio) as Maestro::WritePlotFile calls it
+   (MaestroPlot.cpp:99-116): the AMReX plotfile directory `dir` (format
+   HyperCLaw-V1.1; Header, Level_<l>/Cell_H, Level_<l>/Cell_D_00000) holding
+   components 0..ncomp-1 of the valid cells of mf[0..nlev-1].  Geometry and
+   boxes are the context's; prob_lo/prob_hi are geom[0].ProbLo()/ProbHi().
+   Cartesian coordinates, one data file per level, little-endian doubles.
+   Returns MX_ERR_IO when a file cannot be written.  Host code: it waits for
+   the context stream. */
+int mx_adv_write_plotfile(mx_adv_ctx* ctx, const char* dir, int nlev,
+                          const mx_multifab* const* mf, int ncomp, const char* const* varnames,
+                          double time, const int* level_steps, const double* prob_lo,
+                          const double* prob_hi);
 
 /* ---- AMR restriction ---------------------------------------------------- */
 /* amrex::average_down(fine, crse, ..., comp, ncomp, ratio) between levels

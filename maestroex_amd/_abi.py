@@ -17,6 +17,7 @@ MX_ERR_HIP = -4
 MX_ERR_UNSUPPORTED = -5
 MX_ERR_NOMEM = -6
 MX_ERR_NOLIB = -7
+MX_ERR_IO = -8
 
 # amrex::BCType
 BC_REFLECT_ODD, BC_INT_DIR, BC_REFLECT_EVEN, BC_FOEXTRAP, BC_EXT_DIR, BC_HOEXTRAP = -1, 0, 1, 2, 3, 4
@@ -31,6 +32,7 @@ ERRORS = {
     MX_ERR_UNSUPPORTED: "unsupported configuration",
     MX_ERR_NOMEM: "workspace allocation failed",
     MX_ERR_NOLIB: "no usable GPU",
+    MX_ERR_IO: "file could not be created or written",
 }
 
 
@@ -221,6 +223,8 @@ SIGNATURES = {
     "modify_scal_force": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc),
                           P(BaseStateDesc), P(MultiFabDesc), P(MultiFabDesc), P(BaseStateDesc), _i,
                           _i, P(SphGeom)],
+    "write_plotfile": [_vp, C.c_char_p, _i, P(P(MultiFabDesc)), _i, P(C.c_char_p), _d, P(_i),
+                       P(_d), P(_d)],
     "fill_patch": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i, _i, P(BCRec), _i],
     "average_down": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i],
     "average_down_faces": [_vp, _i, P(FaceMultiFabDesc), P(FaceMultiFabDesc)],
@@ -257,6 +261,11 @@ LIB_ONLY = {
     "prof_read": [_vp, _i, P(C.c_char_p), P(C.c_double), P(C.c_longlong)],
     "set_tuning": [_vp, C.c_char_p, C.c_longlong],
 }
+
+
+# entry points with no counterpart in the C oracle library (the plotfile
+# writer is checked against a file-level restatement instead)
+NO_ORACLE = {"write_plotfile"}
 
 
 def bind(lib, prefix, names=None, extra=None):

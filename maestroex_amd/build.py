@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = ["mx_ctx.hip", "mx_velocity.hip", "mx_scalar.hip", "mx_edge_fused.hip", "mx_edge1d.hip",
            "mx_graph.hip", "mx_adjoint.hip", "mx_update.hip", "mx_base.hip", "mx_sync.hip",
-           "mx_fill.hip"]
+           "mx_fill.hip", "mx_plotfile.hip"]
 LIB = os.path.join(HERE, "lib", "libmaestroex_amd.so")
 ARCH = os.environ.get("MX_OFFLOAD_ARCH", "gfx950")
 

@@ -930,6 +930,7 @@ const char* mx_adv_error_string(int code) {
         case MX_ERR_UNSUPPORTED: return "unsupported configuration";
         case MX_ERR_NOMEM: return "workspace allocation failed";
         case MX_ERR_NOLIB: return "no usable GPU";
+        case MX_ERR_IO: return "file could not be created or written";
         default: return "unknown error";
     }
 }

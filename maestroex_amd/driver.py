@@ -168,6 +168,21 @@ class Maestro:
             self.ctx, lev, *[C.byref(x) for x in a], comp, int(fullform),
             C.byref(sg) if sg is not None else None), "ModifyScalForce")
 
+    def write_plotfile(self, path, mfs, varnames, time, level_steps, prob_lo, prob_hi):
+        """amrex::WriteMultiLevelPlotfile as Maestro::WritePlotFile calls it
+        (MaestroPlot.cpp:99-116): components 0..len(varnames)-1 of the valid
+        cells of mfs[lev], lev = 0..len(mfs)-1."""
+        nlev = len(mfs)
+        descs = [_mf(m) for m in mfs]
+        arr = (C.POINTER(_abi.MultiFabDesc) * nlev)(*[C.pointer(d) for d in descs])
+        names = (C.c_char_p * len(varnames))(*[v.encode() for v in varnames])
+        steps = (C.c_int * nlev)(*[int(s) for s in level_steps])
+        lo = (C.c_double * 3)(*(list(prob_lo) + [0.0] * (3 - len(prob_lo))))
+        hi = (C.c_double * 3)(*(list(prob_hi) + [0.0] * (3 - len(prob_hi))))
+        self._check(self.fns["write_plotfile"](self.ctx, str(path).encode(), nlev, arr,
+                                               len(varnames), names, float(time), steps, lo, hi),
+                    "write_plotfile")
+
     def FillPatch(self, lev, mf, src, srccomp, destcomp, ncomp, startbccomp, bcs_in,
                   variable_type):
         """Maestro::FillPatch at level 0 (MaestroFillData.cpp:24-47): src is the

@@ -26,7 +26,7 @@ def load(fast=False):
         if not os.path.exists(path):
             raise FileNotFoundError(f"oracle library not built: {path} (run make -C oracle)")
         lib = C.CDLL(path)
-        fns = _abi.bind(lib, "mxo_")
+        fns = _abi.bind(lib, "mxo_", names=[n for n in _abi.SIGNATURES if n not in _abi.NO_ORACLE])
         lib.mxo_omp_threads.argtypes = [C.c_int]
         lib.mxo_omp_threads.restype = C.c_int
         fns["omp_threads"] = lib.mxo_omp_threads
