@@ -192,6 +192,9 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
      "edge_scal_tile"   cell columns per block of the fused kernel, and the
                         block size: 0: 32 x 8 (384 threads), 1: 16 x 16 (384),
                         2 (default): 16 x 12 (256), 3: 16 x 8 (192).
+     "xcd_remap"        1: the staged VelPred, MakeUtrans and flux kernels
+                        give each of the 8 XCDs one contiguous eighth of a
+                        box's blocks; 0 (default): blocks in launch order.
    Unknown keys and out-of-range values return MX_ERR_ARG. */
 int mx_adv_set_tuning(mx_adv_ctx* ctx, const char* key, long long value);
 

@@ -27,6 +27,7 @@ KGeom kgeom(const mx_adv_ctx* c, int lev) {
         g.phys_lo[d] = c->p.phys_bc_lo[d];
         g.phys_hi[d] = c->p.phys_bc_hi[d];
     }
+    g.xcd = c->xcd_remap;
     return g;
 }
 
@@ -811,6 +812,9 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
     } else if (k == "edge_scal_per_box") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->fused_per_box = (int)value;
+    } else if (k == "xcd_remap") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->xcd_remap = (int)value;
     } else if (k == "adjoint_state_fused") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->adj_state_fused = (int)value;

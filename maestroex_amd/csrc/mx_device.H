@@ -435,7 +435,19 @@ struct KGeom {
     int domhi[3];
     int phys_lo[3];
     int phys_hi[3];
+    int xcd;  // 1: grids from grid_for_g (x a multiple of 8), indices from grid_point
 };
+
+// The linear point index of this thread in a grid_for_g launch.  Workgroups
+// are dispatched to the 8 XCDs round robin by linear id; with the grid's x
+// extent a multiple of 8 the XCD is blockIdx.x % 8, and the remap gives each
+// XCD one contiguous eighth of a box (its stencil planes stay in that XCD's
+// L2) instead of every eighth block.
+__device__ __forceinline__ long long grid_point(const KGeom& g) {
+    unsigned b = blockIdx.x;
+    if (g.xcd) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+    return (long long)b * blockDim.x + threadIdx.x;
+}
 
 // region iteration: linear index -> (i, j, k) in box [lo, hi]
 __device__ __forceinline__ bool region_point(long long t, const int* lo, const int* hi, int& i,

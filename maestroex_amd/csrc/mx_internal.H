@@ -87,6 +87,7 @@ struct mx_adv_ctx {
     // kernel selection (mx_adv_set_tuning)
     int fused_edge = 1;
     int fused_tz = 32;
+    int xcd_remap = 0;  // KGeom::xcd of the staged pass kernels
     int fused_tile = 2;     // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8
     int fused_per_box = 0;  // 1: one fused launch per box (comparison mode)
     int adj_state_fused = 1;  // adjoint: face, slope and state adjoints in one gather
@@ -267,6 +268,15 @@ inline dim3 grid_for(long long maxpts, int nbox, int nz, int block) {
     long long gx = (maxpts + block - 1) / block;
     if (gx < 1) gx = 1;
     return dim3((unsigned)gx, (unsigned)nbox, (unsigned)nz);
+}
+
+// grid_for for kernels that take their point index from grid_point: with
+// the XCD remap on, x is rounded up to a multiple of 8 (the extra blocks'
+// points lie past the region and return)
+inline dim3 grid_for_g(const mxd::KGeom& g, long long maxpts, int nbox, int nz, int block) {
+    dim3 d = grid_for(maxpts, nbox, nz, block);
+    if (g.xcd) d.x = (d.x + 7u) & ~7u;
+    return d;
 }
 
 // stages shared between entry points

@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(256) k_rhox_flux(const FluxBox* __restrict__ b
     int lo[3] = {B.lo[0], B.lo[1], B.lo[2]};
     int hi[3] = {B.hi[0], B.hi[1], B.hi[2]};
     hi[d] += 1;
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long idx = grid_point(g);
     int i, j, k;
     if (!region_point(idx, lo, hi, i, j, k)) return;
     if (d == 0)
@@ -552,7 +552,7 @@ __global__ void __launch_bounds__(256) k_rhoh_flux(const FluxBox* __restrict__ b
     int lo[3] = {B.lo[0], B.lo[1], B.lo[2]};
     int hi[3] = {B.hi[0], B.hi[1], B.hi[2]};
     hi[d] += 1;
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long idx = grid_point(g);
     int i, j, k;
     if (!region_point(idx, lo, hi, i, j, k)) return;
     if (d == 0)
@@ -833,7 +833,9 @@ int make_rhox_flux(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux,
         MX_LAUNCH(c, "k_zero_fab", k_zero_fab,  grid_for(zmax, nb, 1, block), dim3(block), tab.dev<FluxBox>(oz), nb);
 
     }
-    MX_LAUNCH(c, "k_rhox_flux", k_rhox_flux,  grid_for(maxpts, nb, c->p.dim, block), dim3(block), tab.dev<FluxBox>(ob), kgeom(c, lev), A);
+    const KGeom kg = kgeom(c, lev);
+    MX_LAUNCH(c, "k_rhox_flux", k_rhox_flux, grid_for_g(kg, maxpts, nb, c->p.dim, block),
+              dim3(block), tab.dev<FluxBox>(ob), kg, A);
 
     return MX_OK;
 }
@@ -865,7 +867,10 @@ int make_rhoh_flux(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux,
     int rc = tab.upload();
     if (rc) return rc;
     const int block = 256;
-    MX_LAUNCH(c, "k_rhoh_flux", k_rhoh_flux,  grid_for(maxpts, (int)boxes.size(), c->p.dim, block), dim3(block), tab.dev<FluxBox>(ob), kgeom(c, lev), A);
+    const KGeom kg = kgeom(c, lev);
+    MX_LAUNCH(c, "k_rhoh_flux", k_rhoh_flux,
+              grid_for_g(kg, maxpts, (int)boxes.size(), c->p.dim, block), dim3(block),
+              tab.dev<FluxBox>(ob), kg, A);
 
     return MX_OK;
 }

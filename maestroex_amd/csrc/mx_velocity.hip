@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(256) k_vp_ppm(const VpBox* __restrict__ boxes,
         lo[d] = B.lo[d] - (d < g.dim ? 1 : 0);
         hi[d] = B.hi[d] + (d < g.dim ? 1 : 0);
     }
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long idx = grid_point(g);
     int i, j, k;
     if (!region_point(idx, lo, hi, i, j, k)) return;
     // every direction is computed before any store (loads and stores share
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(256) k_vp_trans(const VpBox* __restrict__ boxe
     lo[O] -= 1;
     hi[O] += 1;
     hi[D] += 1;
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long idx = grid_point(g);
     int i, j, k;
     if (!region_point(idx, lo, hi, i, j, k)) return;
     switch (p) {
@@ -525,7 +525,7 @@ __global__ void __launch_bounds__(256) k_vp_vel(const VpBox* __restrict__ boxes,
     int lo[3] = {B.lo[0], B.lo[1], B.lo[2]};
     int hi[3] = {B.hi[0], B.hi[1], B.hi[2]};
     hi[d] += 1;
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long idx = grid_point(g);
     int i, j, k;
     if (!region_point(idx, lo, hi, i, j, k)) return;
     if (d == 0)
@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(256) k_utrans_pp(const VpBox* __restrict__ box
     int lo[3] = {B.lo[0], B.lo[1], B.lo[2]};
     int hi[3] = {B.hi[0], B.hi[1], B.hi[2]};
     hi[d] += 1;
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long idx = grid_point(g);
     int i, j, k;
     if (!region_point(idx, lo, hi, i, j, k)) return;
     if (d == 0)
@@ -716,7 +716,7 @@ int velpred_setup(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mu
     const int block = 256;
     const int nb = (int)boxes.size();
 
-    dim3 gr = grid_for(maxpts, nb, dim, block);
+    dim3 gr = grid_for_g(kg, maxpts, nb, dim, block);
     if (ppm == 0)
         MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm<0>,  gr, dim3(block), dp, kg, A);
     else if (ppm == 1)
@@ -735,7 +735,7 @@ int velpred_setup(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mu
 int utrans_from_plan(mx_adv_ctx* c, const VpPlan& plan) {
     const KGeom kg = kgeom(c, plan.lev);
     const int block = 256;
-    const dim3 gr = grid_for(plan.maxpts, plan.nb, c->p.dim, block);
+    const dim3 gr = grid_for_g(kg, plan.maxpts, plan.nb, c->p.dim, block);
     if (c->p.ppm_type == 0)
         MX_LAUNCH(c, "k_utrans", k_utrans_pp<0>, gr, dim3(block), (const VpBox*)plan.dp, kg);
     else
@@ -748,9 +748,9 @@ int velpred_finish(mx_adv_ctx* c, const VpPlan& plan) {
     const KGeom kg = kgeom(c, plan.lev);
     const int block = 256;
     if (c->p.dim == 3)
-        MX_LAUNCH(c, "k_vp_trans", k_vp_trans, grid_for(plan.maxpts, plan.nb, 6, block),
+        MX_LAUNCH(c, "k_vp_trans", k_vp_trans, grid_for_g(kg, plan.maxpts, plan.nb, 6, block),
                   dim3(block), (const VpBox*)plan.dp, kg);
-    MX_LAUNCH(c, "k_vp_vel", k_vp_vel, grid_for(plan.maxpts, plan.nb, c->p.dim, block),
+    MX_LAUNCH(c, "k_vp_vel", k_vp_vel, grid_for_g(kg, plan.maxpts, plan.nb, c->p.dim, block),
               dim3(block), (const VpBox*)plan.dp, kg);
     // AverageDownFaces(umac) is a no-op on a single level
     return MX_OK;

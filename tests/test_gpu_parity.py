@@ -102,6 +102,28 @@ def test_pass_kernel_choices(name, cfg, fused, tz):
     assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
 
 
+XCD_CASES = [CASES[0], CASES[3], CASES[7], CASES[10], CASES[16], CASES[17]]
+
+
+@pytest.mark.parametrize("name,cfg", XCD_CASES, ids=[c[0] for c in XCD_CASES])
+def test_pass_xcd_remap(name, cfg):
+    """The XCD-aware block order of the staged kernels (VelPred, MakeUtrans
+    from its traced states, the fluxes) computes every point once: the whole
+    pass bit-exact, box grids that are not multiples of 8 blocks included."""
+    print(f"[parity] xcd remap {name}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    m.set_tuning("xcd_remap", 1)
+    _run_pass(m, wg)
+    assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
+    assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} sedge")
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
+    assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} etarhoflux")
+
+
 @pytest.mark.parametrize("tile", [0, 1, 2, 3])
 @pytest.mark.parametrize("name,cfg", KERNEL_CHOICE_CASES, ids=[c[0] for c in KERNEL_CHOICE_CASES])
 def test_pass_fused_tile_shapes(name, cfg, tile):
