@@ -201,6 +201,11 @@ int mx_adv_fill_patch(mx_adv_ctx* c, int lev, mx_multifab* mf, const mx_multifab
     if (!mf || !src || !bcs || mf->nfab != nb || src->nfab != nb || !mf->fab || !src->fab ||
         srccomp < 0 || destcomp < 0 || ncomp < 1 || ncomp > 65535)
         return MX_ERR_ARG;
+    // a MultiFab copied onto itself with shifted, overlapping component
+    // ranges would read components the same launch writes
+    if (mf->fab == src->fab || (nb > 0 && mf->fab[0].p == src->fab[0].p))
+        if (srccomp != destcomp && srccomp < destcomp + ncomp && destcomp < srccomp + ncomp)
+            return MX_ERR_ARG;
     const bool is_vel = variable_type == 1;
     HBox dom;
     for (int d = 0; d < 3; ++d) {

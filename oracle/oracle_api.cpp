@@ -622,6 +622,9 @@ int mxo_fill_patch(Ctx* c, int lev, mx_multifab* mf, const mx_multifab* src, int
         destcomp < 0 || ncomp < 1)
         return MX_ERR_ARG;
     std::vector<Arr> m = arrs(mf), s = arrs(src);
+    if (nb > 0 && m[0].p == s[0].p && srccomp != destcomp && srccomp < destcomp + ncomp &&
+        destcomp < srccomp + ncomp)
+        return MX_ERR_ARG;  // overlapping component ranges of one MultiFab
     for (size_t b = 0; b < nb; ++b) {
         const IBox& g = c->grids[lev][b];
         if (!m[b].valid() || !s[b].valid() || m[b].nc < destcomp + ncomp ||

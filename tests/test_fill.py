@@ -162,3 +162,7 @@ def test_errors():
     with pytest.raises(_abi.MxError) as e:  # more components than the FABs hold
         m.FillPatch(0, mf, mf, 0, 0, 2, 0, bcs(2, (0, 0, 2), (0, 0, 2)), 0)
     assert e.value.code == _abi.MX_ERR_ARG
+    mf3 = new_mf(boxes, 3, 2, 3, NP, field)
+    with pytest.raises(_abi.MxError) as e:  # one MultiFab, overlapping component ranges
+        m.FillPatch(0, mf3, mf3, 0, 1, 2, 0, bcs(2, (0, 0, 2), (0, 0, 2)), 0)
+    assert e.value.code == _abi.MX_ERR_ARG

@@ -85,6 +85,9 @@ def test_fill_patch_errors():
     with pytest.raises(_abi.MxError) as e:  # an unknown BCType
         m.FillPatch(0, mf, mf, 0, 0, 1, 0, [((0, 0, 9), (0, 0, 2))], 0)
     assert e.value.code == _abi.MX_ERR_BC
+    with pytest.raises(_abi.MxError) as e:  # one MultiFab, overlapping component ranges
+        m.FillPatch(0, mf, mf, 0, 1, 2, 0, rec * 2, 1)
+    assert e.value.code == _abi.MX_ERR_ARG
     m.sync()
     for x, y in zip(before, mf.to_numpy()):
         assert np.array_equal(x, y)
