@@ -256,6 +256,21 @@ def amr_main(args):
         if nm.value.decode() == "k_average_down":
             sync_kernel_ms = kms.value
     m.fns["prof_enable"](m.ctx, 0)
+    # FillPatch of the level-0 scalars (150 boxes, 4 ghost cells), as after
+    # the update: host planning (box intersections) plus the kernels
+    s0 = fields[0]["scal"].desc()
+    bs, _ = m.bc_setup()
+
+    def fill0():
+        rc = m.fns["fill_patch"](m.ctx, 0, C.byref(s0), C.byref(s0), 0, 0, nsc, bs, 0)
+        if rc:
+            raise RuntimeError(f"fill_patch: error {rc}")
+    fill_ms = time_ms(fill0, args.steps, args.warmup)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fill0()
+    fill_host_ms = (time.perf_counter() - t0) * 1e3 / args.steps  # enqueue time per call
+    m.sync()
     fine_cells = sum(int(np.prod([hi[d] - lo[d] + 1 for d in range(3)]))
                      for bl in h.boxes[1:] for (lo, hi) in bl)
     # bytes: every fine cell read and an eighth as many coarse cells written
@@ -302,6 +317,9 @@ def amr_main(args):
                                          "tz": cands1[best1][1]},
                 "autotuned": tuned,
                 "autotune_candidates_ms": [[c[0], c[1], round(c[2], 4)] for c in cands],
+                "fill_patch_level0": {"ms": round(fill_ms, 4),
+                                      "host_enqueue_ms": round(fill_host_ms, 4),
+                                      "boxes": len(h.boxes[0]), "ncomp": nsc},
                 "restriction": {"ms": round(sync_ms, 4), "kernel_ms": round(sync_kernel_ms, 4),
                                 "fine_cells": fine_cells,
                                 "kernel_gbs": round(sync_bytes / (sync_kernel_ms * 1e-3) / 1e9, 1)

@@ -78,9 +78,17 @@ int Table::upload() {
         base = nullptr;
         return MX_OK;
     }
-    // FNV-1a over the bytes, then a full comparison
+    // an FNV-1a-style hash over 8-byte words (tables of many thousand
+    // entries are hashed on every call), then a full comparison
     unsigned long long h = 1469598103934665603ULL;
-    for (size_t i = 0; i < bytes; ++i) h = (h ^ (unsigned char)c->htab[i]) * 1099511628211ULL;
+    size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+        unsigned long long w;
+        std::memcpy(&w, c->htab.data() + i, 8);
+        h = (h ^ w) * 1099511628211ULL;
+        h ^= h >> 29;
+    }
+    for (; i < bytes; ++i) h = (h ^ (unsigned char)c->htab[i]) * 1099511628211ULL;
     for (const auto& e : c->tab_cache) {
         if (e.hash == h && e.bytes.size() == bytes &&
             std::memcmp(e.bytes.data(), c->htab.data(), bytes) == 0) {
@@ -898,6 +906,12 @@ int mx_adv_set_level(mx_adv_ctx* c, int lev, int nbox, const mx_box* boxes) {
     for (auto it = c->plans.begin(); it != c->plans.end();) {
         if (std::get<0>(it->first) == lev)
             it = c->plans.erase(it);
+        else
+            ++it;
+    }
+    for (auto it = c->fill_plans.begin(); it != c->fill_plans.end();) {
+        if (std::get<0>(it->first) == lev)
+            it = c->fill_plans.erase(it);
         else
             ++it;
     }

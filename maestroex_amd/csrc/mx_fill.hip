@@ -261,41 +261,64 @@ int mx_adv_fill_patch(mx_adv_ctx* c, int lev, mx_multifab* mf, const mx_multifab
         }
     }
 
-    // 1. ParallelCopy from the source valid boxes and their periodic images
+    // 1. ParallelCopy from the source valid boxes and their periodic images;
+    //    the box intersections are planned once per (level, ghost widths)
     {
-        int nsh[3] = {0, 0, 0};
-        for (int d = 0; d < dim; ++d) nsh[d] = c->p.is_periodic[d] ? 1 : 0;
+        const auto key = std::make_tuple(lev, ng[0], ng[1], ng[2]);
+        auto it = c->fill_plans.find(key);
+        if (it == c->fill_plans.end()) {
+            int nsh[3] = {0, 0, 0};
+            for (int d = 0; d < dim; ++d) nsh[d] = c->p.is_periodic[d] ? 1 : 0;
+            std::vector<mx_adv_ctx::FillCopy> plan;
+            for (int b = 0; b < nb; ++b) {
+                HBox gb = hbox(grids[b]);  // the FAB's box: grown by ng (checked above)
+                for (int d = 0; d < 3; ++d) {
+                    gb.lo[d] -= ng[d];
+                    gb.hi[d] += ng[d];
+                }
+                for (int f = 0; f < nb; ++f)
+                    for (int sz = -nsh[2]; sz <= nsh[2]; ++sz)
+                        for (int sy = -nsh[1]; sy <= nsh[1]; ++sy)
+                            for (int sx = -nsh[0]; sx <= nsh[0]; ++sx) {
+                                const int s[3] = {sx * (dom.hi[0] - dom.lo[0] + 1),
+                                                  sy * (dom.hi[1] - dom.lo[1] + 1),
+                                                  sz * (dom.hi[2] - dom.lo[2] + 1)};
+                                HBox v = hbox(grids[f]);
+                                for (int d = 0; d < 3; ++d) {
+                                    v.lo[d] += s[d];
+                                    v.hi[d] += s[d];
+                                }
+                                HBox r;
+                                if (!intersect(gb, v, r)) continue;
+                                mx_adv_ctx::FillCopy q;
+                                q.b = b;
+                                q.f = f;
+                                for (int d = 0; d < 3; ++d) {
+                                    q.sh[d] = s[d];
+                                    q.lo[d] = r.lo[d];
+                                    q.hi[d] = r.hi[d];
+                                }
+                                plan.push_back(q);
+                            }
+            }
+            it = c->fill_plans.emplace(key, std::move(plan)).first;
+        }
         std::vector<CopyTask> tasks;
-        for (int b = 0; b < nb; ++b) {
-            const HBox fb = fab_box(mf->fab[b]);
-            for (int f = 0; f < nb; ++f)
-                for (int sz = -nsh[2]; sz <= nsh[2]; ++sz)
-                    for (int sy = -nsh[1]; sy <= nsh[1]; ++sy)
-                        for (int sx = -nsh[0]; sx <= nsh[0]; ++sx) {
-                            const int s[3] = {sx * (dom.hi[0] - dom.lo[0] + 1),
-                                              sy * (dom.hi[1] - dom.lo[1] + 1),
-                                              sz * (dom.hi[2] - dom.lo[2] + 1)};
-                            if (b == f && !s[0] && !s[1] && !s[2] &&
-                                mf->fab[b].p == src->fab[f].p && destcomp == srccomp)
-                                continue;  // a FAB's own valid cells
-                            HBox v = hbox(grids[f]);
-                            for (int d = 0; d < 3; ++d) {
-                                v.lo[d] += s[d];
-                                v.hi[d] += s[d];
-                            }
-                            HBox r;
-                            if (!intersect(fb, v, r)) continue;
-                            CopyTask T;
-                            for (int d = 0; d < 3; ++d) {
-                                T.lo[d] = r.lo[d];
-                                T.hi[d] = r.hi[d];
-                                T.sh[d] = s[d];
-                            }
-                            T.blk0 = 0;
-                            T.dst = to_da(mf->fab[b]);
-                            T.src = to_da(src->fab[f]);
-                            tasks.push_back(T);
-                        }
+        tasks.reserve(it->second.size());
+        for (const auto& q : it->second) {
+            if (q.b == q.f && !q.sh[0] && !q.sh[1] && !q.sh[2] &&
+                mf->fab[q.b].p == src->fab[q.f].p && destcomp == srccomp)
+                continue;  // a FAB's own valid cells
+            CopyTask T;
+            for (int d = 0; d < 3; ++d) {
+                T.lo[d] = q.lo[d];
+                T.hi[d] = q.hi[d];
+                T.sh[d] = q.sh[d];
+            }
+            T.blk0 = 0;
+            T.dst = to_da(mf->fab[q.b]);
+            T.src = to_da(src->fab[q.f]);
+            tasks.push_back(T);
         }
         std::vector<int> blk;
         if ((rc = flatten(tasks, blk))) return rc;

@@ -70,6 +70,15 @@ struct mx_adv_ctx {
         std::vector<GhostCand> cands;
     };
     std::map<std::tuple<int, int, long long>, GhostPlan> plans;
+    // FillPatch copy plans, keyed by (lev, ghost widths): every intersection
+    // of a destination FAB's grown box with a source valid box or one of its
+    // periodic images
+    struct FillCopy {
+        int b, f;
+        int sh[3];
+        int lo[3], hi[3];
+    };
+    std::map<std::tuple<int, int, int, int>, std::vector<FillCopy>> fill_plans;
 
     // opt-in kernel timing: a HIP event pair around every launch
     struct ProfRec {

@@ -91,3 +91,23 @@ def test_fill_patch_errors():
     m.sync()
     for x, y in zip(before, mf.to_numpy()):
         assert np.array_equal(x, y)
+
+
+def test_fill_patch_plan_reuse():
+    """Later calls on one context reuse the level's copy plan with other
+    MultiFabs: each call still matches the oracle"""
+    import maestroex_amd as mx
+    print("[parity] fill_patch plan reuse", flush=True)
+    dim, n, mg, per, ng, nc, vt, rec = CASES["3d_walls_vel"]
+    params, boxes = layout(dim, n=n, max_grid=mg, periodic=per)
+    mo = pyoracle.oracle_maestro(params, [boxes])
+    mg_ = mx.gpu_maestro(params, [boxes])
+    for seed in (5, 6, 7):
+        outs = []
+        for be, m in ((NP, mo), (mx.device_backend(), mg_)):
+            mf = new_mf(boxes, nc, ng, dim, be, random_field(seed))
+            m.FillPatch(0, mf, mf, 0, 0, nc, 0, rec, vt)
+            m.sync()
+            outs.append(mf.to_numpy())
+        for b, (x, y) in enumerate(zip(*outs)):
+            assert np.array_equal(x, y), f"seed {seed}: FAB {b}"
