@@ -191,7 +191,8 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
      "edge_scal_tz"     planes per z-chunk of the fused kernel (default 32).
      "edge_scal_tile"   cell columns per block of the fused kernel, and the
                         block size: 0: 32 x 8 (384 threads), 1: 16 x 16 (384),
-                        2 (default): 16 x 12 (256), 3: 16 x 8 (192).
+                        2: 16 x 12 (256), 3: 16 x 8 (192), 4: 16 x 11 (256);
+                        -1 (default): 16 x 11 for ppm_type 1, else 16 x 12.
      "xcd_remap"        1: the staged VelPred, MakeUtrans and flux kernels
                         give each of the 8 XCDs one contiguous eighth of a
                         box's blocks; 0 (default): blocks in launch order.
@@ -375,7 +376,7 @@ int mx_adv_pass_graph_destroy(mx_adv_graph* g);
 
 /* Autotuner of the fused edge-state kernel (no counterpart in the
    reference): runs the hierarchy pass once, then times the species edge-state
-   launch for every tile shape (16x12, 16x8, 32x8 columns) and z-chunk (16,
+   launch for every tile shape (16x11, 16x12, 16x8, 32x8 columns) and z-chunk (16,
    32, 64, 128 planes), keeps the fastest in the context ("edge_scal_tile",
    "edge_scal_tz") and reports every candidate. */
 #define MX_TUNE_MAX 16

@@ -815,7 +815,7 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
         if (value < 1 || value > 4096) return MX_ERR_ARG;
         c->fused_tz = (int)value;
     } else if (k == "edge_scal_tile") {
-        if (value < 0 || value > 3) return MX_ERR_ARG;
+        if (value < -1 || value > 4) return MX_ERR_ARG;
         c->fused_tile = (int)value;
     } else if (k == "edge_scal_per_box") {
         if (value != 0 && value != 1) return MX_ERR_ARG;

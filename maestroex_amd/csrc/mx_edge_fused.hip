@@ -910,12 +910,21 @@ int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int 
 
 // the tile shape of `fused_tile`, with the block size (columns rounded up to
 // whole waves) and the register budget (waves per SIMD):
-//   0: 32 x 8 (384 threads), 1: 16 x 16 (384), 2: 16 x 12 (256), 3: 16 x 8 (192)
-// all at 3 waves per SIMD (168 VGPRs), so a CU holds 2, 2, 3 or 4 blocks
+//   0: 32 x 8 (384 threads), 1: 16 x 16 (384), 2: 16 x 12 (256), 3: 16 x 8 (192),
+//   4: 16 x 11 (256)
+// all at 3 waves per SIMD (168 VGPRs), so a CU holds 2, 2, 3, 4 or 3 blocks.
+// -1 (the default) is 16 x 11 for ppm_type 1 and 16 x 12 otherwise: with
+// ppm_type 1 the 16 x 11 tile's face phases (19 x 13 x-faces, 18 x 14
+// y-faces) fit one pass of 256 threads, where 16 x 12 leaves one wave a
+// second pass while the others wait at the barrier
+int fused_tile_of(const mx_adv_ctx* c) {
+    if (c->fused_tile >= 0) return c->fused_tile;
+    return c->p.ppm_type == 1 ? 4 : 2;
+}
 int launch_tile(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int is_vel,
                 const mx_bcrec* bcs, int start_scomp, int start_bccomp, int num_comp,
                 int is_conservative) {
-    switch (c->fused_tile) {
+    switch (fused_tile_of(c)) {
         case 1:
             return launch_fused<16, 16, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
                                            num_comp, is_conservative);
@@ -925,6 +934,10 @@ int launch_tile(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int i
         case 3:
             return launch_fused<16, 8, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
                                           num_comp, is_conservative);
+        case 4:
+            return launch_fused<16, 11, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
+                                           num_comp, is_conservative);
+
         default:
             return launch_fused<32, 8, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
                                           num_comp, is_conservative);

@@ -173,7 +173,7 @@ int mx_adv_autotune(mx_adv_ctx* c, int nlev, const mx_adv_pass_args* args,
     for (int i = 0; i < nlev; ++i)
         levs.push_back({i, &args[i].scal, &args[i].sedge, &args[i].umac, &args[i].scal_force});
     const mx_bcrec* bs = c->bcs_s.data();
-    const int tiles[] = {2, 3, 0};
+    const int tiles[] = {4, 2, 3, 0};
     const int tzs[] = {16, 32, 64, 128};
     const int saved_tile = c->fused_tile, saved_tz = c->fused_tz, saved_prof = c->prof_on;
     c->prof_on = 0;

@@ -97,7 +97,8 @@ struct mx_adv_ctx {
     int fused_edge = 1;
     int fused_tz = 32;
     int xcd_remap = 0;  // KGeom::xcd of the staged pass kernels
-    int fused_tile = 2;     // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8
+    int fused_tile = -1;    // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8,
+                            // 4: 16 x 11; -1: 16 x 11 for ppm_type 1, else 16 x 12
     int fused_per_box = 0;  // 1: one fused launch per box (comparison mode)
     int adj_state_fused = 1;  // adjoint: face, slope and state adjoints in one gather
     int adj_face_sums = 0;    // adjoint: predictor-state face adjoints stored once per face

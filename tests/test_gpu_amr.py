@@ -101,9 +101,9 @@ def test_autotune_keeps_results():
     hg, m = gpu_hierarchy(1)
     cands, best = m.autotune(fields(hg))
     print("[autotune]", cands, "best", best, flush=True)
-    # 3 tile shapes x 4 z-chunk lengths
-    assert len(cands) == 12 and 0 <= best < len(cands)
-    assert sorted({t for t, _, _ in cands}) == [0, 2, 3]
+    # 4 tile shapes x 4 z-chunk lengths
+    assert len(cands) == 16 and 0 <= best < len(cands)
+    assert sorted({t for t, _, _ in cands}) == [0, 2, 3, 4]
     assert all(ms > 0 for _, _, ms in cands)
     assert cands[best][2] == min(ms for _, _, ms in cands)
     clear_outputs(hg)

@@ -124,12 +124,12 @@ def test_pass_xcd_remap(name, cfg):
     assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} etarhoflux")
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("name,cfg", KERNEL_CHOICE_CASES, ids=[c[0] for c in KERNEL_CHOICE_CASES])
 def test_pass_fused_tile_shapes(name, cfg, tile):
     """The fused kernel's tile shapes and block sizes (32 x 8 and 16 x 16 on
-    384 threads, 16 x 12 on 256, 16 x 8 on 192), with 5-plane z-chunks:
-    bit-exact, partial tiles included."""
+    384 threads, 16 x 12 and 16 x 11 on 256, 16 x 8 on 192), with 5-plane
+    z-chunks: bit-exact, partial tiles included."""
     print(f"[parity] fused tile {tile} {name}", flush=True)
     wc, wg = pair(**cfg)
     _run_pass(oracle_for(wc), wc)
