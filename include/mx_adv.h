@@ -193,6 +193,10 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
                         block size: 0: 32 x 8 (384 threads), 1: 16 x 16 (384),
                         2: 16 x 12 (256), 3: 16 x 8 (192), 4: 16 x 11 (256);
                         -1 (default): 16 x 11 for ppm_type 1, else 16 x 12.
+     "velpred_ppm_all"  1 (default): VelPred's PPM kernel computes every
+                        velocity component of a cell in one thread (3-D,
+                        ppm_type 1/2, no traced forces); 0: one component
+                        per thread.
      "xcd_remap"        1: the staged VelPred, MakeUtrans and flux kernels
                         give each of the 8 XCDs one contiguous eighth of a
                         box's blocks; 0 (default): blocks in launch order.

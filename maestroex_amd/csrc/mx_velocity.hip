@@ -214,6 +214,37 @@ __device__ __forceinline__ void vp_ppm_dir(const VpBox& B, const KGeom& g, const
     }
 }
 
+// every velocity component of a cell in one thread (ppm_type 1/2, no traced
+// forces, 3-D): the tracing velocity and the index work are shared, and all
+// loads come before the 18 stores
+template <int PPM>
+__global__ void __launch_bounds__(256) k_vp_ppm_all(const VpBox* __restrict__ boxes, KGeom g,
+                                                    VpArgs A) {
+    const VpBox& B = boxes[blockIdx.y];
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = B.lo[d] - 1;
+        hi[d] = B.hi[d] + 1;
+    }
+    long long idx = grid_point(g);
+    int i, j, k;
+    if (!region_point(idx, lo, hi, i, j, k)) return;
+    double r0[3][3], r1[3][3], f0, f1;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        vp_ppm_dir<PPM, 0>(B, g, A, n, i, j, k, r0[n][0], r1[n][0], f0, f1);
+        vp_ppm_dir<PPM, 1>(B, g, A, n, i, j, k, r0[n][1], r1[n][1], f0, f1);
+        vp_ppm_dir<PPM, 2>(B, g, A, n, i, j, k, r0[n][2], r1[n][2], f0, f1);
+    }
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            B.Ip[n](i, j, k, d) = r0[n][d];
+            B.Im[n](i, j, k, d) = r1[n][d];
+        }
+}
+
 template <int PPM>
 __global__ void __launch_bounds__(256) k_vp_ppm(const VpBox* __restrict__ boxes, KGeom g, VpArgs A) {
     const VpBox& B = boxes[blockIdx.y];
@@ -717,7 +748,13 @@ int velpred_setup(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mu
     const int nb = (int)boxes.size();
 
     dim3 gr = grid_for_g(kg, maxpts, nb, dim, block);
-    if (ppm == 0)
+    if (c->vp_ppm_all && dim == 3 && ppm != 0 && c->p.ppm_trace_forces == 0) {
+        const dim3 g1 = grid_for_g(kg, maxpts, nb, 1, block);
+        if (ppm == 1)
+            MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm_all<1>, g1, dim3(block), dp, kg, A);
+        else
+            MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm_all<2>, g1, dim3(block), dp, kg, A);
+    } else if (ppm == 0)
         MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm<0>,  gr, dim3(block), dp, kg, A);
     else if (ppm == 1)
         MX_LAUNCH(c, "k_vp_ppm", k_vp_ppm<1>,  gr, dim3(block), dp, kg, A);

@@ -124,6 +124,27 @@ def test_pass_xcd_remap(name, cfg):
     assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} etarhoflux")
 
 
+VP_ALL_CASES = [CASES[0], CASES[1], CASES[7], CASES[9], CASES[10], CASES[16], CASES[8]]
+
+
+@pytest.mark.parametrize("name,cfg", VP_ALL_CASES, ids=[c[0] for c in VP_ALL_CASES])
+def test_pass_velpred_ppm_per_component(name, cfg):
+    """VelPred's PPM with one velocity component per thread (the default
+    computes all three in one thread; traced forces, ppm_type 0 and 2-D
+    always take this kernel): the whole pass bit-exact."""
+    print(f"[parity] velpred ppm per component {name}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    m.set_tuning("velpred_ppm_all", 0)
+    _run_pass(m, wg)
+    assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
+    assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} sedge")
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
+
+
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("name,cfg", KERNEL_CHOICE_CASES, ids=[c[0] for c in KERNEL_CHOICE_CASES])
 def test_pass_fused_tile_shapes(name, cfg, tile):
