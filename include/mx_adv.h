@@ -197,6 +197,9 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
                         velocity component of a cell in one thread (3-D,
                         ppm_type 1/2, no traced forces); 0: one component
                         per thread.
+     "velpred_trans_all" 1 (default): VelPred's transverse kernel computes
+                        the six transverse states of a cell in one thread
+                        (3-D); 0: one (direction, correction) pair per thread.
      "xcd_remap"        1: the staged VelPred, MakeUtrans and flux kernels
                         give each of the 8 XCDs one contiguous eighth of a
                         box's blocks; 0 (default): blocks in launch order.

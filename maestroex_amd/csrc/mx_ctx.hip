@@ -820,6 +820,9 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
     } else if (k == "edge_scal_per_box") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->fused_per_box = (int)value;
+    } else if (k == "velpred_trans_all") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->vp_trans_all = (int)value;
     } else if (k == "velpred_ppm_all") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->vp_ppm_all = (int)value;

@@ -98,6 +98,7 @@ struct mx_adv_ctx {
     int fused_tz = 32;
     int xcd_remap = 0;  // KGeom::xcd of the staged pass kernels
     int vp_ppm_all = 1;  // VelPred's PPM kernel: 1 = all components per thread
+    int vp_trans_all = 1;  // VelPred's transverse kernel: 1 = all six states per thread
     int fused_tile = -1;    // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8,
                             // 4: 16 x 11; -1: 16 x 11 for ppm_type 1, else 16 x 12
     int fused_per_box = 0;  // 1: one fused launch per box (comparison mode)

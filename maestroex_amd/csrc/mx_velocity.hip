@@ -378,7 +378,8 @@ __device__ __forceinline__ double vp_uimh(const VpBox& B, const KGeom& g, int n,
 
 // transverse state on D-faces of component O corrected by direction T
 template <int D, int T>
-__device__ __forceinline__ void vp_trans(const VpBox& B, const KGeom& g, int i, int j, int k) {
+__device__ __forceinline__ double vp_trans_value(const VpBox& B, const KGeom& g, int i, int j,
+                                                int k) {
     constexpr int O = 3 - D - T;
     const int im = i - e(D, 0), jm = j - e(D, 1), km = k - e(D, 2);
     const double dt6 = g.dt / 6.0;
@@ -433,7 +434,57 @@ __device__ __forceinline__ void vp_trans(const VpBox& B, const KGeom& g, int i, 
     const double tv = B.tr[D](i, j, k);
     double res = tv > 0.0 ? l : r;
     res = dabs(tv) < g.rel_eps ? 0.5 * (l + r) : res;
-    B.T[pair_index(D, T)](i, j, k) = res;
+    return res;
+}
+
+template <int D, int T>
+__device__ __forceinline__ void vp_trans(const VpBox& B, const KGeom& g, int i, int j, int k) {
+    B.T[pair_index(D, T)](i, j, k) = vp_trans_value<D, T>(B, g, i, j, k);
+}
+
+// is (i,j,k) in the region of pair (D, T): the box grown by one in O = 3-D-T
+// and by one face on the high side of D
+template <int D, int T>
+__device__ __forceinline__ bool in_pair(const VpBox& B, int i, int j, int k) {
+    constexpr int O = 3 - D - T;
+    const int p[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int lo = B.lo[d] - (d == O ? 1 : 0);
+        const int hi = B.hi[d] + (d == O ? 1 : 0) + (d == D ? 1 : 0);
+        if (p[d] < lo || p[d] > hi) return false;
+    }
+    return true;
+}
+
+// all six transverse states of the points of one cell in one thread (3-D):
+// every load before the stores
+__global__ void __launch_bounds__(256) k_vp_trans_all(const VpBox* __restrict__ boxes, KGeom g) {
+    const VpBox& B = boxes[blockIdx.y];
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = B.lo[d] - 1;
+        hi[d] = B.hi[d] + 1;
+    }
+    long long idx = grid_point(g);
+    int i, j, k;
+    if (!region_point(idx, lo, hi, i, j, k)) return;
+    const bool in0 = in_pair<0, 1>(B, i, j, k), in1 = in_pair<0, 2>(B, i, j, k);
+    const bool in2 = in_pair<1, 0>(B, i, j, k), in3 = in_pair<1, 2>(B, i, j, k);
+    const bool in4 = in_pair<2, 0>(B, i, j, k), in5 = in_pair<2, 1>(B, i, j, k);
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
+    if (in0) v0 = vp_trans_value<0, 1>(B, g, i, j, k);
+    if (in1) v1 = vp_trans_value<0, 2>(B, g, i, j, k);
+    if (in2) v2 = vp_trans_value<1, 0>(B, g, i, j, k);
+    if (in3) v3 = vp_trans_value<1, 2>(B, g, i, j, k);
+    if (in4) v4 = vp_trans_value<2, 0>(B, g, i, j, k);
+    if (in5) v5 = vp_trans_value<2, 1>(B, g, i, j, k);
+    if (in0) B.T[0](i, j, k) = v0;
+    if (in1) B.T[1](i, j, k) = v1;
+    if (in2) B.T[2](i, j, k) = v2;
+    if (in3) B.T[3](i, j, k) = v3;
+    if (in4) B.T[4](i, j, k) = v4;
+    if (in5) B.T[5](i, j, k) = v5;
 }
 
 __global__ void __launch_bounds__(256) k_vp_trans(const VpBox* __restrict__ boxes, KGeom g) {
@@ -784,7 +835,10 @@ int utrans_from_plan(mx_adv_ctx* c, const VpPlan& plan) {
 int velpred_finish(mx_adv_ctx* c, const VpPlan& plan) {
     const KGeom kg = kgeom(c, plan.lev);
     const int block = 256;
-    if (c->p.dim == 3)
+    if (c->p.dim == 3 && c->vp_trans_all)
+        MX_LAUNCH(c, "k_vp_trans", k_vp_trans_all, grid_for_g(kg, plan.maxpts, plan.nb, 1, block),
+                  dim3(block), (const VpBox*)plan.dp, kg);
+    else if (c->p.dim == 3)
         MX_LAUNCH(c, "k_vp_trans", k_vp_trans, grid_for_g(kg, plan.maxpts, plan.nb, 6, block),
                   dim3(block), (const VpBox*)plan.dp, kg);
     MX_LAUNCH(c, "k_vp_vel", k_vp_vel, grid_for_g(kg, plan.maxpts, plan.nb, c->p.dim, block),

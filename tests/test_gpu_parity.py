@@ -145,6 +145,24 @@ def test_pass_velpred_ppm_per_component(name, cfg):
     assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
 
 
+@pytest.mark.parametrize("name,cfg", VP_ALL_CASES, ids=[c[0] for c in VP_ALL_CASES])
+def test_pass_velpred_trans_per_pair(name, cfg):
+    """VelPred's transverse states one (direction, correction) pair per
+    thread (the default computes all six of a cell in one thread): the
+    whole pass bit-exact."""
+    print(f"[parity] velpred trans per pair {name}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    m.set_tuning("velpred_trans_all", 0)
+    _run_pass(m, wg)
+    assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
+    assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} sedge")
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
+
+
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("name,cfg", KERNEL_CHOICE_CASES, ids=[c[0] for c in KERNEL_CHOICE_CASES])
 def test_pass_fused_tile_shapes(name, cfg, tile):
