@@ -84,12 +84,14 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--max-grid", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time only the headline pass (no ppm_type 2, AMR or adjoint fields)")
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner pick the fused edge kernel's launch shape (the "
                          "hierarchy mode always does)")
     ap.add_argument("--no-autotune", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="kernel choice passed to mx_adv_set_tuning (repeatable)")
     ap.add_argument("--amr", action="store_true",
@@ -125,33 +127,66 @@ def committed_traffic(kernel, workload_tag):
     return None
 
 
-def cpu_baseline(args):
+def host_cpu():
+    """(nproc, CPU model name) of this host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count(), model
+
+
+def cpu_threads():
+    """Host threads of the CPU baseline: the host-thread share of one GPU on
+    the GPU box (OMP_NUM_THREADS there, 16), at most the CPUs this process
+    may run on; MX_CPU_THREADS overrides."""
+    n = int(os.environ.get("MX_CPU_THREADS", "0")) or int(os.environ.get("OMP_NUM_THREADS", "0"))
+    n = n or 16
+    return max(1, min(n, len(os.sched_getaffinity(0))))
+
+
+def cpu_baseline(args, ppm=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from maestroex_amd.fabs import Backend
     from maestroex_amd.workloads import Workload
-    cores = int(os.environ.get("MX_CPU_THREADS", "0")) or len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
+    ppm = args.ppm if ppm is None else ppm
+    cores = cpu_threads()
     pyoracle.set_threads(cores, fast=True)
+    lib, _ = pyoracle.load(True)
+    lib.mxo_alloc_ns.restype = C.c_longlong
+    lib.mxo_alloc_ns.argtypes = [C.c_int]
     n = args.cpu_n
-    w = Workload(dim=3, n=n, max_grid=n, ppm_type=args.ppm, backend=Backend("numpy"))
+    w = Workload(dim=3, n=n, max_grid=n, ppm_type=ppm, backend=Backend("numpy"))
     m = pyoracle.oracle_maestro(w.params, [w.boxes], fast=True)
     m.advance_pass(0, w.pass_fields())  # warm-up (page-in, thread start)
-    times = []
+    times, excl = [], []
     for _ in range(args.cpu_reps):
+        lib.mxo_alloc_ns(1)
         t0 = time.perf_counter()
         m.advance_pass(0, w.pass_fields())
-        times.append(time.perf_counter() - t0)
-    times.sort()
-    med = times[len(times) // 2]
+        t = time.perf_counter() - t0
+        times.append(t)
+        excl.append(t - lib.mxo_alloc_ns(1) * 1e-9)
+    med = sorted(times)[len(times) // 2]
+    med_x = sorted(excl)[len(excl) // 2]
+    nproc, model = host_cpu()
     return {
         "value": round(n ** 3 / med / 1e9, 6),
         "unit": "Gcell-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"one {n}^3 box of the same construction (4 ghosts, 5 scalars, ppm_type {args.ppm}), "
-                  f"median of {args.cpu_reps} passes after 1 warm-up, OpenMP {cores} threads, "
-                  f"g++ -O3 -fopenmp -ffp-contract=off; {med:.2f} s per pass",
+        "sample": f"one {n}^3 box of the same construction (4 ghosts, 5 scalars, ppm_type {ppm}), "
+                  f"median of {args.cpu_reps} passes after 1 warm-up, OpenMP {cores} threads "
+                  f"(the host-thread share of one GPU), g++ -O3 -fopenmp -ffp-contract=off; "
+                  f"{med:.3f} s per pass allocation-inclusive, {med_x:.3f} s allocation-exclusive",
+        "alloc_exclusive_value": round(n ** 3 / med_x / 1e9, 6),
+        "host_nproc": nproc,
+        "host_cpu": model,
     }
 
 
@@ -195,7 +230,7 @@ def time_ms(fn, steps, warmup):
     return start.elapsed_time(stop) / steps
 
 
-def amr_main(args):
+def amr_main(args, emit=True):
     """Batched AMR hierarchy: level 0 256^3 in uneven boxes of 32-64 cells
     per edge, level 1 (ratio 2) 150 boxes of 16-48, level 2 200 boxes of
     8-32, all 5 scalars + 3 velocities.  The hierarchy pass is timed as one
@@ -327,10 +362,12 @@ def amr_main(args):
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
-def adjoint_main(args):
+def adjoint_main(args, emit=True):
     """Differentiable step (mx_adv_scal_step / _vjp): 128^3 periodic, 4
     scalars, a fixed divergence-free MAC velocity (max |u| = 1, CFL 0.5).
     Times one forward step and one adjoint step (tape + reverse sweeps), and
@@ -402,7 +439,9 @@ def adjoint_main(args):
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 # algorithmic bytes per cell of the update stages (3-D planar, NumSpec = 3,
@@ -733,7 +772,7 @@ def fill_cpu_baseline(args, ng):
     import pyoracle
     from maestroex_amd import _abi
     from maestroex_amd.fabs import Backend, Geometry, MultiFab, make_params, split_domain
-    cores = min(int(os.environ.get("MX_CPU_THREADS", "0")) or len(os.sched_getaffinity(0)), 16)
+    cores = cpu_threads()
     pyoracle.set_threads(cores, fast=True)
     n = args.cpu_n
     out_bc = [_abi.PHYS_INTERIOR, _abi.PHYS_INTERIOR, _abi.PHYS_OUTFLOW]
@@ -765,7 +804,7 @@ def update_cpu_baseline(args, nspec, bcd):
     import numpy as np
     import pyoracle
     from maestroex_amd.fabs import Backend, Geometry, make_params, split_domain
-    cores = min(int(os.environ.get("MX_CPU_THREADS", "0")) or len(os.sched_getaffinity(0)), 16)
+    cores = cpu_threads()
     pyoracle.set_threads(cores, fast=True)
     n = args.cpu_n
     geom = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 1], [0] * 3, [0] * 3)
@@ -1078,6 +1117,10 @@ def main():
     roofline["frac_of_measured_copy"] = round(achieved / copy_gbs, 4) if achieved else None
     print(f"[bench] device copy {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
+    extras = {}
+    if not args.headline_only and not args.spherical:
+        extras = rest_of_metric(args, w, fields)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args)
@@ -1104,7 +1147,45 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
+
+
+def rest_of_metric(args, w, fields):
+    """The other parts of the metric, on the same run: the same 256^3
+    workload with ppm_type 2 (graph launch per pass), the batched AMR
+    hierarchy's per-cell cost against a single 128^3 box, and the adjoint /
+    forward time ratio of the differentiable step (128^3, 4 scalars)."""
+    import copy
+    import maestroex_amd as mx
+    out = {}
+    # ppm_type 2: a second context on the same fields (4 ghost cells)
+    p2 = mx._abi.Params()
+    C.memmove(C.byref(p2), C.byref(w.params), C.sizeof(mx._abi.Params))
+    p2.ppm_type = 2
+    m2 = mx.gpu_maestro(p2, [w.boxes])
+    g2 = m2.pass_graph([fields])
+    ms2 = time_ms(g2.launch, args.steps, args.warmup)
+    g2.close()
+    cells = w.valid_cells()
+    out["ppm2"] = {"ms_per_step": round(ms2, 4),
+                   "value": round(cells / (ms2 * 1e-3) / 1e9, 4),
+                   "pass_frac": round(PASS_BYTES_PER_CELL[3] * cells / (ms2 * 1e-3) / 1e9
+                                      / HBM_PEAK_GBS, 4)}
+    m2.close()
+    print(f"[bench] ppm_type 2: {out['ppm2']}", file=sys.stderr, flush=True)
+    a = copy.copy(args)
+    a.ppm = 1
+    amr = amr_main(a, emit=False)
+    out["amr"] = {k: amr["amr"][k] for k in ("graph_ms", "ns_per_cell", "single_box_ns_per_cell",
+                                              "per_cell_ratio_vs_single_box")}
+    out["amr"]["cells"] = amr["config"]["global_batch"]
+    out["amr"]["value"] = amr["value"]
+    print(f"[bench] AMR: {out['amr']}", file=sys.stderr, flush=True)
+    adj = adjoint_main(a, emit=False)
+    out["adjoint"] = dict(adj["adjoint"])
+    print(f"[bench] adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":

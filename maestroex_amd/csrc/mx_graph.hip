@@ -21,6 +21,16 @@ struct mx_adv_graph {
     hipGraphExec_t exec = nullptr;
     char* arena = nullptr;
     char* ws = nullptr;
+    size_t ws_cap = 0;
+    // what a re-capture needs: the per-step values the kernels take by value
+    // at capture time, the pass arguments with graph-owned copies of the
+    // caller's FAB descriptor arrays, and the recorded box tables
+    double dt = 0.0, rel_eps = 0.0;
+    int nlev = 0;
+    std::vector<mx_adv_pass_args> args;
+    std::vector<std::vector<mx_fab>> fabs;
+    std::vector<std::vector<char>> tabs;
+    std::vector<size_t> offs;
 };
 
 namespace {
@@ -48,6 +58,74 @@ struct TabModeGuard {
     }
 };
 
+// graph-owned copies of the FAB descriptor arrays the pass arguments point
+// at, so that a later re-capture does not read caller memory that may be gone
+void keep_mf(mx_adv_graph* g, mx_multifab& m) {
+    if (m.nfab <= 0 || !m.fab) return;
+    g->fabs.emplace_back(m.fab, m.fab + m.nfab);
+    m.fab = g->fabs.back().data();
+}
+void keep_face(mx_adv_graph* g, mx_face_multifab& f) {
+    for (int d = 0; d < 3; ++d) keep_mf(g, f.dir[d]);
+}
+void keep_args(mx_adv_graph* g, const mx_adv_pass_args* args, int nlev) {
+    g->args.assign(args, args + nlev);
+    size_t n = 0;
+    for (int i = 0; i < nlev; ++i) n += 6 + 1 + 8 * 3;
+    g->fabs.reserve(n);  // no reallocation: the data() pointers stay valid
+    for (mx_adv_pass_args& a : g->args) {
+        for (mx_multifab* m : {&a.utilde, &a.ufull, &a.vel_force, &a.w0_cart, &a.scal,
+                               &a.scal_force, &a.etarhoflux})
+            keep_mf(g, *m);
+        for (mx_face_multifab* f :
+             {&a.w0mac, &a.r0mac, &a.rh0mac, &a.h0mac, &a.utrans, &a.umac, &a.sedge, &a.sflux})
+            keep_face(g, *f);
+    }
+}
+
+// Stream capture of the pass with every box table taken from the graph's
+// arena and the graph's workspace (so the captured addresses are those of
+// the first capture); on a stream of our own when the context uses the null
+// stream.  The caller holds a TabModeGuard.
+int capture(mx_adv_graph* g, hipGraph_t* out) {
+    mx_adv_ctx* c = g->c;
+    *out = nullptr;
+    c->rec_tabs = g->tabs;
+    c->rec_offs = g->offs;
+    c->rec_arena = g->arena;
+    c->rec_next = 0;
+    char* ws_saved = c->ws;
+    const size_t cap_saved = c->ws_cap, off_saved = c->ws_off;
+    c->ws = g->ws;
+    c->ws_cap = g->ws_cap;
+    c->ws_off = 0;
+    hipStream_t saved = c->stream;
+    hipStream_t cap = saved;
+    int rc = MX_OK;
+    if (!cap && hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) rc = MX_ERR_HIP;
+    if (!rc) {
+        c->stream = cap;
+        c->tab_mode = 2;
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            rc = advance_pass_levels(c, 0, g->nlev, g->args.data());
+            hipGraph_t graph = nullptr;
+            e = hipStreamEndCapture(cap, &graph);
+            *out = graph;
+        }
+        if (e != hipSuccess) rc = rc ? rc : MX_ERR_HIP;
+        if (!rc && c->rec_next != c->rec_tabs.size()) rc = MX_ERR_ARG;
+        c->stream = saved;
+        if (!saved) (void)hipStreamDestroy(cap);
+    }
+    // the workspace must not have moved: the graph owns it
+    if (!rc && c->ws != g->ws) rc = MX_ERR_ARG;
+    c->ws = ws_saved;
+    c->ws_cap = cap_saved;
+    c->ws_off = off_saved;
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -64,17 +142,21 @@ int mx_adv_pass_graph_create(mx_adv_ctx* c, int nlev, const mx_adv_pass_args* ar
     int rc;
     mx_adv_graph* g = new mx_adv_graph();
     g->c = c;
+    g->nlev = nlev;
+    g->dt = c->p.dt;
+    g->rel_eps = c->p.rel_eps;
+    keep_args(g, args, nlev);
     TabModeGuard guard(c);
 
     // 1) an eager run sizes the workspace (its address then stays fixed), a
     //    second one records every box table
-    if ((rc = advance_pass_levels(c, 0, nlev, args))) {
+    if ((rc = advance_pass_levels(c, 0, nlev, g->args.data()))) {
         graph_free(g);
         return rc;
     }
     c->tab_mode = 1;
     c->rec_tabs.clear();
-    if ((rc = advance_pass_levels(c, 0, nlev, args))) {
+    if ((rc = advance_pass_levels(c, 0, nlev, g->args.data()))) {
         graph_free(g);
         return rc;
     }
@@ -86,11 +168,11 @@ int mx_adv_pass_graph_create(mx_adv_ctx* c, int nlev, const mx_adv_pass_args* ar
 
     // 2) the recorded tables in one device arena
     size_t total = 0;
-    c->rec_offs.clear();
     for (const auto& t : c->rec_tabs) {
-        c->rec_offs.push_back(total);
+        g->offs.push_back(total);
         total += (t.size() + 255) & ~(size_t)255;
     }
+    g->tabs.swap(c->rec_tabs);
     if (total) {
         if (hipMalloc((void**)&g->arena, total) != hipSuccess) {
             g->arena = nullptr;
@@ -98,57 +180,73 @@ int mx_adv_pass_graph_create(mx_adv_ctx* c, int nlev, const mx_adv_pass_args* ar
             return MX_ERR_NOMEM;
         }
         std::vector<char> host(total, 0);
-        for (size_t i = 0; i < c->rec_tabs.size(); ++i)
-            if (!c->rec_tabs[i].empty())
-                std::memcpy(host.data() + c->rec_offs[i], c->rec_tabs[i].data(),
-                            c->rec_tabs[i].size());
+        for (size_t i = 0; i < g->tabs.size(); ++i)
+            if (!g->tabs[i].empty())
+                std::memcpy(host.data() + g->offs[i], g->tabs[i].data(), g->tabs[i].size());
         if (hipMemcpy(g->arena, host.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
             graph_free(g);
             return MX_ERR_HIP;
         }
     }
-    c->rec_arena = g->arena;
-    c->rec_next = 0;
+    // the graph's kernels use the context workspace as it is now: hand it to
+    // the graph so that later eager calls allocate their own
+    g->ws = c->ws;
+    g->ws_cap = c->ws_cap;
+    c->ws = nullptr;
+    c->ws_cap = 0;
+    c->ws_off = 0;
 
-    // 3) capture on a stream of our own when the context uses the null stream
-    hipStream_t saved = c->stream;
-    hipStream_t cap = saved;
-    if (!cap && hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
-        graph_free(g);
-        return MX_ERR_HIP;
-    }
-    c->stream = cap;
-    c->tab_mode = 2;
-    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        rc = advance_pass_levels(c, 0, nlev, args);
-        hipGraph_t graph = nullptr;
-        e = hipStreamEndCapture(cap, &graph);
-        g->graph = graph;
-    }
-    c->stream = saved;
-    if (!saved) (void)hipStreamDestroy(cap);
-    if (e != hipSuccess) rc = rc ? rc : MX_ERR_HIP;
-    if (!rc && c->rec_next != c->rec_tabs.size()) rc = MX_ERR_ARG;
+    // 3) capture, then instantiate
+    hipGraph_t graph = nullptr;
+    rc = capture(g, &graph);
+    g->graph = graph;
     if (!rc && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
         rc = MX_ERR_HIP;
     if (rc) {
         graph_free(g);
         return rc;
     }
-    // the graph's kernels use the context workspace as it is now: hand it to
-    // the graph so that later eager calls allocate their own
-    g->ws = c->ws;
-    c->ws = nullptr;
-    c->ws_cap = 0;
-    c->ws_off = 0;
     *out = g;
     return MX_OK;
 }
 
 int mx_adv_pass_graph_launch(mx_adv_graph* g) {
     if (!g || !g->exec) return MX_ERR_ARG;
-    if (hipGraphLaunch(g->exec, g->c->stream) != hipSuccess) return MX_ERR_HIP;
+    mx_adv_ctx* c = g->c;
+    if (c->p.dt != g->dt || c->p.rel_eps != g->rel_eps) {
+        // dt and rel_eps changed since capture (mx_adv_set_dt, every step of
+        // the reference's time loop): the kernels take them by value, so the
+        // pass is captured again with the new values and the executable
+        // graph updated in place (re-instantiated if the update is refused)
+        hipGraph_t fresh = nullptr;
+        int rc;
+        {
+            TabModeGuard guard(c);
+            rc = capture(g, &fresh);
+        }
+        if (rc) {
+            if (fresh) (void)hipGraphDestroy(fresh);
+            return rc;
+        }
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(g->exec, fresh, &err_node, &res) != hipSuccess ||
+            res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            hipGraphExec_t ex = nullptr;
+            if (hipGraphInstantiate(&ex, fresh, nullptr, nullptr, 0) != hipSuccess) {
+                (void)hipGraphDestroy(fresh);
+                return MX_ERR_HIP;
+            }
+            (void)hipGraphExecDestroy(g->exec);
+            g->exec = ex;
+        }
+        (void)hipGraphDestroy(g->graph);
+        g->graph = fresh;
+        g->dt = c->p.dt;
+        g->rel_eps = c->p.rel_eps;
+    }
+    if (hipGraphLaunch(g->exec, c->stream) != hipSuccess) return MX_ERR_HIP;
     return MX_OK;
 }
 

@@ -15,6 +15,7 @@
 namespace mxo {
 
 std::atomic<int> g_oob{0};
+std::atomic<long long> g_alloc_ns{0};
 
 void PPM(const Ctx& c, const IBox& bx, const Arr& s, const Arr& u,
          const Arr& v, const Arr& w, const Arr& Ip, const Arr& Im,
@@ -654,6 +655,11 @@ int mxo_convert_rhoh_to_h(Ctx* c, int lev, mx_multifab* scal, int flag) {
     if (rc) return rc;
     if (!scal) return MX_ERR_ARG;
     return finish(ConvertByRho(*c, lev, arrs(scal), c->p.comp_rhoh, 1, flag != 0));
+}
+
+// nanoseconds spent in temporary allocation since the last reset
+long long mxo_alloc_ns(int reset) {
+    return reset ? g_alloc_ns.exchange(0) : g_alloc_ns.load();
 }
 
 int mxo_omp_threads(int n) {

@@ -10,6 +10,7 @@
 #define MXO_ORACLE_CORE_H
 
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -27,6 +28,9 @@ inline double ab(double a) { return std::fabs(a); }
 inline double cs(double a, double b) { return std::copysign(a, b); }
 
 extern std::atomic<int> g_oob;  // set when a bounds-checked access misses
+// wall time spent allocating and initialising temporaries (Fab::define), so
+// that a caller can report allocation-inclusive and -exclusive times
+extern std::atomic<long long> g_alloc_ns;
 
 struct IBox {
     int lo[3];
@@ -110,6 +114,7 @@ struct Fab {
     std::vector<double> v;
     Arr a;
     void define(const IBox& region, int ncomp, double init = 0.0) {
+        const auto t0 = std::chrono::steady_clock::now();
         long long n0 = region.hi[0] - region.lo[0] + 1;
         long long n1 = region.hi[1] - region.lo[1] + 1;
         long long n2 = region.hi[2] - region.lo[2] + 1;
@@ -123,6 +128,9 @@ struct Fab {
             a.e[d] = region.hi[d] + 1;
         }
         a.nc = ncomp;
+        g_alloc_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count();
     }
 };
 

@@ -454,7 +454,10 @@ void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double r
               const double* const du[3], double* dsnew) {
     const Grid g{n};
     const long long N3 = (long long)n * n * n;
+    // components are independent: one OpenMP thread each (the double and
+    // dual instantiations keep no shared state)
     if (!ds) {
+#pragma omp parallel for schedule(dynamic, 1)
         for (int c = 0; c < ncomp; ++c)
             step_comp<double>(g, s + c * N3, u, snew + c * N3, dt, dx, rel_eps, nullptr, ppm);
         return;
@@ -466,8 +469,9 @@ void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double r
         for (long long x = 0; x < N3; ++x) ud[d][x] = {u[d][x], du[d] ? du[d][x] : 0.0};
         up[d] = ud[d].data();
     }
-    std::vector<Dual> sd(N3), out(N3);
+#pragma omp parallel for schedule(dynamic, 1)
     for (int c = 0; c < ncomp; ++c) {
+        std::vector<Dual> sd(N3), out(N3);
         for (long long x = 0; x < N3; ++x) sd[x] = {s[c * N3 + x], ds[c * N3 + x]};
         step_comp<Dual>(g, sd.data(), up, out.data(), dt, dx, rel_eps, nullptr, ppm);
         for (long long x = 0; x < N3; ++x) {

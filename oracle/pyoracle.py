@@ -76,14 +76,14 @@ def step_op_count(s, u, dt, dx, rel_eps):
     return dict(zip(keys, out.tolist()))
 
 
-def scal_step(s, u, dt, dx, rel_eps, ds=None, du=None, ppm_type=1):
+def scal_step(s, u, dt, dx, rel_eps, ds=None, du=None, ppm_type=1, fast=False):
     """The differentiable scalar step on a periodic N^3 box (oracle_step.cpp),
     ppm_type 1 or 2.
     s: (ncomp, N, N, N) indexed [c, k, j, i]; u: three (N, N, N) face arrays
     (face index = the cell above it; face N is face 0).  Returns snew, or
     (snew, dsnew) when the direction ds (and optionally du) is given."""
     import numpy as np
-    lib, _ = load(False)
+    lib, _ = load(fast)
     s = np.ascontiguousarray(s, dtype=np.float64)
     u = [np.ascontiguousarray(x, dtype=np.float64) for x in u]
     ncomp, n = s.shape[0], s.shape[1]

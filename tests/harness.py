@@ -63,6 +63,23 @@ def compare_face(a, b, region="valid", comps=None):
                 ndiff=sum(r["ndiff"] for r in res), n=sum(r["n"] for r in res))
 
 
+def record_parity(case, array, res):
+    """Print and keep the parity metrics of one output array: max relative
+    error ||gpu - cpu||_inf / ||cpu||_inf and the count of differing values.
+    Lines go to stdout and, when the folder gpurun_out/ exists at the repo
+    root (a GPU run), to gpurun_out/parity_metrics.jsonl."""
+    import json
+    import os
+    rec = {"case": case, "array": array, "max_rel": res["max_rel"], "max_abs": res["max_abs"],
+           "ndiff": res["ndiff"], "n": res["n"]}
+    print(f"[parity-metric] {json.dumps(rec)}", flush=True)
+    out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+    if os.path.isdir(out):
+        with open(os.path.join(out, "parity_metrics.jsonl"), "a") as f:
+            f.write(json.dumps(rec) + "\n")
+    return res
+
+
 def assert_bitexact(res, what):
     assert res["ndiff"] == 0, f"{what}: {res['ndiff']} of {res['n']} values differ (max rel {res['max_rel']:.3e})"
 

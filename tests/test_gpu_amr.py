@@ -110,3 +110,40 @@ def test_autotune_keeps_results():
     m.advance_pass_levels(fields(hg))
     m.sync()
     compare(hc, hg, "tuned")
+
+
+@pytest.mark.parametrize("nlev", [1, 3])
+def test_graph_replay_follows_set_dt(nlev):
+    """dt and rel_eps change every step of the reference's time loop
+    (EstDt, MaestroDt.cpp:481-483).  A graph captured at one dt and replayed
+    after mx_adv_set_dt must compute the pass at the new dt: each replay is
+    compared bit for bit with the oracle's pass at that dt."""
+    import torch
+    print(f"[parity] graph replay after set_dt nlev={nlev}", flush=True)
+    hg, m = gpu_hierarchy(1)
+    hc = Hierarchy(ppm_type=1, backend=Backend("numpy"))
+    mo = pyoracle.oracle_maestro(hc.params, hc.boxes)
+    fg = fields(hg)[:nlev]
+    g = m.pass_graph(fg)
+    m.sync()
+    scratch = torch.zeros(1 << 16, dtype=torch.float64, device="cuda:0")
+    for scale in (0.5, 0.75, 0.75, 1.0):
+        dt, rel_eps = hg.dt * scale, hg.rel_eps * (1.0 + scale)
+        mo.set_dt(dt, rel_eps)
+        for lev in range(nlev):
+            mo.advance_pass(lev, hc.pass_fields(lev))
+        m.set_dt(dt, rel_eps)
+        clear_outputs(hg)
+        scratch.add_(1.0)
+        torch.cuda.synchronize()
+        g.launch()
+        m.sync()
+        for lev in range(nlev):
+            fc, fgl = hc.pass_fields(lev), hg.pass_fields(lev)
+            for name in OUTS:
+                assert_bitexact(compare_face(fc[name], fgl[name], region="all" if name in
+                                             ("utrans", "umac") else "valid",
+                                             comps=None if name in ("utrans", "umac") else
+                                             [0, 1] + list(range(4, hc.nscal))),
+                                f"dt x{scale} lev {lev} {name}")
+    g.close()
