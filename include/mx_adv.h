@@ -50,7 +50,9 @@ enum {
     MX_ERR_UNSUPPORTED = -5, /* configuration the path does not implement */
     MX_ERR_NOMEM = -6,       /* workspace allocation failed */
     MX_ERR_NOLIB = -7,       /* no usable GPU */
-    MX_ERR_IO = -8           /* a file could not be created or written */
+    MX_ERR_IO = -8,          /* a file could not be created or written */
+    MX_ERR_SCRATCH = -9      /* the call needs more device memory than the fixed
+                                scratch of mx_adv_advance_pass_noalloc holds */
 };
 
 /* amrex::BCType values (mathematical boundary types, per component) */
@@ -358,6 +360,24 @@ typedef struct {
 } mx_adv_pass_args;
 
 int mx_adv_advance_pass(mx_adv_ctx* ctx, int lev, const mx_adv_pass_args* a);
+
+/* The same pass with no device allocation.  No counterpart in the reference:
+   it is for callers that cannot allocate device memory during the call
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  Results equal mx_adv_advance_pass on the same arguments.
+   The box tables and VelPred's temporaries live in a fixed device array of
+   the library, 256 MiB per GPU (16 MiB of tables, 240 MiB of workspace),
+   held from the first call on a GPU until the process ends.  A call whose
+   tables or workspace do not fit returns MX_ERR_SCRATCH before anything is
+   launched; the workspace is about 192 bytes per cell of the level's boxes
+   grown by one (3-D), so levels up to about 1.3 million such cells fit.
+   Tables go to the device by stream-ordered copies from host memory the
+   context keeps until its next such call, which first waits for the context's
+   stream.  Every call on one GPU shares that GPU's array and overwrites it:
+   calls that overlap on one GPU (several streams or threads) race.  Must be
+   called with the context's device current, and not while a pass graph is
+   being created; kernel timing (mx_adv_prof_enable) is not recorded. */
+int mx_adv_advance_pass_noalloc(mx_adv_ctx* ctx, int lev, const mx_adv_pass_args* a);
 
 /* ---- a whole level hierarchy (batched) ---------------------------------- */
 /* No single counterpart in the reference: the reference runs each stage over

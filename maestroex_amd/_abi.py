@@ -18,6 +18,7 @@ MX_ERR_UNSUPPORTED = -5
 MX_ERR_NOMEM = -6
 MX_ERR_NOLIB = -7
 MX_ERR_IO = -8
+MX_ERR_SCRATCH = -9
 
 # amrex::BCType
 BC_REFLECT_ODD, BC_INT_DIR, BC_REFLECT_EVEN, BC_FOEXTRAP, BC_EXT_DIR, BC_HOEXTRAP = -1, 0, 1, 2, 3, 4
@@ -33,6 +34,7 @@ ERRORS = {
     MX_ERR_NOMEM: "workspace allocation failed",
     MX_ERR_NOLIB: "no usable GPU",
     MX_ERR_IO: "file could not be created or written",
+    MX_ERR_SCRATCH: "needs more device memory than the no-alloc scratch holds",
 }
 
 
@@ -248,6 +250,7 @@ class TuneReport(C.Structure):
 # entry points only the HIP library has
 LIB_ONLY = {
     "advance_pass_levels": [_vp, _i, P(PassArgs)],
+    "advance_pass_noalloc": [_vp, _i, P(PassArgs)],
     "pass_graph_create": [_vp, _i, P(PassArgs), P(_vp)],
     "pass_graph_launch": [_vp],
     "pass_graph_destroy": [_vp],

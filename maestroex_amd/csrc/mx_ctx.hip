@@ -33,6 +33,11 @@ KGeom kgeom(const mx_adv_ctx* c, int lev) {
 
 int ws_reserve(mx_adv_ctx* c, size_t bytes) {
     c->ws_off = 0;
+    if (c->dry) {
+        c->dry_ws = std::max(c->dry_ws, bytes);
+        return MX_OK;
+    }
+    if (c->ext_ws) return bytes <= c->ext_ws_cap ? MX_OK : MX_ERR_NOMEM;
     if (bytes <= c->ws_cap) return MX_OK;
     MX_HIP_CHECK(hipStreamSynchronize(c->stream));
     if (c->ws) MX_HIP_CHECK(hipFree(c->ws));
@@ -59,11 +64,31 @@ hipEvent_t prof_event(mx_adv_ctx* c) {
 double* ws_take(mx_adv_ctx* c, size_t ndouble) {
     size_t off = (c->ws_off + 255) & ~(size_t)255;
     c->ws_off = off + ndouble * sizeof(double);
-    return reinterpret_cast<double*>(c->ws + off);
+    // a dry run only sizes the workspace: its views are never dereferenced
+    const uintptr_t base = c->dry ? 0 : reinterpret_cast<uintptr_t>(c->ext_ws ? c->ext_ws : c->ws);
+    return reinterpret_cast<double*>(base + off);
 }
 
 int Table::upload() {
     size_t bytes = c->htab.size();
+    if (c->dry) {
+        c->dry_tab += (bytes + 255) & ~(size_t)255;
+        base = nullptr;
+        return MX_OK;
+    }
+    if (c->ext_tab) {
+        // caller-given device memory: one region per table of the call, in
+        // order; the host bytes are kept until the stream has copied them
+        const size_t off = (c->ext_tab_off + 255) & ~(size_t)255;
+        if (off + bytes > c->ext_tab_cap) return MX_ERR_NOMEM;
+        base = c->ext_tab + off;
+        c->ext_tab_off = off + bytes;
+        if (bytes == 0) return MX_OK;
+        c->ext_host.push_back(c->htab);
+        MX_HIP_CHECK(hipMemcpyAsync(base, c->ext_host.back().data(), bytes, hipMemcpyHostToDevice,
+                                    c->stream));
+        return MX_OK;
+    }
     if (c->tab_mode == 2) {
         // replay under capture: the recorded table of this position (the
         // same call sequence over the same descriptors; the bytes are not
@@ -955,6 +980,8 @@ const char* mx_adv_error_string(int code) {
         case MX_ERR_NOMEM: return "workspace allocation failed";
         case MX_ERR_NOLIB: return "no usable GPU";
         case MX_ERR_IO: return "file could not be created or written";
+        case MX_ERR_SCRATCH:
+            return "the call needs more device memory than the no-alloc scratch (256 MiB) holds";
         default: return "unknown error";
     }
 }

@@ -107,6 +107,21 @@ struct mx_adv_ctx {
     int adj_pow2 = 1;         // adjoint: mask indexing when n is a power of two
     int adj_trace_rolled = 0; // adjoint: trace kernel with one direction per loop trip
 
+    // calls that allocate nothing (mx_adv_advance_pass_noalloc).  With `dry`
+    // set an entry point runs its host logic only: every launch is skipped,
+    // a table upload adds its size to dry_tab and a workspace request raises
+    // dry_ws.  With ext_tab / ext_ws set, tables and workspace are placed in
+    // that caller-given device memory instead of the context's own; the host
+    // bytes of each table stay in ext_host until the stream has passed the
+    // copies (the next such call synchronises first).
+    int dry = 0;
+    size_t dry_tab = 0, dry_ws = 0;
+    char* ext_tab = nullptr;
+    size_t ext_tab_cap = 0, ext_tab_off = 0;
+    char* ext_ws = nullptr;
+    size_t ext_ws_cap = 0;
+    std::vector<std::vector<char>> ext_host;
+
     // box tables under graph capture: 0 eager (one device table, copied at
     // every call); 1 record (eager, and keep each table's bytes in order);
     // 2 replay (each table is the next recorded one, already in rec_arena)
@@ -130,6 +145,7 @@ hipEvent_t prof_event(mx_adv_ctx* c);
 // launch on the context stream; with timing on, bracket it with events
 #define MX_LAUNCH(c, name, kern, grid, block, ...)                                  \
     do {                                                                            \
+        if ((c)->dry) break;                                                        \
         hipEvent_t e0_ = nullptr;                                                   \
         if ((c)->prof_on) {                                                         \
             e0_ = mxh::prof_event(c);                                               \

@@ -305,6 +305,14 @@ class Maestro:
         a = self.pass_args(f, keep)
         self._check(self.fns["advance_pass"](self.ctx, lev, C.byref(a)), "advance_pass")
 
+    def advance_pass_noalloc(self, lev, f):
+        """mx_adv_advance_pass_noalloc: the same pass with its tables and
+        workspace in the library's fixed device scratch (no allocation)."""
+        keep = []
+        a = self.pass_args(f, keep)
+        self._check(self.fns["advance_pass_noalloc"](self.ctx, lev, C.byref(a)),
+                    "advance_pass_noalloc")
+
     def _levels_args(self, fields):
         keep = []
         arr = (_abi.PassArgs * len(fields))(*[self.pass_args(f, keep) for f in fields])

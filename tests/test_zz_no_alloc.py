@@ -1,0 +1,82 @@
+"""The pass entry point that allocates nothing (mx_adv_advance_pass_noalloc)
+against mx_adv_advance_pass on identical inputs, through the C-ABI: two
+calls in a row on one GPU, the second on a smaller level (both share the
+library's fixed scratch), every output bit-exact; a level too large for the
+scratch is refused with MX_ERR_SCRATCH before anything is launched."""
+import numpy as np
+import pytest
+
+from harness import assert_bitexact, compare_face, compare_mf
+from maestroex_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+OUTS = ("utrans", "umac", "sedge", "sflux")
+
+
+def _pair(cfg):
+    import maestroex_amd as mx
+    from maestroex_amd.workloads import Workload
+    dev = mx.device_backend("cuda:0")
+    we = Workload(backend=dev, **cfg)
+    wn = Workload(backend=dev, **cfg)
+    return we, wn
+
+
+def _compare(we, wn, what):
+    comps = [0, 1] + list(range(4, we.nscal))
+    fe, fn = we.pass_fields(), wn.pass_fields()
+    assert_bitexact(compare_face(fe["utrans"], fn["utrans"], region="all"), f"{what} utrans")
+    assert_bitexact(compare_face(fe["umac"], fn["umac"], region="all"), f"{what} umac")
+    assert_bitexact(compare_face(fe["sedge"], fn["sedge"], comps=comps), f"{what} sedge")
+    assert_bitexact(compare_face(fe["sflux"], fn["sflux"], comps=comps), f"{what} sflux")
+    assert_bitexact(compare_mf(fe["etarhoflux"], fn["etarhoflux"], region="all"),
+                    f"{what} etarhoflux")
+
+
+def _run(cfg, what):
+    import maestroex_amd as mx
+    print(f"[no-alloc] {what}: {cfg}", flush=True)
+    we, wn = _pair(cfg)
+    me = mx.gpu_maestro(we.params, [we.boxes])
+    mn = mx.gpu_maestro(wn.params, [wn.boxes])
+    me.advance_pass(0, we.pass_fields())
+    mn.advance_pass_noalloc(0, wn.pass_fields())
+    me.sync()
+    mn.sync()
+    _compare(we, wn, what)
+    # the outputs are not trivially equal (the pass wrote them)
+    assert np.any(wn.pass_fields()["umac"][0].to_numpy()[0] != 0.0)
+    me.close()
+    mn.close()
+
+
+def test_noalloc_matches_pass_two_calls_second_smaller():
+    _run(dict(dim=3, n=48, max_grid=24, ppm_type=1, seed=5), "48^3 in 8 boxes, ppm 1")
+    _run(dict(dim=3, n=32, max_grid=32, ppm_type=2, seed=6), "32^3 in 1 box, ppm 2")
+
+
+def test_noalloc_spherical_and_2d():
+    _run(dict(dim=3, n=40, max_grid=20, ppm_type=2, spherical=True, seed=7),
+         "40^3 spherical, ppm 2")
+    _run(dict(dim=2, n=64, max_grid=32, ppm_type=1, seed=8), "2-D 64^2 in 4 boxes")
+
+
+def test_noalloc_rejects_a_level_beyond_the_scratch():
+    import maestroex_amd as mx
+    from maestroex_amd.workloads import Workload
+    # one 112^3 box: VelPred's workspace over the box grown by one is
+    # 114^3 * 192 B = 303 MB, beyond the 240 MiB workspace of the scratch
+    w = Workload(dim=3, n=112, max_grid=112, ppm_type=1, seed=9,
+                 backend=mx.device_backend("cuda:0"))
+    m = mx.gpu_maestro(w.params, [w.boxes])
+    with pytest.raises(_abi.MxError) as ei:
+        m.advance_pass_noalloc(0, w.pass_fields())
+    assert ei.value.code == _abi.MX_ERR_SCRATCH
+    m.sync()
+    # nothing was launched: the outputs keep their initial zeros
+    for name in OUTS:
+        for d in range(3):
+            for a in w.pass_fields()[name][d].to_numpy():
+                assert not np.any(a != 0.0), name
+    m.close()
