@@ -576,6 +576,58 @@ def test_scal_step_conserves_and_keeps_constants():
     assert np.max(np.abs(outc - 0.7)) <= 1e-14
 
 
+@pytest.mark.parametrize("ppm", [1, 2])
+def test_scal_step_equals_the_pass_stages(ppm):
+    """The differentiable step (oracle_step.cpp) is bit-identical to the
+    oracle's own stages on the same periodic box: MakeEdgeScal (advective,
+    no force) of the species, MakeRhoXFlux with species_pred_type 2
+    (F = umac (rho X)_edge) and UpdateScal of the species."""
+    from maestroex_amd.adjoint import periodic_mac_velocity
+    from maestroex_amd.fabs import Geometry, make_params
+    n, nspec, ng = 16, 3, 4
+    g = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 1], [0] * 3, [0] * 3)
+    dt, eps = 0.5 / n, 1e-8
+    params = make_params(g, ppm_type=ppm, nspec=nspec, species_pred_type=2, dt=dt,
+                         rel_eps=eps)
+    boxes = [((0, 0, 0), (n - 1, n - 1, n - 1))]
+    u = periodic_mac_velocity(n, seed=5)
+    rng = np.random.default_rng(9)
+    x = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    sp = np.stack([1.5 + 0.3 * np.sin(2 * np.pi * (X + 0.3 * c)) * np.cos(2 * np.pi * (Y - Z))
+                   + 0.05 * rng.random((n, n, n)) for c in range(nspec)])
+    first = params.first_spec
+    nscal = params.nscal
+    idx = np.arange(-ng, n + ng) % n
+    state = MultiFab(boxes, nscal, ng, 3, NP)
+    full = np.zeros((nscal, n, n, n))
+    full[first:first + nspec] = sp
+    full[params.comp_rho] = sp.sum(axis=0)
+    state.arrays[0][...] = full[:, idx][:, :, idx][:, :, :, idx]
+    umac = FaceMultiFab(boxes, 1, 1, 3, NP)
+    for d in range(3):
+        fidx = np.arange(-1, n + 2) % n
+        sl = [np.arange(-1, n + 1) % n] * 3
+        sl[2 - d] = fidx
+        umac[d].arrays[0][0] = u[d][sl[0]][:, sl[1]][:, :, sl[2]]
+    force = MultiFab(boxes, nscal, 1, 3, NP)
+    sedge = FaceMultiFab(boxes, nscal, 0, 3, NP)
+    sflux = FaceMultiFab(boxes, nscal, 0, 3, NP)
+    eta = MultiFab(boxes, 1, 1, 3, NP, nodal_dir=2)
+    snew = MultiFab(boxes, nscal, 0, 3, NP)
+    m = pyoracle.oracle_maestro(params, [boxes])
+    bs, _ = m.bc_setup()
+    m.MakeEdgeScal(0, state, sedge, umac, force, 0, bs, first, first, nspec, 0)
+    from maestroex_amd.fabs import BaseState
+    zero = BaseState(np.zeros((1, n + 1)), NP)
+    m.MakeRhoXFlux(0, sflux, eta, sedge, umac, zero, zero, None, zero, zero, None, zero, zero,
+                   first, nspec)
+    m.UpdateScal(0, state, snew, sflux, force, first, nspec)
+    got = snew.arrays[0][first:first + nspec]
+    want = pyoracle.scal_step(sp, u, dt, [1.0 / n] * 3, eps, ppm_type=ppm)
+    assert np.array_equal(got, want), float(np.max(np.abs(got - want)))
+
+
 def test_step_op_count():
     """The op-counting instantiation of the oracle's step: the phase counts
     are fixed by the formulas (predictor 9, corner 78, edge 75, flux and
