@@ -71,6 +71,8 @@ def kernel_bytes_per_cell(name, nc, dim):
         "k_es_edge": 8 * (nc * (2 * D + 6 + 1 + D) + D),
         # fused MakeEdgeScal: read s, force, umac (D); write sedge (D)
         "k_es_fused": 8 * (nc * (2 + D) + D),
+        # fused VelPred: read utilde, ufull, utrans, force (D each); write umac (D)
+        "k_vp_fused": 8 * (5 * D),
     }
     return table.get(name)
 
@@ -81,6 +83,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ppm", type=int, default=1)
+    ap.add_argument("--dim", type=int, default=3,
+                    help="2: the 2-D workload (256^2 as one box, periodic x, outflow y)")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--max-grid", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -160,8 +164,9 @@ def cpu_baseline(args, ppm=None):
     lib, _ = pyoracle.load(True)
     lib.mxo_alloc_ns.restype = C.c_longlong
     lib.mxo_alloc_ns.argtypes = [C.c_int]
-    n = args.cpu_n
-    w = Workload(dim=3, n=n, max_grid=n, ppm_type=ppm, backend=Backend("numpy"))
+    dim = getattr(args, "dim", 3)
+    n = args.cpu_n if dim == 3 else args.n
+    w = Workload(dim=dim, n=n, max_grid=n, ppm_type=ppm, backend=Backend("numpy"))
     m = pyoracle.oracle_maestro(w.params, [w.boxes], fast=True)
     m.advance_pass(0, w.pass_fields())  # warm-up (page-in, thread start)
     times, excl = [], []
@@ -175,16 +180,17 @@ def cpu_baseline(args, ppm=None):
     med = sorted(times)[len(times) // 2]
     med_x = sorted(excl)[len(excl) // 2]
     nproc, model = host_cpu()
+    cells = n ** dim
     return {
-        "value": round(n ** 3 / med / 1e9, 6),
+        "value": round(cells / med / 1e9, 6),
         "unit": "Gcell-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"one {n}^3 box of the same construction (4 ghosts, 5 scalars, ppm_type {ppm}), "
+        "sample": f"one {n}^{dim} box of the same construction (4 ghosts, 5 scalars, ppm_type {ppm}), "
                   f"median of {args.cpu_reps} passes after 1 warm-up, OpenMP {cores} threads "
                   f"(the host-thread share of one GPU), g++ -O3 -fopenmp -ffp-contract=off; "
                   f"{med:.3f} s per pass allocation-inclusive, {med_x:.3f} s allocation-exclusive",
-        "alloc_exclusive_value": round(n ** 3 / med_x / 1e9, 6),
+        "alloc_exclusive_value": round(cells / med_x / 1e9, 6),
         "host_nproc": nproc,
         "host_cpu": model,
     }
@@ -1008,6 +1014,12 @@ def main():
         workload = (f"3-D spherical star 384^3 FP64 in {len(w.boxes)} uneven boxes (edges 48-96), "
                     f"4 ghosts, 5 scalars + 3 velocities, radial w0 = 0.05 r (1 - r), ppm_type 2")
         bytes_per_cell = PASS_BYTES_PER_CELL_SPHERICAL
+    elif args.dim == 2:
+        args.max_grid = args.n
+        w = Workload(dim=2, n=args.n, max_grid=args.n, ppm_type=args.ppm, backend=dev)
+        workload = (f"2-D {args.n}^2 FP64 as one box, 4 ghosts, periodic x, outflow y, "
+                    f"5 scalars + 2 velocities, ppm_type {args.ppm}")
+        bytes_per_cell = PASS_BYTES_PER_CELL[2]
     else:
         w = Workload(dim=3, n=args.n, max_grid=args.max_grid, ppm_type=args.ppm, backend=dev)
         workload = (f"3-D {args.n}^3 FP64 as {len(w.boxes)} boxes of {args.max_grid}^3, "
@@ -1083,7 +1095,7 @@ def main():
         nc = 5.0 / 3.0
     else:
         nc = 1
-    bpc = kernel_bytes_per_cell(dname, nc, 3)
+    bpc = kernel_bytes_per_cell(dname, nc, args.dim)
     achieved = bpc * cells / (avg_ms * 1e-3) / 1e9 if bpc else None
     roofline = {
         "bound": "hbm",
@@ -1118,7 +1130,7 @@ def main():
     print(f"[bench] device copy {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
 
     extras = {}
-    if not args.headline_only and not args.spherical:
+    if not args.headline_only and not args.spherical and args.dim == 3:
         extras = rest_of_metric(args, w, fields)
 
     cpu = None

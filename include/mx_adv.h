@@ -236,7 +236,9 @@ int mx_adv_slope(mx_adv_ctx* ctx, int lev, int dir, const mx_box* bx,
 
 /* Maestro::MakeUtrans (Maestro.H:1172-1177, MaestroMakeUtrans.cpp:6-648),
    including its closing FillBoundary + FillUmacGhost.  w0_cart is the
-   cell-centred w0 (Maestro.H:1903); w0mac is read only when spherical. */
+   cell-centred w0 (Maestro.H:1903) with at least one ghost cell: the
+   vertical Riemann solve reads it at the face's index, the top face's
+   included, as the reference does; w0mac is read only when spherical. */
 int mx_adv_make_utrans(mx_adv_ctx* ctx, int lev, const mx_multifab* utilde,
                        const mx_multifab* ufull, mx_face_multifab* utrans,
                        const mx_face_multifab* w0mac,
@@ -248,7 +250,8 @@ int mx_adv_addw0(mx_adv_ctx* ctx, int lev, mx_face_multifab* uedge,
                  const mx_face_multifab* w0mac, const mx_basestate* w0,
                  double mult);
 
-/* Maestro::VelPred (Maestro.H:1711-1718, MaestroVelPred.cpp:5-292). */
+/* Maestro::VelPred (Maestro.H:1711-1718, MaestroVelPred.cpp:5-292).
+   w0_cart as for MakeUtrans (one ghost cell). */
 int mx_adv_velpred(mx_adv_ctx* ctx, int lev, const mx_multifab* utilde,
                    const mx_multifab* ufull, const mx_face_multifab* utrans,
                    mx_face_multifab* umac, const mx_face_multifab* w0mac,
@@ -365,11 +368,13 @@ int mx_adv_advance_pass(mx_adv_ctx* ctx, int lev, const mx_adv_pass_args* a);
    it is for callers that cannot allocate device memory during the call
    (code under HIP graph capture, tight serving loops, harnesses that forbid
    hipMalloc).  Results equal mx_adv_advance_pass on the same arguments.
-   The box tables and VelPred's temporaries live in a fixed device array of
-   the library, 256 MiB per GPU (16 MiB of tables, 240 MiB of workspace),
-   held from the first call on a GPU until the process ends.  A call whose
-   tables or workspace do not fit returns MX_ERR_SCRATCH before anything is
-   launched; the workspace is about 192 bytes per cell of the level's boxes
+   The box tables and the temporaries of the staged VelPred kernels live in
+   a fixed device array of the library, 256 MiB per GPU (16 MiB of tables,
+   240 MiB of workspace), held from the first call on a GPU until the
+   process ends.  A call whose tables or workspace do not fit returns
+   MX_ERR_SCRATCH before anything is launched.  The fused VelPred kernel (3-D,
+   ppm_type 1/2, forces not traced: the default there) needs no workspace;
+   the staged kernels need about 192 bytes per cell of the level's boxes
    grown by one (3-D), so levels up to about 1.3 million such cells fit.
    Tables go to the device by stream-ordered copies from host memory the
    context keeps until its next such call, which first waits for the context's
@@ -380,14 +385,22 @@ int mx_adv_advance_pass(mx_adv_ctx* ctx, int lev, const mx_adv_pass_args* a);
 int mx_adv_advance_pass_noalloc(mx_adv_ctx* ctx, int lev, const mx_adv_pass_args* a);
 
 /* ---- a whole level hierarchy (batched) ---------------------------------- */
-/* No single counterpart in the reference: the reference runs each stage over
-   levels 0..finest_level in turn (MaestroAdvection.cpp:7-84 loops inside every
-   function).  Here args[lev] holds level lev's fields, levels 0..nlev-1; each
-   level follows the single-level semantics of mx_adv_advance_pass (ghost
-   cells filled by the caller; FillBoundary within the level), so the result
-   on every level equals mx_adv_advance_pass on that level alone.  The scalar
-   edge states of all levels go to one fused launch per MakeEdgeScal call
-   (species, Rho, RhoH), over one box/tile work list. */
+/* The pass over the hierarchy levels 0..nlev-1 (args[lev] holds level lev's
+   fields), with the reference's multi-level semantics (finest_level =
+   nlev - 1; reflux_type 1, the default): each stage runs on every level, and
+   between the stages the face data are synchronised as the reference does --
+   utrans: AverageDownFaces + FillPatchUedge after MakeUtrans and after Addw0
+   (MaestroMakeUtrans.cpp:641-647, MaestroFill3dData.cpp:449-455); umac:
+   AverageDownFaces after VelPred (MaestroVelPred.cpp:291), AverageDownFaces +
+   FillPatchUedge after Addw0; sflux: AverageDownFaces after MakeRhoXFlux and
+   MakeRhoHFlux (MaestroMakeFlux.cpp:459-462, 1061-1064).  Cell-centred ghost
+   cells of the inputs are filled by the caller, as in the reference.  The
+   levels must be properly nested: every ghost face of a level > 0 box lies
+   over a coarse face of a coarser-level FAB (its valid faces or its one
+   ghost face), as AMReX's grid generation guarantees.  nlev == 1 is
+   mx_adv_advance_pass on level 0.  The scalar edge states of all levels go
+   to one fused launch per MakeEdgeScal call (species, Rho, RhoH), over one
+   box/tile work list. */
 int mx_adv_advance_pass_levels(mx_adv_ctx* ctx, int nlev, const mx_adv_pass_args* args);
 
 /* The same hierarchy pass captured once as a HIP graph and replayed with one
@@ -580,6 +593,17 @@ int mx_adv_average_down(mx_adv_ctx* ctx, int lev, const mx_multifab* fine, mx_mu
    fine boxes must hold one value in both (as after FillBoundary). */
 int mx_adv_average_down_faces(mx_adv_ctx* ctx, int lev, const mx_face_multifab* fine,
                               mx_face_multifab* crse);
+
+/* Maestro::FillPatchUedge (MaestroFillData.cpp:455-833) over levels
+   0..nlev-1 (uedge[lev] holds level lev's one-component face data, one
+   ghost face): on every level lev > 0, the ghost faces no box of the level
+   holds are filled from level lev - 1 (piecewise constant on the fine faces
+   of each coarse face, the level's own valid faces where they exist, and
+   linear interpolation in the normal direction between coarse-aligned
+   faces, piecewise constant across); then, on every level, FillBoundary
+   with periodicity and FillUmacGhost.  Coarse faces no coarse FAB holds read
+   as 1e200, as in the reference. */
+int mx_adv_fill_patch_uedge(mx_adv_ctx* ctx, int nlev, mx_face_multifab* uedge);
 
 /* ---- the update that closes the step ------------------------------------ */
 /* Maestro::UpdateScal (Maestro.H:306-311, MaestroAdvection.cpp:86-297) on

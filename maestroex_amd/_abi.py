@@ -230,6 +230,7 @@ SIGNATURES = {
     "fill_patch": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i, _i, P(BCRec), _i],
     "average_down": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i],
     "average_down_faces": [_vp, _i, P(FaceMultiFabDesc), P(FaceMultiFabDesc)],
+    "fill_patch_uedge": [_vp, _i, P(FaceMultiFabDesc)],
     "convert_rhox_to_x": [_vp, _i, P(MultiFabDesc), _i],
     "convert_rhoh_to_h": [_vp, _i, P(MultiFabDesc), _i],
     "update_scal": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc),

@@ -509,7 +509,7 @@ const mx_adv_ctx::GhostPlan& ghost_plan(mx_adv_ctx* c, int lev, int ndir, const 
 
 namespace mxh {
 
-int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge) {
+int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge, bool extrapolate) {
     const int dim = c->p.dim;
     const auto& grids = c->grids[lev];
     const int block = 256;
@@ -550,6 +550,11 @@ int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge) {
                 const bool bounded = q < dim && !c->p.is_periodic[q];
                 sd.plo[q] = bounded ? c->p.domain_lo[lev][q] : -(1 << 30);
                 sd.phi[q] = bounded ? c->p.domain_hi[lev][q] + (q == d ? 1 : 0) : (1 << 30);
+                if (!extrapolate) {
+                    // an empty region: every uncovered face is left as it is
+                    sd.plo[q] = 1 << 30;
+                    sd.phi[q] = -(1 << 30);
+                }
             }
             maxpts = std::max(maxpts, npts(r));
             slabs.push_back(sd);
@@ -640,7 +645,7 @@ __global__ void k_addw0(const AddTask* __restrict__ tasks, const double* __restr
 namespace mxh {
 
 int addw0(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge, const mx_face_multifab* w0mac,
-          const mx_basestate* w0, double mult) {
+          const mx_basestate* w0, double mult, bool fill) {
     const int dim = c->p.dim;
     const auto& grids = c->grids[lev];
     std::vector<AddTask> tasks;
@@ -689,7 +694,7 @@ int addw0(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge, const mx_face_m
     MX_LAUNCH(c, "k_addw0", k_addw0,  grid_for(maxpts, (int)tasks.size(), 1, block), dim3(block), tab.dev<AddTask>(off), c->p.spherical ? nullptr : w0->p,
                        c->p.spherical ? 0LL : (long long)lev * w0->len, mult, c->p.spherical);
 
-    return fill_face_ghosts(c, lev, uedge);
+    return fill ? fill_face_ghosts(c, lev, uedge) : MX_OK;
 }
 
 }  // namespace mxh
@@ -851,6 +856,12 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
     } else if (k == "velpred_ppm_all") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->vp_ppm_all = (int)value;
+    } else if (k == "velpred_fused") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->vp_fused = (int)value;
+    } else if (k == "velpred_fused_tz") {
+        if (value < 1 || value > 4096) return MX_ERR_ARG;
+        c->vp_fused_tz = (int)value;
     } else if (k == "xcd_remap") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->xcd_remap = (int)value;

@@ -280,6 +280,41 @@ __device__ __forceinline__ void ppm_trace(double s0, double sm, double sp, doubl
     }
 }
 
+// PPM type 1 face value between cells sa (left) and sb (right) from the two
+// cells' van Leer slopes (MaestroPPM.cpp:86-99: sm of the right cell, which
+// equals sp of the left cell)
+__device__ __forceinline__ double face1(double sa, double sb, double dla, double dlb) {
+    double e = 0.5 * (sb + sa) - (dlb - dla) / 6.0;
+    e = dmax(e, dmin(sb, sa));
+    e = dmin(e, dmax(sb, sa));
+    return e;
+}
+
+// does the cell at `pos` take a PPM boundary stencil?
+template <int PPM>
+__device__ __forceinline__ bool ppm_special(int pos, int dlo, int dhi, int bclo, int bchi) {
+    constexpr int R = PPM == 2 ? 2 : 1;
+    return (is_ho(bclo) && pos >= dlo && pos <= dlo + R) ||
+           (is_ho(bchi) && pos <= dhi && pos >= dhi - R);
+}
+
+// the characteristic trace of ppm_trace with sigma of each face given
+__device__ __forceinline__ void trace_sig(double s0, double sm, double sp, double u_hi,
+                                          double u_lo, double sig_hi, double sig_lo,
+                                          double rel_eps, double& Ip, double& Im) {
+    double s6 = 6.0 * s0 - 3.0 * (sm + sp);
+    if (u_hi > rel_eps) {
+        Ip = sp - 0.5 * sig_hi * (sp - sm - (1.0 - 2.0 / 3.0 * sig_hi) * s6);
+    } else {
+        Ip = s0;
+    }
+    if (u_lo < -rel_eps) {
+        Im = sm + 0.5 * sig_lo * (sp - sm + (1.0 - 2.0 / 3.0 * sig_lo) * s6);
+    } else {
+        Im = s0;
+    }
+}
+
 // load the 9-point stencil along a direction
 template <int PPM>
 __device__ __forceinline__ void load_stencil(const double* q, long long st, double* v) {

@@ -99,6 +99,8 @@ struct mx_adv_ctx {
     int xcd_remap = 0;  // KGeom::xcd of the staged pass kernels
     int vp_ppm_all = 1;  // VelPred's PPM kernel: 1 = all components per thread
     int vp_trans_all = 1;  // VelPred's transverse kernel: 1 = all six states per thread
+    int vp_fused = 1;      // VelPred: 1 = the fused z-marching kernel where it applies
+    int vp_fused_tz = 32;  // planes per chunk of the fused VelPred kernel
     int fused_tile = -1;    // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8,
                             // 4: 16 x 11; -1: 16 x 11 for ppm_type 1, else 16 x 12
     int fused_per_box = 0;  // 1: one fused launch per box (comparison mode)
@@ -308,12 +310,37 @@ inline dim3 grid_for_g(const mxd::KGeom& g, long long maxpts, int nbox, int nz, 
 }
 
 // stages shared between entry points
-int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge);
+// extrapolate: ghost faces no box of the level covers take the value of the
+// box's nearest valid face (single-level calls on a finer level); off after
+// the coarse-fine fill of FillPatchUedge, which has set them
+int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge,
+                     bool extrapolate = true);
+// fill: FillBoundary + FillUmacGhost afterwards (finest_level == 0); a
+// hierarchy pass synchronises the levels instead
 int addw0(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge, const mx_face_multifab* w0mac,
-          const mx_basestate* w0, double mult);
+          const mx_basestate* w0, double mult, bool fill = true);
+// Maestro::FillPatchUedge over levels 0..nlev-1 (uedge[lev] per level)
+int fill_patch_uedge(mx_adv_ctx* c, int nlev, const mx_face_multifab* uedge);
+// the workspace VelPred's temporaries need on one level
+size_t velpred_ws_bytes(const mx_adv_ctx* c, int lev);
+int check_phys(const mx_adv_ctx* c);
 int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
                 const mx_face_multifab* utrans, const mx_face_multifab* w0mac,
-                const mx_multifab* w0_cart);
+                const mx_multifab* w0_cart, bool fill = true);
+// fused 3-D VelPred (ppm_type 1/2, forces not traced): the boxes of every
+// listed level in one launch
+struct VpLevel {
+    int lev;
+    const mx_multifab* utilde;
+    const mx_multifab* ufull;
+    const mx_face_multifab* utrans;
+    const mx_face_multifab* umac;
+    const mx_face_multifab* w0mac;
+    const mx_multifab* force;
+    const mx_multifab* w0_cart;
+};
+bool velpred_fused_ok(const mx_adv_ctx* c);
+int velpred_fused(mx_adv_ctx* c, const std::vector<VpLevel>& levs);
 int velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
             const mx_face_multifab* utrans, const mx_face_multifab* umac,
             const mx_face_multifab* w0mac, const mx_multifab* force, const mx_multifab* w0_cart);
@@ -331,8 +358,8 @@ struct VpPlan {
 int velpred_setup(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
                   const mx_face_multifab* utrans, const mx_face_multifab* umac,
                   const mx_face_multifab* w0mac, const mx_multifab* force,
-                  const mx_multifab* w0_cart, VpPlan& plan);
-int utrans_from_plan(mx_adv_ctx* c, const VpPlan& plan);
+                  const mx_multifab* w0_cart, VpPlan& plan, bool reserve = true);
+int utrans_from_plan(mx_adv_ctx* c, const VpPlan& plan, bool fill = true);
 int velpred_finish(mx_adv_ctx* c, const VpPlan& plan);
 int make_edge_scal(mx_adv_ctx* c, int lev, const mx_multifab* state,
                    const mx_face_multifab* sedge, const mx_face_multifab* umac,

@@ -913,7 +913,7 @@ int pass_check(mx_adv_ctx* c, int lev, const mx_adv_pass_args* a) {
     if ((rc = check_mf(c, lev, &a->utilde, dim, g, -1))) return rc;
     if ((rc = check_mf(c, lev, &a->ufull, dim, 1, -1))) return rc;
     if ((rc = check_mf(c, lev, &a->vel_force, dim, fg, -1))) return rc;
-    if ((rc = check_mf(c, lev, &a->w0_cart, dim, 0, -1))) return rc;
+    if ((rc = check_mf(c, lev, &a->w0_cart, dim, sph ? 0 : 1, -1))) return rc;
     if ((rc = check_mf(c, lev, &a->scal, p.nscal, g, -1))) return rc;
     if ((rc = check_mf(c, lev, &a->scal_force, p.nscal, fg, -1))) return rc;
     if ((rc = check_face(c, lev, &a->utrans, 1, 1))) return rc;
@@ -943,19 +943,79 @@ int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_arg
     const bool sph = p.dim == 3 && p.spherical;
     const mx_bcrec* bs = c->bcs_s.data();
     const int nbs = (int)c->bcs_s.size();
-    for (int i = 0; i < nlev; ++i) {
+    // a hierarchy (levels 0..nlev-1, nlev > 1) synchronises its face data
+    // between levels; a single level fills its ghost faces on its own
+    const bool multi = nlev > 1;
+    if (multi && lev0 != 0) return MX_ERR_ARG;
+    std::vector<VpPlan> vps(nlev);
+    // the fused VelPred kernel keeps its temporaries on chip: MakeUtrans
+    // reconstructs its own normal states and VelPred follows in one launch
+    const bool vfused = velpred_fused_ok(c);
+    std::vector<VpLevel> vlevs;
+    for (int i = 0; i < nlev && vfused; ++i) {
+        const mx_adv_pass_args* A = &a[i];
+        vlevs.push_back({lev0 + i, &A->utilde, &A->ufull, &A->utrans, &A->umac, &A->w0mac,
+                         &A->vel_force, &A->w0_cart});
+    }
+    if (multi && !vfused) {
+        // every level's VelPred temporaries stay live until the levels'
+        // utrans are synchronised
+        size_t need = 0;
+        for (int i = 0; i < nlev; ++i) need += velpred_ws_bytes(c, lev0 + i);
+        if ((rc = ws_reserve(c, need))) return rc;
+    }
+    for (int i = 0; i < nlev && vfused; ++i) {
+        const int lev = lev0 + i;
+        const mx_adv_pass_args* A = &a[i];
+        if ((rc = make_utrans(c, lev, &A->utilde, &A->ufull, &A->utrans, &A->w0mac, &A->w0_cart,
+                              !multi)))
+            return rc;
+        if ((rc = addw0(c, lev, &A->utrans, &A->w0mac, &A->w0, 1.0, !multi))) return rc;
+        if (multi) continue;
+        std::vector<VpLevel> one(1, vlevs[i]);
+        if ((rc = velpred_fused(c, one))) return rc;
+        if ((rc = addw0(c, lev, &A->umac, &A->w0mac, &A->w0, 1.0))) return rc;
+    }
+    for (int i = 0; i < nlev && !vfused; ++i) {
         const int lev = lev0 + i;
         const mx_adv_pass_args* A = &a[i];
         // VelPred's reconstruction runs first and MakeUtrans reads its traced
         // normal-direction states (the numbers MakeUtrans would compute)
-        VpPlan vp;
+        VpPlan& vp = vps[i];
         if ((rc = velpred_setup(c, lev, &A->utilde, &A->ufull, &A->utrans, &A->umac, &A->w0mac,
-                                &A->vel_force, &A->w0_cart, vp)))
+                                &A->vel_force, &A->w0_cart, vp, !multi)))
             return rc;
-        if ((rc = utrans_from_plan(c, vp))) return rc;
-        if ((rc = addw0(c, lev, &A->utrans, &A->w0mac, &A->w0, 1.0))) return rc;
+        if ((rc = utrans_from_plan(c, vp, !multi))) return rc;
+        if ((rc = addw0(c, lev, &A->utrans, &A->w0mac, &A->w0, 1.0, !multi))) return rc;
+        if (multi) continue;
         if ((rc = velpred_finish(c, vp))) return rc;
         if ((rc = addw0(c, lev, &A->umac, &A->w0mac, &A->w0, 1.0))) return rc;
+    }
+    // AverageDownFaces over the hierarchy, then FillPatchUedge
+    auto sync_levels = [&](mx_face_multifab mx_adv_pass_args::*field, bool fill) -> int {
+        std::vector<mx_face_multifab> f(nlev);
+        for (int i = 0; i < nlev; ++i) f[i] = a[i].*field;
+        for (int lev = nlev - 2; lev >= 0; --lev) {
+            const int r = mx_adv_average_down_faces(c, lev, &f[lev + 1], &f[lev]);
+            if (r) return r;
+        }
+        return fill ? fill_patch_uedge(c, nlev, f.data()) : MX_OK;
+    };
+    if (multi) {
+        // The reference synchronises utrans after MakeUtrans and again after
+        // Addw0 (MaestroMakeUtrans.cpp:641-647, MaestroFill3dData.cpp:449-455),
+        // and umac after VelPred and again after Addw0 (MaestroVelPred.cpp:291).
+        // Both syncs read valid faces only and the second rewrites every face
+        // the first one wrote, so one sync after the w0 addition gives the
+        // same values.
+        if ((rc = sync_levels(&mx_adv_pass_args::utrans, true))) return rc;
+        if (vfused && (rc = velpred_fused(c, vlevs))) return rc;
+        for (int i = 0; i < nlev; ++i) {
+            const mx_adv_pass_args* A = &a[i];
+            if (!vfused && (rc = velpred_finish(c, vps[i]))) return rc;
+            if ((rc = addw0(c, lev0 + i, &A->umac, &A->w0mac, &A->w0, 1.0, false))) return rc;
+        }
+        if ((rc = sync_levels(&mx_adv_pass_args::umac, true))) return rc;
     }
     const int groups[3][2] = {{p.first_spec, p.num_spec}, {p.comp_rho, 1}, {p.comp_rhoh, 1}};
     if (fused_edge_scal_ok(c, 0)) {
@@ -987,6 +1047,11 @@ int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_arg
                                  rh0m, &A->rhoh0, &A->rhoh0_edge, rh0m, h0m, h0m)))
             return rc;
     }
+    // reflux_type 1: AverageDownFaces(sflux) after MakeRhoXFlux and after
+    // MakeRhoHFlux (MaestroMakeFlux.cpp:459-462, 1061-1064).  MakeRhoHFlux
+    // writes only the RhoH component and reads no flux, so one restriction
+    // of every component at the end gives the same values.
+    if (multi && (rc = sync_levels(&mx_adv_pass_args::sflux, false))) return rc;
     return MX_OK;
 }
 

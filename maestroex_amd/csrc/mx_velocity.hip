@@ -660,6 +660,10 @@ __global__ void __launch_bounds__(256) k_utrans_pp(const VpBox* __restrict__ box
         utrans_pp_face<PPM, 2>(B, g, i, j, k);
 }
 
+}  // namespace
+
+namespace mxh {
+
 int check_phys(const mx_adv_ctx* c) {
     for (int d = 0; d < c->p.dim; ++d) {
         if (c->p.phys_bc_lo[d] < MX_PHYS_INTERIOR || c->p.phys_bc_lo[d] > MX_PHYS_NOSLIPWALL)
@@ -670,13 +674,9 @@ int check_phys(const mx_adv_ctx* c) {
     return MX_OK;
 }
 
-}  // namespace
-
-namespace mxh {
-
 int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
                 const mx_face_multifab* utrans, const mx_face_multifab* w0mac,
-                const mx_multifab* w0_cart) {
+                const mx_multifab* w0_cart, bool fill) {
     const int dim = c->p.dim;
     const auto& grids = c->grids[lev];
     const bool sph = dim == 3 && c->p.spherical;
@@ -721,13 +721,13 @@ int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mult
     else
         MX_LAUNCH(c, "k_utrans", k_utrans<2>,  gr, dim3(block), dp, kg, A);
 
-    return fill_face_ghosts(c, lev, utrans);
+    return fill ? fill_face_ghosts(c, lev, utrans) : MX_OK;
 }
 
 int velpred_setup(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
                   const mx_face_multifab* utrans, const mx_face_multifab* umac,
                   const mx_face_multifab* w0mac, const mx_multifab* force,
-                  const mx_multifab* w0_cart, VpPlan& plan) {
+                  const mx_multifab* w0_cart, VpPlan& plan, bool reserve) {
     const int dim = c->p.dim;
     const auto& grids = c->grids[lev];
     const bool sph = dim == 3 && c->p.spherical;
@@ -738,12 +738,12 @@ int velpred_setup(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mu
     if (rc) return rc;
 
     auto grown1 = [&](HBox b) { return grow(b, dim, 1); };
-    // workspace: Ip/Im (or slopes) per component, traced forces, transverse states
-    size_t need = 2 * dim * tmp_bytes(grids, dim, grown1);
-    if (tf) need += 2 * dim * tmp_bytes(grids, dim, grown1);
-    if (dim == 3) need += 6 * tmp_bytes(grids, 1, grown1);
-    rc = ws_reserve(c, need);
-    if (rc) return rc;
+    // reserve = false: the caller reserved the workspace of several levels
+    // and each level takes its temporaries in turn
+    if (reserve) {
+        rc = ws_reserve(c, velpred_ws_bytes(c, lev));
+        if (rc) return rc;
+    }
 
     std::vector<DA> Ip[3], Im[3], Ipf[3], Imf[3], T[6];
     for (int n = 0; n < dim; ++n) {
@@ -819,8 +819,22 @@ int velpred_setup(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mu
     return MX_OK;
 }
 
+// VelPred's temporaries on one level: Ip/Im (or slopes) per component,
+// traced forces, transverse states, each over the boxes grown by one
+size_t velpred_ws_bytes(const mx_adv_ctx* c, int lev) {
+    const int dim = c->p.dim;
+    const auto& grids = c->grids[lev];
+    const bool tf = c->p.ppm_trace_forces == 1 && c->p.ppm_type != 0;
+    auto grown1 = [&](HBox b) { return grow(b, dim, 1); };
+    size_t need = 2 * dim * tmp_bytes(grids, dim, grown1);
+    if (tf) need += 2 * dim * tmp_bytes(grids, dim, grown1);
+    if (dim == 3) need += 6 * tmp_bytes(grids, 1, grown1);
+    return need;
+}
+
 // MakeUtrans from the traced states of velpred_setup, then the face ghosts
-int utrans_from_plan(mx_adv_ctx* c, const VpPlan& plan) {
+// (fill = false: the caller synchronises the levels instead)
+int utrans_from_plan(mx_adv_ctx* c, const VpPlan& plan, bool fill) {
     const KGeom kg = kgeom(c, plan.lev);
     const int block = 256;
     const dim3 gr = grid_for_g(kg, plan.maxpts, plan.nb, c->p.dim, block);
@@ -828,7 +842,7 @@ int utrans_from_plan(mx_adv_ctx* c, const VpPlan& plan) {
         MX_LAUNCH(c, "k_utrans", k_utrans_pp<0>, gr, dim3(block), (const VpBox*)plan.dp, kg);
     else
         MX_LAUNCH(c, "k_utrans", k_utrans_pp<1>, gr, dim3(block), (const VpBox*)plan.dp, kg);
-    return fill_face_ghosts(c, plan.lev, plan.utrans);
+    return fill ? fill_face_ghosts(c, plan.lev, plan.utrans) : MX_OK;
 }
 
 // VelPred after its PPM kernel: transverse states and MAC velocities
@@ -850,6 +864,11 @@ int velpred_finish(mx_adv_ctx* c, const VpPlan& plan) {
 int velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_multifab* ufull,
             const mx_face_multifab* utrans, const mx_face_multifab* umac,
             const mx_face_multifab* w0mac, const mx_multifab* force, const mx_multifab* w0_cart) {
+    if (velpred_fused_ok(c)) {
+        std::vector<VpLevel> levs(1);
+        levs[0] = {lev, utilde, ufull, utrans, umac, w0mac, force, w0_cart};
+        return velpred_fused(c, levs);
+    }
     VpPlan plan;
     int rc = velpred_setup(c, lev, utilde, ufull, utrans, umac, w0mac, force, w0_cart, plan);
     if (rc) return rc;
@@ -873,7 +892,7 @@ int mx_adv_make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde,
     if (dim == 3 && c->p.spherical) {
         if ((rc = check_face(c, lev, w0mac, 1, 0))) return rc;
     } else {
-        if ((rc = check_mf(c, lev, w0_cart, dim, 0, -1))) return rc;
+        if ((rc = check_mf(c, lev, w0_cart, dim, 1, -1))) return rc;
     }
     return make_utrans(c, lev, utilde, ufull, utrans, w0mac, w0_cart);
 }
@@ -895,7 +914,7 @@ int mx_adv_velpred(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_m
     if (dim == 3 && c->p.spherical) {
         if ((rc = check_face(c, lev, w0mac, 1, 0))) return rc;
     } else {
-        if ((rc = check_mf(c, lev, w0_cart, dim, 0, -1))) return rc;
+        if ((rc = check_mf(c, lev, w0_cart, dim, 1, -1))) return rc;
     }
     return velpred(c, lev, utilde, ufull, utrans, umac, w0mac, force, w0_cart);
 }

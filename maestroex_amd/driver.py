@@ -212,6 +212,13 @@ class Maestro:
         self._check(self.fns["average_down_faces"](self.ctx, lev, *[C.byref(x) for x in a]),
                     "average_down_faces")
 
+    def FillPatchUedge(self, uedge):
+        """Maestro::FillPatchUedge (MaestroFillData.cpp:455-833): uedge[lev]
+        is level lev's FaceMultiFab (one component, one ghost face)."""
+        descs = [_face(f) for f in uedge]
+        arr = (_abi.FaceMultiFabDesc * len(descs))(*descs)
+        self._check(self.fns["fill_patch_uedge"](self.ctx, len(descs), arr), "fill_patch_uedge")
+
     def AverageDown(self, mfs, comp, ncomp):
         """Maestro::AverageDown (MaestroFillData.cpp:145-154): mfs[lev] per level."""
         for lev in range(len(mfs) - 2, -1, -1):

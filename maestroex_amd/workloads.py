@@ -382,15 +382,43 @@ def _random_boxes(rng, n, count, lo, hi, region_frac=1.0, align=1, tries=200000)
     return boxes
 
 
+def _nested_boxes(rng, parents, count, lo, hi, ratio=2, align=2, buf=2, tries=200000):
+    """`count` non-overlapping boxes with random edges lo..hi (multiples of
+    `align`, at even corners), each inside one refined parent box shrunk by
+    `buf` fine cells."""
+    occ = []
+    boxes = []
+    t = 0
+    while len(boxes) < count and t < tries:
+        t += 1
+        plo, phi = parents[int(rng.integers(0, len(parents)))]
+        room = [ratio * (phi[d] - plo[d] + 1) - 2 * buf for d in range(3)]
+        top = [min(hi, room[d]) // align for d in range(3)]
+        if any(top[d] * align < lo for d in range(3)):
+            continue
+        e = [align * int(rng.integers(-(-lo // align), top[d] + 1)) for d in range(3)]
+        a = [ratio * plo[d] + buf + align * int(rng.integers(0, (room[d] - e[d]) // align + 1))
+             for d in range(3)]
+        b = [a[d] + e[d] - 1 for d in range(3)]
+        if any(all(a[d] <= q[1][d] and q[0][d] <= b[d] for d in range(3)) for q in occ):
+            continue
+        occ.append((a, b))
+        boxes.append((tuple(a), tuple(b)))
+    return boxes
+
+
 class Hierarchy:
     """An uneven AMR hierarchy with the field construction of Workload,
     evaluated per box (no full-level arrays), refinement ratio 2.
 
     level 0: n0^3 split into boxes whose edges are lo0..hi0 along each axis;
-    level l > 0: `counts[l-1]` non-overlapping boxes with random even edges
-    `edges[l-1]` at even corners inside the first `fracs[l-1]` of the level's
-    index space (each coarsens onto whole cells of level l - 1; boxes are not
-    nested in the coarser level's boxes).
+    level 1: `counts[0]` non-overlapping boxes with random even edges
+    `edges[0]` at even corners inside the first `fracs[0]` of the level's
+    index space (each coarsens onto whole cells of level 0);
+    level l > 1: `counts[l-1]` non-overlapping boxes with random even edges
+    `edges[l-1]` at even corners, each inside one box of level l - 1 with at
+    least one coarse cell of that box around it (proper nesting: every ghost
+    face of the level lies over a coarser-level face).
     Periodic in x and y, outflow in z; w0 = 0.01 z on every level; one dt
     (CFL 0.5 on the finest level) for all levels.  Every level follows the
     single-level semantics of the pass: ghost cells hold the analytic fields
@@ -414,8 +442,12 @@ class Hierarchy:
               for x in segs[0]]
         self.boxes = [b0]
         for l in range(1, nlev):
-            self.boxes.append(_random_boxes(rng, self.ns[l], counts[l - 1], *edges[l - 1],
-                                            region_frac=fracs[l - 1], align=2))
+            if l == 1:
+                self.boxes.append(_random_boxes(rng, self.ns[l], counts[l - 1], *edges[l - 1],
+                                                region_frac=fracs[l - 1], align=2))
+            else:
+                self.boxes.append(_nested_boxes(rng, self.boxes[l - 1], counts[l - 1],
+                                                *edges[l - 1]))
         geoms = [Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 0],
                           [_abi.PHYS_INTERIOR, _abi.PHYS_INTERIOR, _abi.PHYS_OUTFLOW],
                           [_abi.PHYS_INTERIOR, _abi.PHYS_INTERIOR, _abi.PHYS_OUTFLOW])

@@ -118,7 +118,7 @@ void BCSetup(const Ctx& c, std::vector<mx_bcrec>& bcs_s,
 // level, which the reference fills by FillPatch from the coarser level) takes
 // the value of the box's nearest valid face.
 void FillBoundaryFace(const Ctx& c, int lev, const std::vector<Arr>& f,
-                      int ndir) {
+                      int ndir, bool extrapolate) {
     const int dim = c.dim();
     const std::vector<IBox>& grids = c.grids[lev];
     const IBox dom = c.domain(lev);
@@ -180,6 +180,7 @@ void FillBoundaryFace(const Ctx& c, int lev, const std::vector<Arr>& f,
         for (int k = fabbox.lo[2]; k <= fabbox.hi[2]; ++k)
             for (int j = fabbox.lo[1]; j <= fabbox.hi[1]; ++j)
                 for (int i = fabbox.lo[0]; i <= fabbox.hi[0]; ++i) {
+                    if (!extrapolate) continue;
                     if (contains(valid, i, j, k) || covered[at(i, j, k)]) continue;
                     const int q[3] = {i, j, k};
                     bool phys = false;
@@ -403,9 +404,12 @@ void FillUmacGhost(const Ctx& c, int lev, const std::vector<Arr> uedge[3]) {
 }
 
 // FillBoundary of each face MultiFab then FillUmacGhost, as MakeUtrans and
-// Addw0 do when finest_level == 0 (MaestroMakeUtrans.cpp:631-640).
-void FillFaceGhosts(const Ctx& c, int lev, const std::vector<Arr> uedge[3]) {
-    for (int d = 0; d < c.dim(); ++d) FillBoundaryFace(c, lev, uedge[d], d);
+// Addw0 do when finest_level == 0 (MaestroMakeUtrans.cpp:631-640), and as
+// FillPatchUedge does on every level after its coarse-fine fill
+// (MaestroFillData.cpp:824-831; there without the extrapolation of faces no
+// box covers, which that fill has set).
+void FillFaceGhosts(const Ctx& c, int lev, const std::vector<Arr> uedge[3], bool extrapolate) {
+    for (int d = 0; d < c.dim(); ++d) FillBoundaryFace(c, lev, uedge[d], d, extrapolate);
     FillUmacGhost(c, lev, uedge);
 }
 
@@ -414,7 +418,7 @@ void FillFaceGhosts(const Ctx& c, int lev, const std::vector<Arr> uedge[3]) {
 int MakeUtrans(const Ctx& c, int lev, const std::vector<Arr>& utilde,
                const std::vector<Arr>& ufull, const std::vector<Arr> utrans[3],
                const std::vector<Arr> w0mac[3], const std::vector<Arr>& w0_cart,
-               const std::vector<mx_bcrec>& bcs_u) {
+               const std::vector<mx_bcrec>& bcs_u, bool fill_ghosts) {
     const int dim = c.dim();
     const IBox dom = c.domain(lev);
     const double* dx = c.p.dx[lev];
@@ -550,14 +554,16 @@ int MakeUtrans(const Ctx& c, int lev, const std::vector<Arr>& utilde,
         }
     }
 
-    FillFaceGhosts(c, lev, utrans);
+    // finest_level == 0; with finer levels the caller runs AverageDownFaces
+    // and FillPatchUedge over the hierarchy instead (:641-647)
+    if (fill_ghosts) FillFaceGhosts(c, lev, utrans, true);
     return MX_OK;
 }
 
 // ---------------------------------------------------------------------------
 // Maestro::Addw0 (MaestroFill3dData.cpp:357-461), one level.
 int Addw0(const Ctx& c, int lev, const std::vector<Arr> uedge[3],
-          const std::vector<Arr> w0mac[3], const BaseArr& w0, double mult) {
+          const std::vector<Arr> w0mac[3], const BaseArr& w0, double mult, bool fill_ghosts) {
     const int dim = c.dim();
     const std::vector<IBox>& grids = c.grids[lev];
     for (size_t b = 0; b < grids.size(); ++b) {
@@ -581,7 +587,9 @@ int Addw0(const Ctx& c, int lev, const std::vector<Arr> uedge[3],
             }
         }
     }
-    FillFaceGhosts(c, lev, uedge);
+    // finest_level == 0 (MaestroFill3dData.cpp:424-448); with finer levels
+    // the caller runs AverageDownFaces and FillPatchUedge (:449-455)
+    if (fill_ghosts) FillFaceGhosts(c, lev, uedge, true);
     return MX_OK;
 }
 

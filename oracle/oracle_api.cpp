@@ -26,13 +26,17 @@ void Slope(int dir, int slope_order, const IBox& bx, const Arr& s,
            int bc_start_comp);
 void BCSetup(const Ctx& c, std::vector<mx_bcrec>& bcs_s,
              std::vector<mx_bcrec>& bcs_u);
-void FillFaceGhosts(const Ctx& c, int lev, const std::vector<Arr> uedge[3]);
+void FillFaceGhosts(const Ctx& c, int lev, const std::vector<Arr> uedge[3],
+                    bool extrapolate = true);
 int MakeUtrans(const Ctx& c, int lev, const std::vector<Arr>& utilde,
                const std::vector<Arr>& ufull, const std::vector<Arr> utrans[3],
                const std::vector<Arr> w0mac[3], const std::vector<Arr>& w0_cart,
-               const std::vector<mx_bcrec>& bcs_u);
+               const std::vector<mx_bcrec>& bcs_u, bool fill_ghosts = true);
 int Addw0(const Ctx& c, int lev, const std::vector<Arr> uedge[3],
-          const std::vector<Arr> w0mac[3], const BaseArr& w0, double mult);
+          const std::vector<Arr> w0mac[3], const BaseArr& w0, double mult,
+          bool fill_ghosts = true);
+int FillPatchUedgeCF(const Ctx& c, int lev, const std::vector<Arr> crse[3],
+                     const std::vector<Arr> fine[3]);
 int VelPred(const Ctx& c, int lev, const std::vector<Arr>& utilde,
             const std::vector<Arr>& ufull, const std::vector<Arr> utrans[3],
             const std::vector<Arr> umac[3], const std::vector<Arr> w0mac[3],
@@ -444,6 +448,140 @@ int mxo_advance_pass(Ctx* c, int lev, const mx_adv_pass_args* a) {
                       base(&a->rhoh0_edge), rh0mac.d, base(&a->rhoh0), base(&a->rhoh0_edge),
                       rh0mac.d, h0mac.d, h0mac.d);
     return finish(rc);
+}
+
+// The pass over levels 0..nlev-1 (mx_adv_advance_pass_levels), in the
+// reference's order with finest_level = nlev - 1 > 0: every stage loops over
+// the levels, and the face data are synchronised between levels where the
+// reference does it (reflux_type 1, the default, _cpp_parameters:24):
+//   MakeUtrans; AverageDownFaces(utrans); FillPatchUedge(utrans)
+//     (MaestroMakeUtrans.cpp:641-647)
+//   Addw0(utrans, +1); AverageDownFaces; FillPatchUedge (MaestroFill3dData.cpp:449-455)
+//   VelPred; AverageDownFaces(umac) (MaestroVelPred.cpp:291)
+//   Addw0(umac, +1); AverageDownFaces; FillPatchUedge
+//   MakeEdgeScal x 3 (scalars: no restriction, MaestroMakeEdgeScalars.cpp:294-301)
+//   MakeRhoXFlux; AverageDownFaces(sflux) (MaestroMakeFlux.cpp:459-462)
+//   MakeRhoHFlux; AverageDownFaces(sflux) (MaestroMakeFlux.cpp:1061-1064)
+// nlev == 1 is the single-level pass.
+int mxo_advance_pass_levels(Ctx* c, int nlev, const mx_adv_pass_args* a) {
+    if (!c || !a || nlev < 1 || nlev - 1 > c->p.finest_level) return MX_ERR_ARG;
+    if (nlev == 1) return mxo_advance_pass(c, 0, a);
+    for (int lev = 0; lev < nlev; ++lev) {
+        int rc = check_level(c, lev);
+        if (rc) return rc;
+    }
+    std::vector<mx_bcrec> bs, bu;
+    BCSetup(*c, bs, bu);
+    const mx_adv_params& p = c->p;
+    const bool sph = p.dim == 3 && p.spherical;
+    std::vector<Face> utrans(nlev), umac(nlev), sedge(nlev), sflux(nlev), w0mac(nlev), r0mac(nlev),
+        rh0mac(nlev), h0mac(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        utrans[l] = faces(&a[l].utrans);
+        umac[l] = faces(&a[l].umac);
+        sedge[l] = faces(&a[l].sedge);
+        sflux[l] = faces(&a[l].sflux);
+        w0mac[l] = faces(&a[l].w0mac);
+        if (sph) {
+            r0mac[l] = faces(&a[l].r0mac);
+            rh0mac[l] = faces(&a[l].rh0mac);
+            h0mac[l] = faces(&a[l].h0mac);
+        }
+    }
+    auto average_down_faces = [&](std::vector<Face>& f) {
+        for (int lev = nlev - 2; lev >= 0; --lev) AverageDownFaces(*c, lev, f[lev + 1].d, f[lev].d);
+    };
+    // Maestro::FillPatchUedge (MaestroFillData.cpp:455-833)
+    auto fill_patch_uedge = [&](std::vector<Face>& f) {
+        for (int lev = 0; lev < nlev; ++lev) {
+            if (lev > 0) FillPatchUedgeCF(*c, lev, f[lev - 1].d, f[lev].d);
+            FillFaceGhosts(*c, lev, f[lev].d, false);
+        }
+    };
+    (void)fill_patch_uedge;
+    int rc;
+    for (int lev = 0; lev < nlev; ++lev) {
+        rc = MakeUtrans(*c, lev, arrs(&a[lev].utilde), arrs(&a[lev].ufull), utrans[lev].d,
+                        w0mac[lev].d, arrs(&a[lev].w0_cart), bu, false);
+        if (rc) return finish(rc);
+    }
+    average_down_faces(utrans);
+    fill_patch_uedge(utrans);
+    for (int lev = 0; lev < nlev; ++lev) {
+        rc = Addw0(*c, lev, utrans[lev].d, w0mac[lev].d, base(&a[lev].w0), 1.0, false);
+        if (rc) return finish(rc);
+    }
+    average_down_faces(utrans);
+    fill_patch_uedge(utrans);
+    for (int lev = 0; lev < nlev; ++lev) {
+        rc = VelPred(*c, lev, arrs(&a[lev].utilde), arrs(&a[lev].ufull), utrans[lev].d,
+                     umac[lev].d, w0mac[lev].d, arrs(&a[lev].vel_force), arrs(&a[lev].w0_cart), bu);
+        if (rc) return finish(rc);
+    }
+    average_down_faces(umac);
+    for (int lev = 0; lev < nlev; ++lev) {
+        rc = Addw0(*c, lev, umac[lev].d, w0mac[lev].d, base(&a[lev].w0), 1.0, false);
+        if (rc) return finish(rc);
+    }
+    average_down_faces(umac);
+    fill_patch_uedge(umac);
+    const int groups[3][2] = {{p.first_spec, p.num_spec}, {p.comp_rho, 1}, {p.comp_rhoh, 1}};
+    for (const auto& gr : groups)
+        for (int lev = 0; lev < nlev; ++lev) {
+            rc = MakeEdgeScal(*c, lev, arrs(&a[lev].scal), sedge[lev].d, umac[lev].d,
+                              arrs(&a[lev].scal_force), false, bs.data(), gr[0], gr[0], gr[1],
+                              false);
+            if (rc) return finish(rc);
+        }
+    for (int lev = 0; lev < nlev; ++lev) {
+        auto eta = arrs(&a[lev].etarhoflux);
+        for (size_t b = 0; b < eta.size(); ++b) {
+            const Arr& e = eta[b];
+            IBox whole;
+            for (int d = 0; d < 3; ++d) {
+                whole.lo[d] = e.b[d];
+                whole.hi[d] = e.e[d] - 1;
+            }
+            par_for(whole, [&](int i, int j, int k) { e(i, j, k) = 0.0; });
+        }
+    }
+    for (int lev = 0; lev < nlev; ++lev) {
+        const mx_adv_pass_args* A = &a[lev];
+        rc = MakeRhoXFlux(*c, lev, sflux[lev].d, arrs(&A->etarhoflux), sedge[lev].d, umac[lev].d,
+                          base(&A->rho0), base(&A->rho0_edge), r0mac[lev].d, base(&A->rho0),
+                          base(&A->rho0_edge), r0mac[lev].d, base(&A->rho0_predicted_edge),
+                          base(&A->w0), p.first_spec, p.num_spec);
+        if (rc) return finish(rc);
+    }
+    average_down_faces(sflux);
+    for (int lev = 0; lev < nlev; ++lev) {
+        const mx_adv_pass_args* A = &a[lev];
+        rc = MakeRhoHFlux(*c, lev, sflux[lev].d, sedge[lev].d, umac[lev].d, base(&A->rho0),
+                          base(&A->rho0_edge), r0mac[lev].d, base(&A->rho0), base(&A->rho0_edge),
+                          r0mac[lev].d, base(&A->rhoh0), base(&A->rhoh0_edge), rh0mac[lev].d,
+                          base(&A->rhoh0), base(&A->rhoh0_edge), rh0mac[lev].d, h0mac[lev].d,
+                          h0mac[lev].d);
+        if (rc) return finish(rc);
+    }
+    average_down_faces(sflux);
+    return finish(MX_OK);
+}
+
+// Maestro::FillPatchUedge (MaestroFillData.cpp:455-833) over levels
+// 0..nlev-1; uedge[lev] holds level lev's face data (one component)
+int mxo_fill_patch_uedge(Ctx* c, int nlev, mx_face_multifab* uedge) {
+    if (!c || !uedge || nlev < 1 || nlev - 1 > c->p.finest_level) return MX_ERR_ARG;
+    std::vector<Face> f(nlev);
+    for (int lev = 0; lev < nlev; ++lev) {
+        int rc = check_level(c, lev);
+        if (rc) return rc;
+        f[lev] = faces(&uedge[lev]);
+    }
+    for (int lev = 0; lev < nlev; ++lev) {
+        if (lev > 0) FillPatchUedgeCF(*c, lev, f[lev - 1].d, f[lev].d);
+        FillFaceGhosts(*c, lev, f[lev].d, nlev == 1);
+    }
+    return finish(MX_OK);
 }
 
 int mxo_update_scal(Ctx* c, int lev, const mx_multifab* stateold, mx_multifab* statenew,

@@ -27,6 +27,10 @@ def load(fast=False):
             raise FileNotFoundError(f"oracle library not built: {path} (run make -C oracle)")
         lib = C.CDLL(path)
         fns = _abi.bind(lib, "mxo_", names=[n for n in _abi.SIGNATURES if n not in _abi.NO_ORACLE])
+        lv = lib.mxo_advance_pass_levels
+        lv.argtypes = _abi.LIB_ONLY["advance_pass_levels"]
+        lv.restype = C.c_int
+        fns["advance_pass_levels"] = lv
         lib.mxo_omp_threads.argtypes = [C.c_int]
         lib.mxo_omp_threads.restype = C.c_int
         fns["omp_threads"] = lib.mxo_omp_threads

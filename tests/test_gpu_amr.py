@@ -1,9 +1,11 @@
 """GPU parity of the batched hierarchy pass: every box of an uneven AMR
-hierarchy (three levels, refinement ratio 2) in one pass call, eagerly, as
-one graph launch (replayed several times with other device work between the
-replays, outputs cleared before each), with one fused launch per box, and
-after the autotuner changed the launch shape -- all bit-exact against the
-oracle's level-by-level pass."""
+hierarchy (three levels, refinement ratio 2, properly nested) in one pass
+call, eagerly, as one graph launch (replayed several times with other device
+work between the replays, outputs cleared before each), with one fused
+launch per box, and after the autotuner changed the launch shape -- all
+bit-exact against the oracle's hierarchy pass, which runs the reference's
+stages over all levels with AverageDownFaces and FillPatchUedge between
+them.  FillPatchUedge alone is compared on random face data."""
 import numpy as np
 import pytest
 
@@ -20,8 +22,7 @@ OUTS = ("utrans", "umac", "sedge", "sflux")
 def oracle_hierarchy(ppm):
     h = Hierarchy(ppm_type=ppm, backend=Backend("numpy"))
     m = pyoracle.oracle_maestro(h.params, h.boxes)
-    for lev in range(h.nlev):
-        m.advance_pass(lev, h.pass_fields(lev))
+    m.advance_pass_levels([h.pass_fields(lev) for lev in range(h.nlev)])
     return h
 
 
@@ -130,8 +131,7 @@ def test_graph_replay_follows_set_dt(nlev):
     for scale in (0.5, 0.75, 0.75, 1.0):
         dt, rel_eps = hg.dt * scale, hg.rel_eps * (1.0 + scale)
         mo.set_dt(dt, rel_eps)
-        for lev in range(nlev):
-            mo.advance_pass(lev, hc.pass_fields(lev))
+        mo.advance_pass_levels([hc.pass_fields(lev) for lev in range(nlev)])
         m.set_dt(dt, rel_eps)
         clear_outputs(hg)
         scratch.add_(1.0)
@@ -147,3 +147,36 @@ def test_graph_replay_follows_set_dt(nlev):
                                              [0, 1] + list(range(4, hc.nscal))),
                                 f"dt x{scale} lev {lev} {name}")
     g.close()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_fill_patch_uedge_parity(seed):
+    """FillPatchUedge over the three-level hierarchy on random face data
+    (valid faces and ghosts), every face compared bit for bit."""
+    import maestroex_amd as mx
+    from maestroex_amd.fabs import FaceMultiFab
+    print(f"[parity] FillPatchUedge seed={seed}", flush=True)
+    h = Hierarchy(ppm_type=1, backend=Backend("numpy"))
+    rng = np.random.default_rng(seed)
+    host = []
+    for lev in range(h.nlev):
+        f = FaceMultiFab(h.boxes[lev], 1, 1, 3, Backend("numpy"))
+        for d in range(3):
+            for b in range(len(h.boxes[lev])):
+                f[d].arrays[b][...] = rng.standard_normal(f[d].arrays[b].shape)
+        host.append(f)
+    dev = []
+    bk = mx.device_backend()
+    for lev in range(h.nlev):
+        f = FaceMultiFab(h.boxes[lev], 1, 1, 3, bk)
+        for d in range(3):
+            for b in range(len(h.boxes[lev])):
+                f[d].arrays[b] = bk.from_numpy(host[lev][d].arrays[b])
+        dev.append(f)
+    mo = pyoracle.oracle_maestro(h.params, h.boxes)
+    mo.FillPatchUedge(host)
+    mg = mx.gpu_maestro(h.params, h.boxes)
+    mg.FillPatchUedge(dev)
+    mg.sync()
+    for lev in range(h.nlev):
+        assert_bitexact(compare_face(host[lev], dev[lev], region="all"), f"lev {lev}")

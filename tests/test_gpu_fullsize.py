@@ -138,8 +138,7 @@ def test_config2_amr_hierarchy():
     nth = _oracle_threads()
     t0 = time.perf_counter()
     mo = pyoracle.oracle_maestro(hc.params, hc.boxes, fast=True)
-    for lev in range(hc.nlev):
-        mo.advance_pass(lev, hc.pass_fields(lev))
+    mo.advance_pass_levels([hc.pass_fields(lev) for lev in range(hc.nlev)])
     _log(f"config 2: oracle pass {time.perf_counter() - t0:.1f}s on {nth} threads")
     mg = mx.gpu_maestro(hg.params, hg.boxes)
     fields = [hg.pass_fields(l) for l in range(hg.nlev)]

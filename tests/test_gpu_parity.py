@@ -136,6 +136,7 @@ def test_pass_velpred_ppm_per_component(name, cfg):
     wc, wg = pair(**cfg)
     _run_pass(oracle_for(wc), wc)
     m = gpu_for(wg)
+    m.set_tuning("velpred_fused", 0)
     m.set_tuning("velpred_ppm_all", 0)
     _run_pass(m, wg)
     assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
@@ -154,6 +155,7 @@ def test_pass_velpred_trans_per_pair(name, cfg):
     wc, wg = pair(**cfg)
     _run_pass(oracle_for(wc), wc)
     m = gpu_for(wg)
+    m.set_tuning("velpred_fused", 0)
     m.set_tuning("velpred_trans_all", 0)
     _run_pass(m, wg)
     assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
@@ -161,6 +163,31 @@ def test_pass_velpred_trans_per_pair(name, cfg):
     comps = [0, 1] + list(range(4, wc.nscal))
     assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} sedge")
     assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
+
+
+VP_FUSED_CASES = VP_ALL_CASES + [CASES[11], CASES[17], KERNEL_CHOICE_CASES[3]]
+
+
+@pytest.mark.parametrize("fused,tz", [(0, 32), (1, 1), (1, 5), (1, 32)])
+@pytest.mark.parametrize("name,cfg", VP_FUSED_CASES, ids=[c[0] for c in VP_FUSED_CASES])
+def test_pass_velpred_fused(name, cfg, fused, tz):
+    """VelPred as the fused z-marching kernel (the default for 3-D
+    ppm_type 1/2 without traced forces) with z-chunks of 1, 5 and 32 planes,
+    and as the staged kernels: the whole pass bit-exact, walls, inflow,
+    spherical w0mac and partial tiles included."""
+    print(f"[parity] velpred fused={fused} tz={tz} {name}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    m.set_tuning("velpred_fused", fused)
+    m.set_tuning("velpred_fused_tz", tz)
+    _run_pass(m, wg)
+    assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
+    assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} sedge")
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
+    assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} etarhoflux")
 
 
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])

@@ -34,12 +34,15 @@ def _compare(we, wn, what):
                     f"{what} etarhoflux")
 
 
-def _run(cfg, what):
+def _run(cfg, what, staged=False):
     import maestroex_amd as mx
     print(f"[no-alloc] {what}: {cfg}", flush=True)
     we, wn = _pair(cfg)
     me = mx.gpu_maestro(we.params, [we.boxes])
     mn = mx.gpu_maestro(wn.params, [wn.boxes])
+    if staged:
+        me.set_tuning("velpred_fused", 0)
+        mn.set_tuning("velpred_fused", 0)
     me.advance_pass(0, we.pass_fields())
     mn.advance_pass_noalloc(0, wn.pass_fields())
     me.sync()
@@ -56,6 +59,14 @@ def test_noalloc_matches_pass_two_calls_second_smaller():
     _run(dict(dim=3, n=32, max_grid=32, ppm_type=2, seed=6), "32^3 in 1 box, ppm 2")
 
 
+def test_noalloc_with_staged_velpred():
+    """The staged VelPred kernels take their temporaries from the scratch."""
+    _run(dict(dim=3, n=40, max_grid=20, ppm_type=1, seed=10), "40^3 staged VelPred",
+         staged=True)
+    _run(dict(dim=3, n=24, max_grid=24, ppm_type=2, seed=11), "24^3 staged VelPred",
+         staged=True)
+
+
 def test_noalloc_spherical_and_2d():
     _run(dict(dim=3, n=40, max_grid=20, ppm_type=2, spherical=True, seed=7),
          "40^3 spherical, ppm 2")
@@ -65,11 +76,13 @@ def test_noalloc_spherical_and_2d():
 def test_noalloc_rejects_a_level_beyond_the_scratch():
     import maestroex_amd as mx
     from maestroex_amd.workloads import Workload
-    # one 112^3 box: VelPred's workspace over the box grown by one is
-    # 114^3 * 192 B = 303 MB, beyond the 240 MiB workspace of the scratch
+    # one 112^3 box with the staged VelPred kernels: their workspace over the
+    # box grown by one is 114^3 * 192 B = 303 MB, beyond the 240 MiB
+    # workspace of the scratch (the fused VelPred kernel needs none)
     w = Workload(dim=3, n=112, max_grid=112, ppm_type=1, seed=9,
                  backend=mx.device_backend("cuda:0"))
     m = mx.gpu_maestro(w.params, [w.boxes])
+    m.set_tuning("velpred_fused", 0)
     with pytest.raises(_abi.MxError) as ei:
         m.advance_pass_noalloc(0, w.pass_fields())
     assert ei.value.code == _abi.MX_ERR_SCRATCH
