@@ -70,6 +70,7 @@ struct FTile {
 
 struct FComp {
     int bclo[3], bchi[3];
+    int comp;  // state / edge-state component
 };
 
 struct FArgs {
@@ -207,7 +208,7 @@ __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_
     const bool last_z = (z1 == bhi2);
 
     const int vc = blockIdx.z;
-    const int comp = A.start_scomp + vc;
+    const int comp = comps[vc].comp;
     const int bcx0 = comps[vc].bclo[0], bcx1 = comps[vc].bchi[0];
     const int bcy0 = comps[vc].bclo[1], bcy1 = comps[vc].bchi[1];
     const int bcz0 = comps[vc].bclo[2], bcz1 = comps[vc].bchi[2];
@@ -788,16 +789,17 @@ int launch_one(mx_adv_ctx* c, const FBox* db, const FTile* dtl, size_t ntile, co
 
 template <int TX, int TY, int WPE>
 int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int is_vel,
-                 const mx_bcrec* bcs, int start_scomp, int start_bccomp, int num_comp,
-                 int is_conservative) {
+                 const mx_bcrec* bcs, const std::vector<int>& scomps,
+                 const std::vector<int>& bccomps, int is_conservative) {
+    const int num_comp = (int)scomps.size();
     const long long lim = 0x7fffffffLL;
     std::vector<FBox> boxes;
     std::vector<FTile> tiles, btiles;  // x/y-free tiles, then the others
     bool act[2][2] = {{false, false}, {false, false}};
     for (int vc = 0; vc < num_comp; ++vc)
         for (int d = 0; d < 2; ++d) {
-            act[d][0] = act[d][0] || bc_acts(bcs[start_bccomp + vc].lo[d]);
-            act[d][1] = act[d][1] || bc_acts(bcs[start_bccomp + vc].hi[d]);
+            act[d][0] = act[d][0] || bc_acts(bcs[bccomps[vc]].lo[d]);
+            act[d][1] = act[d][1] || bc_acts(bcs[bccomps[vc]].hi[d]);
         }
     for (const EdgeLevel& L : levs) {
         const auto& grids = c->grids[L.lev];
@@ -846,13 +848,15 @@ int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int 
     if (tiles.empty()) return MX_OK;
     if (tiles.size() > 0x7fffffffULL) return MX_ERR_UNSUPPORTED;
     std::vector<FComp> comps(num_comp);
-    for (int vc = 0; vc < num_comp; ++vc)
+    for (int vc = 0; vc < num_comp; ++vc) {
         for (int d = 0; d < 3; ++d) {
-            comps[vc].bclo[d] = bcs[start_bccomp + vc].lo[d];
-            comps[vc].bchi[d] = bcs[start_bccomp + vc].hi[d];
+            comps[vc].bclo[d] = bcs[bccomps[vc]].lo[d];
+            comps[vc].bchi[d] = bcs[bccomps[vc]].hi[d];
         }
+        comps[vc].comp = scomps[vc];
+    }
     FArgs A;
-    A.start_scomp = start_scomp;
+    A.start_scomp = scomps[0];
     A.num_comp = num_comp;
     A.is_vel = is_vel;
     A.tz = tz;
@@ -887,34 +891,24 @@ int fused_tile_of(const mx_adv_ctx* c) {
     return c->p.ppm_type == 1 ? 4 : 2;
 }
 int launch_tile(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int is_vel,
-                const mx_bcrec* bcs, int start_scomp, int start_bccomp, int num_comp,
-                int is_conservative) {
+                const mx_bcrec* bcs, const std::vector<int>& sc, const std::vector<int>& bc,
+                int cons) {
     switch (fused_tile_of(c)) {
-        case 1:
-            return launch_fused<16, 16, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
-                                           num_comp, is_conservative);
-        case 2:
-            return launch_fused<16, 12, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
-                                           num_comp, is_conservative);
-        case 3:
-            return launch_fused<16, 8, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
-                                          num_comp, is_conservative);
-        case 4:
-            return launch_fused<16, 11, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
-                                           num_comp, is_conservative);
-
-        default:
-            return launch_fused<32, 8, 3>(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp,
-                                          num_comp, is_conservative);
+        case 1: return launch_fused<16, 16, 3>(c, levs, tz, is_vel, bcs, sc, bc, cons);
+        case 2: return launch_fused<16, 12, 3>(c, levs, tz, is_vel, bcs, sc, bc, cons);
+        case 3: return launch_fused<16, 8, 3>(c, levs, tz, is_vel, bcs, sc, bc, cons);
+        case 4: return launch_fused<16, 11, 3>(c, levs, tz, is_vel, bcs, sc, bc, cons);
+        default: return launch_fused<32, 8, 3>(c, levs, tz, is_vel, bcs, sc, bc, cons);
     }
 }
 
 }  // namespace
 
-int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int is_vel,
-                                const mx_bcrec* bcs, int start_scomp, int start_bccomp,
-                                int num_comp, int is_conservative) {
-    if (levs.empty()) return MX_OK;
+int make_edge_scal_fused_list(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int is_vel,
+                              const mx_bcrec* bcs, const std::vector<int>& scomps,
+                              const std::vector<int>& bccomps, int is_conservative) {
+    if (levs.empty() || scomps.empty()) return MX_OK;
+    if (scomps.size() != bccomps.size()) return MX_ERR_ARG;
     const int tz = c->fused_tz > 0 ? c->fused_tz : 32;
     if (c->fused_per_box) {
         // comparison mode: the same work as one launch per box
@@ -923,14 +917,23 @@ int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& lev
             for (size_t b = 0; b < c->grids[L.lev].size(); ++b) {
                 one[0] = L;
                 one[0].box = (int)b;
-                int rc = launch_tile(c, one, tz, is_vel, bcs, start_scomp, start_bccomp,
-                                     num_comp, is_conservative);
+                int rc = launch_tile(c, one, tz, is_vel, bcs, scomps, bccomps, is_conservative);
                 if (rc) return rc;
             }
         return MX_OK;
     }
-    return launch_tile(c, levs, tz, is_vel, bcs, start_scomp, start_bccomp, num_comp,
-                       is_conservative);
+    return launch_tile(c, levs, tz, is_vel, bcs, scomps, bccomps, is_conservative);
+}
+
+int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int is_vel,
+                                const mx_bcrec* bcs, int start_scomp, int start_bccomp,
+                                int num_comp, int is_conservative) {
+    std::vector<int> sc(num_comp), bc(num_comp);
+    for (int vc = 0; vc < num_comp; ++vc) {
+        sc[vc] = start_scomp + vc;
+        bc[vc] = start_bccomp + vc;
+    }
+    return make_edge_scal_fused_list(c, levs, is_vel, bcs, sc, bc, is_conservative);
 }
 
 int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,

@@ -101,6 +101,10 @@ struct mx_adv_ctx {
     int vp_trans_all = 1;  // VelPred's transverse kernel: 1 = all six states per thread
     int vp_fused = 1;      // VelPred: 1 = the fused z-marching kernel where it applies
     int vp_fused_tz = 32;  // planes per chunk of the fused VelPred kernel
+    int vp_fused_comp = 1; // fused VelPred: 1 = one velocity component per block
+    int utrans_line = 0;   // MakeUtrans (3-D, ppm_type 1/2): 1 = along lines of faces
+    int utrans_tz = 32;    // faces per chunk of a line
+    int vp_fused_tile = 0; // its tile (waves per SIMD): 0 16x11 (3), 1 16x8 (3), 2 16x11 (4), 3 32x8 (3)
     int fused_tile = -1;    // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8,
                             // 4: 16 x 11; -1: 16 x 11 for ppm_type 1, else 16 x 12
     int fused_per_box = 0;  // 1: one fused launch per box (comparison mode)
@@ -396,6 +400,11 @@ bool fused_edge_scal_ok(const mx_adv_ctx* c, int is_conservative);
 int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int is_vel,
                                 const mx_bcrec* bcs, int start_scomp, int start_bccomp,
                                 int num_comp, int is_conservative);
+// the same over a list of components (state component, BCRec index), all in
+// one launch
+int make_edge_scal_fused_list(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int is_vel,
+                              const mx_bcrec* bcs, const std::vector<int>& scomps,
+                              const std::vector<int>& bccomps, int is_conservative);
 int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
                          const mx_face_multifab* sedge, const mx_face_multifab* umac,
                          const mx_multifab* force, int is_vel, const mx_bcrec* bcs,

@@ -1022,9 +1022,12 @@ int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_arg
         std::vector<EdgeLevel> levs;
         for (int i = 0; i < nlev; ++i)
             levs.push_back({lev0 + i, &a[i].scal, &a[i].sedge, &a[i].umac, &a[i].scal_force});
+        // MakeEdgeScal's three calls (species, Rho, RhoH) read the same
+        // inputs and write disjoint components: one launch covers them all
+        std::vector<int> sc;
         for (const auto& gr : groups)
-            if ((rc = make_edge_scal_fused_levels(c, levs, 0, bs, gr[0], gr[0], gr[1], 0)))
-                return rc;
+            for (int n = 0; n < gr[1]; ++n) sc.push_back(gr[0] + n);
+        if ((rc = make_edge_scal_fused_list(c, levs, 0, bs, sc, sc, 0))) return rc;
     } else {
         for (int i = 0; i < nlev; ++i)
             for (const auto& gr : groups)

@@ -784,6 +784,521 @@ __global__ void __launch_bounds__(vf_nt(TX, TY)) __attribute__((amdgpu_waves_per
     }
 }
 
+// ---------------------------------------------------------------------------
+// One velocity component per block.  umac on N-faces depends on component N
+// alone: its interface states iface(N, N), the transverse states of
+// component N (pairs (T, 3 - N - T)), their corrections (uimh(T', N)) and
+// the N-component force.  So the blocks of blockIdx.z = N compute the chain
+// of component N only and write umac_N: the column state of one component
+// instead of three (the staged kernels' work, split by output).
+//   N = 0: pairs uimhyz (y-faces of plane q - 1, by z) and uimhzy (z-face q,
+//          by y); umac on the x-faces of plane q - 1;
+//   N = 1: pairs vimhxz (x-faces of plane q - 1, by z) and vimhzx (z-face q,
+//          by x); vmac on the y-faces of plane q - 1;
+//   N = 2: pairs wimhxy (x-faces of plane q, by y) and wimhyx (y-faces of
+//          plane q, by x); wmac on z-face q.
+// the block's LDS, one set for the three component bodies
+template <int PPM, int TX, int TY>
+struct VPShared {
+    static constexpr int G = PPM == 2 ? 3 : 2;
+    static constexpr int W = TX + 2;
+    static constexpr int H = TY + 2;
+    static constexpr int NE = W * H;
+    static constexpr int SW = W + 2 * G;
+    static constexpr int SH = H + 2 * G;
+    static constexpr int FO = PPM == 2 ? 1 : 0;
+    static constexpr int FXW = W + 1 + 2 * FO;
+    static constexpr int FYH = H + 1 + 2 * FO;
+    static constexpr int NF = FXW * H > W * FYH ? FXW * H : W * FYH;
+    double S[SW * SH];
+    double UL[3][(W + 1) * H];
+    double VL[3][W * (H + 1)];
+    double FX[NF];    // x-face values; then uimh(0, N); then transverse states
+    double FY[NF];    // y-face values; then uimh(1, N); then transverse states
+    double P[4][NE];  // Ipx, Imx, Ipy, Imy; then final corrections
+    double Q[2][NE];  // transverse corrections across cells
+    double SC[NE];    // plane q - 1 cell values (inflow overrides)
+    double FL[NE];    // plane q - 1 force (component N)
+};
+
+template <int PPM, int TX, int TY, bool XYF, int N>
+__device__ __forceinline__ void vp_comp(const VFBox& B, const VFTile& tl, const KGeom& g,
+                                        const VFArgs& A, VPShared<PPM, TX, TY>& sh) {
+    constexpr int NT = vf_nt(TX, TY);
+    constexpr int G = PPM == 2 ? 3 : 2;
+    constexpr int W = TX + 2;
+    constexpr int H = TY + 2;
+    constexpr int NE = W * H;
+    constexpr int SW = W + 2 * G;
+    constexpr int SH = H + 2 * G;
+    constexpr int UW = W + 1;
+    constexpr int VH = H + 1;
+    constexpr int FO = PPM == 2 ? 1 : 0;
+    constexpr int FXW = W + 1 + 2 * FO;
+    constexpr int FYH = H + 1 + 2 * FO;
+    constexpr int NF = FXW * H > W * FYH ? FXW * H : W * FYH;
+
+    static_assert(NF == VPShared<PPM, TX, TY>::NF, "LDS layout");
+    double* S = sh.S;
+    auto& UL = sh.UL;
+    auto& VL = sh.VL;
+    double* FX = sh.FX;
+    double* FY = sh.FY;
+    auto& P = sh.P;
+    auto& Q = sh.Q;
+    double* SC = sh.SC;
+    double* FL = sh.FL;
+
+    const int bhi0 = B.hi[0], bhi1 = B.hi[1], bhi2 = B.hi[2];
+    const int X0 = B.lo[0] + tl.tx * TX;
+    const int Y0 = B.lo[1] + tl.ty * TY;
+    const int nx = min(TX, bhi0 - X0 + 1);
+    const int ny = min(TY, bhi1 - Y0 + 1);
+    const int z0 = B.lo[2] + tl.tzc * A.tz;
+    const int z1 = min(z0 + A.tz - 1, bhi2);
+    const bool last_x = (X0 + nx - 1 == bhi0);
+    const bool last_y = (Y0 + ny - 1 == bhi1);
+    const bool last_z = (z1 == bhi2);
+    const int dlx = B.domlo[0], dhx = B.domhi[0];
+    const int dly = B.domlo[1], dhy = B.domhi[1];
+    const int dlz = B.domlo[2], dhz = B.domhi[2];
+    const int pxl = A.plo[0], pxh = A.phi[0], pyl = A.plo[1], pyh = A.phi[1];
+    const int pzl = A.plo[2], pzh = A.phi[2];
+    const bool sph = A.spherical != 0;
+    const int bcx0 = A.bclo[N][0], bcx1 = A.bchi[N][0];
+    const int bcy0 = A.bclo[N][1], bcy1 = A.bchi[N][1];
+    const int bcz0 = A.bclo[N][2], bcz1 = A.bchi[N][2];
+
+    const DA utv = B.ut, ufv = B.uf, fov = B.fo;
+    const DA trx = B.tr[0], tryv = B.tr[1], trz = B.tr[2];
+    const DA umo = B.um[N];
+    const int sjs = (int)utv.js, sks = (int)utv.ks;
+    const int xjs = (int)trx.js, xks = (int)trx.ks;
+    const int yjs = (int)tryv.js, yks = (int)tryv.ks;
+
+    const int tid = threadIdx.x;
+    const int a = tid % W;  // local column: cell (X0 + a - 1, Y0 + b - 1)
+    const int b = tid / W;
+    const bool col = (tid < NE) && (a <= nx + 1) && (b <= ny + 1);
+    const int I = X0 + a - 1;
+    const int J = Y0 + b - 1;
+    const int si = (a + G) + (b + G) * SW;
+    const int ui = a + b * UW;
+    const int vi = a + b * W;
+
+    const double dt = g.dt;
+    const double rel_eps = g.rel_eps;
+    const double dt2 = 0.5 * dt;
+    const double dt4 = 0.25 * dt;
+    const double dt6 = dt / 6.0;
+    const double hx = B.dx[0], hy = B.dx[1], hz = B.dx[2];
+    const double t6x = dt6 / hx, t6y = dt6 / hy, t6z = dt6 / hz;
+    const double t4x = dt4 / hx, t4y = dt4 / hy, t4z = dt4 / hz;
+
+    gcd* sPl = (gcd*)utv.p + utv.off(X0 - 1 - G, Y0 - 1 - G, z0 - 1, N);
+    gcd* xPl = (gcd*)trx.p + trx.off(X0 - 1, Y0 - 1, z0 - 1);
+    gcd* yPl = (gcd*)tryv.p + tryv.off(X0 - 1, Y0 - 1, z0 - 1);
+    gcd* sCol = (gcd*)utv.p;
+    gcd* ufCol = (gcd*)ufv.p;
+    gcd* foCol = (gcd*)fov.p;
+    gcd* zCol = (gcd*)trz.p;
+    gd* oCol = (gd*)umo.p;
+    if (col) {
+        sCol += utv.off(I, J, z0 - 1 + G, N);
+        ufCol += ufv.off(I, J, z0 - 1);
+        foCol += fov.off(I, J, z0 - 1, N);
+        zCol += trz.off(I, J, z0 - 1);
+        oCol += umo.off(I, J, z0 - 1);
+    }
+    const int ufks = (int)ufv.ks, ufns = (int)ufv.ns;
+    const int foks = (int)fov.ks, zks = (int)trz.ks, oks = (int)umo.ks;
+
+    double v[9];
+#pragma unroll
+    for (int jj = 0; jj < 9; ++jj) v[jj] = 0.0;
+    double zf0 = 0.0, zf1 = 0.0, zf2 = 0.0, zsl = 0.0;
+    double Ipz_prev = 0.0;
+    double zl = 0.0, zr = 0.0;          // iface(2, N) on z-face q
+    double uz_q = 0.0, uz_p = 0.0;      // uimh(2, N) on z-faces q and q - 1 (N < 2)
+    double xl = 0.0, xr = 0.0, yl = 0.0, yr = 0.0;  // iface(0, N), iface(1, N), plane q
+    double xl_p = 0.0, xr_p = 0.0, yl_p = 0.0, yr_p = 0.0;  // the same, plane p
+    double tc_q = 0.0, tc_p = 0.0;      // own-column correction (pair on z-faces)
+    double pz_q = 0.0, pz_p = 0.0;      // transverse state on z-faces q, q - 1
+    double ez0_q = 0.0, ez0_p = 0.0, ez1_q = 0.0, ez1_p = 0.0;  // wmac corrections
+    double trz_q = 0.0, trz_p = 0.0;
+    double fo_q = 0.0, fo_p = 0.0;
+    if (col)
+        for (int jj = 5 - G; jj <= 4 + G; ++jj) v[jj] = utv(I, J, z0 - 6 + jj, N);
+    int slot_q = 0, slot_p = 2;
+
+    constexpr int NS = (SW * SH + NT - 1) / NT;
+    constexpr int NU = (UW * H + NT - 1) / NT;
+    constexpr int NV = (W * VH + NT - 1) / NT;
+    bool okS[NS], okU[NU], okV[NV];
+    int offS[NS], offU[NU], offV[NV];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int idx = tid + k * NT;
+        const int sx = idx % SW, sy = idx / SW;
+        okS[k] = idx < SW * SH && sx <= nx + 1 + 2 * G && sy <= ny + 1 + 2 * G;
+        offS[k] = sx + sy * sjs;
+    }
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+        const int idx = tid + k * NT;
+        const int f = idx % UW, r = idx / UW;
+        okU[k] = idx < UW * H && f <= nx + 2 && r <= ny + 1;
+        offU[k] = f + r * xjs;
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const int idx = tid + k * NT;
+        const int cc = idx % W, r = idx / W;
+        okV[k] = idx < W * VH && cc <= nx + 1 && r <= ny + 2;
+        offV[k] = cc + r * yjs;
+    }
+    double pS[NS], pU[NU], pV[NV];
+    double pcs = 0.0, pfo = 0.0, pz = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) pS[k] = okS[k] ? sPl[offS[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) pU[k] = okU[k] ? xPl[offU[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) pV[k] = okV[k] ? yPl[offV[k]] : 0.0;
+    if (col) {
+        pcs = *sCol;
+        pfo = *foCol;
+        pz = *zCol;
+    }
+
+    for (int q = z0 - 1; q <= z1 + 1; ++q) {
+        const bool zface = q >= z0;
+        const bool pplane = q >= z0 + 1;
+        const bool first = q == z0 - 1;
+        double* ULq = UL[slot_q];
+        double* VLq = VL[slot_q];
+        const double* ULp = UL[slot_p];
+        const double* VLp = VL[slot_p];
+
+        // ---- A: stage plane q; shift the column window; issue the loads
+        //      of plane q + 1 (the tracing velocity of plane q is read here:
+        //      B1 and B2 consume it)
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            if (okS[k]) S[tid + k * NT] = pS[k];
+#pragma unroll
+        for (int k = 0; k < NU; ++k)
+            if (okU[k]) ULq[tid + k * NT] = pU[k];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            if (okV[k]) VLq[tid + k * NT] = pV[k];
+        double ufx = 0.0, ufy = 0.0, ufz = 0.0;
+        if (col) {
+            ufx = ufCol[0];
+            ufy = ufCol[ufns];
+            ufz = ufCol[2 * ufns];
+            SC[tid] = v[4];  // plane q - 1 (before the shift)
+            FL[tid] = fo_q;
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) v[jj] = v[jj + 1];
+            v[4 + G] = pcs;
+            fo_p = fo_q;
+            fo_q = pfo;
+            trz_p = trz_q;
+            trz_q = pz;
+        }
+        if (q <= z1) {
+            sPl += sks;
+            xPl += xks;
+            yPl += yks;
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                if (okS[k]) pS[k] = sPl[offS[k]];
+#pragma unroll
+            for (int k = 0; k < NU; ++k)
+                if (okU[k]) pU[k] = xPl[offU[k]];
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                if (okV[k]) pV[k] = yPl[offV[k]];
+            if (col) {
+                sCol += sks;
+                foCol += foks;
+                zCol += zks;
+                pcs = *sCol;
+                pfo = *foCol;
+                pz = *zCol;
+            }
+        }
+        if (col) ufCol += ufks;
+        __syncthreads();
+
+        // ---- B1: face values of plane q; PPM of cell q in z
+        for (int idx = tid; idx < FXW * H; idx += NT) {
+            const int f = idx % FXW - FO, r = idx / FXW;
+            if (f <= nx + 2 + FO && r <= ny + 1) {
+                const double* sr = S + (f + G) + (r + G) * SW;
+                if (PPM == 1) {
+                    const double sa2 = sr[-2], sa = sr[-1], sb = sr[0], sb2 = sr[1];
+                    FX[idx] = face1(sa, sb, vl_slope(sa2, sa, sb), vl_slope(sa, sb, sb2));
+                } else {
+                    FX[idx] = cs_face(sr[-2], sr[-1], sr[0], sr[1]);
+                }
+            }
+        }
+        for (int idx = tid; idx < W * FYH; idx += NT) {
+            const int c = idx % W, f = idx / W - FO;
+            if (c <= nx + 1 && f <= ny + 2 + FO) {
+                const double* sr = S + (c + G) + (f + G) * SW;
+                if (PPM == 1) {
+                    const double sa2 = sr[-2 * SW], sa = sr[-SW], sb = sr[0], sb2 = sr[SW];
+                    FY[idx] = face1(sa, sb, vl_slope(sa2, sa, sb), vl_slope(sa, sb, sb2));
+                } else {
+                    FY[idx] = cs_face(sr[-2 * SW], sr[-SW], sr[0], sr[SW]);
+                }
+            }
+        }
+        if (col) {
+            double sm, sp, Ipz, Imz;
+            if (PPM == 1) {
+                if (first) {
+                    const double d0 = vl_slope(v[2], v[3], v[4]);
+                    zsl = vl_slope(v[3], v[4], v[5]);
+                    zf0 = face1(v[3], v[4], d0, zsl);
+                }
+                const double d1 = vl_slope(v[4], v[5], v[6]);
+                zf1 = face1(v[4], v[5], zsl, d1);
+                zsl = d1;
+                sm = zf0;
+                sp = zf1;
+                cw_limit(v[4], sm, sp);
+            } else {
+                if (first) {
+                    zf0 = cs_face(v[1], v[2], v[3], v[4]);
+                    zf1 = cs_face(v[2], v[3], v[4], v[5]);
+                    zf2 = cs_face(v[3], v[4], v[5], v[6]);
+                }
+                const double f3 = cs_face(v[4], v[5], v[6], v[7]);
+                cs_cell(v, zf0, zf1, zf2, f3, 1.e-10, sm, sp);
+                zf0 = zf1;
+                zf1 = zf2;
+                zf2 = f3;
+            }
+            if (ppm_special<PPM>(q, dlz, dhz, bcz0, bcz1))
+                ppm_edges<PPM, 2>(v, q, dlz, dhz, bcz0, bcz1, sm, sp);
+            if (PPM == 1) zf0 = zf1;
+            const double sgz = dabs(ufz) * dt / hz;
+            trace_sig(v[4], sm, sp, ufz, ufz, sgz, sgz, rel_eps, Ipz, Imz);
+            if (zface) {
+                zl = Ipz_prev;
+                zr = Imz;
+                vf_iface_bc(q, dlz, dhz, pzl, pzh, N == 2, v[3], v[4], zl, zr);
+                if (N != 2) uz_q = vf_upwind(zl, zr, trz_q, rel_eps);
+            }
+            Ipz_prev = Ipz;
+        }
+        __syncthreads();
+
+        // ---- B2: PPM of plane q in x and y
+        if (col) {
+            double sm, sp, Ip, Im;
+            double w9[9];
+#pragma unroll
+            for (int o = -4; o <= 4; ++o) w9[4 + o] = (o >= -G && o <= G) ? S[si + o] : 0.0;
+            vf_cell_edges<PPM, 0>(w9, FX + (a + FO) + b * FXW, 1, I, dlx, dhx, bcx0, bcx1, !XYF,
+                                  sm, sp);
+            const double sgx = dabs(ufx) * dt / hx;
+            trace_sig(w9[4], sm, sp, ufx, ufx, sgx, sgx, rel_eps, Ip, Im);
+            P[0][tid] = Ip;
+            P[1][tid] = Im;
+#pragma unroll
+            for (int o = -4; o <= 4; ++o) w9[4 + o] = (o >= -G && o <= G) ? S[si + o * SW] : 0.0;
+            vf_cell_edges<PPM, 1>(w9, FY + a + (b + FO) * W, W, J, dly, dhy, bcy0, bcy1, !XYF,
+                                  sm, sp);
+            const double sgy = dabs(ufy) * dt / hy;
+            trace_sig(w9[4], sm, sp, ufy, ufy, sgy, sgy, rel_eps, Ip, Im);
+            P[2][tid] = Ip;
+            P[3][tid] = Im;
+        }
+        __syncthreads();
+
+        // ---- C: interface states on the x- and y-faces of plane q; the
+        //      upwind states uimh(0, N) -> FX, uimh(1, N) -> FY where used
+        if (col && a >= 1) {
+            xl = P[0][tid - 1];
+            xr = P[1][tid];
+            if (!XYF) vf_iface_bc(I, dlx, dhx, pxl, pxh, N == 0, S[si - 1], S[si], xl, xr);
+            if (N != 0) FX[tid] = vf_upwind(xl, xr, ULq[ui], rel_eps);
+        }
+        if (col && b >= 1) {
+            yl = P[2][tid - W];
+            yr = P[3][tid];
+            if (!XYF) vf_iface_bc(J, dly, dhy, pyl, pyh, N == 1, S[si - SW], S[si], yl, yr);
+            if (N != 1) FY[tid] = vf_upwind(yl, yr, VLq[vi], rel_eps);
+        }
+        __syncthreads();
+
+        // ---- D1: per-cell corrections of component N
+        if (col) {
+            if (N == 2) {
+                if (b >= 1 && b <= ny)  // (y; w): wimhxy on the x-faces beside it
+                    Q[0][tid] = t6y * (VLq[vi + W] + VLq[vi]) * (FY[tid + W] - FY[tid]);
+                if (a >= 1 && a <= nx)  // (x; w): wimhyx on the y-faces beside it
+                    Q[1][tid] = t6x * (ULq[ui + 1] + ULq[ui]) * (FX[tid + 1] - FX[tid]);
+            } else if (N == 1) {
+                if (a >= 1 && a <= nx)  // (x; v): vimhzx on its z-faces
+                    tc_q = t6x * (ULq[ui + 1] + ULq[ui]) * (FX[tid + 1] - FX[tid]);
+                if (pplane)  // (z; v) of plane p: vimhxz on the x-faces beside it
+                    Q[0][tid] = t6z * (trz_q + trz_p) * (uz_q - uz_p);
+            } else {
+                if (b >= 1 && b <= ny)  // (y; u): uimhzy on its z-faces
+                    tc_q = t6y * (VLq[vi + W] + VLq[vi]) * (FY[tid + W] - FY[tid]);
+                if (pplane)  // (z; u) of plane p: uimhyz on the y-faces beside it
+                    Q[0][tid] = t6z * (trz_q + trz_p) * (uz_q - uz_p);
+            }
+        }
+        __syncthreads();
+
+        // ---- D2: transverse states of component N
+        if (N == 2) {
+            if (col && a >= 1 && b >= 1 && b <= ny) {  // wimhxy, x-faces of plane q
+                double l = xl - Q[0][tid - 1];
+                double r = xr - Q[0][tid];
+                if (!XYF) vf_trans_bc(I, dlx, dhx, pxl, pxh, S[si - 1], S[si], l, r);
+                FX[tid] = vf_upwind(l, r, ULq[ui], rel_eps);
+            }
+            if (col && b >= 1 && a >= 1 && a <= nx) {  // wimhyx, y-faces of plane q
+                double l = yl - Q[1][tid - W];
+                double r = yr - Q[1][tid];
+                if (!XYF) vf_trans_bc(J, dly, dhy, pyl, pyh, S[si - SW], S[si], l, r);
+                FY[tid] = vf_upwind(l, r, VLq[vi], rel_eps);
+            }
+        } else if (N == 1) {
+            if (pplane && col && a >= 1) {  // vimhxz, x-faces of plane p
+                double l = xl_p - Q[0][tid - 1];
+                double r = xr_p - Q[0][tid];
+                if (!XYF) vf_trans_bc(I, dlx, dhx, pxl, pxh, SC[tid - 1], SC[tid], l, r);
+                FX[tid] = vf_upwind(l, r, ULp[ui], rel_eps);
+            }
+            if (zface && col && a >= 1 && a <= nx) {  // vimhzx, z-face q
+                double l = zl - tc_p;
+                double r = zr - tc_q;
+                vf_trans_bc(q, dlz, dhz, pzl, pzh, v[3], v[4], l, r);
+                pz_q = vf_upwind(l, r, trz_q, rel_eps);
+            }
+        } else {
+            if (pplane && col && b >= 1) {  // uimhyz, y-faces of plane p
+                double l = yl_p - Q[0][tid - W];
+                double r = yr_p - Q[0][tid];
+                if (!XYF) vf_trans_bc(J, dly, dhy, pyl, pyh, SC[tid - W], SC[tid], l, r);
+                FY[tid] = vf_upwind(l, r, VLp[vi], rel_eps);
+            }
+            if (zface && col && b >= 1 && b <= ny) {  // uimhzy, z-face q
+                double l = zl - tc_p;
+                double r = zr - tc_q;
+                vf_trans_bc(q, dlz, dhz, pzl, pzh, v[3], v[4], l, r);
+                pz_q = vf_upwind(l, r, trz_q, rel_eps);
+            }
+        }
+        __syncthreads();
+
+        // ---- E1: per-cell final corrections
+        if (N == 2) {
+            if (col && a >= 1 && a <= nx && b >= 1 && b <= ny) {
+                ez0_q = t4x * (ULq[ui + 1] + ULq[ui]) * (FX[tid + 1] - FX[tid]);
+                ez1_q = t4y * (VLq[vi + W] + VLq[vi]) * (FY[tid + W] - FY[tid]);
+            }
+        } else if (N == 1) {
+            if (pplane && col && a >= 1 && a <= nx) {  // vmac's: vimhxz across x, vimhzx across z
+                P[0][tid] = t4x * (ULp[ui + 1] + ULp[ui]) * (FX[tid + 1] - FX[tid]);
+                P[1][tid] = t4z * (trz_q + trz_p) * (pz_q - pz_p);
+            }
+        } else {
+            if (pplane && col && b >= 1 && b <= ny) {  // umac's: uimhyz across y, uimhzy across z
+                P[0][tid] = t4y * (VLp[vi + W] + VLp[vi]) * (FY[tid + W] - FY[tid]);
+                P[1][tid] = t4z * (trz_q + trz_p) * (pz_q - pz_p);
+            }
+        }
+        __syncthreads();
+
+        // ---- E2: the MAC velocity of component N
+        if (N == 0) {
+            if (pplane && col && a >= 1 && b >= 1 && b <= ny && (a <= nx || last_x)) {
+                double vl = xl_p - P[0][tid - 1];
+                vl = vl - P[1][tid - 1];
+                double vr = xr_p - P[0][tid];
+                vr = vr - P[1][tid];
+                const double umacl = vl + dt2 * FL[tid - 1];
+                const double umacr = vr + dt2 * FL[tid];
+                const double w = sph ? B.w0m[0](I, J, q - 1) : 0.0;
+                double res = vf_riemann(umacl, umacr, sph, w, rel_eps);
+                if (!XYF)
+                    res = vf_vel_bc(res, I, dlx, dhx, pxl, pxh, SC[tid - 1], SC[tid], umacl,
+                                    umacr);
+                oCol[-oks] = res;  // plane p
+            }
+        } else if (N == 1) {
+            if (pplane && col && b >= 1 && a >= 1 && a <= nx && (b <= ny || last_y)) {
+                double vl = yl_p - P[0][tid - W];
+                vl = vl - P[1][tid - W];
+                double vr = yr_p - P[0][tid];
+                vr = vr - P[1][tid];
+                const double umacl = vl + dt2 * FL[tid - W];
+                const double umacr = vr + dt2 * FL[tid];
+                const double w = sph ? B.w0m[1](I, J, q - 1) : 0.0;
+                double res = vf_riemann(umacl, umacr, sph, w, rel_eps);
+                if (!XYF)
+                    res = vf_vel_bc(res, J, dly, dhy, pyl, pyh, SC[tid - W], SC[tid], umacl,
+                                    umacr);
+                oCol[-oks] = res;  // plane p
+            }
+        } else {
+            if (((zface && q <= z1) || (q == z1 + 1 && last_z)) && col && a >= 1 && a <= nx &&
+                b >= 1 && b <= ny) {
+                double vl = zl - ez0_p;
+                vl = vl - ez1_p;
+                double vr = zr - ez0_q;
+                vr = vr - ez1_q;
+                const double umacl = vl + dt2 * fo_p;
+                const double umacr = vr + dt2 * fo_q;
+                // planar: w0_cart of the cell at the face's index (MaestroVelPred.cpp:1880)
+                const double w = sph ? B.w0m[2](I, J, q) : B.w0c(I, J, q, 2);
+                double res = vf_riemann(umacl, umacr, true, w, rel_eps);
+                res = vf_vel_bc(res, q, dlz, dhz, pzl, pzh, v[3], v[4], umacl, umacr);
+                *oCol = res;  // z-face q
+            }
+        }
+
+        // ---- advance the column state to the next plane
+        xl_p = xl;
+        xr_p = xr;
+        yl_p = yl;
+        yr_p = yr;
+        uz_p = uz_q;
+        tc_p = tc_q;
+        pz_p = pz_q;
+        ez0_p = ez0_q;
+        ez1_p = ez1_q;
+        oCol += oks;
+        slot_p = slot_q;
+        slot_q = slot_q == 2 ? 0 : slot_q + 1;
+    }
+}
+
+template <int PPM, int TX, int TY, bool XYF, int WPE>
+__global__ void __launch_bounds__(vf_nt(TX, TY)) __attribute__((amdgpu_waves_per_eu(WPE)))
+    k_vp_fused_comp(const VFBox* __restrict__ boxes, const VFTile* __restrict__ tiles, KGeom g,
+                    VFArgs A) {
+    __shared__ VPShared<PPM, TX, TY> sh;
+    const VFTile tl = tiles[blockIdx.x];
+    const VFBox& B = boxes[tl.box];
+    if (blockIdx.z == 0)
+        vp_comp<PPM, TX, TY, XYF, 0>(B, tl, g, A, sh);
+    else if (blockIdx.z == 1)
+        vp_comp<PPM, TX, TY, XYF, 1>(B, tl, g, A, sh);
+    else
+        vp_comp<PPM, TX, TY, XYF, 2>(B, tl, g, A, sh);
+}
+
 }  // namespace
 
 namespace mxh {
@@ -809,23 +1324,34 @@ bool vf_tile_free(int lo0, int n, int dlo, int dhi, bool act_lo, bool act_hi) {
     return free_lo && free_hi;
 }
 
-template <int TX, int TY, int WPE, bool XYF>
+// COMP: one velocity component per block (blockIdx.z), else all three
+template <int TX, int TY, int WPE, bool COMP, bool XYF>
 void vf_launch_one(mx_adv_ctx* c, const VFBox* db, const VFTile* dt, size_t n, const KGeom& kg,
                    const VFArgs& A, int& rc) {
     if (n == 0 || rc) return;
     auto launch = [&]() -> int {
-        if (c->p.ppm_type == 1)
-            MX_LAUNCH(c, "k_vp_fused", (k_vp_fused<1, TX, TY, XYF, WPE>), dim3((unsigned)n),
-                      dim3(vf_nt(TX, TY)), db, dt, kg, A);
-        else
-            MX_LAUNCH(c, "k_vp_fused", (k_vp_fused<2, TX, TY, XYF, WPE>), dim3((unsigned)n),
-                      dim3(vf_nt(TX, TY)), db, dt, kg, A);
+        if constexpr (COMP) {
+            const dim3 grid((unsigned)n, 1, 3);
+            if (c->p.ppm_type == 1)
+                MX_LAUNCH(c, "k_vp_fused", (k_vp_fused_comp<1, TX, TY, XYF, WPE>), grid,
+                          dim3(vf_nt(TX, TY)), db, dt, kg, A);
+            else
+                MX_LAUNCH(c, "k_vp_fused", (k_vp_fused_comp<2, TX, TY, XYF, WPE>), grid,
+                          dim3(vf_nt(TX, TY)), db, dt, kg, A);
+        } else {
+            if (c->p.ppm_type == 1)
+                MX_LAUNCH(c, "k_vp_fused", (k_vp_fused<1, TX, TY, XYF, WPE>), dim3((unsigned)n),
+                          dim3(vf_nt(TX, TY)), db, dt, kg, A);
+            else
+                MX_LAUNCH(c, "k_vp_fused", (k_vp_fused<2, TX, TY, XYF, WPE>), dim3((unsigned)n),
+                          dim3(vf_nt(TX, TY)), db, dt, kg, A);
+        }
         return MX_OK;
     };
     rc = launch();
 }
 
-template <int TX, int TY, int WPE>
+template <int TX, int TY, int WPE, bool COMP>
 int vf_launch(mx_adv_ctx* c, const std::vector<VpLevel>& levs, int tz) {
     const long long lim = 0x7fffffffLL;
     const bool sph = c->p.spherical != 0;
@@ -908,8 +1434,8 @@ int vf_launch(mx_adv_ctx* c, const std::vector<VpLevel>& levs, int tz) {
     const VFBox* db = tab.dev<VFBox>(ob);
     const VFTile* dtl = tab.dev<VFTile>(ot);
     const KGeom kg = kgeom(c, levs[0].lev);
-    vf_launch_one<TX, TY, WPE, true>(c, db, dtl, nfree, kg, A, rc);
-    vf_launch_one<TX, TY, WPE, false>(c, db, dtl + nfree, tiles.size() - nfree, kg, A, rc);
+    vf_launch_one<TX, TY, WPE, COMP, true>(c, db, dtl, nfree, kg, A, rc);
+    vf_launch_one<TX, TY, WPE, COMP, false>(c, db, dtl + nfree, tiles.size() - nfree, kg, A, rc);
     return rc;
 }
 
@@ -920,7 +1446,16 @@ int velpred_fused(mx_adv_ctx* c, const std::vector<VpLevel>& levs) {
     int rc = check_phys(c);
     if (rc) return rc;
     const int tz = c->vp_fused_tz > 0 ? c->vp_fused_tz : 32;
-    return vf_launch<16, 11, 2>(c, levs, tz);
+    // all three components per block: 16 x 11 columns at 2 waves per SIMD
+    // (256 VGPRs); one component per block: the tile shape (cell columns)
+    // and register budget (waves per SIMD) of `velpred_fused_tile`
+    if (!c->vp_fused_comp) return vf_launch<16, 11, 2, false>(c, levs, tz);
+    switch (c->vp_fused_tile) {
+        case 1: return vf_launch<16, 8, 3, true>(c, levs, tz);
+        case 2: return vf_launch<16, 11, 4, true>(c, levs, tz);
+        case 3: return vf_launch<32, 8, 3, true>(c, levs, tz);
+        default: return vf_launch<16, 11, 3, true>(c, levs, tz);
+    }
 }
 
 }  // namespace mxh

@@ -102,6 +102,30 @@ def test_pass_kernel_choices(name, cfg, fused, tz):
     assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
 
 
+UT_CASES = [CASES[0], CASES[1], CASES[7], CASES[9], CASES[10], CASES[11], CASES[15],
+            CASES[16], KERNEL_CHOICE_CASES[3]]
+
+
+@pytest.mark.parametrize("line,tz", [(0, 32), (1, 1), (1, 3), (1, 7)])
+@pytest.mark.parametrize("name,cfg", UT_CASES, ids=[c[0] for c in UT_CASES])
+def test_pass_utrans_line(name, cfg, line, tz):
+    """MakeUtrans per face (the default) and along lines of faces (3-D,
+    ppm_type 1/2) cut into chunks of 1, 3 and 7 faces (chunk ends at the
+    domain boundary stencils, at inflow faces and inside boxes): utrans and
+    the whole pass bit-exact."""
+    print(f"[parity] utrans line={line} tz={tz} {name}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    m.set_tuning("utrans_line", line)
+    m.set_tuning("utrans_tz", tz)
+    _run_pass(m, wg)
+    assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
+    assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
+
+
 XCD_CASES = [CASES[0], CASES[3], CASES[7], CASES[10], CASES[16], CASES[17]]
 
 
@@ -168,19 +192,24 @@ def test_pass_velpred_trans_per_pair(name, cfg):
 VP_FUSED_CASES = VP_ALL_CASES + [CASES[11], CASES[17], KERNEL_CHOICE_CASES[3]]
 
 
-@pytest.mark.parametrize("fused,tz", [(0, 32), (1, 1), (1, 5), (1, 32)])
+@pytest.mark.parametrize("fused,tz,comp,tile", [(0, 32, 1, 0), (1, 1, 1, 0), (1, 5, 1, 0),
+                                                (1, 32, 1, 0), (1, 5, 0, 0), (1, 5, 1, 1),
+                                                (1, 5, 1, 2), (1, 5, 1, 3)])
 @pytest.mark.parametrize("name,cfg", VP_FUSED_CASES, ids=[c[0] for c in VP_FUSED_CASES])
-def test_pass_velpred_fused(name, cfg, fused, tz):
+def test_pass_velpred_fused(name, cfg, fused, tz, comp, tile):
     """VelPred as the fused z-marching kernel (the default for 3-D
-    ppm_type 1/2 without traced forces) with z-chunks of 1, 5 and 32 planes,
-    and as the staged kernels: the whole pass bit-exact, walls, inflow,
-    spherical w0mac and partial tiles included."""
-    print(f"[parity] velpred fused={fused} tz={tz} {name}", flush=True)
+    ppm_type 1/2 without traced forces) -- one velocity component per block
+    (every tile shape and register budget) or all three -- with z-chunks of
+    1, 5 and 32 planes, and as the staged kernels: the whole pass bit-exact,
+    walls, inflow, spherical w0mac and partial tiles included."""
+    print(f"[parity] velpred fused={fused} tz={tz} comp={comp} tile={tile} {name}", flush=True)
     wc, wg = pair(**cfg)
     _run_pass(oracle_for(wc), wc)
     m = gpu_for(wg)
     m.set_tuning("velpred_fused", fused)
     m.set_tuning("velpred_fused_tz", tz)
+    m.set_tuning("velpred_fused_comp", comp)
+    m.set_tuning("velpred_fused_tile", tile)
     _run_pass(m, wg)
     assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
     assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
