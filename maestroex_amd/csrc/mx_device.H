@@ -41,6 +41,11 @@ struct DA {
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dabs(double a) { return fabs(a); }
+// min / max of two values that are both >= +0 and not NaN: every such pair
+// orders totally and equal values have equal bits, so the hardware min / max
+// (one instruction) returns what dmin / dmax return
+__device__ __forceinline__ double nn_min(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ double nn_max(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ double dsign1(double b) { return copysign(1.0, b); }
 
 // ---------------------------------------------------------------------------
@@ -52,7 +57,8 @@ __device__ __forceinline__ double vl_slope(double a, double b, double c) {
     double dsl = 2.0 * (b - a);
     double dsr = 2.0 * (c - b);
     double r = 0.0;
-    if (dsl * dsr > 0.0) r = dsign1(dsc) * dmin(dabs(dsc), dmin(dabs(dsl), dabs(dsr)));
+    // sign(dsc) * m with m >= +0 is m with the sign bit of dsc
+    if (dsl * dsr > 0.0) r = copysign(nn_min(dabs(dsc), nn_min(dabs(dsl), dabs(dsr))), dsc);
     return r;
 }
 
@@ -103,8 +109,8 @@ __device__ __forceinline__ void cs_cell(const double* v, double sedgel, double s
         double dafacep = sedgerr - sedger;
         double dabarm = s0 - v[3];
         double dabarp = v[5] - s0;
-        double dafacemin = dmin(dabs(dafacem), dabs(dafacep));
-        double dabarmin = dmin(dabs(dabarm), dabs(dabarp));
+        double dafacemin = nn_min(dabs(dafacem), dabs(dafacep));
+        double dabarmin = nn_min(dabs(dabarm), dabs(dabarp));
         double dachkm, dachkp;
         if (dafacemin >= dabarmin) {
             dachkm = dafacem;
@@ -123,7 +129,7 @@ __device__ __forceinline__ void cs_cell(const double* v, double sedgel, double s
         double sgn = dsign1(D2);
         double D2LIM =
             dmax(dmin(sgn * D2, dmin(C * sgn * D2L, dmin(C * sgn * D2R, C * sgn * D2C))), 0.0);
-        double D2ABS = dmax(dabs(D2), 1.e-10);
+        double D2ABS = nn_max(dabs(D2), 1.e-10);
         alpham = alpham * D2LIM / D2ABS;
         alphap = alphap * D2LIM / D2ABS;
     } else {
@@ -347,7 +353,7 @@ __device__ __forceinline__ double slope_point(const double* v, int order, int po
         double del = 0.5 * (v[5] - v[3]);
         double dpls = 2.0 * (v[5] - v[4]);
         double dmn = 2.0 * (v[4] - v[3]);
-        double slim = dmin(dabs(dpls), dabs(dmn));
+        double slim = nn_min(dabs(dpls), dabs(dmn));
         slim = dpls * dmn > 0.0 ? slim : 0.0;
         double sflag = dsign1(del);
         out = sflag * dmin(slim, dabs(del));
@@ -358,7 +364,7 @@ __device__ __forceinline__ double slope_point(const double* v, int order, int po
                 del = (v[5] + 3.0 * v[4] - 4.0 * v[3]) / 3.0;
                 dpls = 2.0 * (v[5] - v[4]);
                 dmn = 2.0 * (v[4] - v[3]);
-                slim = dmin(dabs(dpls), dabs(dmn));
+                slim = nn_min(dabs(dpls), dabs(dmn));
                 slim = dpls * dmn > 0.0 ? slim : 0.0;
                 sflag = dsign1(del);
                 out = sflag * dmin(slim, dabs(del));
@@ -376,7 +382,7 @@ __device__ __forceinline__ double slope_point(const double* v, int order, int po
                     dpls = 2.0 * (v[5] - v[4]);
                     dmn = 2.0 * (v[4] - v[3]);
                 }
-                slim = dmin(dabs(dpls), dabs(dmn));
+                slim = nn_min(dabs(dpls), dabs(dmn));
                 slim = dpls * dmn > 0.0 ? slim : 0.0;
                 sflag = dsign1(del);
                 out = sflag * dmin(slim, dabs(del));
@@ -386,25 +392,25 @@ __device__ __forceinline__ double slope_point(const double* v, int order, int po
         double dcen = 0.5 * (v[4] - v[2]);
         double dmn = 2.0 * (v[3] - v[2]);
         double dpls = 2.0 * (v[4] - v[3]);
-        double dlim = dmin(dabs(dmn), dabs(dpls));
+        double dlim = nn_min(dabs(dmn), dabs(dpls));
         dlim = dpls * dmn > 0.0 ? dlim : 0.0;
         double dflag = dsign1(dcen);
         double dxl = dflag * dmin(dlim, dabs(dcen));
         dcen = 0.5 * (v[6] - v[4]);
         dmn = 2.0 * (v[5] - v[4]);
         dpls = 2.0 * (v[6] - v[5]);
-        dlim = dmin(dabs(dmn), dabs(dpls));
+        dlim = nn_min(dabs(dmn), dabs(dpls));
         dlim = dpls * dmn > 0.0 ? dlim : 0.0;
         dflag = dsign1(dcen);
         double dxr = dflag * dmin(dlim, dabs(dcen));
         dcen = 0.5 * (v[5] - v[3]);
         dmn = 2.0 * (v[4] - v[3]);
         dpls = 2.0 * (v[5] - v[4]);
-        dlim = dmin(dabs(dmn), dabs(dpls));
+        dlim = nn_min(dabs(dmn), dabs(dpls));
         dlim = dpls * dmn > 0.0 ? dlim : 0.0;
         dflag = dsign1(dcen);
         double ds = 4.0 / 3.0 * dcen - (dxr + dxl) / 6.0;
-        out = dflag * dmin(dabs(ds), dlim);
+        out = dflag * nn_min(dabs(ds), dlim);
         if (lo_ho) {
             if (pos == ilo - 1) {
                 out = 0.0;
@@ -412,7 +418,7 @@ __device__ __forceinline__ double slope_point(const double* v, int order, int po
                 double del = -16.0 / 15.0 * v[3] + 0.5 * v[4] + 2.0 / 3.0 * v[5] - 0.1 * v[6];
                 dmn = 2.0 * (v[4] - v[3]);
                 dpls = 2.0 * (v[5] - v[4]);
-                double slim = dmin(dabs(dpls), dabs(dmn));
+                double slim = nn_min(dabs(dpls), dabs(dmn));
                 slim = dpls * dmn > 0.0 ? (D == 0 ? dlim : slim) : 0.0;
                 double sflag = dsign1(del);
                 out = sflag * dmin(slim, dabs(del));
@@ -420,12 +426,12 @@ __device__ __forceinline__ double slope_point(const double* v, int order, int po
                 double del = -16.0 / 15.0 * v[2] + 0.5 * v[3] + 2.0 / 3.0 * v[4] - 0.1 * v[5];
                 dmn = 2.0 * (v[3] - v[2]);
                 dpls = 2.0 * (v[4] - v[3]);
-                double slim = dmin(dabs(dpls), dabs(dmn));
+                double slim = nn_min(dabs(dpls), dabs(dmn));
                 slim = dpls * dmn > 0.0 ? (D == 0 ? dlim : slim) : 0.0;
                 double sflag = dsign1(del);
                 dxl = sflag * dmin(slim, dabs(del));
                 ds = 4.0 / 3.0 * dcen - (dxr + dxl) / 6.0;
-                out = dflag * dmin(dabs(ds), dlim);
+                out = dflag * nn_min(dabs(ds), dlim);
             }
         }
         if (hi_ho) {
@@ -435,7 +441,7 @@ __device__ __forceinline__ double slope_point(const double* v, int order, int po
                 double del = -(-16.0 / 15.0 * v[5] + 0.5 * v[4] + 2.0 / 3.0 * v[3] - 0.1 * v[2]);
                 dmn = 2.0 * (v[4] - v[3]);
                 dpls = 2.0 * (v[5] - v[4]);
-                double slim = dmin(dabs(dpls), dabs(dmn));
+                double slim = nn_min(dabs(dpls), dabs(dmn));
                 slim = dpls * dmn > 0.0 ? (D == 0 ? dlim : slim) : 0.0;
                 double sflag = dsign1(del);
                 out = sflag * dmin(slim, dabs(del));
@@ -443,12 +449,12 @@ __device__ __forceinline__ double slope_point(const double* v, int order, int po
                 double del = -(-16.0 / 15.0 * v[6] + 0.5 * v[5] + 2.0 / 3.0 * v[4] - 0.1 * v[3]);
                 dmn = 2.0 * (v[5] - v[4]);
                 dpls = 2.0 * (v[6] - v[5]);
-                double slim = dmin(dabs(dpls), dabs(dmn));
+                double slim = nn_min(dabs(dpls), dabs(dmn));
                 slim = dpls * dmn > 0.0 ? (D == 0 ? dlim : slim) : 0.0;
                 double sflag = dsign1(del);
                 dxr = sflag * dmin(slim, dabs(del));
                 ds = 4.0 / 3.0 * dcen - (dxl + dxr) / 6.0;
-                out = dflag * dmin(dabs(ds), dlim);
+                out = dflag * nn_min(dabs(ds), dlim);
             }
         }
     }
