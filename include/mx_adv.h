@@ -145,6 +145,8 @@ typedef struct {
     int comp_pi;             /* Pi */
     int first_spec;          /* FirstSpec */
     int num_spec;            /* NumSpec; FirstAux.. are the remaining comps */
+    int reflux_type;         /* 0 none, 1 (default) AverageDownFaces of the fluxes,
+                                2 flux registers (_cpp_parameters:24) */
 } mx_adv_params;
 
 typedef struct mx_adv_ctx mx_adv_ctx;
@@ -618,6 +620,60 @@ int mx_adv_average_down_faces(mx_adv_ctx* ctx, int lev, const mx_face_multifab* 
    with periodicity and FillUmacGhost.  Coarse faces no coarse FAB holds read
    as 1e200, as in the reference. */
 int mx_adv_fill_patch_uedge(mx_adv_ctx* ctx, int nlev, mx_face_multifab* uedge);
+
+/* ---- flux registers (reflux_type 2) -------------------------------------- */
+/* Maestro::flux_reg_s[lev] (Maestro.H:2021-2025), an amrex::FluxRegister on
+   level lev's BoxArray for the lev / lev - 1 interface.  The context holds
+   one register per level lev >= 1; mx_adv_set_level defines it with Nscal
+   components when params.reflux_type == 2 (MaestroInit.cpp:389-392), or
+   mx_adv_flux_reg_define does so explicitly (the FluxRegister constructor).
+   The register holds, per fine box and side, the faces of the coarsened box
+   on that side; values start at 0.
+
+   With reflux_type == 2, mx_adv_make_rhox_flux / mx_adv_make_rhoh_flux and
+   the passes update the registers as MakeRhoXFlux / MakeRhoHFlux do
+   (MaestroMakeFlux.cpp:420-451, 1035-1058): on level lev, CrseInit into the
+   register of lev + 1 with mult = -dt * area[d] and FineAdd into the
+   register of lev with mult = dt * area[d], area the level's face areas;
+   components start_comp.. and Rho (MakeRhoXFlux) or RhoH (MakeRhoHFlux).
+   The caller then calls mx_adv_flux_reg_reflux where the reference's
+   UpdateScal calls Reflux (MaestroAdvection.cpp:258-278). */
+int mx_adv_flux_reg_define(mx_adv_ctx* ctx, int lev, int ncomp);
+
+/* FluxRegister::setVal: every register value of level lev becomes v. */
+int mx_adv_flux_reg_setval(mx_adv_ctx* ctx, int lev, double v);
+
+/* FluxRegister::CrseInit(mflx, dir, srccomp, destcomp, numcomp, mult) with
+   mflx = level lev - 1's fluxes of direction dir (one mx_multifab, a FAB per
+   coarse box holding its valid faces): every register face of direction dir
+   becomes mflx * mult, taken from the first coarse box (box order) whose
+   valid faces hold it. */
+int mx_adv_flux_reg_crse_init(mx_adv_ctx* ctx, int lev, const mx_multifab* flux, int dir,
+                              int srccomp, int destcomp, int numcomp, double mult);
+
+/* FluxRegister::FineAdd(mflx, dir, srccomp, destcomp, numcomp, mult) with
+   mflx = level lev's fluxes of direction dir: every register face of a fine
+   box adds mult * flux of each fine face on it, one addition per fine face,
+   fine face loops (k, j) for dir 0, (k, i) for dir 1, (j, i) for dir 2,
+   the first outermost. */
+int mx_adv_flux_reg_fine_add(mx_adv_ctx* ctx, int lev, const mx_multifab* flux, int dir,
+                             int srccomp, int destcomp, int numcomp, double mult);
+
+/* FluxRegister::Reflux(state, scale, scomp, dcomp, ncomp, geom[lev - 1]) on
+   level lev - 1's state: for every valid coarse cell and each orientation in
+   order (the low sides of x, y, z, then the high sides), f = the register
+   value on the cell's face across that orientation (its high face for a low
+   side, its low face for a high side; periodic images included; 0 where no
+   register face lies) and s += (-scale * f) / vol on a low side,
+   s += (scale * f) / vol on a high side, vol = the coarse cell volume. */
+int mx_adv_flux_reg_reflux(mx_adv_ctx* ctx, int lev, mx_multifab* state, double scale,
+                           int scomp, int dcomp, int ncomp);
+
+/* Copy every register value of level lev to host memory (test access; no
+   counterpart in the reference): registers in (fine box, orientation) order,
+   each with its components outermost and faces i fastest.  *n receives the
+   count; host may be NULL to query it. */
+int mx_adv_flux_reg_read(mx_adv_ctx* ctx, int lev, double* host, long long cap, long long* n);
 
 /* ---- the update that closes the step ------------------------------------ */
 /* Maestro::UpdateScal (Maestro.H:306-311, MaestroAdvection.cpp:86-297) on

@@ -103,6 +103,7 @@ class Params(C.Structure):
         ("comp_pi", C.c_int),
         ("first_spec", C.c_int),
         ("num_spec", C.c_int),
+        ("reflux_type", C.c_int),
     ]
 
 
@@ -231,6 +232,12 @@ SIGNATURES = {
     "average_down": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i],
     "average_down_faces": [_vp, _i, P(FaceMultiFabDesc), P(FaceMultiFabDesc)],
     "fill_patch_uedge": [_vp, _i, P(FaceMultiFabDesc)],
+    "flux_reg_define": [_vp, _i, _i],
+    "flux_reg_setval": [_vp, _i, _d],
+    "flux_reg_crse_init": [_vp, _i, P(MultiFabDesc), _i, _i, _i, _i, _d],
+    "flux_reg_fine_add": [_vp, _i, P(MultiFabDesc), _i, _i, _i, _i, _d],
+    "flux_reg_reflux": [_vp, _i, P(MultiFabDesc), _d, _i, _i, _i],
+    "flux_reg_read": [_vp, _i, P(_d), C.c_longlong, P(C.c_longlong)],
     "convert_rhox_to_x": [_vp, _i, P(MultiFabDesc), _i],
     "convert_rhoh_to_h": [_vp, _i, P(MultiFabDesc), _i],
     "update_scal": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc),

@@ -22,7 +22,7 @@ ROOT = os.path.dirname(HERE)
 SOURCES = ["mx_ctx.hip", "mx_velocity.hip", "mx_scalar.hip", "mx_edge_fused.hip", "mx_edge1d.hip",
            "mx_graph.hip", "mx_adjoint.hip", "mx_update.hip", "mx_base.hip", "mx_sync.hip",
            "mx_fill.hip", "mx_plotfile.hip", "mx_velpred_fused.hip", "mx_noalloc.hip",
-           "mx_amr.hip"]
+           "mx_amr.hip", "mx_fluxreg.hip"]
 LIB = os.path.join(HERE, "lib", "libmaestroex_amd.so")
 OBJ = os.path.join(HERE, "_build")
 ARCH = os.environ.get("MX_OFFLOAD_ARCH", "gfx950")

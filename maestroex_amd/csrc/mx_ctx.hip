@@ -810,6 +810,7 @@ int mx_adv_create(const mx_adv_params* p, mx_adv_ctx** out) {
     if (p->finest_level < 0 || p->finest_level >= MX_MAX_LEV) return MX_ERR_ARG;
     if (p->nscal <= 0 || p->nscal > MX_MAX_SCAL) return MX_ERR_ARG;
     if (p->spherical && p->dim != 3) return MX_ERR_ARG;
+    if (p->reflux_type < 0 || p->reflux_type > 2) return MX_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MX_ERR_NOLIB;
     mx_adv_ctx* c = new mx_adv_ctx();
@@ -929,6 +930,7 @@ int mx_adv_destroy(mx_adv_ctx* c) {
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     c->ev_pool.clear();
     (void)hipStreamSynchronize(c->stream);
+    for (int lev = 0; lev < MX_MAX_LEV; ++lev) flux_reg_free(c, lev);
     if (c->ws) (void)hipFree(c->ws);
     for (auto& t : c->tab_slots) {
         free_tab_slot(t);
@@ -969,6 +971,9 @@ int mx_adv_set_level(mx_adv_ctx* c, int lev, int nbox, const mx_box* boxes) {
         else
             ++it;
     }
+    // reflux_type 2: the level's flux register, Nscal components, as the
+    // reference makes it with every new level (MaestroInit.cpp:389-392)
+    if (lev > 0 && c->p.reflux_type == 2) return flux_reg_define(c, lev, c->p.nscal);
     return MX_OK;
 }
 

@@ -80,6 +80,20 @@ struct mx_adv_ctx {
     };
     std::map<std::tuple<int, int, int, int>, std::vector<FillCopy>> fill_plans;
 
+    // flux registers (amrex::FluxRegister) of levels 1..finest_level: per
+    // fine box b and orientation o (the low sides of the directions, then the
+    // high sides) the faces of the coarsened box on that side, in coarse face
+    // index space; box[b * 2 * dim + o], values at d + off[...] (components
+    // outermost, i fastest)
+    struct FluxRegDev {
+        int ncomp = 0;
+        std::vector<mx_box> box;
+        std::vector<long long> off;
+        long long total = 0;
+        double* d = nullptr;
+    };
+    FluxRegDev freg[MX_MAX_LEV];
+
     // opt-in kernel timing: a HIP event pair around every launch
     struct ProfRec {
         const char* name;
@@ -413,6 +427,12 @@ int make_edge_scal_fused(mx_adv_ctx* c, int lev, const mx_multifab* state,
 // the pass (mx_adv_advance_pass and its multi-level / graph forms)
 int pass_check(mx_adv_ctx* c, int lev, const mx_adv_pass_args* a);
 int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_args* a);
+
+// flux registers (reflux_type 2)
+int flux_reg_define(mx_adv_ctx* c, int lev, int ncomp);
+void flux_reg_free(mx_adv_ctx* c, int lev);
+int flux_reg_ops(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux, bool rhox,
+                 int start_comp, int num_comp);
 
 void bc_setup_host(mx_adv_ctx* c);
 int check_level(const mx_adv_ctx* c, int lev);

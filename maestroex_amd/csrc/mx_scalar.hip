@@ -836,8 +836,7 @@ int make_rhox_flux(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux,
     const KGeom kg = kgeom(c, lev);
     MX_LAUNCH(c, "k_rhox_flux", k_rhox_flux, grid_for_g(kg, maxpts, nb, c->p.dim, block),
               dim3(block), tab.dev<FluxBox>(ob), kg, A);
-
-    return MX_OK;
+    return flux_reg_ops(c, lev, sflux, true, start_comp, num_comp);
 }
 
 int make_rhoh_flux(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux,
@@ -871,8 +870,7 @@ int make_rhoh_flux(mx_adv_ctx* c, int lev, const mx_face_multifab* sflux,
     MX_LAUNCH(c, "k_rhoh_flux", k_rhoh_flux,
               grid_for_g(kg, maxpts, (int)boxes.size(), c->p.dim, block), dim3(block),
               tab.dev<FluxBox>(ob), kg, A);
-
-    return MX_OK;
+    return flux_reg_ops(c, lev, sflux, false, 0, 0);
 }
 
 }  // namespace mxh
@@ -1053,8 +1051,10 @@ int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_arg
     // reflux_type 1: AverageDownFaces(sflux) after MakeRhoXFlux and after
     // MakeRhoHFlux (MaestroMakeFlux.cpp:459-462, 1061-1064).  MakeRhoHFlux
     // writes only the RhoH component and reads no flux, so one restriction
-    // of every component at the end gives the same values.
-    if (multi && (rc = sync_levels(&mx_adv_pass_args::sflux, false))) return rc;
+    // of every component at the end gives the same values.  reflux_type 2
+    // has updated the flux registers inside the flux stages instead.
+    if (multi && p.reflux_type == 1 && (rc = sync_levels(&mx_adv_pass_args::sflux, false)))
+        return rc;
     return MX_OK;
 }
 

@@ -219,6 +219,46 @@ class Maestro:
         arr = (_abi.FaceMultiFabDesc * len(descs))(*descs)
         self._check(self.fns["fill_patch_uedge"](self.ctx, len(descs), arr), "fill_patch_uedge")
 
+    # amrex::FluxRegister of level lev (Maestro::flux_reg_s[lev]), held by the
+    # context
+    def flux_reg_define(self, lev, ncomp):
+        self._check(self.fns["flux_reg_define"](self.ctx, lev, ncomp), "FluxRegister")
+
+    def flux_reg_setval(self, lev, v):
+        self._check(self.fns["flux_reg_setval"](self.ctx, lev, float(v)), "FluxRegister::setVal")
+
+    def flux_reg_crse_init(self, lev, flux, dir, srccomp, destcomp, numcomp, mult):
+        """FluxRegister::CrseInit with level lev - 1's fluxes of direction
+        dir (the MultiFab flux[dir] of a FaceMultiFab)."""
+        m = _mf(flux)
+        self._check(self.fns["flux_reg_crse_init"](self.ctx, lev, C.byref(m), dir, srccomp,
+                                                   destcomp, numcomp, float(mult)),
+                    "FluxRegister::CrseInit")
+
+    def flux_reg_fine_add(self, lev, flux, dir, srccomp, destcomp, numcomp, mult):
+        """FluxRegister::FineAdd with level lev's fluxes of direction dir."""
+        m = _mf(flux)
+        self._check(self.fns["flux_reg_fine_add"](self.ctx, lev, C.byref(m), dir, srccomp,
+                                                  destcomp, numcomp, float(mult)),
+                    "FluxRegister::FineAdd")
+
+    def flux_reg_reflux(self, lev, state, scale, scomp, dcomp, ncomp):
+        """FluxRegister::Reflux onto level lev - 1's state."""
+        m = _mf(state)
+        self._check(self.fns["flux_reg_reflux"](self.ctx, lev, C.byref(m), float(scale), scomp,
+                                                dcomp, ncomp), "FluxRegister::Reflux")
+
+    def flux_reg_read(self, lev):
+        """Every register value of level lev as a numpy array (registers in
+        (fine box, orientation) order, components outermost, i fastest)."""
+        n = C.c_longlong(0)
+        self._check(self.fns["flux_reg_read"](self.ctx, lev, None, 0, C.byref(n)), "flux_reg_read")
+        out = np.zeros(n.value)
+        self._check(self.fns["flux_reg_read"](self.ctx, lev,
+                                              out.ctypes.data_as(C.POINTER(C.c_double)), n.value,
+                                              C.byref(n)), "flux_reg_read")
+        return out
+
     def AverageDown(self, mfs, comp, ncomp):
         """Maestro::AverageDown (MaestroFillData.cpp:145-154): mfs[lev] per level."""
         for lev in range(len(mfs) - 2, -1, -1):

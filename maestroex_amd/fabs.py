@@ -233,7 +233,7 @@ class BaseState:
 def make_params(geom, ppm_type=1, nspec=3, *, slope_order=4, ppm_trace_forces=0,
                 species_pred_type=1, enthalpy_pred_type=1, spherical=0,
                 use_exact_base_state=0, evolve_base_state=1, dt=0.0, rel_eps=0.0,
-                naux=0, extra_levels=()):
+                naux=0, extra_levels=(), reflux_type=1):
     """mx_adv_params for one level (or several: extra_levels = [Geometry, ...])."""
     p = _abi.Params()
     p.dim = geom.dim
@@ -263,6 +263,7 @@ def make_params(geom, ppm_type=1, nspec=3, *, slope_order=4, ppm_trace_forces=0,
     p.first_spec = 4
     p.num_spec = nspec
     p.nscal = 4 + nspec + naux
+    p.reflux_type = reflux_type
     return p
 
 

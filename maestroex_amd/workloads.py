@@ -428,7 +428,7 @@ class Hierarchy:
 
     def __init__(self, n0=32, level0_edges=(8, 16), counts=(6, 8), edges=((8, 16), (4, 12)),
                  fracs=(0.6, 0.4), ppm_type=1, nspec=3, seed=7, nmodes=16, ng=4, backend=None,
-                 verbose=False):
+                 verbose=False, reflux_type=1):
         from .fabs import Backend
         self.verbose = verbose
         self.backend = bk = backend or Backend("numpy")
@@ -477,7 +477,8 @@ class Hierarchy:
         self.ng = ng
         self.seed = seed
         self.params = make_params(geoms[0], ppm_type=ppm_type, nspec=nspec, dt=self.dt,
-                                  rel_eps=self.rel_eps, extra_levels=geoms[1:])
+                                  rel_eps=self.rel_eps, extra_levels=geoms[1:],
+                                  reflux_type=reflux_type)
         self.levels = [self._level(l) for l in range(nlev)]
 
     # analytic fields -----------------------------------------------------

@@ -19,6 +19,10 @@
 
 namespace mxo {
 
+int FluxRegOps(const Ctx& c, int lev, const std::vector<Arr> sflux[3], bool rhox,
+               int start_comp, int num_comp);
+
+
 void PPM(const Ctx& c, const IBox& bx, const Arr& s, const Arr& u,
          const Arr& v, const Arr& w, const Arr& Ip, const Arr& Im,
          const IBox& dom, const mx_bcrec* bcs, const double* dx, bool is_umac,
@@ -1356,8 +1360,9 @@ int MakeRhoXFlux(const Ctx& c, int lev, const std::vector<Arr> sflux[3],
             });
         }
     }
-    // reflux_type 1: AverageDownFaces(sflux) is a no-op on a single level
-    return MX_OK;
+    // reflux_type 1: AverageDownFaces(sflux) is a no-op on a single level;
+    // reflux_type 2: the flux registers of lev and lev + 1
+    return FluxRegOps(c, lev, sflux, true, start_comp, num_comp);
 }
 
 // Maestro::MakeRhoHFlux (MaestroMakeFlux.cpp:467-1067), one level.
@@ -1464,7 +1469,7 @@ int MakeRhoHFlux(const Ctx& c, int lev, const std::vector<Arr> sflux[3],
             });
         }
     }
-    return MX_OK;
+    return FluxRegOps(c, lev, sflux, false, 0, 0);
 }
 
 // ---------------------------------------------------------------------------

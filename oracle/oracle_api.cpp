@@ -88,6 +88,14 @@ int FillPatchSingleLevel(const Ctx& c, int lev, const std::vector<Arr>& mf,
                          const mx_bcrec* bcs, bool is_vel);
 int AverageDownFaces(const Ctx& c, int lev, const std::vector<Arr> fine[3],
                      const std::vector<Arr> crse[3]);
+int FluxRegDefine(const Ctx& c, int lev, int ncomp);
+int FluxRegSetVal(const Ctx& c, int lev, double v);
+int FluxRegCrseInit(const Ctx& c, int lev, const std::vector<Arr>& flux, int dir, int scomp,
+                    int dcomp, int ncomp, double mult);
+int FluxRegFineAdd(const Ctx& c, int lev, const std::vector<Arr>& flux, int dir, int scomp,
+                   int dcomp, int ncomp, double mult);
+int FluxRegReflux(const Ctx& c, int lev, const std::vector<Arr>& state, double scale, int scomp,
+                  int dcomp, int ncomp);
 int MakeVelForce(const Ctx& c, int lev, const std::vector<Arr>& vel_force,
                  const std::vector<Arr> uedge[3], const std::vector<Arr>& rho,
                  const std::vector<Arr>& gpi, const std::vector<Arr>& w0_force_cart,
@@ -165,6 +173,9 @@ int mxo_set_level(Ctx* c, int lev, int nbox, const mx_box* boxes) {
         if (c->p.dim == 2) bx.lo[2] = bx.hi[2] = 0;
         c->grids[lev].push_back(bx);
     }
+    // reflux_type 2: the level's flux register, Nscal components
+    // (MaestroInit.cpp:389-392)
+    if (lev > 0 && c->p.reflux_type == 2) return FluxRegDefine(*c, lev, c->p.nscal);
     return MX_OK;
 }
 
@@ -553,7 +564,7 @@ int mxo_advance_pass_levels(Ctx* c, int nlev, const mx_adv_pass_args* a) {
                           base(&A->w0), p.first_spec, p.num_spec);
         if (rc) return finish(rc);
     }
-    average_down_faces(sflux);
+    if (p.reflux_type == 1) average_down_faces(sflux);
     for (int lev = 0; lev < nlev; ++lev) {
         const mx_adv_pass_args* A = &a[lev];
         rc = MakeRhoHFlux(*c, lev, sflux[lev].d, sedge[lev].d, umac[lev].d, base(&A->rho0),
@@ -563,8 +574,88 @@ int mxo_advance_pass_levels(Ctx* c, int nlev, const mx_adv_pass_args* a) {
                           h0mac[lev].d);
         if (rc) return finish(rc);
     }
-    average_down_faces(sflux);
+    if (p.reflux_type == 1) average_down_faces(sflux);
     return finish(MX_OK);
+}
+
+// amrex::FluxRegister of level lev (Maestro::flux_reg_s[lev])
+int mxo_flux_reg_define(Ctx* c, int lev, int ncomp) {
+    if (!c) return MX_ERR_ARG;
+    return FluxRegDefine(*c, lev, ncomp);
+}
+
+static int reg_ok(const Ctx* c, int lev) {
+    if (!c || lev < 1 || lev > c->p.finest_level || lev >= MX_MAX_LEV) return MX_ERR_ARG;
+    return c->freg[lev].ncomp > 0 ? MX_OK : MX_ERR_ARG;
+}
+
+int mxo_flux_reg_setval(Ctx* c, int lev, double v) {
+    int rc = reg_ok(c, lev);
+    return rc ? rc : FluxRegSetVal(*c, lev, v);
+}
+
+static int dir_flux_ok(const Ctx* c, int lev, const std::vector<Arr>& f, int dir, int nc) {
+    if (f.size() != c->grids[lev].size()) return MX_ERR_ARG;
+    for (size_t b = 0; b < f.size(); ++b) {
+        const IBox r = nodal(c->grids[lev][b], dir);
+        if (!f[b].valid() || f[b].nc < nc) return MX_ERR_ARG;
+        for (int d = 0; d < 3; ++d)
+            if (f[b].b[d] > r.lo[d] || f[b].e[d] - 1 < r.hi[d]) return MX_ERR_BOX;
+    }
+    return MX_OK;
+}
+
+int mxo_flux_reg_crse_init(Ctx* c, int lev, const mx_multifab* flux, int dir, int srccomp,
+                           int destcomp, int numcomp, double mult) {
+    int rc = reg_ok(c, lev);
+    if (rc) return rc;
+    if (dir < 0 || dir >= c->p.dim || srccomp < 0 || destcomp < 0 || numcomp < 1 ||
+        destcomp + numcomp > c->freg[lev].ncomp)
+        return MX_ERR_ARG;
+    std::vector<Arr> f = arrs(flux);
+    if ((rc = dir_flux_ok(c, lev - 1, f, dir, srccomp + numcomp))) return rc;
+    return finish(FluxRegCrseInit(*c, lev, f, dir, srccomp, destcomp, numcomp, mult));
+}
+
+int mxo_flux_reg_fine_add(Ctx* c, int lev, const mx_multifab* flux, int dir, int srccomp,
+                          int destcomp, int numcomp, double mult) {
+    int rc = reg_ok(c, lev);
+    if (rc) return rc;
+    if (dir < 0 || dir >= c->p.dim || srccomp < 0 || destcomp < 0 || numcomp < 1 ||
+        destcomp + numcomp > c->freg[lev].ncomp)
+        return MX_ERR_ARG;
+    std::vector<Arr> f = arrs(flux);
+    if ((rc = dir_flux_ok(c, lev, f, dir, srccomp + numcomp))) return rc;
+    return finish(FluxRegFineAdd(*c, lev, f, dir, srccomp, destcomp, numcomp, mult));
+}
+
+int mxo_flux_reg_reflux(Ctx* c, int lev, mx_multifab* state, double scale, int scomp, int dcomp,
+                        int ncomp) {
+    int rc = reg_ok(c, lev);
+    if (rc) return rc;
+    if (scomp < 0 || dcomp < 0 || ncomp < 1 || scomp + ncomp > c->freg[lev].ncomp)
+        return MX_ERR_ARG;
+    std::vector<Arr> s = arrs(state);
+    if (s.size() != c->grids[lev - 1].size()) return MX_ERR_ARG;
+    for (size_t b = 0; b < s.size(); ++b)
+        if (!s[b].valid() || s[b].nc < dcomp + ncomp) return MX_ERR_ARG;
+    return finish(FluxRegReflux(*c, lev, s, scale, scomp, dcomp, ncomp));
+}
+
+int mxo_flux_reg_read(Ctx* c, int lev, double* host, long long cap, long long* n) {
+    int rc = reg_ok(c, lev);
+    if (rc) return rc;
+    if (!n) return MX_ERR_ARG;
+    long long total = 0;
+    for (const auto& v : c->freg[lev].val) total += (long long)v.size();
+    *n = total;
+    if (!host) return MX_OK;
+    if (cap < total) return MX_ERR_ARG;
+    for (const auto& v : c->freg[lev].val) {
+        std::copy(v.begin(), v.end(), host);
+        host += v.size();
+    }
+    return MX_OK;
 }
 
 // Maestro::FillPatchUedge (MaestroFillData.cpp:455-833) over levels

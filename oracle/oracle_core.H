@@ -184,9 +184,22 @@ inline BaseArr base(const mx_basestate* s) {
     return a;
 }
 
+// amrex::FluxRegister of level lev (the lev / lev - 1 interface): one face
+// box per (fine box b, orientation o), o = d for the low side of direction d
+// and dim + d for the high side (amrex::Orientation's order); the face box is
+// in the coarse level's face index space, one face thick.
+struct FluxReg {
+    int ncomp = 0;
+    std::vector<IBox> box;                 // [b * 2 * dim + o]
+    std::vector<std::vector<double>> val;  // same index; i fastest, comp outermost
+};
+
 struct Ctx {
     mx_adv_params p;
     std::vector<IBox> grids[MX_MAX_LEV];
+    // the flux registers are updated inside the flux stages (as the
+    // reference's MakeRhoXFlux / MakeRhoHFlux do), which see the Ctx as const
+    mutable FluxReg freg[MX_MAX_LEV];
     int dim() const { return p.dim; }
     IBox domain(int lev) const {
         IBox b = make_box(p.domain_lo[lev], p.domain_hi[lev]);
