@@ -1091,8 +1091,10 @@ def main():
     # all boxes (x the components the launch covers, folded into bytes/cell)
     ncomp_launch = {"k_es_ppm": None, "k_es_trans": None, "k_es_edge": None, "k_es_fused": None}
     if dname in ncomp_launch:
-        # MakeEdgeScal launches cover 3 species, then rho, then rhoh: average
-        nc = 5.0 / 3.0
+        # the pass's 5 edge-state components (3 species, rho', (rho h)')
+        # over the kernel's launches per pass (1: all in one launch; 3:
+        # species, then rho, then rhoh)
+        nc = 5.0 / (dcnt / args.steps)
     else:
         nc = 1
     bpc = kernel_bytes_per_cell(dname, nc, args.dim)
