@@ -158,7 +158,8 @@ typedef struct mx_adv_ctx mx_adv_ctx;
 int mx_adv_create(const mx_adv_params* params, mx_adv_ctx** out);
 int mx_adv_destroy(mx_adv_ctx* ctx);
 
-/* grids[lev]: the level's BoxArray (Maestro.H, AmrCore::grids). */
+/* grids[lev]: the level's BoxArray (Maestro.H, AmrCore::grids): disjoint
+   boxes inside the level's domain (MX_ERR_BOX otherwise). */
 int mx_adv_set_level(mx_adv_ctx* ctx, int lev, int nbox, const mx_box* boxes);
 
 /* dt and rel_eps, set per step in the reference by EstDt (MaestroDt.cpp:481-483). */
@@ -568,13 +569,43 @@ int mx_adv_convert_rhoh_to_h(mx_adv_ctx* ctx, int lev, mx_multifab* scal, int fl
         with an ext_dir side returns MX_ERR_BC (the reference aborts) before
         anything is written.
    Every FAB of mf is its valid box grown by one ghost width.  Level > 0
-   (FillPatchTwoLevels, coarse-fine interpolation) returns
-   MX_ERR_UNSUPPORTED.  There is no time argument: a time strictly between
-   t_old and t_new, for which GetData hands over both MultiFabs, has no
-   counterpart here. */
+   is mx_adv_fill_patch_two_levels; here it returns MX_ERR_UNSUPPORTED.
+   There is no time argument: a time strictly between t_old and t_new, for
+   which GetData hands over both MultiFabs, has no counterpart here. */
 int mx_adv_fill_patch(mx_adv_ctx* ctx, int lev, mx_multifab* mf, const mx_multifab* src,
                       int srccomp, int destcomp, int ncomp, const mx_bcrec* bcs,
                       int variable_type);
+
+/* Maestro::FillPatch at a level lev > 0 (MaestroFillData.cpp:48-66):
+   amrex::FillPatchTwoLevels with cell_cons_interp and the boundary functions
+   of mx_adv_fill_patch.  `crse` is level lev - 1's and `fine` level lev's
+   MultiFab that GetData picks (`fine` may be `mf`; `crse` needs its valid
+   cells only).  Steps:
+     1. fine patches: per FAB of mf, its cells in the fine domain (grown
+        without bound in the periodic directions) that no fine valid box or
+        periodic image holds;
+     2. per fine patch a coarse patch, the patch coarsened and grown by one
+        coarse cell, filled as FillPatchSingleLevel on level lev - 1 does
+        (copy from crse's valid boxes and images; the boundary rules with
+        the coarse domain, the patch's components numbered 0..ncomp-1 for
+        FillExtBC's Pi test, as in the reference);
+     3. CellConservativeLinear with linear limiting: per coarse cell and
+        direction the central slope (one-sided at a domain side with
+        ext_dir / hoextrap the coarse patch reaches), bounded by twice the
+        one-sided differences (zero at an extremum), and one factor per
+        direction shared by the ncomp components, the minimum of 1 and the
+        bounded-over-central ratios; a fine cell gets the coarse value plus
+        (i - ic r + 1/2) / r - 1/2 times each limited slope, x, y, z in order;
+     4. FillPatchSingleLevel on level lev from `fine` (mx_adv_fill_patch's
+        steps).
+   A coarse patch cell inside the domain that no level lev - 1 box holds
+   (levels not properly nested for this ghost width) returns MX_ERR_BOX
+   before anything is written; so do the checks of mx_adv_fill_patch.
+   ncomp is at most 32. */
+int mx_adv_fill_patch_two_levels(mx_adv_ctx* ctx, int lev, mx_multifab* mf,
+                                 const mx_multifab* crse, const mx_multifab* fine, int srccomp,
+                                 int destcomp, int ncomp, const mx_bcrec* bcs,
+                                 int variable_type);
 
 /* ---- Plotfile -------------------------------------------------------------- */
 /* amrex::WriteMultiLevelPlotfile(plotfilename, nlevels, mf, varnames, geom,

@@ -229,6 +229,8 @@ SIGNATURES = {
     "write_plotfile": [_vp, C.c_char_p, _i, P(P(MultiFabDesc)), _i, P(C.c_char_p), _d, P(_i),
                        P(_d), P(_d)],
     "fill_patch": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i, _i, P(BCRec), _i],
+    "fill_patch_two_levels": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), P(MultiFabDesc), _i, _i,
+                              _i, P(BCRec), _i],
     "average_down": [_vp, _i, P(MultiFabDesc), P(MultiFabDesc), _i, _i],
     "average_down_faces": [_vp, _i, P(FaceMultiFabDesc), P(FaceMultiFabDesc)],
     "fill_patch_uedge": [_vp, _i, P(FaceMultiFabDesc)],

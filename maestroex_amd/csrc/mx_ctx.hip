@@ -958,6 +958,18 @@ int mx_adv_set_level(mx_adv_ctx* c, int lev, int nbox, const mx_box* boxes) {
         }
         c->grids[lev].push_back(bx);
     }
+    // a BoxArray's boxes are disjoint
+    for (int a = 0; a < nbox; ++a)
+        for (int b = a + 1; b < nbox; ++b) {
+            const mx_box &x = c->grids[lev][a], &y = c->grids[lev][b];
+            bool overlap = true;
+            for (int d = 0; d < 3; ++d)
+                if (std::max(x.lo[d], y.lo[d]) > std::min(x.hi[d], y.hi[d])) overlap = false;
+            if (overlap) {
+                c->grids[lev].clear();
+                return MX_ERR_BOX;
+            }
+        }
     // plans depend on the BoxArray
     for (auto it = c->plans.begin(); it != c->plans.end();) {
         if (std::get<0>(it->first) == lev)

@@ -199,6 +199,21 @@ class Maestro:
                                            destcomp, ncomp, recs, int(variable_type)),
                     "FillPatch")
 
+    def FillPatchTwoLevels(self, lev, mf, crse_src, fine_src, srccomp, destcomp, ncomp,
+                           startbccomp, bcs_in, variable_type):
+        """Maestro::FillPatch on a level lev > 0 (MaestroFillData.cpp:48-66):
+        fine_src is level lev's and crse_src level lev - 1's MultiFab that
+        GetData picks for the fill time (mf itself, possibly)."""
+        if not isinstance(bcs_in, C.Array):
+            bcs_in = bcrec_array(bcs_in)
+        recs = bcrec_array(bcrec_to_list(bcs_in)[startbccomp:startbccomp + ncomp])
+        if len(recs) != ncomp:
+            raise _abi.MxError(_abi.MX_ERR_ARG, "FillPatch: bcs_in shorter than startbccomp + ncomp")
+        a = [_mf(mf), _mf(crse_src), _mf(fine_src)]
+        self._check(self.fns["fill_patch_two_levels"](self.ctx, lev, *[C.byref(x) for x in a],
+                                                      srccomp, destcomp, ncomp, recs,
+                                                      int(variable_type)), "FillPatchTwoLevels")
+
     def average_down(self, lev, fine, crse, comp, ncomp):
         """amrex::average_down between levels lev + 1 and lev (one step of
         Maestro::AverageDown)."""

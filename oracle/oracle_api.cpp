@@ -83,6 +83,9 @@ int ConvertByRho(const Ctx& c, int lev, const std::vector<Arr>& scal, int comp0,
                  bool flag);
 int AverageDown(const Ctx& c, int lev, const std::vector<Arr>& fine,
                 const std::vector<Arr>& crse, int comp, int ncomp);
+int FillPatchTwoLevels(const Ctx& c, int lev, const std::vector<Arr>& mf,
+                       const std::vector<Arr>& crse, const std::vector<Arr>& fine, int scomp,
+                       int dcomp, int ncomp, const mx_bcrec* bcs, bool is_vel);
 int FillPatchSingleLevel(const Ctx& c, int lev, const std::vector<Arr>& mf,
                          const std::vector<Arr>& src, int scomp, int dcomp, int ncomp,
                          const mx_bcrec* bcs, bool is_vel);
@@ -173,6 +176,19 @@ int mxo_set_level(Ctx* c, int lev, int nbox, const mx_box* boxes) {
         if (c->p.dim == 2) bx.lo[2] = bx.hi[2] = 0;
         c->grids[lev].push_back(bx);
     }
+    // a BoxArray's boxes are disjoint
+    for (int a = 0; a < nbox; ++a)
+        for (int b = a + 1; b < nbox; ++b) {
+            const IBox &x = c->grids[lev][a], &y = c->grids[lev][b];
+            bool overlap = true;
+            for (int d = 0; d < 3; ++d)
+                if ((x.lo[d] > y.lo[d] ? x.lo[d] : y.lo[d]) > (x.hi[d] < y.hi[d] ? x.hi[d] : y.hi[d]))
+                    overlap = false;
+            if (overlap) {
+                c->grids[lev].clear();
+                return MX_ERR_BOX;
+            }
+        }
     // reflux_type 2: the level's flux register, Nscal components
     // (MaestroInit.cpp:389-392)
     if (lev > 0 && c->p.reflux_type == 2) return FluxRegDefine(*c, lev, c->p.nscal);
@@ -576,6 +592,46 @@ int mxo_advance_pass_levels(Ctx* c, int nlev, const mx_adv_pass_args* a) {
     }
     if (p.reflux_type == 1) average_down_faces(sflux);
     return finish(MX_OK);
+}
+
+// Maestro::FillPatch on level lev > 0 (FillPatchTwoLevels with
+// cell_cons_interp): crse is level lev - 1's source, fine level lev's
+int mxo_fill_patch_two_levels(Ctx* c, int lev, mx_multifab* mf, const mx_multifab* crse,
+                              const mx_multifab* fine, int srccomp, int destcomp, int ncomp,
+                              const mx_bcrec* bcs, int variable_type) {
+    int rc = check_level(c, lev);
+    if (rc) return rc;
+    if (lev < 1 || c->grids[lev - 1].empty()) return MX_ERR_ARG;
+    const size_t nb = c->grids[lev].size(), ncb = c->grids[lev - 1].size();
+    if (!mf || !crse || !fine || !bcs || mf->nfab != (int)nb || fine->nfab != (int)nb ||
+        crse->nfab != (int)ncb || srccomp < 0 || destcomp < 0 || ncomp < 1 ||
+        ncomp > MX_MAX_SCAL)
+        return MX_ERR_ARG;
+    std::vector<Arr> m = arrs(mf), f = arrs(fine), cr = arrs(crse);
+    if (m[0].p == f[0].p && srccomp != destcomp && srccomp < destcomp + ncomp &&
+        destcomp < srccomp + ncomp)
+        return MX_ERR_ARG;
+    for (size_t b = 0; b < ncb; ++b) {
+        const IBox& g = c->grids[lev - 1][b];
+        if (!cr[b].valid() || cr[b].nc < srccomp + ncomp) return MX_ERR_ARG;
+        for (int d = 0; d < 3; ++d)
+            if (cr[b].b[d] > g.lo[d] || cr[b].e[d] - 1 < g.hi[d]) return MX_ERR_BOX;
+    }
+    for (size_t b = 0; b < nb; ++b) {
+        const IBox& g = c->grids[lev][b];
+        if (!m[b].valid() || !f[b].valid() || m[b].nc < destcomp + ncomp ||
+            f[b].nc < srccomp + ncomp)
+            return MX_ERR_ARG;
+        for (int d = 0; d < 3; ++d) {
+            const int ng = g.lo[d] - m[b].b[d];
+            if (ng < 0 || m[b].e[d] - 1 - g.hi[d] != ng || f[b].b[d] > g.lo[d] ||
+                f[b].e[d] - 1 < g.hi[d])
+                return MX_ERR_BOX;
+            if (b > 0 && ng != c->grids[lev][0].lo[d] - m[0].b[d]) return MX_ERR_BOX;
+        }
+    }
+    return finish(FillPatchTwoLevels(*c, lev, m, cr, f, srccomp, destcomp, ncomp, bcs,
+                                     variable_type == 1));
 }
 
 // amrex::FluxRegister of level lev (Maestro::flux_reg_s[lev])

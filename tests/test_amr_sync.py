@@ -32,7 +32,7 @@ def two_level(periodic=False):
     g0 = Geometry(3, [16] * 3, [1.0 / 16] * 3, per, [bc] * 3, [bc] * 3)
     g1 = Geometry(3, [32] * 3, [1.0 / 32] * 3, per, [bc] * 3, [bc] * 3)
     b0 = [((0, 0, 0), (7, 15, 15)), ((8, 0, 0), (15, 15, 15))]
-    b1 = [((8, 8, 8), (15, 23, 15)), ((16, 8, 8), (23, 15, 15)), ((4, 20, 0), (11, 27, 9))]
+    b1 = [((8, 8, 8), (15, 23, 15)), ((16, 8, 8), (23, 15, 15)), ((4, 24, 0), (11, 31, 9))]
     params = make_params(g0, extra_levels=[g1])
     return params, [b0, b1]
 
