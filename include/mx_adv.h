@@ -215,10 +215,6 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
      "velpred_fused_tile" tile and register budget of the per-component
                         fused VelPred: 0 (default): 16 x 11 columns, 3 waves
                         per SIMD; 1: 16 x 8, 3; 2: 16 x 11, 4; 3: 32 x 8, 3.
-     "utrans_line"      1: 3-D MakeUtrans with ppm_type 1/2 runs along lines
-                        of faces (each cell's PPM once); 0 (default): one
-                        face per thread.
-     "utrans_tz"        faces per chunk of a line (default 32).
      "xcd_remap"        1: the staged VelPred, MakeUtrans and flux kernels
                         give each of the 8 XCDs one contiguous eighth of a
                         box's blocks; 0 (default): blocks in launch order.

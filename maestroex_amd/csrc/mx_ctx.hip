@@ -860,12 +860,6 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
     } else if (k == "velpred_fused") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->vp_fused = (int)value;
-    } else if (k == "utrans_line") {
-        if (value != 0 && value != 1) return MX_ERR_ARG;
-        c->utrans_line = (int)value;
-    } else if (k == "utrans_tz") {
-        if (value < 1 || value > 4096) return MX_ERR_ARG;
-        c->utrans_tz = (int)value;
     } else if (k == "velpred_fused_comp") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->vp_fused_comp = (int)value;

@@ -116,8 +116,6 @@ struct mx_adv_ctx {
     int vp_fused = 1;      // VelPred: 1 = the fused z-marching kernel where it applies
     int vp_fused_tz = 32;  // planes per chunk of the fused VelPred kernel
     int vp_fused_comp = 1; // fused VelPred: 1 = one velocity component per block
-    int utrans_line = 0;   // MakeUtrans (3-D, ppm_type 1/2): 1 = along lines of faces
-    int utrans_tz = 32;    // faces per chunk of a line
     int vp_fused_tile = 0; // its tile (waves per SIMD): 0 16x11 (3), 1 16x8 (3), 2 16x11 (4), 3 32x8 (3)
     int fused_tile = -1;    // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8,
                             // 4: 16 x 11; -1: 16 x 11 for ppm_type 1, else 16 x 12

@@ -172,106 +172,6 @@ __global__ void __launch_bounds__(256) k_utrans(const UtBox* __restrict__ boxes,
         utrans_face<PPM, 2>(B, g, A, i, j, k);
 }
 
-// MakeUtrans along lines (ppm_type 1/2): one thread per line of D-faces,
-// marching along D over a chunk of them with a register window of the normal
-// velocity component, so each cell's PPM, slope and face value is evaluated
-// once instead of once per adjacent face.  The per-cell arithmetic is that of
-// ppm_edges (type 1: the face value shared by the two cells beside it;
-// boundary stencils through the full routine) and the Riemann solve is
-// utrans_riemann, so the results equal k_utrans's.
-struct UtLine {
-    int box;
-    int d;       // face direction
-    int f0, f1;  // faces f0 .. f1 along d
-    int t0lo, t0n, t1lo, t1n;  // the transverse ranges (first fastest)
-};
-
-template <int PPM, int D>
-__device__ __forceinline__ void utrans_line(const UtBox& B, const KGeom& g, const UtArgs& A,
-                                            const UtLine& L, int a0, int a1) {
-    constexpr int G = PPM == 2 ? 3 : 2;
-    // transverse directions: the first one fastest across the threads
-    constexpr int T0 = D == 0 ? 1 : 0;
-    constexpr int T1 = D == 2 ? 1 : 2;
-    int p[3];
-    p[T0] = a0;
-    p[T1] = a1;
-    const int bclo = A.bclo[D][D], bchi = A.bchi[D][D];
-    const int dlo = g.domlo[D], dhi = g.domhi[D];
-    const long long st = B.ut.stride(D);
-    const long long fst = B.uf.stride(D);
-    p[D] = L.f0 - 1;
-    const double* up = B.ut.p + B.ut.off(p[0], p[1], p[2], D);  // cell f0 - 1
-    const double* fp = B.uf.p + B.uf.off(p[0], p[1], p[2], D);
-    double v[9];
-#pragma unroll
-    for (int jj = 0; jj < 9; ++jj) v[jj] = 0.0;
-#pragma unroll
-    for (int jj = 5 - G; jj <= 4 + G; ++jj) v[jj] = up[(jj - 5) * st];
-    double f0v = 0.0, f1v = 0.0, f2v = 0.0, slv = 0.0, Ip_prev = 0.0;
-    for (int s = L.f0 - 1; s <= L.f1; ++s) {
-#pragma unroll
-        for (int jj = 0; jj < 8; ++jj) v[jj] = v[jj + 1];
-        v[4 + G] = up[G * st];  // cell s + G
-        double sm, sp;
-        if (PPM == 1) {
-            if (s == L.f0 - 1) {
-                const double d0 = vl_slope(v[2], v[3], v[4]);
-                slv = vl_slope(v[3], v[4], v[5]);
-                f0v = face1(v[3], v[4], d0, slv);
-            }
-            const double d1 = vl_slope(v[4], v[5], v[6]);
-            f1v = face1(v[4], v[5], slv, d1);
-            slv = d1;
-            sm = f0v;
-            sp = f1v;
-            cw_limit(v[4], sm, sp);
-            f0v = f1v;
-        } else {
-            if (s == L.f0 - 1) {
-                f0v = cs_face(v[1], v[2], v[3], v[4]);
-                f1v = cs_face(v[2], v[3], v[4], v[5]);
-                f2v = cs_face(v[3], v[4], v[5], v[6]);
-            }
-            const double f3 = cs_face(v[4], v[5], v[6], v[7]);
-            cs_cell(v, f0v, f1v, f2v, f3, 1.e-10, sm, sp);
-            f0v = f1v;
-            f1v = f2v;
-            f2v = f3;
-        }
-        if (ppm_special<PPM>(s, dlo, dhi, bclo, bchi))
-            ppm_edges<PPM, D>(v, s, dlo, dhi, bclo, bchi, sm, sp);
-        const double uc = *fp;
-        double Ip, Im;
-        ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[D], g.rel_eps, Ip, Im);
-        if (s >= L.f0) {
-            p[D] = s;
-            utrans_riemann<D>(B, g, p[0], p[1], p[2], Ip_prev, Im);
-        }
-        Ip_prev = Ip;
-        up += st;
-        fp += fst;
-    }
-}
-
-template <int PPM>
-__global__ void __launch_bounds__(256) k_utrans_line(const UtBox* __restrict__ boxes,
-                                                     const UtLine* __restrict__ lines, KGeom g,
-                                                     UtArgs A) {
-    const UtLine L = lines[blockIdx.y];
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)L.t0n * L.t1n) return;
-    const int a0 = L.t0lo + (int)(idx % L.t0n);
-    const int a1 = L.t1lo + (int)(idx / L.t0n);
-    const UtBox& B = boxes[L.box];
-    if (L.d == 0)
-        utrans_line<PPM, 0>(B, g, A, L, a0, a1);
-    else if (L.d == 1)
-        utrans_line<PPM, 1>(B, g, A, L, a0, a1);
-    else
-        utrans_line<PPM, 2>(B, g, A, L, a0, a1);
-}
-
 // ---------------------------------------------------------------------------
 // VelPred
 
@@ -813,44 +713,6 @@ int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mult
     KGeom kg = kgeom(c, lev);
     const int block = 256;
     const UtBox* dp = tab.dev<UtBox>(off);
-    if (c->utrans_line && dim == 3 && c->p.ppm_type != 0) {
-        // lines of faces, cut into chunks of at most `utrans_tz` faces of
-        // near-equal length
-        std::vector<UtLine> lines;
-        long long maxl = 0;
-        for (size_t b = 0; b < grids.size(); ++b) {
-            HBox g = hbox(grids[b]);
-            for (int d = 0; d < 3; ++d) {
-                const int t0 = d == 0 ? 1 : 0, t1 = d == 2 ? 1 : 2;
-                const int nf = g.hi[d] - g.lo[d] + 2;
-                const int nch = (nf + c->utrans_tz - 1) / c->utrans_tz;
-                for (int q = 0; q < nch; ++q) {
-                    UtLine L;
-                    L.box = (int)b;
-                    L.d = d;
-                    L.f0 = g.lo[d] + (int)((long long)nf * q / nch);
-                    L.f1 = g.lo[d] + (int)((long long)nf * (q + 1) / nch) - 1;
-                    L.t0lo = g.lo[t0];
-                    L.t0n = g.hi[t0] - g.lo[t0] + 1;
-                    L.t1lo = g.lo[t1];
-                    L.t1n = g.hi[t1] - g.lo[t1] + 1;
-                    maxl = std::max(maxl, (long long)L.t0n * L.t1n);
-                    lines.push_back(L);
-                }
-            }
-        }
-        Table lt(c);
-        const size_t ol = lt.add(lines);
-        if ((rc = lt.upload())) return rc;
-        const dim3 lg((unsigned)((maxl + block - 1) / block), (unsigned)lines.size());
-        if (c->p.ppm_type == 1)
-            MX_LAUNCH(c, "k_utrans", k_utrans_line<1>, lg, dim3(block), dp,
-                      lt.dev<UtLine>(ol), kg, A);
-        else
-            MX_LAUNCH(c, "k_utrans", k_utrans_line<2>, lg, dim3(block), dp,
-                      lt.dev<UtLine>(ol), kg, A);
-        return fill ? fill_face_ghosts(c, lev, utrans) : MX_OK;
-    }
     dim3 gr = grid_for(maxpts, (int)boxes.size(), dim, block);
     if (c->p.ppm_type == 0)
         MX_LAUNCH(c, "k_utrans", k_utrans<0>,  gr, dim3(block), dp, kg, A);

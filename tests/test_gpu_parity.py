@@ -102,30 +102,6 @@ def test_pass_kernel_choices(name, cfg, fused, tz):
     assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
 
 
-UT_CASES = [CASES[0], CASES[1], CASES[7], CASES[9], CASES[10], CASES[11], CASES[15],
-            CASES[16], KERNEL_CHOICE_CASES[3]]
-
-
-@pytest.mark.parametrize("line,tz", [(0, 32), (1, 1), (1, 3), (1, 7)])
-@pytest.mark.parametrize("name,cfg", UT_CASES, ids=[c[0] for c in UT_CASES])
-def test_pass_utrans_line(name, cfg, line, tz):
-    """MakeUtrans per face (the default) and along lines of faces (3-D,
-    ppm_type 1/2) cut into chunks of 1, 3 and 7 faces (chunk ends at the
-    domain boundary stencils, at inflow faces and inside boxes): utrans and
-    the whole pass bit-exact."""
-    print(f"[parity] utrans line={line} tz={tz} {name}", flush=True)
-    wc, wg = pair(**cfg)
-    _run_pass(oracle_for(wc), wc)
-    m = gpu_for(wg)
-    m.set_tuning("utrans_line", line)
-    m.set_tuning("utrans_tz", tz)
-    _run_pass(m, wg)
-    assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
-    assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
-    comps = [0, 1] + list(range(4, wc.nscal))
-    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
-
-
 XCD_CASES = [CASES[0], CASES[3], CASES[7], CASES[10], CASES[16], CASES[17]]
 
 
