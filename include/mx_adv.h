@@ -215,6 +215,10 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
      "velpred_fused_tile" tile and register budget of the per-component
                         fused VelPred: 0 (default): 16 x 11 columns, 3 waves
                         per SIMD; 1: 16 x 8, 3; 2: 16 x 11, 4; 3: 32 x 8, 3.
+     "rhoh_flux_epilogue" 1 (default): in the pass, MakeRhoHFlux's planar
+                        branches that read only the RhoH edge state
+                        (enthalpy_pred_type 0, 1, 3) run as the fused edge
+                        kernel's epilogue; 0: the flux kernel.
      "xcd_remap"        1: the staged VelPred, MakeUtrans and flux kernels
                         give each of the 8 XCDs one contiguous eighth of a
                         box's blocks; 0 (default): blocks in launch order.

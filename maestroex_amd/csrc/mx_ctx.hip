@@ -860,6 +860,9 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
     } else if (k == "velpred_fused") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->vp_fused = (int)value;
+    } else if (k == "rhoh_flux_epilogue") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->rhoh_flux_epilogue = (int)value;
     } else if (k == "velpred_fused_comp") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->vp_fused_comp = (int)value;

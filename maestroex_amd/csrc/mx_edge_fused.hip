@@ -61,6 +61,10 @@ struct FBox {
     // geometry of the box's level
     double dx[3];
     int domlo[3], domhi[3];
+    // the flux epilogue (FComp::flux): the fluxes and the level's base
+    // state at cell heights (bc_a, bc_b: old, new) and at edges (be_a, be_b)
+    DA sf[3];
+    const double *bc_a, *bc_b, *be_a, *be_b;
 };
 
 // one block's work: a tile of one box, chunk tzc of its planes
@@ -71,6 +75,10 @@ struct FTile {
 struct FComp {
     int bclo[3], bchi[3];
     int comp;  // state / edge-state component
+    // flux epilogue of this component (MakeRhoHFlux's planar branches,
+    // MaestroMakeFlux.cpp:557-620): 0 none; 1 umac (rhoh0_edge + sedge)
+    // (enthalpy_pred_type 1, 3); 2 umac sedge (enthalpy_pred_type 0)
+    int flux;
 };
 
 struct FArgs {
@@ -79,6 +87,22 @@ struct FArgs {
     int is_vel;
     int tz;  // planes per chunk
 };
+
+// the flux of edge state `se` on d-face (i, j, k) (MakeRhoHFlux, planar,
+// MaestroMakeFlux.cpp:557-620: umac (rhoh0_edge + sedge), the base state at
+// the face's height, averaged over old and new, from the edge-centred state on
+// vertical faces)
+__device__ __forceinline__ void flux_store(const FBox& B, int d, int i, int j, int k, int comp,
+                                           int kind, double um, double se, bool vertical) {
+    double f;
+    if (kind == 1) {
+        const double b0 = vertical ? 0.5 * (B.be_a[k] + B.be_b[k]) : 0.5 * (B.bc_a[k] + B.bc_b[k]);
+        f = um * (b0 + se);
+    } else {
+        f = um * se;
+    }
+    B.sf[d](i, j, k, comp) = f;
+}
 
 // predictor-style boundary overrides on a (left, right) pair at face `pos`
 __device__ __forceinline__ void bc_pair(int pos, int dlo, int dhi, int bclo, int bchi,
@@ -209,6 +233,7 @@ __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_
 
     const int vc = blockIdx.z;
     const int comp = comps[vc].comp;
+    const int flux = comps[vc].flux;
     const int bcx0 = comps[vc].bclo[0], bcx1 = comps[vc].bchi[0];
     const int bcy0 = comps[vc].bclo[1], bcy1 = comps[vc].bchi[1];
     const int bcz0 = comps[vc].bclo[2], bcz1 = comps[vc].bchi[2];
@@ -675,6 +700,7 @@ __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_
                 res = bc_edge(res, I, dlx, dhx, bcx0, bcx1, vn0, SC[tid - 1], SC[tid], sedgel,
                               sedger);
             e0Col[-e0ks] = res;  // plane p
+            if (flux) flux_store(boxes[bid], 0, I, J, q - 1, comp, flux, u0, res, false);
         }
         if (pplane && col && b >= 1 && a >= 1 && a <= nx && (b <= ny || last_y)) {  // y-faces, p
             double l = sly_p - P2[2 * NE + tid - W];
@@ -693,6 +719,7 @@ __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_
                 res = bc_edge(res, J, dly, dhy, bcy0, bcy1, vn1, SC[tid - W], SC[tid], sedgel,
                               sedger);
             e1Col[-e1ks] = res;  // plane p
+            if (flux) flux_store(boxes[bid], 1, I, J, q - 1, comp, flux, u0, res, false);
         }
         if (((zface && q <= z1) || (q == z1 + 1 && last_z)) && col && a >= 1 && a <= nx &&
             b >= 1 && b <= ny) {  // z-face q
@@ -709,6 +736,7 @@ __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_
             double res = upwind(sedgel, sedger, w_q, rel_eps);
             res = bc_edge(res, q, dlz, dhz, bcz0, bcz1, vn2, v[3], v[4], sedgel, sedger);
             *e2Col = res;  // z-face q
+            if (flux) flux_store(boxes[bid], 2, I, J, q, comp, flux, w_q, res, true);
         }
 
         // ---- advance the column state to the next plane
@@ -819,7 +847,12 @@ int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int 
             for (int d = 0; d < 3; ++d) {
                 B.um[d] = to_da(L.umac->dir[d].fab[b]);
                 B.se[d] = to_da(L.sedge->dir[d].fab[b]);
+                B.sf[d] = L.sflux ? to_da(L.sflux->dir[d].fab[b]) : null_da();
             }
+            B.bc_a = L.bc_a;
+            B.bc_b = L.bc_b;
+            B.be_a = L.be_a;
+            B.be_b = L.be_b;
             // the kernel keeps the plane strides in 32 bits
             const DA* views[] = {&B.s,     &B.fo,    &B.um[0], &B.um[1],
                                  &B.um[2], &B.se[0], &B.se[1], &B.se[2]};
@@ -854,6 +887,7 @@ int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int 
             comps[vc].bchi[d] = bcs[bccomps[vc]].hi[d];
         }
         comps[vc].comp = scomps[vc];
+        comps[vc].flux = (levs[0].sflux && scomps[vc] == levs[0].flux_comp) ? levs[0].flux_kind : 0;
     }
     FArgs A;
     A.start_scomp = scomps[0];

@@ -116,6 +116,8 @@ struct mx_adv_ctx {
     int vp_fused = 1;      // VelPred: 1 = the fused z-marching kernel where it applies
     int vp_fused_tz = 32;  // planes per chunk of the fused VelPred kernel
     int vp_fused_comp = 1; // fused VelPred: 1 = one velocity component per block
+    int rhoh_flux_epilogue = 1;  // the pass: MakeRhoHFlux (planar, ept 0/1/3) in the
+                                 // fused edge kernel's epilogue
     int vp_fused_tile = 0; // its tile (waves per SIMD): 0 16x11 (3), 1 16x8 (3), 2 16x11 (4), 3 32x8 (3)
     int fused_tile = -1;    // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8,
                             // 4: 16 x 11; -1: 16 x 11 for ppm_type 1, else 16 x 12
@@ -407,6 +409,13 @@ struct EdgeLevel {
     const mx_face_multifab* umac;
     const mx_multifab* force;
     int box = -1;  // one box of the level, or -1 for all
+    // optional flux epilogue for state component flux_comp (MakeRhoHFlux's
+    // planar branch of kind flux_kind: 1 umac (rhoh0_edge + sedge), 2 umac
+    // sedge) into sflux, with the level's base states (cell, edge; old, new)
+    int flux_comp = -1;
+    int flux_kind = 0;
+    const mx_face_multifab* sflux = nullptr;
+    const double *bc_a = nullptr, *bc_b = nullptr, *be_a = nullptr, *be_b = nullptr;
 };
 bool fused_edge_scal_ok(const mx_adv_ctx* c, int is_conservative);
 int make_edge_scal_fused_levels(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int is_vel,

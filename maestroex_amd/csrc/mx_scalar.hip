@@ -1016,10 +1016,27 @@ int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_arg
         if ((rc = sync_levels(&mx_adv_pass_args::umac, true))) return rc;
     }
     const int groups[3][2] = {{p.first_spec, p.num_spec}, {p.comp_rho, 1}, {p.comp_rhoh, 1}};
+    // MakeRhoHFlux's planar branches that read only the RhoH edge state run
+    // as the fused edge kernel's epilogue (the flux from the edge state in
+    // registers); the register updates of reflux_type 2 follow as usual
+    const int ept = p.enthalpy_pred_type;
+    const bool rhoh_epilogue = fused_edge_scal_ok(c, 0) && c->rhoh_flux_epilogue && !sph &&
+                               (ept == 0 || ept == 1 || ept == 3);
     if (fused_edge_scal_ok(c, 0)) {
         std::vector<EdgeLevel> levs;
-        for (int i = 0; i < nlev; ++i)
-            levs.push_back({lev0 + i, &a[i].scal, &a[i].sedge, &a[i].umac, &a[i].scal_force});
+        for (int i = 0; i < nlev; ++i) {
+            EdgeLevel L{lev0 + i, &a[i].scal, &a[i].sedge, &a[i].umac, &a[i].scal_force};
+            if (rhoh_epilogue) {
+                const long long o = (long long)(lev0 + i) * a[i].rhoh0.len;
+                const long long oe = (long long)(lev0 + i) * a[i].rhoh0_edge.len;
+                L.flux_comp = p.comp_rhoh;
+                L.flux_kind = ept == 0 ? 2 : 1;
+                L.sflux = &a[i].sflux;
+                L.bc_a = L.bc_b = a[i].rhoh0.p + o;       // old and new: the pass's one state
+                L.be_a = L.be_b = a[i].rhoh0_edge.p + oe;
+            }
+            levs.push_back(L);
+        }
         // MakeEdgeScal's three calls (species, Rho, RhoH) read the same
         // inputs and write disjoint components: one launch covers them all
         std::vector<int> sc;
@@ -1043,10 +1060,14 @@ int advance_pass_levels(mx_adv_ctx* c, int lev0, int nlev, const mx_adv_pass_arg
                                  &A->rho0, &A->rho0_edge, r0m, &A->rho0, &A->rho0_edge, r0m,
                                  &A->rho0_predicted_edge, &A->w0, p.first_spec, p.num_spec, 1)))
             return rc;
-        if ((rc = make_rhoh_flux(c, lev, &A->sflux, &A->sedge, &A->umac, &A->rho0, &A->rho0_edge,
-                                 r0m, &A->rho0, &A->rho0_edge, r0m, &A->rhoh0, &A->rhoh0_edge,
-                                 rh0m, &A->rhoh0, &A->rhoh0_edge, rh0m, h0m, h0m)))
+        if (rhoh_epilogue) {
+            if ((rc = flux_reg_ops(c, lev, &A->sflux, false, 0, 0))) return rc;
+        } else if ((rc = make_rhoh_flux(c, lev, &A->sflux, &A->sedge, &A->umac, &A->rho0,
+                                        &A->rho0_edge, r0m, &A->rho0, &A->rho0_edge, r0m,
+                                        &A->rhoh0, &A->rhoh0_edge, rh0m, &A->rhoh0,
+                                        &A->rhoh0_edge, rh0m, h0m, h0m))) {
             return rc;
+        }
     }
     // reflux_type 1: AverageDownFaces(sflux) after MakeRhoXFlux and after
     // MakeRhoHFlux (MaestroMakeFlux.cpp:459-462, 1061-1064).  MakeRhoHFlux

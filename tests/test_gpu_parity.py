@@ -102,6 +102,27 @@ def test_pass_kernel_choices(name, cfg, fused, tz):
     assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
 
 
+@pytest.mark.parametrize("epilogue", [0, 1])
+@pytest.mark.parametrize("ept,spt", [(0, 1), (1, 1), (2, 1), (3, 2), (4, 3), (5, 1), (6, 2),
+                                     (1, 3)])
+def test_pass_enthalpy_pred_types(ept, spt, epilogue):
+    """The whole pass for every enthalpy_pred_type (with species_pred_type
+    1-3): MakeRhoHFlux's planar branches that read only the RhoH edge state
+    (ept 0, 1, 3) run as the fused edge kernel's epilogue by default, the
+    others as the flux kernel; every output bit-exact, walls included."""
+    cfg = dict(CASES[9][1], enthalpy_pred_type=ept, species_pred_type=spt)
+    print(f"[parity] pass ept={ept} spt={spt} epilogue={epilogue}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    m.set_tuning("rhoh_flux_epilogue", epilogue)
+    _run_pass(m, wg)
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"ept {ept} sedge")
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"ept {ept} sflux")
+    assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"ept {ept} eta")
+
+
 XCD_CASES = [CASES[0], CASES[3], CASES[7], CASES[10], CASES[16], CASES[17]]
 
 
