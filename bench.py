@@ -1098,6 +1098,11 @@ def main():
     else:
         nc = 1
     bpc = kernel_bytes_per_cell(dname, nc, args.dim)
+    if (dname == "k_es_fused" and not args.spherical and nc == 5 and
+            "rhoh_flux_epilogue=0" not in args.tune):
+        # the (rho h)' blocks also write the enthalpy flux (the pass's
+        # MakeRhoHFlux for enthalpy_pred_type 1 in the kernel's epilogue)
+        bpc += 8 * args.dim
     achieved = bpc * cells / (avg_ms * 1e-3) / 1e9 if bpc else None
     roofline = {
         "bound": "hbm",

@@ -27,6 +27,8 @@ Ghost cells are filled as FillPatch would: periodic images in periodic
 directions, first-order extrapolation (copy of the nearest valid cell) at
 the outflow boundary.
 """
+import sys
+
 import numpy as np
 
 from . import _abi
@@ -560,7 +562,7 @@ class Hierarchy:
 
         fng = 1
         if self.verbose:
-            print(f"[hierarchy] level {l}: {len(boxes)} boxes", flush=True)
+            print(f"[hierarchy] level {l}: {len(boxes)} boxes", file=sys.stderr, flush=True)
         out["utilde"] = cellmf(vel, 3, ng)
         out["ufull"] = cellmf(ufull, 3, ng)
         self._vkey = None
