@@ -2,11 +2,13 @@
 // and its forward-mode derivative (test infrastructure only, see
 // oracle_core.H).
 //
-// The step, on one fully periodic box [0, N)^3 with a fixed MAC velocity
-// (face N is face 0), for each scalar component:
+// The step, on the cells of a domain with a fixed MAC velocity, each
+// direction periodic or outflow (foextrap ghost cells and the foextrap
+// rules at the domain faces), for each scalar component:
 //   sedge = MakeEdgeScal(s, umac, force = 0, advective form, ppm_type 1 or 2)
 //           (MaestroMakeEdgeScalars.cpp: PPM, Predictor :662-863, Transverse
-//            :865-1359, Edges :1361-1667; interior formulas only, periodic)
+//            :865-1359, Edges :1361-1667; interior PPM formulas: no
+//            ext_dir or hoextrap sides)
 //   F     = umac * sedge                     (MakeRhoXFlux, predict_rhoX,
 //                                             MaestroMakeFlux.cpp:118-140)
 //   snew  = s + dt * (-div F + 0)            (UpdateScal, MaestroAdvection.cpp:165-183)
@@ -14,6 +16,8 @@
 // value and one directional derivative) gives the Jacobian-vector product
 // with the limiter branches taken at the primal values.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "oracle_core.H"
@@ -256,11 +260,49 @@ T upwind(T l, T r, T u, double thr) {
     return res;
 }
 
+// The step's index space: n[d] cells per direction, each direction periodic
+// (faces n[d] and 0 coincide, cells wrap) or outflow (foextrap: faces
+// 0 .. n[d], a ghost cell takes the value of the nearest valid cell).
+// Cell arrays hold the n0 n1 n2 valid cells, the face arrays of direction d
+// one more point along d when d is outflow; face (i,j,k) of direction d is
+// the face below cell (i,j,k).
 struct Grid {
-    int n;
+    int n[3];
+    int per[3];
+    int fe(int d, int q) const { return n[q] + (q == d && !per[q] ? 1 : 0); }
+    long long flen(int d) const { return (long long)fe(d, 0) * fe(d, 1) * fe(d, 2); }
+    long long cells() const { return (long long)n[0] * n[1] * n[2]; }
+    int wrap(int x, int q) const { return ((x % n[q]) + n[q]) % n[q]; }
+    // state read: periodic image, or the nearest valid cell (the foextrap
+    // ghost fill, MaestroBCFill.cpp:8-171)
     long long at(int i, int j, int k) const {
-        auto w = [&](int x) { return ((x % n) + n) % n; };
-        return (long long)w(i) + (long long)n * ((long long)w(j) + (long long)n * w(k));
+        auto w = [&](int x, int q) {
+            return per[q] ? wrap(x, q) : (x < 0 ? 0 : (x >= n[q] ? n[q] - 1 : x));
+        };
+        return (long long)w(i, 0) + (long long)n[0] * ((long long)w(j, 1) + (long long)n[1] * w(k, 2));
+    }
+    // face (i,j,k) of direction d; outflow positions must lie in the array
+    long long face(int d, int i, int j, int k) const {
+        const int p[3] = {i, j, k};
+        long long r = 0, m = 1;
+        for (int q = 0; q < 3; ++q) {
+            int x = p[q];
+            if (per[q]) {
+                x = wrap(x, q);
+            } else if (x < 0 || x >= fe(d, q)) {
+                std::fprintf(stderr, "oracle step: face (%d,%d,%d) of direction %d out of range\n",
+                             i, j, k, d);
+                std::abort();
+            }
+            r += m * x;
+            m *= fe(d, q);
+        }
+        return r;
+    }
+    // 1: d-face at the low outflow side, 2: at the high one, else 0
+    int bnd(int d, int pos) const {
+        if (per[d]) return 0;
+        return pos == 0 ? 1 : (pos == n[d] ? 2 : 0);
     }
 };
 
@@ -270,32 +312,43 @@ inline void step_e(int d, int& a, int& b, int& c) {
     c = d == 2;
 }
 
-// one step of one component; s, u[3] and the output are N^3 arrays
+// the boundary rule of a d-face at an outflow side, for the predictor,
+// transverse and edge states alike (MaestroMakeEdgeScalars.cpp:709-738,
+// :931-961, :1400-1430): the state from outside the domain takes the
+// inside one (foextrap)
+template <class T>
+inline void outflow_face(int b, T& l, T& r) {
+    if (b == 1) l = r;
+    else if (b == 2) r = l;
+}
+
+// one step of one component; s and snew are cell arrays, u[d] face arrays
 // marks (op counting only): the counter after each of the five phases
 template <class T>
 void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double dt,
                const double dx[3], double rel_eps, long long* marks = nullptr, int ppm = 1) {
-    const int n = g.n;
-    const long long N3 = (long long)n * n * n;
+    const long long NC = g.cells();
+    auto cell = [&](int i, int j, int k) { return g.at(i, j, k); };
     std::vector<T> Ip[3], Im[3], simh[3], corner[6], se[3];
-    for (int d = 0; d < 3; ++d) {
-        Ip[d].resize(N3);
-        Im[d].resize(N3);
-        simh[d].resize(N3);
-        se[d].resize(N3);
-    }
-    for (auto& c : corner) c.resize(N3);
     auto pair_index = [](int d, int t) { return d * 2 + (t > d ? t - 1 : t); };
+    for (int d = 0; d < 3; ++d) {
+        Ip[d].resize(NC);
+        Im[d].resize(NC);
+        simh[d].resize(g.flen(d));
+        se[d].resize(g.flen(d));
+        for (int t = 0; t < 3; ++t)
+            if (t != d) corner[pair_index(d, t)].resize(g.flen(d));
+    }
 
-    // PPM in every direction (faces are indexed by the cell on their high side)
-    for (int k = 0; k < n; ++k)
-        for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i)
+    // PPM in every direction
+    for (int k = 0; k < g.n[2]; ++k)
+        for (int j = 0; j < g.n[1]; ++j)
+            for (int i = 0; i < g.n[0]; ++i)
                 for (int d = 0; d < 3; ++d) {
                     int a, b, c;
                     step_e(d, a, b, c);
                     T v[7];
-                    for (int o = -3; o <= 3; ++o) v[o + 3] = s[g.at(i + o * a, j + o * b, k + o * c)];
+                    for (int o = -3; o <= 3; ++o) v[o + 3] = s[cell(i + o * a, j + o * b, k + o * c)];
                     T sm, sp;
                     if (ppm == 1) {
                         ppm1(v + 3, sm, sp);
@@ -307,86 +360,104 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
                         const T frr = cs_face(v[3], v[4], v[5], v[6]);
                         cs_cell(v + 3, fl, f0, fr, frr, sm, sp);
                     }
-                    trace(v[3], sm, sp, u[d][g.at(i + a, j + b, k + c)], u[d][g.at(i, j, k)], dt,
-                          dx[d], rel_eps, Ip[d][g.at(i, j, k)], Im[d][g.at(i, j, k)]);
+                    trace(v[3], sm, sp, u[d][g.face(d, i + a, j + b, k + c)], u[d][g.face(d, i, j, k)],
+                          dt, dx[d], rel_eps, Ip[d][cell(i, j, k)], Im[d][cell(i, j, k)]);
                 }
     if (marks) marks[0] = g_ops;
+    // loops over the d-faces: f(d, i, j, k, b) with b the boundary side
+    auto faces = [&](int d, auto&& f) {
+        for (int k = 0; k < g.fe(d, 2); ++k)
+            for (int j = 0; j < g.fe(d, 1); ++j)
+                for (int i = 0; i < g.fe(d, 0); ++i) f(i, j, k, g.bnd(d, d == 0 ? i : (d == 1 ? j : k)));
+    };
     // predictor and upwind states on faces (3-D threshold 0)
-    for (int k = 0; k < n; ++k)
-        for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i)
-                for (int d = 0; d < 3; ++d) {
-                    int a, b, c;
-                    step_e(d, a, b, c);
-                    const long long f = g.at(i, j, k);
-                    simh[d][f] = upwind(Ip[d][g.at(i - a, j - b, k - c)], Im[d][f], u[d][f], 0.0);
-                }
+    for (int d = 0; d < 3; ++d) {
+        int a, b, c;
+        step_e(d, a, b, c);
+        faces(d, [&](int i, int j, int k, int bd) {
+            T l = cst<T>(0.0), r = cst<T>(0.0);
+            if (bd != 1) l = Ip[d][cell(i - a, j - b, k - c)];
+            if (bd != 2) r = Im[d][cell(i, j, k)];
+            outflow_face(bd, l, r);
+            const long long f = g.face(d, i, j, k);
+            simh[d][f] = upwind(l, r, u[d][f], 0.0);
+        });
+    }
     if (marks) marks[1] = g_ops;
-    // corner-coupled states simh_dt on d-faces, corrected by t
+    // corner-coupled states simh_dt on d-faces, corrected by t; the
+    // transverse term of cell (i,j,k) in direction t
     const double dt6 = dt / 6.0;
-    for (int k = 0; k < n; ++k)
-        for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i)
-                for (int d = 0; d < 3; ++d)
-                    for (int t = 0; t < 3; ++t) {
-                        if (t == d) continue;
-                        int a, b, c, p, q, r;
-                        step_e(d, a, b, c);
-                        step_e(t, p, q, r);
-                        const long long f = g.at(i, j, k);
-                        const long long Lc = g.at(i - a, j - b, k - c);  // cell below the face
-                        const long long L1 = g.at(i - a + p, j - b + q, k - c + r);
-                        const long long R1 = g.at(i + p, j + q, k + r);
-                        const double ht = dx[t];
-                        T l = Ip[d][Lc];
-                        T rr = Im[d][f];
-                        l = l - (dt6 / ht) * (u[t][L1] + u[t][Lc]) * (simh[t][L1] - simh[t][Lc]);
-                        rr = rr - (dt6 / ht) * (u[t][R1] + u[t][f]) * (simh[t][R1] - simh[t][f]);
-                        corner[pair_index(d, t)][f] = upwind(l, rr, u[d][f], rel_eps);
-                    }
+    auto tterm = [&](int t, int i, int j, int k) {
+        int p, q, r;
+        step_e(t, p, q, r);
+        const long long f0 = g.face(t, i, j, k), f1 = g.face(t, i + p, j + q, k + r);
+        return (dt6 / dx[t]) * (u[t][f1] + u[t][f0]) * (simh[t][f1] - simh[t][f0]);
+    };
+    for (int d = 0; d < 3; ++d)
+        for (int t = 0; t < 3; ++t) {
+            if (t == d) continue;
+            int a, b, c;
+            step_e(d, a, b, c);
+            faces(d, [&](int i, int j, int k, int bd) {
+                T l = cst<T>(0.0), rr = cst<T>(0.0);
+                if (bd != 1) l = Ip[d][cell(i - a, j - b, k - c)] - tterm(t, i - a, j - b, k - c);
+                if (bd != 2) rr = Im[d][cell(i, j, k)] - tterm(t, i, j, k);
+                outflow_face(bd, l, rr);
+                const long long f = g.face(d, i, j, k);
+                corner[pair_index(d, t)][f] = upwind(l, rr, u[d][f], rel_eps);
+            });
+        }
     if (marks) marks[2] = g_ops;
     // edge states: d-face terms in the order of the reference (x: y, z;
-    // y: x, z; z: x, y), force zero
+    // y: x, z; z: x, y), force zero; the edge term of cell (i,j,k) from the
+    // corner state on its t-faces corrected by o
     const double dt4 = 0.25 * dt;
     const double dt2 = 0.5 * dt;
-    for (int k = 0; k < n; ++k)
-        for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i)
-                for (int d = 0; d < 3; ++d) {
-                    int a, b, c;
-                    step_e(d, a, b, c);
-                    const long long f = g.at(i, j, k);
-                    const long long Lc = g.at(i - a, j - b, k - c);
-                    T l = Ip[d][Lc];
-                    T rr = Im[d][f];
-                    for (int t = 0; t < 3; ++t) {
-                        if (t == d) continue;
-                        const int o = 3 - d - t;
-                        int p, q, r;
-                        step_e(t, p, q, r);
-                        const std::vector<T>& ct = corner[pair_index(t, o)];
-                        const long long L1 = g.at(i - a + p, j - b + q, k - c + r);
-                        const long long R1 = g.at(i + p, j + q, k + r);
-                        const double ht = dx[t];
-                        l = l - (dt4 / ht) * (u[t][L1] + u[t][Lc]) * (ct[L1] - ct[Lc]);
-                        rr = rr - (dt4 / ht) * (u[t][R1] + u[t][f]) * (ct[R1] - ct[f]);
-                    }
-                    T sedgel = l + dt2 * 0.0;
-                    T sedger = rr + dt2 * 0.0;
-                    se[d][f] = upwind(sedgel, sedger, u[d][f], rel_eps);
-                }
+    auto eterm = [&](int t, int o, int i, int j, int k) {
+        int p, q, r;
+        step_e(t, p, q, r);
+        const std::vector<T>& ct = corner[pair_index(t, o)];
+        const long long f0 = g.face(t, i, j, k), f1 = g.face(t, i + p, j + q, k + r);
+        return (dt4 / dx[t]) * (u[t][f1] + u[t][f0]) * (ct[f1] - ct[f0]);
+    };
+    for (int d = 0; d < 3; ++d) {
+        int a, b, c;
+        step_e(d, a, b, c);
+        faces(d, [&](int i, int j, int k, int bd) {
+            T l = cst<T>(0.0), rr = cst<T>(0.0);
+            if (bd != 1) {
+                l = Ip[d][cell(i - a, j - b, k - c)];
+                for (int t = 0; t < 3; ++t)
+                    if (t != d) l = l - eterm(t, 3 - d - t, i - a, j - b, k - c);
+            }
+            if (bd != 2) {
+                rr = Im[d][cell(i, j, k)];
+                for (int t = 0; t < 3; ++t)
+                    if (t != d) rr = rr - eterm(t, 3 - d - t, i, j, k);
+            }
+            T sedgel = l + dt2 * 0.0;
+            T sedger = rr + dt2 * 0.0;
+            outflow_face(bd, sedgel, sedger);
+            const long long f = g.face(d, i, j, k);
+            se[d][f] = upwind(sedgel, sedger, u[d][f], rel_eps);
+        });
+    }
     if (marks) marks[3] = g_ops;
     // fluxes and the update
-    for (int k = 0; k < n; ++k)
-        for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i) {
-                const long long cc = g.at(i, j, k);
-                T fx1 = u[0][g.at(i + 1, j, k)] * se[0][g.at(i + 1, j, k)];
-                T fx0 = u[0][cc] * se[0][cc];
-                T fy1 = u[1][g.at(i, j + 1, k)] * se[1][g.at(i, j + 1, k)];
-                T fy0 = u[1][cc] * se[1][cc];
-                T fz1 = u[2][g.at(i, j, k + 1)] * se[2][g.at(i, j, k + 1)];
-                T fz0 = u[2][cc] * se[2][cc];
+    for (int k = 0; k < g.n[2]; ++k)
+        for (int j = 0; j < g.n[1]; ++j)
+            for (int i = 0; i < g.n[0]; ++i) {
+                const long long cc = cell(i, j, k);
+                auto fl = [&](int d, int i2, int j2, int k2) {
+                    const long long f = g.face(d, i2, j2, k2);
+                    return u[d][f] * se[d][f];
+                };
+                T fx1 = fl(0, i + 1, j, k);
+                T fx0 = fl(0, i, j, k);
+                T fy1 = fl(1, i, j + 1, k);
+                T fy0 = fl(1, i, j, k);
+                T fz1 = fl(2, i, j, k + 1);
+                T fz0 = fl(2, i, j, k);
                 T divterm = (fx1 - fx0) / dx[0];
                 divterm = divterm + (fy1 - fy0) / dx[1];
                 divterm = divterm + (fz1 - fz0) / dx[2];
@@ -405,7 +476,7 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
 // one PPM type 1 face value (limits included), averaged over the calls.
 void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const double* s,
                  const double* const u[3], double out[7]) {
-    const Grid g{n};
+    const Grid g{{n, n, n}, {1, 1, 1}};
     const long long N3 = (long long)n * n * n;
     std::vector<Cnt> sc(N3), out_c(N3), uc[3];
     const Cnt* up[3];
@@ -447,38 +518,50 @@ void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const dou
     out[6] = (double)faces / (double)(3 * N3);
 }
 
-// the step (T = double) or its directional derivative (ds, du != null):
-// s, snew: ncomp x N^3; u[d]: N^3 (face N is face 0)
-void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double rel_eps,
-              const double* s, const double* const u[3], double* snew, const double* ds,
-              const double* const du[3], double* dsnew) {
-    const Grid g{n};
-    const long long N3 = (long long)n * n * n;
+// the step (T = double) or its directional derivative (ds, du != null) on
+// n[0] x n[1] x n[2] cells, direction d periodic (per[d]) or outflow:
+// s, snew: ncomp cell arrays; u[d]: the face array of direction d (one more
+// point along d when d is outflow; du[d] may be null)
+void ScalStepBC(const int n[3], const int per[3], int ncomp, int ppm, double dt,
+                const double dx[3], double rel_eps, const double* s, const double* const u[3],
+                double* snew, const double* ds, const double* const du[3], double* dsnew) {
+    const Grid g{{n[0], n[1], n[2]}, {per[0] ? 1 : 0, per[1] ? 1 : 0, per[2] ? 1 : 0}};
+    const long long NC = g.cells();
     // components are independent: one OpenMP thread each (the double and
     // dual instantiations keep no shared state)
     if (!ds) {
 #pragma omp parallel for schedule(dynamic, 1)
         for (int c = 0; c < ncomp; ++c)
-            step_comp<double>(g, s + c * N3, u, snew + c * N3, dt, dx, rel_eps, nullptr, ppm);
+            step_comp<double>(g, s + c * NC, u, snew + c * NC, dt, dx, rel_eps, nullptr, ppm);
         return;
     }
     std::vector<Dual> ud[3];
     const Dual* up[3];
     for (int d = 0; d < 3; ++d) {
-        ud[d].resize(N3);
-        for (long long x = 0; x < N3; ++x) ud[d][x] = {u[d][x], du[d] ? du[d][x] : 0.0};
+        const long long nf = g.flen(d);
+        ud[d].resize(nf);
+        for (long long x = 0; x < nf; ++x) ud[d][x] = {u[d][x], du[d] ? du[d][x] : 0.0};
         up[d] = ud[d].data();
     }
 #pragma omp parallel for schedule(dynamic, 1)
     for (int c = 0; c < ncomp; ++c) {
-        std::vector<Dual> sd(N3), out(N3);
-        for (long long x = 0; x < N3; ++x) sd[x] = {s[c * N3 + x], ds[c * N3 + x]};
+        std::vector<Dual> sd(NC), out(NC);
+        for (long long x = 0; x < NC; ++x) sd[x] = {s[c * NC + x], ds[c * NC + x]};
         step_comp<Dual>(g, sd.data(), up, out.data(), dt, dx, rel_eps, nullptr, ppm);
-        for (long long x = 0; x < N3; ++x) {
-            snew[c * N3 + x] = out[x].v;
-            dsnew[c * N3 + x] = out[x].d;
+        for (long long x = 0; x < NC; ++x) {
+            snew[c * NC + x] = out[x].v;
+            dsnew[c * NC + x] = out[x].d;
         }
     }
+}
+
+// the step on a periodic N^3 box: s, snew: ncomp x N^3; u[d]: N^3 (face N
+// is face 0)
+void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double rel_eps,
+              const double* s, const double* const u[3], double* snew, const double* ds,
+              const double* const du[3], double* dsnew) {
+    const int nn[3] = {n, n, n}, per[3] = {1, 1, 1};
+    ScalStepBC(nn, per, ncomp, ppm, dt, dx, rel_eps, s, u, snew, ds, du, dsnew);
 }
 
 }  // namespace mxo

@@ -80,6 +80,53 @@ def step_op_count(s, u, dt, dx, rel_eps):
     return dict(zip(keys, out.tolist()))
 
 
+def scal_step_bc(s, u, dt, dx, rel_eps, periodic, ds=None, du=None, ppm_type=1, fast=False):
+    """The differentiable scalar step (oracle_step.cpp) on the cells of a
+    domain whose direction d is periodic (periodic[d]) or outflow (foextrap),
+    ppm_type 1 or 2.
+    s: (ncomp, nz, ny, nx) indexed [c, k, j, i]; u: three face arrays, u[d]
+    one point longer along d when d is outflow (faces 0..n[d]).  Returns
+    snew, or (snew, dsnew) when the direction ds (and optionally du) is
+    given."""
+    import numpy as np
+    lib, _ = load(fast)
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    ncomp = s.shape[0]
+    n = [s.shape[3], s.shape[2], s.shape[1]]
+    per = [1 if p else 0 for p in periodic]
+    u = [np.ascontiguousarray(x, dtype=np.float64) for x in u]
+    for d in range(3):
+        want = [n[2], n[1], n[0]]
+        if not per[d]:
+            want[2 - d] += 1
+        if list(u[d].shape) != want:
+            raise ValueError(f"u[{d}] has shape {u[d].shape}, expected {tuple(want)}")
+    null = C.POINTER(C.c_double)()
+    snew = np.empty_like(s)
+    dsnew = None
+    dsp = null
+    dup = [null, null, null]
+    keep = []
+    if ds is not None:
+        ds = np.ascontiguousarray(ds, dtype=np.float64)
+        keep.append(ds)
+        dsp = _dptr(ds)
+        dsnew = np.empty_like(s)
+        for d in range(3):
+            if du is not None and du[d] is not None:
+                a = np.ascontiguousarray(du[d], dtype=np.float64)
+                keep.append(a)
+                dup[d] = _dptr(a)
+    f = lib.mxo_scal_step_bc
+    f.restype = C.c_int
+    rc = f((C.c_int * 3)(*n), (C.c_int * 3)(*per), ncomp, ppm_type, C.c_double(dt),
+           (C.c_double * 3)(*dx), C.c_double(rel_eps), _dptr(s), *[_dptr(x) for x in u],
+           _dptr(snew), dsp, *dup, _dptr(dsnew) if dsnew is not None else null)
+    if rc:
+        raise _abi.MxError(rc, "oracle.scal_step_bc")
+    return snew if ds is None else (snew, dsnew)
+
+
 def scal_step(s, u, dt, dx, rel_eps, ds=None, du=None, ppm_type=1, fast=False):
     """The differentiable scalar step on a periodic N^3 box (oracle_step.cpp),
     ppm_type 1 or 2.

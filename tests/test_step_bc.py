@@ -1,0 +1,127 @@
+"""The oracle's differentiable step on domains with outflow sides
+(oracle/oracle_step.cpp, ScalStepBC), pinned on the CPU:
+
+* with every direction periodic it is the periodic step, bit for bit;
+* on an uneven box list with outflow (foextrap) sides it equals the oracle's
+  own stages bit for bit: FillPatch's foextrap ghost cells, MakeEdgeScal
+  with the foextrap rules at the domain faces
+  (MaestroMakeEdgeScalars.cpp:709-738, :931-961, :1446-1480), MakeRhoXFlux
+  with species_pred_type 2 and UpdateScal of the species;
+* a constant field stays constant under a divergence-free velocity;
+* reflecting the domain in x maps the step onto itself (the high-side rule
+  mirrors the low-side one);
+* its dual-number derivative equals central finite differences.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from maestroex_amd.fabs import Backend, BaseState, FaceMultiFab, MultiFab
+from maestroex_amd.workloads import uneven_boxes
+from step_bc_cases import (cell_mf, divergence, face_mf, gather_cells, mac_velocity,
+                           scalar_fields, step_params)
+
+NP = Backend("numpy")
+
+
+def test_all_periodic_is_the_periodic_step():
+    from maestroex_amd.adjoint import periodic_mac_velocity
+    n = 8
+    u = periodic_mac_velocity(n, seed=3)
+    s = np.random.default_rng(2).random((2, n, n, n))
+    for ppm in (1, 2):
+        a = pyoracle.scal_step(s, u, 0.5 / n, [1.0 / n] * 3, 1e-8, ppm_type=ppm)
+        b = pyoracle.scal_step_bc(s, u, 0.5 / n, [1.0 / n] * 3, 1e-8, [1, 1, 1], ppm_type=ppm)
+        assert np.array_equal(a, b)
+
+
+def test_velocity_is_divergence_free():
+    for per in ([0, 0, 0], [1, 0, 1]):
+        u = mac_velocity((12, 10, 14), per, seed=1)
+        assert np.max(np.abs(divergence(u, per))) <= 1e-12 * 14
+
+
+@pytest.mark.parametrize("ppm", [1, 2])
+@pytest.mark.parametrize("per", [(0, 0, 0), (1, 0, 1)])
+def test_outflow_step_equals_the_pass_stages(ppm, per):
+    n, nspec, ng = 16, 2, 4
+    nn = (n, n, n)
+    boxes = uneven_boxes(n, 2, 6, 10, seed=4)
+    u = mac_velocity(nn, per, seed=5)
+    sp = scalar_fields(nn, nspec, seed=6)
+    dt, eps = 0.5 / n, 1e-8
+    params = step_params(nn, per, ppm, nspec, dt, eps)
+    first, nscal = params.first_spec, params.nscal
+    full = np.zeros((nscal,) + sp.shape[1:])
+    full[first:first + nspec] = sp
+    full[params.comp_rho] = sp.sum(axis=0)
+    state = cell_mf(boxes, full, ng, per, NP)
+    umac = face_mf(boxes, u, 1, per, NP)
+    force = MultiFab(boxes, nscal, 1, 3, NP)
+    sedge = FaceMultiFab(boxes, nscal, 0, 3, NP)
+    sflux = FaceMultiFab(boxes, nscal, 0, 3, NP)
+    eta = MultiFab(boxes, 1, 1, 3, NP, nodal_dir=2)
+    snew = MultiFab(boxes, nscal, 0, 3, NP)
+    m = pyoracle.oracle_maestro(params, [boxes])
+    bs, _ = m.bc_setup()
+    m.MakeEdgeScal(0, state, sedge, umac, force, 0, bs, first, first, nspec, 0)
+    zero = BaseState(np.zeros((1, n + 1)), NP)
+    m.MakeRhoXFlux(0, sflux, eta, sedge, umac, zero, zero, None, zero, zero, None, zero, zero,
+                   first, nspec)
+    m.UpdateScal(0, state, snew, sflux, force, first, nspec)
+    got = gather_cells(snew, nn)[first:first + nspec]
+    want = pyoracle.scal_step_bc(sp, u, dt, [1.0 / n] * 3, eps, per, ppm_type=ppm)
+    assert np.array_equal(got, want), float(np.max(np.abs(got - want)))
+
+
+@pytest.mark.parametrize("ppm", [1, 2])
+def test_outflow_step_keeps_constants(ppm):
+    nn = (12, 10, 14)
+    per = [0, 0, 0]
+    u = mac_velocity(nn, per, seed=8)
+    c = np.full((1, nn[2], nn[1], nn[0]), 0.7)
+    out = pyoracle.scal_step_bc(c, u, 0.4 / 14, [1.0 / q for q in nn], 1e-8, per, ppm_type=ppm)
+    assert np.max(np.abs(out - 0.7)) <= 1e-13
+
+
+@pytest.mark.parametrize("ppm", [1, 2])
+def test_outflow_step_mirror_symmetry(ppm):
+    """Reflect x: s(i) -> s(n-1-i), u_x(f) -> -u_x(n-f), u_y, u_z reflected.
+    The step of the reflected input is the reflected step."""
+    nn = (12, 10, 14)
+    per = [0, 0, 0]
+    u = mac_velocity(nn, per, seed=9)
+    s = scalar_fields(nn, 2, seed=10)
+    dx = [1.0 / q for q in nn]
+    dt = 0.4 / 14
+    a = pyoracle.scal_step_bc(s, u, dt, dx, 1e-8, per, ppm_type=ppm)
+    um = (-u[0][:, :, ::-1].copy(), u[1][:, :, ::-1].copy(), u[2][:, :, ::-1].copy())
+    b = pyoracle.scal_step_bc(s[:, :, :, ::-1].copy(), um, dt, dx, 1e-8, per, ppm_type=ppm)
+    scale = np.max(np.abs(a))
+    assert np.max(np.abs(b[:, :, :, ::-1] - a)) <= 1e-13 * scale
+
+
+def test_outflow_jvp_matches_finite_differences():
+    nn = (10, 8, 12)
+    per = [0, 1, 0]
+    u = mac_velocity(nn, per, seed=11)
+    s = scalar_fields(nn, 2, seed=12)
+    rng = np.random.default_rng(13)
+    ds = rng.standard_normal(s.shape)
+    du = [rng.standard_normal(x.shape) for x in u]
+    dx = [1.0 / q for q in nn]
+    dt, eps = 0.4 / 12, 1e-8
+    for ppm in (1, 2):
+        s1 = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, ppm_type=ppm)
+        s2, js = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, ds, du, ppm_type=ppm)
+        assert np.array_equal(s1, s2)
+        # h = 1e-7: the Colella-Sekora limiter's square-root branch has a
+        # large second derivative near its switch, so the O(h^2) error of
+        # the central difference at 1e-6 reaches 1e-5 at a few cells
+        h = 1e-7
+        fp = pyoracle.scal_step_bc(s + h * ds, [u[d] + h * du[d] for d in range(3)], dt, dx, eps,
+                                   per, ppm_type=ppm)
+        fm = pyoracle.scal_step_bc(s - h * ds, [u[d] - h * du[d] for d in range(3)], dt, dx, eps,
+                                   per, ppm_type=ppm)
+        fd = (fp - fm) / (2 * h)
+        assert np.max(np.abs(fd - js)) <= 1e-6 * np.max(np.abs(js))
