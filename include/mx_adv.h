@@ -765,6 +765,33 @@ int mx_adv_scal_step(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, mx_fab* snew, int
 int mx_adv_scal_step_vjp(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, const mx_fab* gbar,
                          mx_fab* sbar, mx_fab* ubar, int ncomp);
 
+/* The same step over the level-0 box list of a context (no single
+   counterpart: the composition above on MultiFabs, as DensityAdvance runs
+   it per FAB).  The boxes must tile the domain; each direction is periodic
+   or has outflow on both sides (phys_bc Outflow: foextrap ghost cells and
+   the foextrap rules at the domain faces, MaestroMakeEdgeScalars.cpp:
+   709-738, 931-961, 1446-1480); other boundaries return
+   MX_ERR_UNSUPPORTED.  s needs >= 3 ghost cells (>= 4 for ppm_type 2),
+   umac >= 1 ghost face; the call fills them (FillPatch at level 0, then
+   FillBoundary + FillUmacGhost).  A face two boxes share must hold one
+   value in both. */
+int mx_adv_scal_step_boxes(mx_adv_ctx* ctx, mx_multifab* s, mx_face_multifab* umac,
+                           mx_multifab* snew, int ncomp);
+
+/* Hand-written adjoint of mx_adv_scal_step_boxes (no counterpart in the
+   reference): gbar = dL/dsnew on the valid cells gives sbar = dL/ds on the
+   valid cells and ubar[d] = dL/dumac[d] on every face of direction d a box
+   holds (faces shared by boxes, and periodic images, all get the one
+   value of their face).  It reads the valid cells of s and the faces
+   lo..hi of umac (and the face at an outflow high side); no ghosts needed.
+   The state adjoint of a cell on an outflow side includes the ghost cells
+   that copy it.  Piecewise differentiation as in mx_adv_scal_step_vjp.
+   Deterministic.  Workspace: about 50 ncomp + 8 (ppm_type 2: 62 ncomp + 8)
+   arrays of the domain grown by 7 points per outflow direction. */
+int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* ctx, const mx_multifab* s,
+                               const mx_face_multifab* umac, const mx_multifab* gbar,
+                               mx_multifab* sbar, mx_face_multifab* ubar, int ncomp);
+
 #ifdef __cplusplus
 }
 #endif

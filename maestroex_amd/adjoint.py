@@ -1,11 +1,17 @@
-"""Differentiable scalar advection on a periodic box (mx_adv_scal_step and its
-hand-written adjoint mx_adv_scal_step_vjp), as torch.autograd.Functions.
+"""Differentiable scalar advection (mx_adv_scal_step / mx_adv_scal_step_boxes
+and their hand-written adjoints), as torch.autograd.Functions.
 
 * `ScalStepper(n, ncomp, dt, rel_eps, ppm_type=1)`: a C-ABI context for one
   fully periodic box [0, n)^3 (dx = 1/n, ppm_type 1 or 2) with device buffers of the
   step; `step(s, u)` and `vjp(s, u, gbar)` take dense torch tensors:
   s (ncomp, n, n, n) indexed [c, k, j, i] and u = (ux, uy, uz), each
   (n, n, n) holding the faces 0..n-1 of its direction (face n is face 0).
+* `BoxStepper(boxes, n, periodic, ncomp, dt, rel_eps, ppm_type=1)`: the same
+  step over a box list that tiles the domain n = (nx, ny, nz), each
+  direction periodic or outflow (foextrap); `step(s, u)` and
+  `vjp(s, u, gbar)` take the same dense tensors as ScalStepper's, u[d]
+  holding faces 0..n[d] along an outflow d; `step_mf` and `vjp_mf` run on
+  the box MultiFabs themselves.
 * `ScalStepFn.apply(stepper, s, ux, uy, uz)`: one step, differentiable in s
   and the velocity.
 * `Rollout.apply(stepper, nsteps, every, s0, ux, uy, uz)`: nsteps steps with
@@ -142,6 +148,120 @@ class ScalStepper:
                                               C.byref(self._fab(sb, [0] * 3, nc)), fub, nc),
                     "scal_step_vjp")
         return sb, ub
+
+
+class BoxStepper:
+    """Device MultiFabs and a C-ABI context for the step over a box list."""
+
+    NG = {1: 3, 2: 4}
+
+    def __init__(self, boxes, n, periodic, ncomp, dt, rel_eps, ppm_type=1, device="cuda:0",
+                 spherical=0):
+        import torch
+        import maestroex_amd as mx
+        from .fabs import Backend, FaceMultiFab, MultiFab
+        if ppm_type not in self.NG:
+            raise ValueError(f"ppm_type {ppm_type}: the step supports 1 and 2")
+        self.torch = torch
+        self.boxes = [tuple(map(tuple, b)) for b in boxes]
+        self.n = list(n)
+        self.periodic = [1 if p else 0 for p in periodic]
+        self.ncomp, self.dt, self.rel_eps, self.ppm_type = ncomp, dt, rel_eps, ppm_type
+        lo = [_abi.PHYS_INTERIOR if p else _abi.PHYS_OUTFLOW for p in self.periodic]
+        g = Geometry(3, self.n, [1.0 / q for q in self.n], self.periodic, lo, lo)
+        p = make_params(g, ppm_type=ppm_type, nspec=ncomp, dt=dt, rel_eps=rel_eps,
+                        species_pred_type=2, spherical=spherical)
+        self.m = mx.gpu_maestro(p, [self.boxes])
+        self.lib, self.fns = mx.load()
+        self.bk = bk = Backend("torch", torch.device(device))
+        ng = self.NG[ppm_type]
+        self.s = MultiFab(self.boxes, ncomp, ng, 3, bk)
+        self.snew = MultiFab(self.boxes, ncomp, 0, 3, bk)
+        self.u = FaceMultiFab(self.boxes, 1, 1, 3, bk)
+        self.g = MultiFab(self.boxes, ncomp, 0, 3, bk)
+        self.sb = MultiFab(self.boxes, ncomp, 0, 3, bk)
+        self.ub = FaceMultiFab(self.boxes, 1, 0, 3, bk)
+
+    def _check(self, rc, where):
+        if rc != _abi.MX_OK:
+            raise _abi.MxError(rc, where)
+
+    def step_mf(self, s, u, snew):
+        """snew = step(s) on MultiFabs (s: >= 3 ghosts, 4 for ppm_type 2;
+        u: >= 1 ghost face; both ghost layers are filled by the call)."""
+        self._check(self.fns["scal_step_boxes"](self.m.ctx, C.byref(s.desc()), C.byref(u.desc()),
+                                                C.byref(snew.desc()), self.ncomp),
+                    "scal_step_boxes")
+
+    def vjp_mf(self, s, u, gbar, sbar, ubar):
+        self._check(self.fns["scal_step_vjp_boxes"](self.m.ctx, C.byref(s.desc()),
+                                                    C.byref(u.desc()), C.byref(gbar.desc()),
+                                                    C.byref(sbar.desc()), C.byref(ubar.desc()),
+                                                    self.ncomp), "scal_step_vjp_boxes")
+
+    # dense global tensors <-> the box MultiFabs
+    def _put_cells(self, mf, a):
+        for b, (lo, hi) in enumerate(self.boxes):
+            reg = mf.valid_region(b)
+            mf.arrays[b][(slice(None),) + reg] = a[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1,
+                                                   lo[0]:hi[0] + 1]
+
+    def _get_cells(self, mf):
+        n = self.n
+        out = self.torch.empty((self.ncomp, n[2], n[1], n[0]), dtype=self.torch.float64,
+                               device=self.bk.device)
+        for b, (lo, hi) in enumerate(self.boxes):
+            reg = mf.valid_region(b)
+            out[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = \
+                mf.arrays[b][(slice(None),) + reg]
+        return out
+
+    def _face_index(self, d, lo, hi):
+        """Per direction (x, y, z) the global face indices of a box's faces
+        lo..hi+1 along d (periodic: face n is face 0)."""
+        idx = []
+        for q in range(3):
+            top = hi[q] + (1 if q == d else 0)
+            r = self.torch.arange(lo[q], top + 1, device=self.bk.device)
+            if q == d and self.periodic[q]:
+                r = r % self.n[q]
+            idx.append(r)
+        return idx
+
+    def _put_faces(self, fm, u):
+        for d in range(3):
+            for b, (lo, hi) in enumerate(self.boxes):
+                ix, iy, iz = self._face_index(d, lo, hi)
+                reg = fm[d].valid_region(b)
+                fm[d].arrays[b][(0,) + reg] = u[d][iz][:, iy][:, :, ix]
+
+    def _get_faces(self, fm):
+        out = []
+        for d in range(3):
+            shp = [self.n[2], self.n[1], self.n[0]]
+            if not self.periodic[d]:
+                shp[2 - d] += 1
+            a = self.torch.empty(shp, dtype=self.torch.float64, device=self.bk.device)
+            for b, (lo, hi) in enumerate(self.boxes):
+                ix, iy, iz = self._face_index(d, lo, hi)
+                reg = fm[d].valid_region(b)
+                v = fm[d].arrays[b][(0,) + reg]
+                a[iz[:, None, None], iy[None, :, None], ix[None, None, :]] = v
+            out.append(a)
+        return out
+
+    def step(self, s, u):
+        self._put_cells(self.s, s)
+        self._put_faces(self.u, u)
+        self.step_mf(self.s, self.u, self.snew)
+        return self._get_cells(self.snew)
+
+    def vjp(self, s, u, gbar):
+        self._put_cells(self.s, s)
+        self._put_faces(self.u, u)
+        self._put_cells(self.g, gbar)
+        self.vjp_mf(self.s, self.u, self.g, self.sb, self.ub)
+        return self._get_cells(self.sb), self._get_faces(self.ub)
 
 
 def _autograd():

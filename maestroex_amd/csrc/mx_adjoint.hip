@@ -44,11 +44,14 @@ constexpr double C23 = 2.0 / 3.0;
 
 __host__ __device__ constexpr int pidx(int d, int t) { return d * 2 + (t > d ? t - 1 : t); }
 
+struct AB;
+
 // periodic indexing of a dense N^3 array (N^3 fits 31 bits; checked on the
 // host, so all index arithmetic is 32-bit).  P2: N is a power of two and a
 // coordinate within one period of [0, N) wraps with a mask.
 template <bool P2>
 struct PerT {
+    static constexpr bool kGen = false;
     int n, m, lg;
     __device__ __forceinline__ int w(int x) const {
         if (P2) return x & m;
@@ -62,6 +65,10 @@ struct PerT {
     __device__ __forceinline__ int nb(int i, int j, int k, int d, int o) const {
         return at(i + (d == 0 ? o : 0), j + (d == 1 ? o : 0), k + (d == 2 ? o : 0));
     }
+    // no domain sides: every face is interior, every point a cell
+    __device__ __forceinline__ int bnd(int, int) const { return 0; }
+    __device__ __forceinline__ bool cell(int, int, int) const { return true; }
+    __device__ static PerT make(const AB& A);
 };
 
 template <bool P2>
@@ -71,6 +78,41 @@ __device__ __forceinline__ PerT<P2> make_per(int n) {
     P.m = n - 1;
     P.lg = 31 - __clz(n);
     return P;
+}
+
+// The general layout: nn[d] cells per direction, each direction periodic
+// (positions wrap) or outflow.  Along an outflow direction the kernels visit
+// positions 0 .. nn[d] (cells 0 .. nn[d]-1 and face nn[d]; the values at the
+// position-nn[d] cell are unused), stored after pad[d] points of padding on
+// each side that hold the state's foextrap copies and zero in every other
+// array, so that reads up to pad[d] points outside need no test.
+struct GenT {
+    static constexpr bool kGen = true;
+    int nn[3], per[3], pad[3];
+    int sj, sk;
+    __device__ __forceinline__ int w(int x, int d) const {
+        if (per[d]) return x < 0 ? x + nn[d] : (x >= nn[d] ? x - nn[d] : x);
+        return x + pad[d];
+    }
+    __device__ __forceinline__ int at(int i, int j, int k) const {
+        return w(i, 0) + sj * w(j, 1) + sk * w(k, 2);
+    }
+    __device__ __forceinline__ int nb(int i, int j, int k, int d, int o) const {
+        return at(i + (d == 0 ? o : 0), j + (d == 1 ? o : 0), k + (d == 2 ? o : 0));
+    }
+    // 1: a d-face at the low outflow side, 2: at the high one, else 0
+    __device__ __forceinline__ int bnd(int d, int pos) const {
+        if (per[d]) return 0;
+        return pos == 0 ? 1 : (pos == nn[d] ? 2 : 0);
+    }
+    __device__ __forceinline__ bool cell(int i, int j, int k) const {
+        return i < nn[0] && j < nn[1] && k < nn[2];
+    }
+    __device__ static GenT make(const AB& A);
+};
+
+__host__ __device__ __forceinline__ int posd(int d, int i, int j, int k) {
+    return d == 0 ? i : (d == 1 ? j : k);
 }
 
 struct AB {
@@ -85,24 +127,51 @@ struct AB {
     double* DV[3][4];  // ppm_type 2: limiter adjoints of s(p - 2d, p - d, p + d, p + 2d)
     double *DSA[3], *DSB[3], *DLA[3], *DLB[3], *QM[3], *Q0[3], *QP[3];
     DA SBF;            // state adjoint: the caller's FAB
-    long long cs;      // N^3
+    long long cs;      // stored points per component (N^3 when periodic)
+    long long npts;    // visited points: ext[0] ext[1] ext[2]
+    int ext[3];
     int n, ncomp;
+    GenT gen;          // the general layout (outflow sides)
+    int fold;          // ghost reach of the state adjoint (general layout)
     double dt, rel_eps, dx[3];
     // per-direction factors, evaluated once on the host by the same
     // expressions: dt/h, (dt/4)/h (edge terms), (dt/6)/h (transverse terms)
     double fdt[3], f4[3], f6[3];
 };
 
-__device__ __forceinline__ bool point(const AB& A, int& i, int& j, int& k, long long& p) {
+template <bool P2>
+__device__ __forceinline__ PerT<P2> PerT<P2>::make(const AB& A) {
+    return make_per<P2>(A.n);
+}
+__device__ __forceinline__ GenT GenT::make(const AB& A) { return A.gen; }
+
+// the visited point of this thread: its position (i,j,k) and stored index p
+template <class PT>
+__device__ __forceinline__ bool point(const AB& A, const PT& P, int& i, int& j, int& k,
+                                      long long& p) {
     const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (unsigned)A.cs) return false;
-    const unsigned n = (unsigned)A.n;
-    const unsigned r = q / n;
-    i = (int)(q - r * n);
-    k = (int)(r / n);
-    j = (int)(r - (unsigned)k * n);
-    p = q;
+    if (q >= (unsigned)A.npts) return false;
+    const unsigned e0 = (unsigned)A.ext[0], e1 = (unsigned)A.ext[1];
+    const unsigned r = q / e0;
+    i = (int)(q - r * e0);
+    k = (int)(r / e1);
+    j = (int)(r - (unsigned)k * e1);
+    p = PT::kGen ? (long long)P.at(i, j, k) : (long long)q;
     return true;
+}
+
+// the foextrap rule at an outflow side (1: low, 2: high) for the predictor,
+// corner-coupled and edge states (MaestroMakeEdgeScalars.cpp:709-738,
+// :931-961, :1446-1480): the outside state takes the inside one
+__device__ __forceinline__ double upwind_b(double l, double r, double u, double thr, int b);
+__device__ __forceinline__ void bfix(int b, double& wl, double& wr) {
+    if (b == 1) {
+        wr = wl + wr;
+        wl = 0.0;
+    } else if (b == 2) {
+        wl = wl + wr;
+        wr = 0.0;
+    }
 }
 
 __device__ __forceinline__ void weights(double u, double thr, double& wl, double& wr) {
@@ -122,6 +191,12 @@ __device__ __forceinline__ double upwind(double l, double r, double u, double th
     double res = (u > 0.0) ? l : r;
     res = (dabs(u) > thr) ? res : 0.5 * (l + r);
     return res;
+}
+
+__device__ __forceinline__ double upwind_b(double l, double r, double u, double thr, int b) {
+    if (b == 1) l = r;
+    else if (b == 2) r = l;
+    return upwind(l, r, u, thr);
 }
 
 __device__ __forceinline__ double dabs_d(double x) { return signbit(x) ? -1.0 : 1.0; }
@@ -150,15 +225,15 @@ __device__ __forceinline__ void tape_dir(const AB& A, const PT& P, const double*
     ppm_cell<PPM, D>(A, P, S, i, j, k, pos, ip, im);
     double ipm, imm;
     ppm_cell<PPM, D>(A, P, S, i - (D == 0), j - (D == 1), k - (D == 2), pos - 1, ipm, imm);
-    h = upwind(ipm, im, A.U[D][p], 0.0);
+    h = upwind_b(ipm, im, A.U[D][p], 0.0, P.bnd(D, pos));
 }
 
-template <int PPM, bool P2>
+template <int PPM, class PT>
 __global__ void __launch_bounds__(NTB) k_tape_ppm(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
     double ip[3], imv[3], h[3];
@@ -185,12 +260,12 @@ __device__ __forceinline__ double tt(const AB& A, const PT& P, long long co, int
 }
 
 // A3: corner-coupled states on d-faces, corrected by t
-template <bool P2>
+template <class PT>
 __global__ void __launch_bounds__(NTB) k_tape_corner(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     double cor[6];
     for (int d = 0; d < 3; ++d) {
@@ -201,7 +276,7 @@ __global__ void __launch_bounds__(NTB) k_tape_corner(AB A) {
             double r = A.IM[d][co + p];
             l = l - tt(A, P, co, im, jm, km, t);
             r = r - tt(A, P, co, i, j, k, t);
-            cor[pidx(d, t)] = upwind(l, r, A.U[d][p], A.rel_eps);
+            cor[pidx(d, t)] = upwind_b(l, r, A.U[d][p], A.rel_eps, P.bnd(d, posd(d, i, j, k)));
         }
     }
     for (int q = 0; q < 6; ++q) A.COR[q][co + p] = cor[q];
@@ -226,16 +301,17 @@ __device__ __forceinline__ double edge_bar(const AB& A, const PT& P, long long c
     const double Fb = A.fdt[d] * (A.G[co + c] - A.G[co + P.nb(i, j, k, d, -1)]);
     const double u = A.U[d][c];
     weights(u, A.rel_eps, wl, wr);
+    bfix(P.bnd(d, posd(d, i, j, k)), wl, wr);
     return Fb * u;
 }
 
 // A4: edge states
-template <bool P2>
+template <class PT>
 __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     const double dt2 = 0.5 * A.dt;
     double se[3];
@@ -251,7 +327,7 @@ __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
         }
         const double sedgel = l + dt2 * 0.0;
         const double sedger = r + dt2 * 0.0;
-        se[d] = upwind(sedgel, sedger, A.U[d][p], A.rel_eps);
+        se[d] = upwind_b(sedgel, sedger, A.U[d][p], A.rel_eps, P.bnd(d, posd(d, i, j, k)));
     }
     // B1 (it needs only the output adjoint and the velocity): the flux,
     // update and edge-state upwinding give the per-cell edge-term adjoints;
@@ -296,6 +372,8 @@ __device__ __forceinline__ void face_totals(const AB& A, const PT& P, long long 
     const double seb = Fb * u0;
     double wl, wr;
     weights(u0, A.rel_eps, wl, wr);
+    const int bt = P.bnd(t, posd(t, i, j, k));
+    bfix(bt, wl, wr);
     ub = want_u ? Fb * A.SE[t][co + f] : 0.0;
     slb = wl * seb;
     srb = wr * seb;
@@ -331,6 +409,7 @@ __device__ __forceinline__ void face_totals(const AB& A, const PT& P, long long 
     }
     double wl0, wr0;
     weights(u0, 0.0, wl0, wr0);
+    bfix(bt, wl0, wr0);
     slb = slb + wl0 * hb;
     srb = srb + wr0 * hb;
 }
@@ -350,6 +429,7 @@ __device__ __forceinline__ void corner_bar(const AB& A, const PT& P, long long c
     const double kb = ebm * b * (u0 + um) - eb0 * b * (up + u0);
     double wl, wr;
     weights(u0, A.rel_eps, wl, wr);
+    bfix(P.bnd(d, posd(d, i, j, k)), wl, wr);
     lb = wl * kb;
     rb = wr * kb;
 }
@@ -357,12 +437,12 @@ __device__ __forceinline__ void corner_bar(const AB& A, const PT& P, long long c
 // B4: per-cell transverse-term adjoints: the transverse term of cell p in
 // direction o enters the corner states on the d-faces p (right state) and
 // p + d (left state), d != o
-template <bool P2>
+template <class PT>
 __global__ void __launch_bounds__(NTB) k_adj_tterm(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     double tb[3];
     for (int o = 0; o < 3; ++o) {
@@ -382,12 +462,12 @@ __global__ void __launch_bounds__(NTB) k_adj_tterm(AB A) {
 
 // B3 + B5 stored once per face: the left and right predictor-state adjoints
 // of every d-face (for the trace variant that reads them)
-template <bool P2>
+template <class PT>
 __global__ void __launch_bounds__(NTB) k_adj_fsum(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     double l[3], r[3], unused;
 #pragma unroll
@@ -438,12 +518,12 @@ __device__ __forceinline__ TraceBar trace_bar(const AB& A, double s0, double sm,
 
 // B6: characteristic trace and CW limiter of cell p in direction d (FS: the
 // face adjoints are read from k_adj_fsum's arrays instead of recomputed)
-template <bool P2, bool FS, bool ROLL>
+template <class PT, bool FS, bool ROLL>
 __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
     // ROLL: one direction per loop trip with its stores at the end of the
@@ -724,12 +804,12 @@ __device__ __forceinline__ void cs_cell_bar(const double* v, double fl, double f
 // p + d (FRB), the direct adjoints of the cells p - 2d .. p + 2d other than p
 // (DV[d][0..3] for offsets -2, -1, 1, 2), the velocity terms, and adds its
 // direct term of s(p) to S0S.
-template <bool P2>
+template <class PT>
 __global__ void __launch_bounds__(NTB) k_adj_trace2(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
     double s0s = 0.0;
@@ -770,18 +850,42 @@ __global__ void __launch_bounds__(NTB) k_adj_trace2(AB A) {
     A.S0S[co + p] = s0s;
 }
 
+// The state adjoint of a cell, from the gather g(i, j, k) of the adjoints
+// of every use of the state at position (i, j, k).  Periodic: the cell's
+// own position.  General layout: a cell on an outflow side also collects
+// the positions outside it whose ghost value is a copy of it (foextrap),
+// up to the stencils' reach A.fold.
+template <class PT, class F>
+__device__ __forceinline__ double fold_state(const AB& A, const PT& P, int i, int j, int k, F g) {
+    if constexpr (!PT::kGen) {
+        return g(i, j, k);
+    } else {
+        const int q[3] = {i, j, k};
+        int lo[3], hi[3];
+        bool edge = false;
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = hi[d] = q[d];
+            if (P.per[d]) continue;
+            if (q[d] == 0) lo[d] = -A.fold;
+            if (q[d] == P.nn[d] - 1) hi[d] = q[d] + A.fold;
+            edge = edge || lo[d] != q[d] || hi[d] != q[d];
+        }
+        if (!edge) return g(i, j, k);
+        double s = 0.0;
+        for (int c = lo[2]; c <= hi[2]; ++c)
+            for (int b = lo[1]; b <= hi[1]; ++b)
+                for (int a = lo[0]; a <= hi[0]; ++a) s += g(a, b, c);
+        return s;
+    }
+}
+
 // B7-B9 for ppm_type 2: the state adjoint of cell p gathers, per direction,
 // the direct limiter terms of the cells p - 2d .. p + 2d and the face
 // adjoints of the faces p - d .. p + 2d through their Colella-Sekora values
-template <bool P2>
-__global__ void __launch_bounds__(NTB) k_adj_state2(AB A) {
-    int i, j, k;
-    long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
-    const long long co = (long long)blockIdx.y * A.cs;
-    const double* S = A.S + co;
-    double s = A.G[co + p] + A.S0S[co + p];
+template <class PT>
+__device__ __forceinline__ double state2_at(const AB& A, const PT& P, long long co,
+                                            const double* S, int i, int j, int k) {
+    double s = A.G[co + P.at(i, j, k)] + A.S0S[co + P.at(i, j, k)];
 #pragma unroll 1
     for (int d = 0; d < 3; ++d) {
         // direct limiter terms: cell c gives DV[o](c) to s(c + o)
@@ -802,7 +906,19 @@ __global__ void __launch_bounds__(NTB) k_adj_state2(AB A) {
             s += fb * g[2 - fo];
         }
     }
-    A.SBF(i, j, k, (int)blockIdx.y) = s;
+    return s;
+}
+
+template <class PT>
+__global__ void __launch_bounds__(NTB) k_adj_state2(AB A) {
+    const PT P = PT::make(A);
+    int i, j, k;
+    long long p;
+    if (!point(A, P, i, j, k, p) || !P.cell(i, j, k)) return;
+    const long long co = (long long)blockIdx.y * A.cs;
+    const double* S = A.S + co;
+    A.SBF(i, j, k, (int)blockIdx.y) = fold_state(
+        A, P, i, j, k, [&](int a, int b, int c) { return state2_at(A, P, co, S, a, b, c); });
 }
 
 // B7: adjoints through the PPM type 1 face value of d-face (i,j,k) (cells
@@ -877,12 +993,12 @@ __device__ __forceinline__ void slope_bar(const PT& P, const double* S, int i, i
 }
 
 // B7: face-value adjoints of face p (per direction)
-template <bool P2>
+template <class PT>
 __global__ void __launch_bounds__(NTB) k_adj_face(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
     double bsa[3], bsb[3], bla[3], blb[3];
@@ -898,12 +1014,12 @@ __global__ void __launch_bounds__(NTB) k_adj_face(AB A) {
 }
 
 // B8: van Leer slope adjoints of cell p; its slope feeds faces p and p + d
-template <bool P2>
+template <class PT>
 __global__ void __launch_bounds__(NTB) k_adj_slope(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     const double* S = A.S + co;
     double qm[3], q0[3], qp[3];
@@ -923,15 +1039,10 @@ __global__ void __launch_bounds__(NTB) k_adj_slope(AB A) {
 // B7 + B8 + B9 in one gather: the state adjoint of cell p with the face
 // adjoints of faces p - d .. p + 2d and the slope adjoints of cells
 // p - d .. p + d recomputed per direction
-template <bool P2>
-__global__ void __launch_bounds__(NTB) k_adj_state_fused(AB A) {
-    int i, j, k;
-    long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
-    const long long co = (long long)blockIdx.y * A.cs;
-    const double* S = A.S + co;
-    double s = A.G[co + p] + A.S0S[co + p];
+template <class PT>
+__device__ __forceinline__ double state_fused_at(const AB& A, const PT& P, long long co,
+                                                 const double* S, int i, int j, int k) {
+    double s = A.G[co + P.at(i, j, k)] + A.S0S[co + P.at(i, j, k)];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const int di = d == 0, dj = d == 1, dk = d == 2;
@@ -947,16 +1058,28 @@ __global__ void __launch_bounds__(NTB) k_adj_state_fused(AB A) {
                       flb[m] + fla[m + 1], qm[m], q0[m], qp[m]);
         s += fsb[1] + fsa[2] + qp[0] + q0[1] + qm[2];
     }
-    A.SBF(i, j, k, (int)blockIdx.y) = s;
+    return s;
+}
+
+template <class PT>
+__global__ void __launch_bounds__(NTB) k_adj_state_fused(AB A) {
+    const PT P = PT::make(A);
+    int i, j, k;
+    long long p;
+    if (!point(A, P, i, j, k, p) || !P.cell(i, j, k)) return;
+    const long long co = (long long)blockIdx.y * A.cs;
+    const double* S = A.S + co;
+    A.SBF(i, j, k, (int)blockIdx.y) = fold_state(
+        A, P, i, j, k, [&](int a, int b, int c) { return state_fused_at(A, P, co, S, a, b, c); });
 }
 
 // B9: state adjoint
-template <bool P2>
+template <class PT>
 __global__ void __launch_bounds__(NTB) k_adj_state(AB A) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     const long long co = (long long)blockIdx.y * A.cs;
     double s = A.G[co + p] + A.S0S[co + p];
     for (int d = 0; d < 3; ++d) {
@@ -970,12 +1093,12 @@ __global__ void __launch_bounds__(NTB) k_adj_state(AB A) {
 // B10: velocity adjoint per face, summed over components in order
 // into the caller's face FABs; the duplicate face n of a periodic face FAB
 // carries no separate value and is set to zero
-template <bool P2, bool STORED>
+template <class PT, bool STORED>
 __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, DA ub0, DA ub1, DA ub2, int has_n) {
+    const PT P = PT::make(A);
     int i, j, k;
     long long p;
-    if (!point(A, i, j, k, p)) return;
-    const PerT<P2> P = make_per<P2>(A.n);
+    if (!point(A, P, i, j, k, p)) return;
     double r[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -996,6 +1119,21 @@ __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, DA ub0, DA ub1, DA ub2, i
             s += ub + A.UHI[d][co + cm];
         }
         r[d] = s;
+    }
+    if constexpr (PT::kGen) {
+        // faces 0 .. nn[d] along d, cells across it
+        const int q[3] = {i, j, k};
+        bool ok[3];
+        for (int d = 0; d < 3; ++d) {
+            ok[d] = true;
+            for (int t = 0; t < 3; ++t)
+                if (t != d && q[t] >= P.nn[t]) ok[d] = false;
+            if (P.per[d] && q[d] >= P.nn[d]) ok[d] = false;
+        }
+        if (ok[0]) ub0(i, j, k) = r[0];
+        if (ok[1]) ub1(i, j, k) = r[1];
+        if (ok[2]) ub2(i, j, k) = r[2];
+        return;
     }
     ub0(i, j, k) = r[0];
     ub1(i, j, k) = r[1];
@@ -1074,6 +1212,281 @@ __global__ void __launch_bounds__(NTB) k_fill_zero(double* a, long long n) {
     if (p < n) a[p] = 0.0;
 }
 
+// the tape and reverse kernels of the step adjoint.  PT indexes the tape,
+// state and velocity sweeps, PC the transverse-term and trace gathers
+// (which run faster with compare indexing on MI355X, measured, whatever n
+// is); the velocity adjoint goes to ub0..ub2 (has_n: the periodic FABs
+// that hold face n)
+template <class PT, class PC>
+int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, DA ub0, DA ub1, DA ub2, int has_n) {
+    const bool ppm2 = c->p.ppm_type == 2;
+    // the three-kernel state gather has no ghost folding: the general
+    // layout takes the fused one
+    const bool state_fused = c->adj_state_fused || PT::kGen;
+    if (ppm2)
+        MX_LAUNCH(c, "k_tape_ppm", (k_tape_ppm<2, PT>), gc, dim3(NTB), A);
+    else
+        MX_LAUNCH(c, "k_tape_ppm", (k_tape_ppm<1, PT>), gc, dim3(NTB), A);
+    MX_LAUNCH(c, "k_tape_corner", k_tape_corner<PT>, gc, dim3(NTB), A);
+    MX_LAUNCH(c, "k_tape_edge", k_tape_edge<PT>, gc, dim3(NTB), A);
+    MX_LAUNCH(c, "k_adj_tterm", k_adj_tterm<PC>, gc, dim3(NTB), A);
+    if (ppm2) {
+        // ppm_type 2: the Colella-Sekora trace and state sweeps (the kernel
+        // arrangement options apply to ppm_type 1)
+        MX_LAUNCH(c, "k_adj_trace", k_adj_trace2<PC>, gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_state", k_adj_state2<PT>, gc, dim3(NTB), A);
+    } else if (c->adj_face_sums) {
+        MX_LAUNCH(c, "k_adj_fsum", k_adj_fsum<PT>, gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<PC, true, false>), gc, dim3(NTB), A);
+    } else if (c->adj_trace_rolled) {
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<PC, false, true>), gc, dim3(NTB), A);
+    } else {
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<PC, false, false>), gc, dim3(NTB), A);
+    }
+    if (ppm2) {
+        // the state sweep ran above
+    } else if (state_fused) {
+        MX_LAUNCH(c, "k_adj_state_fused", k_adj_state_fused<PT>, gc, dim3(NTB), A);
+    } else {
+        MX_LAUNCH(c, "k_adj_face", k_adj_face<PT>, gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_slope", k_adj_slope<PT>, gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_state", k_adj_state<PT>, gc, dim3(NTB), A);
+    }
+    if (c->adj_face_sums && !ppm2)
+        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<PT, false>), g1, dim3(NTB), A, ub0, ub1, ub2, has_n);
+    else
+        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<PT, true>), g1, dim3(NTB), A, ub0, ub1, ub2, has_n);
+    return MX_OK;
+}
+
+// arrays of the tape and the reverse sweeps: `take(m)` hands out m
+// components of cs points each; returns how many components it took (with
+// take == nullptr: counts only)
+template <class Take>
+long long ab_arrays(const mx_adv_ctx* c, AB& A, int ncomp, bool gen, Take take) {
+    const bool ppm2 = c->p.ppm_type == 2;
+    const bool fs = c->adj_face_sums && !ppm2;
+    const bool staged_state = !ppm2 && !c->adj_state_fused && !gen;
+    long long n = 0;
+    auto t = [&](double*& dst) {
+        dst = take(ncomp);
+        n += ncomp;
+    };
+    for (int d = 0; d < 3; ++d) {
+        t(A.IP[d]);
+        t(A.IM[d]);
+        t(A.SIMH[d]);
+        t(A.SE[d]);
+        t(A.TB[d]);
+        t(A.UBC[d]);
+        if (ppm2)
+            for (int o = 0; o < 4; ++o) t(A.DV[d][o]);
+        if (fs) {
+            t(A.SLB[d]);
+            t(A.SRB[d]);
+            t(A.ULO[d]);
+        }
+        t(A.FLB[d]);
+        t(A.FRB[d]);
+        t(A.UHI[d]);
+        if (staged_state) {
+            t(A.DSA[d]);
+            t(A.DSB[d]);
+            t(A.DLA[d]);
+            t(A.DLB[d]);
+            t(A.QM[d]);
+            t(A.Q0[d]);
+            t(A.QP[d]);
+        }
+    }
+    for (int q = 0; q < 6; ++q) {
+        t(A.COR[q]);
+        t(A.EB[q]);
+    }
+    t(A.S0S);
+    return n;
+}
+
+void ab_factors(const mx_adv_ctx* c, AB& A, int ncomp) {
+    A.ncomp = ncomp;
+    A.dt = c->p.dt;
+    A.rel_eps = c->p.rel_eps;
+    for (int d = 0; d < 3; ++d) {
+        A.dx[d] = c->p.dx[0][d];
+        A.fdt[d] = A.dt / A.dx[d];
+        const double dt4 = 0.25 * A.dt, dt6 = A.dt / 6.0;
+        A.f4[d] = dt4 / A.dx[d];
+        A.f6[d] = dt6 / A.dx[d];
+    }
+}
+
+// ---------------------------------------------------------------- box lists
+// The step over the level-0 box list of a context: the boxes tile the
+// domain, each direction periodic or outflow on both sides.
+struct StepDomain {
+    int lo[3], nn[3], per[3];
+};
+
+int box_domain(const mx_adv_ctx* c, int ncomp, StepDomain& D) {
+    const mx_adv_params& p = c->p;
+    if (p.dim != 3 || (p.ppm_type != 1 && p.ppm_type != 2) || p.ppm_trace_forces != 0)
+        return MX_ERR_UNSUPPORTED;
+    if (ncomp < 1 || ncomp > MX_MAX_SCAL) return MX_ERR_ARG;
+    long long dom = 1;
+    for (int d = 0; d < 3; ++d) {
+        D.lo[d] = p.domain_lo[0][d];
+        D.nn[d] = p.domain_hi[0][d] - p.domain_lo[0][d] + 1;
+        if (p.is_periodic[d])
+            D.per[d] = 1;
+        else if (p.phys_bc_lo[d] == MX_PHYS_OUTFLOW && p.phys_bc_hi[d] == MX_PHYS_OUTFLOW)
+            D.per[d] = 0;
+        else
+            return MX_ERR_UNSUPPORTED;
+        if (D.nn[d] < 4) return MX_ERR_UNSUPPORTED;
+        dom *= D.nn[d];
+    }
+    // set_level keeps the boxes disjoint and inside the domain, so they
+    // tile it exactly when their cells add up to it
+    long long cells = 0;
+    for (const mx_box& b : c->grids[0]) cells += npts(hbox(b));
+    if (cells != dom) return MX_ERR_UNSUPPORTED;
+    return MX_OK;
+}
+
+int nfab_ok(const mx_adv_ctx* c, const mx_multifab* m) {
+    return m && m->fab && m->nfab == (int)c->grids[0].size();
+}
+
+// snew = s + dt (-div(u sedge) + 0) on one box per task
+struct UpdTask {
+    DA s, snew, u[3], e[3];
+    int lo[3], hi[3];
+};
+
+__global__ void __launch_bounds__(NTB) k_box_update(const UpdTask* __restrict__ tasks, int ncomp,
+                                                    double dt, double dx0, double dx1, double dx2) {
+    const UpdTask& t = tasks[blockIdx.y];
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!region_point(idx, t.lo, t.hi, i, j, k)) return;
+    for (int c = 0; c < ncomp; ++c) {
+        const double fx1 = t.u[0](i + 1, j, k) * t.e[0](i + 1, j, k, c);
+        const double fx0 = t.u[0](i, j, k) * t.e[0](i, j, k, c);
+        const double fy1 = t.u[1](i, j + 1, k) * t.e[1](i, j + 1, k, c);
+        const double fy0 = t.u[1](i, j, k) * t.e[1](i, j, k, c);
+        const double fz1 = t.u[2](i, j, k + 1) * t.e[2](i, j, k + 1, c);
+        const double fz0 = t.u[2](i, j, k) * t.e[2](i, j, k, c);
+        double divterm = (fx1 - fx0) / dx0;
+        divterm += (fy1 - fy0) / dx1;
+        divterm += (fz1 - fz0) / dx2;
+        t.snew(i, j, k, c) = t.s(i, j, k, c) + dt * (-divterm + 0.0);
+    }
+}
+
+// the dense layout of the general adjoint: E[d] stored points per
+// direction, the domain's cell lo at stored index pad[d]
+struct Dense {
+    double* p;
+    long long cs;  // per component
+    int E[3], pad[3], dlo[3], nn[3], per[3];
+    __device__ __forceinline__ long long at(int x, int y, int z) const {
+        return (long long)(x - dlo[0] + pad[0]) +
+               (long long)E[0] * ((long long)(y - dlo[1] + pad[1]) +
+                                  (long long)E[1] * (long long)(z - dlo[2] + pad[2]));
+    }
+};
+
+// a box's region of a FAB <-> the dense layout: cells (nodal -1) or the
+// faces lo..hi+1 of direction `nodal`; a face position nn[d] along a
+// periodic direction is face 0
+struct XTask {
+    DA f;
+    int lo[3], hi[3];
+};
+
+template <bool TO_DENSE>
+__global__ void __launch_bounds__(NTB) k_box_xfer(const XTask* __restrict__ tasks, Dense D,
+                                                  int ncomp) {
+    const XTask& t = tasks[blockIdx.y];
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!region_point(idx, t.lo, t.hi, i, j, k)) return;
+    int q[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d)
+        if (D.per[d] && q[d] - D.dlo[d] >= D.nn[d]) q[d] -= D.nn[d];
+    const long long o = D.at(q[0], q[1], q[2]);
+    for (int c = 0; c < ncomp; ++c) {
+        if (TO_DENSE)
+            D.p[c * D.cs + o] = t.f(i, j, k, c);
+        else
+            t.f(i, j, k, c) = D.p[c * D.cs + o];
+    }
+}
+
+// every stored point of a dense array that is not a valid cell (nodal -1)
+// or face of direction `nodal`: the nearest valid cell's value (clamp) or
+// zero
+__global__ void __launch_bounds__(NTB) k_dense_outside(Dense D, int ncomp, int nodal, int clamp) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = (long long)D.E[0] * D.E[1] * D.E[2];
+    if (q >= tot) return;
+    const long long r = q / D.E[0];
+    const int x[3] = {(int)(q - r * D.E[0]) - D.pad[0], (int)(r % D.E[1]) - D.pad[1],
+                      (int)(r / D.E[1]) - D.pad[2]};
+    bool inside = true;
+    int cl[3];
+    for (int d = 0; d < 3; ++d) {
+        const int top = D.nn[d] - 1 + (d == nodal && !D.per[d] ? 1 : 0);
+        if (x[d] < 0 || x[d] > top) inside = false;
+        cl[d] = x[d] < 0 ? 0 : (x[d] > D.nn[d] - 1 ? D.nn[d] - 1 : x[d]);
+    }
+    if (inside) return;
+    const long long from = (long long)(cl[0] + D.pad[0]) +
+                           (long long)D.E[0] * ((long long)(cl[1] + D.pad[1]) +
+                                                (long long)D.E[1] * (long long)(cl[2] + D.pad[2]));
+    for (int c = 0; c < ncomp; ++c) D.p[c * D.cs + q] = clamp ? D.p[c * D.cs + from] : 0.0;
+}
+
+// zero the padding shell (the stored points outside the visited ones) of
+// every array of a table, one array per blockIdx.y
+__global__ void __launch_bounds__(NTB) k_shell_zero(double* const* __restrict__ arrs, int E0,
+                                                    int E1, int E2, int x0, int x1, int x2,
+                                                    int p0, int p1, int p2) {
+    double* a = arrs[blockIdx.y];
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long plane = (long long)E0 * E1;
+    const long long n1 = 2LL * p2 * plane;              // low and high z slabs
+    const long long n2 = (long long)x2 * 2 * p1 * E0;   // y slabs inside them
+    const long long n3 = (long long)x2 * x1 * 2 * p0;   // x slabs inside both
+    int i, j, k;
+    if (q < n1) {
+        const int t = (int)(q / plane);
+        const long long r = q - (long long)t * plane;
+        k = t < p2 ? t : x2 + t;
+        j = (int)(r / E0);
+        i = (int)(r - (long long)j * E0);
+    } else if ((q -= n1) < n2) {
+        const long long row = 2LL * p1 * E0;
+        const int t = (int)(q / row);
+        const long long r = q - (long long)t * row;
+        const int rr = (int)(r / E0);
+        k = p2 + t;
+        j = rr < p1 ? rr : x1 + rr;
+        i = (int)(r - (long long)rr * E0);
+    } else if ((q -= n2) < n3) {
+        const int t = (int)(q / (2 * p0));
+        const int rr = (int)(q - (long long)t * 2 * p0);
+        k = p2 + t / x1;
+        j = p1 + t % x1;
+        i = rr < p0 ? rr : x0 + rr;
+    } else {
+        return;
+    }
+    a[(long long)i + (long long)E0 * ((long long)j + (long long)E1 * k)] = 0.0;
+}
+
+dim3 grid1(long long npts) { return dim3((unsigned)((npts + NTB - 1) / NTB)); }
+
 // the step's preconditions: one periodic box [0, n)^3 at level 0
 // ghosts: the step reads s and umac ghosts (>= 3 cells for ppm_type 1, >= 4
 // for ppm_type 2, >= 1 face); the
@@ -1111,8 +1524,6 @@ bool dense_fab(const mx_fab& f, int n, int ncomp) {
            f.kstride == n2 && (ncomp == 1 || f.nstride == n2 * n);
 }
 
-dim3 grid1(long long npts) { return dim3((unsigned)((npts + NTB - 1) / NTB)); }
-
 int periodic_fill(mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp, int n) {
     std::vector<FillTask> tasks;
     long long maxpts = 0;
@@ -1141,52 +1552,24 @@ int periodic_fill(mx_adv_ctx* c, const mx_fab* s, const mx_fab* u, int ncomp, in
     return MX_OK;
 }
 
-// the tape and reverse kernels of the step adjoint
-template <bool P2>
-int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, mx_fab* ubar) {
-    const bool ppm2 = c->p.ppm_type == 2;
-    if (ppm2)
-        MX_LAUNCH(c, "k_tape_ppm", (k_tape_ppm<2, P2>), gc, dim3(NTB), A);
-    else
-        MX_LAUNCH(c, "k_tape_ppm", (k_tape_ppm<1, P2>), gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_tape_corner", k_tape_corner<P2>, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_tape_edge", k_tape_edge<P2>, gc, dim3(NTB), A);
-    // the transverse-term and trace gathers run faster with compare
-    // indexing on MI355X (measured), whatever n is
-    MX_LAUNCH(c, "k_adj_tterm", k_adj_tterm<false>, gc, dim3(NTB), A);
-    if (ppm2) {
-        // ppm_type 2: the Colella-Sekora trace and state sweeps (the kernel
-        // arrangement options apply to ppm_type 1)
-        MX_LAUNCH(c, "k_adj_trace", k_adj_trace2<false>, gc, dim3(NTB), A);
-        MX_LAUNCH(c, "k_adj_state", k_adj_state2<P2>, gc, dim3(NTB), A);
-    } else if (c->adj_face_sums) {
-        MX_LAUNCH(c, "k_adj_fsum", k_adj_fsum<P2>, gc, dim3(NTB), A);
-        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, true, false>), gc, dim3(NTB), A);
-    } else if (c->adj_trace_rolled) {
-        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, false, true>), gc, dim3(NTB), A);
-    } else {
-        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<false, false, false>), gc, dim3(NTB), A);
+// did the views taken since ws_reserve stay inside the workspace?
+bool ws_fits(const mx_adv_ctx* c) {
+    if (c->dry) return true;
+    return c->ws_off <= (c->ext_ws ? c->ext_ws_cap : c->ws_cap);
+}
+
+mx_fab dense_desc(double* p, const int ext[3], const int begin[3], int ncomp) {
+    mx_fab f;
+    f.p = p;
+    f.jstride = ext[0];
+    f.kstride = (long long)ext[0] * ext[1];
+    f.nstride = (long long)ext[0] * ext[1] * ext[2];
+    for (int q = 0; q < 3; ++q) {
+        f.begin[q] = begin[q];
+        f.end[q] = begin[q] + ext[q];
     }
-    if (ppm2) {
-        // the state sweep ran above
-    } else if (c->adj_state_fused) {
-        MX_LAUNCH(c, "k_adj_state_fused", k_adj_state_fused<P2>, gc, dim3(NTB), A);
-    } else {
-        MX_LAUNCH(c, "k_adj_face", k_adj_face<P2>, gc, dim3(NTB), A);
-        MX_LAUNCH(c, "k_adj_slope", k_adj_slope<P2>, gc, dim3(NTB), A);
-        MX_LAUNCH(c, "k_adj_state", k_adj_state<P2>, gc, dim3(NTB), A);
-    }
-    // velocity adjoint, into the caller's face FABs
-    int has_n = 0;
-    for (int d = 0; d < 3; ++d)
-        if (ubar[d].end[d] > A.n) has_n |= 1 << d;
-    if (c->adj_face_sums && !ppm2)
-        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<P2, false>), g1, dim3(NTB), A, to_da(ubar[0]),
-                  to_da(ubar[1]), to_da(ubar[2]), has_n);
-    else
-        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<P2, true>), g1, dim3(NTB), A, to_da(ubar[0]),
-                  to_da(ubar[1]), to_da(ubar[2]), has_n);
-    return MX_OK;
+    f.ncomp = ncomp;
+    return f;
 }
 
 }  // namespace
@@ -1207,26 +1590,11 @@ int mx_adv_scal_step(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, mx_fab* snew, int n
     if ((rc = ws_reserve(c, (3 * fn * ncomp + cn * ncomp) * sizeof(double) + 8192))) return rc;
     mx_fab se[3], fo;
     for (int d = 0; d < 3; ++d) {
-        se[d].p = ws_take(c, (size_t)(fn * ncomp));
-        const int ex[3] = {n + (d == 0), n + (d == 1), n + (d == 2)};
-        se[d].jstride = ex[0];
-        se[d].kstride = (long long)ex[0] * ex[1];
-        se[d].nstride = fn;
-        for (int q = 0; q < 3; ++q) {
-            se[d].begin[q] = 0;
-            se[d].end[q] = ex[q];
-        }
-        se[d].ncomp = ncomp;
+        const int ex[3] = {n + (d == 0), n + (d == 1), n + (d == 2)}, b0[3] = {0, 0, 0};
+        se[d] = dense_desc(ws_take(c, (size_t)(fn * ncomp)), ex, b0, ncomp);
     }
-    fo.p = ws_take(c, (size_t)(cn * ncomp));
-    fo.jstride = n + 2;
-    fo.kstride = (long long)(n + 2) * (n + 2);
-    fo.nstride = cn;
-    for (int q = 0; q < 3; ++q) {
-        fo.begin[q] = -1;
-        fo.end[q] = n + 1;
-    }
-    fo.ncomp = ncomp;
+    const int exf[3] = {n + 2, n + 2, n + 2}, bf[3] = {-1, -1, -1};
+    fo = dense_desc(ws_take(c, (size_t)(cn * ncomp)), exf, bf, ncomp);
     MX_LAUNCH(c, "k_fill_zero", k_fill_zero, grid1(cn * ncomp), dim3(NTB), fo.p, cn * ncomp);
     mx_multifab sm{1, s}, fm{1, &fo};
     mx_face_multifab sem, um;
@@ -1260,14 +1628,12 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
     for (int d = 0; d < 3; ++d)
         if (!covers(ubar[d], v, 1)) return MX_ERR_BOX;
     const long long n3 = (long long)n * n * n;
-    // dense arrays: S and G (ncomp each), 63 per-component intermediates and
-    // adjoints, U (3)
-    const size_t per = (size_t)n3 * sizeof(double) + 256;
-    const bool fs = c->adj_face_sums && c->p.ppm_type == 1;
-    const size_t count = (size_t)ncomp * (63 + (fs ? 9 : 0) + (c->p.ppm_type == 2 ? 12 : 0)) + 3;
-    if ((rc = ws_reserve(c, per * count + 4096))) return rc;
     AB A;
     std::memset(&A, 0, sizeof(A));
+    // dense arrays: S and G (ncomp each), the intermediates and adjoints, U (3)
+    const size_t per = (size_t)n3 * sizeof(double) + 256;
+    const long long count = ab_arrays(c, A, ncomp, false, [](int) { return (double*)nullptr; });
+    if ((rc = ws_reserve(c, per * (size_t)(count + 2 * ncomp + 3) + 4096))) return rc;
     auto take = [&](int m) { return ws_take(c, (size_t)n3 * m); };
     // inputs that are already dense (ncomp, n, n, n) arrays are read in place
     const bool sd = dense_fab(*s, n, ncomp), gd = dense_fab(*gbar, n, ncomp);
@@ -1276,51 +1642,15 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
     double* S = sd ? s->p : take(ncomp);
     double* G = gd ? gbar->p : take(ncomp);
     for (int d = 0; d < 3; ++d) A.U[d] = ud[d] ? umac[d].p : take(1);
-    for (int d = 0; d < 3; ++d) {
-        A.IP[d] = take(ncomp);
-        A.IM[d] = take(ncomp);
-        A.SIMH[d] = take(ncomp);
-        A.SE[d] = take(ncomp);
-        A.TB[d] = take(ncomp);
-        A.UBC[d] = take(ncomp);
-        if (c->p.ppm_type == 2)
-            for (int o = 0; o < 4; ++o) A.DV[d][o] = take(ncomp);
-        if (fs) {
-            A.SLB[d] = take(ncomp);
-            A.SRB[d] = take(ncomp);
-        }
-        A.FLB[d] = take(ncomp);
-        A.FRB[d] = take(ncomp);
-        A.UHI[d] = take(ncomp);
-        if (fs) A.ULO[d] = take(ncomp);
-        A.DSA[d] = take(ncomp);
-        A.DSB[d] = take(ncomp);
-        A.DLA[d] = take(ncomp);
-        A.DLB[d] = take(ncomp);
-        A.QM[d] = take(ncomp);
-        A.Q0[d] = take(ncomp);
-        A.QP[d] = take(ncomp);
-    }
-    for (int q = 0; q < 6; ++q) {
-        A.COR[q] = take(ncomp);
-        A.EB[q] = take(ncomp);
-    }
+    ab_arrays(c, A, ncomp, false, take);
     A.S = S;
     A.G = G;
-    A.S0S = take(ncomp);
     A.SBF = to_da(*sbar);
     A.cs = n3;
+    A.npts = n3;
     A.n = n;
-    A.ncomp = ncomp;
-    A.dt = c->p.dt;
-    A.rel_eps = c->p.rel_eps;
-    for (int d = 0; d < 3; ++d) {
-        A.dx[d] = c->p.dx[0][d];
-        A.fdt[d] = A.dt / A.dx[d];
-        const double dt4 = 0.25 * A.dt, dt6 = A.dt / 6.0;
-        A.f4[d] = dt4 / A.dx[d];
-        A.f6[d] = dt6 / A.dx[d];
-    }
+    for (int d = 0; d < 3; ++d) A.ext[d] = n;
+    ab_factors(c, A, ncomp);
 
     const dim3 g1 = grid1(n3);
     const dim3 gc(g1.x, (unsigned)ncomp);
@@ -1331,8 +1661,290 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
         if (!ud[d])
             MX_LAUNCH(c, "k_to_dense", k_to_dense, g1, dim3(NTB), to_da(umac[d]), 0, 1, n,
                       const_cast<double*>(A.U[d]));
+    int has_n = 0;
+    for (int d = 0; d < 3; ++d)
+        if (ubar[d].end[d] > n) has_n |= 1 << d;
+    const DA u0 = to_da(ubar[0]), u1 = to_da(ubar[1]), u2 = to_da(ubar[2]);
     const bool p2 = c->adj_pow2 && (n & (n - 1)) == 0;
-    return p2 ? vjp_kernels<true>(c, A, g1, gc, ubar) : vjp_kernels<false>(c, A, g1, gc, ubar);
+    return p2 ? vjp_kernels<PerT<true>, PerT<false>>(c, A, g1, gc, u0, u1, u2, has_n)
+              : vjp_kernels<PerT<false>, PerT<false>>(c, A, g1, gc, u0, u1, u2, has_n);
+}
+
+int mx_adv_scal_step_boxes(mx_adv_ctx* c, mx_multifab* s, mx_face_multifab* umac,
+                           mx_multifab* snew, int ncomp) {
+    if (!c || !s || !umac || !snew) return MX_ERR_ARG;
+    StepDomain D;
+    int rc = box_domain(c, ncomp, D);
+    if (rc) return rc;
+    const auto& grids = c->grids[0];
+    const int nb = (int)grids.size();
+    if (!nfab_ok(c, s) || !nfab_ok(c, snew)) return MX_ERR_ARG;
+    for (int d = 0; d < 3; ++d)
+        if (!nfab_ok(c, &umac->dir[d])) return MX_ERR_ARG;
+    const int g = c->p.ppm_type == 2 ? 4 : 3;
+    for (int b = 0; b < nb; ++b) {
+        const HBox v = hbox(grids[b]);
+        if (!covers(s->fab[b], grow(v, 3, g), ncomp) || !covers(snew->fab[b], v, ncomp))
+            return MX_ERR_BOX;
+        for (int d = 0; d < 3; ++d)
+            if (!covers(umac->dir[d].fab[b], grow(nodal(v, d), 3, 1), 1)) return MX_ERR_BOX;
+    }
+    // ghost cells and faces: FillPatch (periodic images, foextrap beyond an
+    // outflow side) and FillBoundary + FillUmacGhost
+    std::vector<mx_bcrec> bcs(ncomp);
+    for (auto& b : bcs)
+        for (int d = 0; d < 3; ++d) b.lo[d] = b.hi[d] = D.per[d] ? MX_BC_INT_DIR : MX_BC_FOEXTRAP;
+    if ((rc = mx_adv_fill_patch(c, 0, s, s, 0, 0, ncomp, bcs.data(), 0))) return rc;
+    if ((rc = fill_face_ghosts(c, 0, umac))) return rc;
+    // edge states per box (no ghosts) and a zero force with one ghost
+    size_t need = 8192;
+    for (int b = 0; b < nb; ++b) {
+        const HBox v = hbox(grids[b]);
+        for (int d = 0; d < 3; ++d) need += (size_t)npts(nodal(v, d)) * ncomp * 8 + 256;
+        need += (size_t)npts(grow(v, 3, 1)) * ncomp * 8 + 256;
+    }
+    if ((rc = ws_reserve(c, need))) return rc;
+    std::vector<mx_fab> se[3], fo(nb);
+    long long maxpts = 0;
+    for (int b = 0; b < nb; ++b) {
+        const HBox v = hbox(grids[b]);
+        for (int d = 0; d < 3; ++d) {
+            const HBox f = nodal(v, d);
+            const int ex[3] = {f.hi[0] - f.lo[0] + 1, f.hi[1] - f.lo[1] + 1,
+                               f.hi[2] - f.lo[2] + 1};
+            se[d].push_back(dense_desc(ws_take(c, (size_t)npts(f) * ncomp), ex, f.lo, ncomp));
+        }
+        const HBox gv = grow(v, 3, 1);
+        const int ex[3] = {gv.hi[0] - gv.lo[0] + 1, gv.hi[1] - gv.lo[1] + 1,
+                           gv.hi[2] - gv.lo[2] + 1};
+        fo[b] = dense_desc(ws_take(c, (size_t)npts(gv) * ncomp), ex, gv.lo, ncomp);
+        maxpts = std::max(maxpts, npts(v));
+    }
+    if (!ws_fits(c)) return MX_ERR_NOMEM;
+    for (int b = 0; b < nb; ++b) {
+        const long long np = npts(grow(hbox(grids[b]), 3, 1)) * ncomp;
+        MX_LAUNCH(c, "k_fill_zero", k_fill_zero, grid1(np), dim3(NTB), fo[b].p, np);
+    }
+    mx_multifab fm{nb, fo.data()};
+    mx_face_multifab sem;
+    for (int d = 0; d < 3; ++d) sem.dir[d] = {nb, se[d].data()};
+    const int saved = c->fused_edge;
+    c->fused_edge = 1;
+    rc = make_edge_scal(c, 0, s, &sem, umac, &fm, 0, bcs.data(), ncomp, 0, 0, ncomp, 0);
+    c->fused_edge = saved;
+    if (rc) return rc;
+    std::vector<UpdTask> tasks(nb);
+    for (int b = 0; b < nb; ++b) {
+        const HBox v = hbox(grids[b]);
+        UpdTask& t = tasks[b];
+        t.s = to_da(s->fab[b]);
+        t.snew = to_da(snew->fab[b]);
+        for (int d = 0; d < 3; ++d) {
+            t.u[d] = to_da(umac->dir[d].fab[b]);
+            t.e[d] = to_da(se[d][b]);
+            t.lo[d] = v.lo[d];
+            t.hi[d] = v.hi[d];
+        }
+    }
+    Table tab(c);
+    const size_t off = tab.add(tasks);
+    if ((rc = tab.upload())) return rc;
+    MX_LAUNCH(c, "k_box_update", k_box_update, grid_for(maxpts, nb, 1, NTB), dim3(NTB),
+              tab.dev<UpdTask>(off), ncomp, c->p.dt, c->p.dx[0][0], c->p.dx[0][1],
+              c->p.dx[0][2]);
+    return MX_OK;
+}
+
+int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* c, const mx_multifab* s, const mx_face_multifab* umac,
+                               const mx_multifab* gbar, mx_multifab* sbar,
+                               mx_face_multifab* ubar, int ncomp) {
+    if (!c || !s || !umac || !gbar || !sbar || !ubar) return MX_ERR_ARG;
+    StepDomain D;
+    int rc = box_domain(c, ncomp, D);
+    if (rc) return rc;
+    const auto& grids = c->grids[0];
+    const int nb = (int)grids.size();
+    if (!nfab_ok(c, s) || !nfab_ok(c, gbar) || !nfab_ok(c, sbar)) return MX_ERR_ARG;
+    for (int d = 0; d < 3; ++d)
+        if (!nfab_ok(c, &umac->dir[d]) || !nfab_ok(c, &ubar->dir[d])) return MX_ERR_ARG;
+    long long maxpts = 0;
+    for (int b = 0; b < nb; ++b) {
+        const HBox v = hbox(grids[b]);
+        if (!covers(s->fab[b], v, ncomp) || !covers(gbar->fab[b], v, ncomp) ||
+            !covers(sbar->fab[b], v, ncomp))
+            return MX_ERR_BOX;
+        for (int d = 0; d < 3; ++d)
+            if (!covers(umac->dir[d].fab[b], nodal(v, d), 1) ||
+                !covers(ubar->dir[d].fab[b], nodal(v, d), 1))
+                return MX_ERR_BOX;
+        maxpts = std::max(maxpts, npts(nodal(nodal(nodal(v, 0), 1), 2)));
+    }
+    // the layout: along an outflow direction positions 0..nn (face nn) and
+    // the reach of the stencils (6 points: ghost positions up to 3 outside,
+    // each read 3 further) on both sides
+    constexpr int PAD = 6;
+    AB A;
+    std::memset(&A, 0, sizeof(A));
+    Dense L;
+    L.p = nullptr;
+    long long cs = 1, npt = 1;
+    for (int d = 0; d < 3; ++d) {
+        L.pad[d] = D.per[d] ? 0 : PAD;
+        A.ext[d] = D.nn[d] + (D.per[d] ? 0 : 1);
+        L.E[d] = A.ext[d] + 2 * L.pad[d];
+        L.dlo[d] = D.lo[d];
+        L.nn[d] = D.nn[d];
+        L.per[d] = D.per[d];
+        cs *= L.E[d];
+        npt *= A.ext[d];
+        A.gen.nn[d] = D.nn[d];
+        A.gen.per[d] = D.per[d];
+        A.gen.pad[d] = L.pad[d];
+    }
+    if (cs > 0x7fffffffLL || npt > 0xffffffffLL) return MX_ERR_UNSUPPORTED;
+    L.cs = cs;
+    A.gen.sj = L.E[0];
+    A.gen.sk = L.E[0] * L.E[1];
+    A.cs = cs;
+    A.npts = npt;
+    A.fold = 3;
+    A.n = D.nn[0];
+    const long long ncell = (long long)D.nn[0] * D.nn[1] * D.nn[2];
+    long long nface[3];
+    for (int d = 0; d < 3; ++d)
+        nface[d] = ncell / D.nn[d] * (D.nn[d] + (D.per[d] ? 0 : 1));
+    const long long count = ab_arrays(c, A, ncomp, true, [](int) { return (double*)nullptr; });
+    size_t need = (size_t)(count + 2 * ncomp + 3) * ((size_t)cs * 8 + 256) +
+                  (size_t)ncell * ncomp * 8 + (nface[0] + nface[1] + nface[2]) * 8 + 8192;
+    if ((rc = ws_reserve(c, need))) return rc;
+    std::vector<double*> padded;  // every array whose shell the sweeps read as zero
+    auto take = [&](int m) {
+        double* p = ws_take(c, (size_t)cs * m);
+        for (int q = 0; q < m; ++q) padded.push_back(p + (size_t)q * cs);
+        return p;
+    };
+    double* S = ws_take(c, (size_t)cs * ncomp);
+    double* G = ws_take(c, (size_t)cs * ncomp);
+    double* U[3];
+    for (int d = 0; d < 3; ++d) U[d] = ws_take(c, (size_t)cs);
+    ab_arrays(c, A, ncomp, true, take);
+    // outputs on the domain's cells and faces
+    const int zero3[3] = {0, 0, 0};
+    mx_fab sbd = dense_desc(ws_take(c, (size_t)ncell * ncomp), D.nn, zero3, ncomp);
+    mx_fab ubd[3];
+    for (int d = 0; d < 3; ++d) {
+        int ex[3] = {D.nn[0], D.nn[1], D.nn[2]};
+        ex[d] += D.per[d] ? 0 : 1;
+        ubd[d] = dense_desc(ws_take(c, (size_t)nface[d]), ex, zero3, 1);
+    }
+    if (!ws_fits(c)) return MX_ERR_NOMEM;
+    A.S = S;
+    A.G = G;
+    for (int d = 0; d < 3; ++d) A.U[d] = U[d];
+    A.SBF = to_da(sbd);
+    ab_factors(c, A, ncomp);
+
+    // gather the inputs: cells of s and gbar, faces lo..hi (and the high
+    // domain face) of umac
+    std::vector<XTask> cells_s(nb), cells_g(nb), faces[3];
+    for (int b = 0; b < nb; ++b) {
+        const HBox v = hbox(grids[b]);
+        cells_s[b].f = to_da(s->fab[b]);
+        cells_g[b].f = to_da(gbar->fab[b]);
+        for (int q = 0; q < 3; ++q) {
+            cells_s[b].lo[q] = cells_g[b].lo[q] = v.lo[q];
+            cells_s[b].hi[q] = cells_g[b].hi[q] = v.hi[q];
+        }
+        for (int d = 0; d < 3; ++d) {
+            XTask t;
+            t.f = to_da(umac->dir[d].fab[b]);
+            for (int q = 0; q < 3; ++q) {
+                t.lo[q] = v.lo[q];
+                t.hi[q] = v.hi[q];
+            }
+            if (!D.per[d] && v.hi[d] == D.lo[d] + D.nn[d] - 1) t.hi[d] += 1;
+            faces[d].push_back(t);
+        }
+    }
+    Table tab(c);
+    const size_t o_s = tab.add(cells_s), o_g = tab.add(cells_g);
+    size_t o_f[3];
+    for (int d = 0; d < 3; ++d) o_f[d] = tab.add(faces[d]);
+    const size_t o_p = tab.add(padded);
+    if ((rc = tab.upload())) return rc;
+    const dim3 gx = grid_for(maxpts, nb, 1, NTB);
+    auto dense_of = [&](double* p) {
+        Dense q = L;
+        q.p = p;
+        return q;
+    };
+    MX_LAUNCH(c, "k_box_xfer", k_box_xfer<true>, gx, dim3(NTB), tab.dev<XTask>(o_s), dense_of(S),
+              ncomp);
+    MX_LAUNCH(c, "k_box_xfer", k_box_xfer<true>, gx, dim3(NTB), tab.dev<XTask>(o_g), dense_of(G),
+              ncomp);
+    for (int d = 0; d < 3; ++d)
+        MX_LAUNCH(c, "k_box_xfer", k_box_xfer<true>, gx, dim3(NTB), tab.dev<XTask>(o_f[d]),
+                  dense_of(U[d]), 1);
+    const dim3 ga = grid1(cs);
+    MX_LAUNCH(c, "k_dense_outside", k_dense_outside, ga, dim3(NTB), dense_of(S), ncomp, -1, 1);
+    MX_LAUNCH(c, "k_dense_outside", k_dense_outside, ga, dim3(NTB), dense_of(G), ncomp, -1, 0);
+    for (int d = 0; d < 3; ++d)
+        MX_LAUNCH(c, "k_dense_outside", k_dense_outside, ga, dim3(NTB), dense_of(U[d]), 1, d, 0);
+    const long long shell = cs - npt;
+    if (shell > 0)
+        MX_LAUNCH(c, "k_shell_zero", k_shell_zero, dim3(grid1(shell).x, (unsigned)padded.size()),
+                  dim3(NTB), tab.dev<double*>(o_p), L.E[0], L.E[1], L.E[2], A.ext[0], A.ext[1],
+                  A.ext[2], L.pad[0], L.pad[1], L.pad[2]);
+
+    const dim3 g1 = grid1(npt);
+    const dim3 gc(g1.x, (unsigned)ncomp);
+    if ((rc = vjp_kernels<GenT, GenT>(c, A, g1, gc, to_da(ubd[0]), to_da(ubd[1]), to_da(ubd[2]),
+                                      0)))
+        return rc;
+
+    // scatter: sbar on every box's cells, ubar on every face a box holds
+    // (faces shared by two boxes, and periodic images, get the same value)
+    std::vector<XTask> out_s(nb), out_u[3];
+    for (int b = 0; b < nb; ++b) {
+        const HBox v = hbox(grids[b]);
+        out_s[b].f = to_da(sbar->fab[b]);
+        for (int q = 0; q < 3; ++q) {
+            out_s[b].lo[q] = v.lo[q];
+            out_s[b].hi[q] = v.hi[q];
+        }
+        for (int d = 0; d < 3; ++d) {
+            XTask t;
+            t.f = to_da(ubar->dir[d].fab[b]);
+            const HBox f = nodal(v, d);
+            for (int q = 0; q < 3; ++q) {
+                t.lo[q] = f.lo[q];
+                t.hi[q] = f.hi[q];
+            }
+            out_u[d].push_back(t);
+        }
+    }
+    Table tab2(c);
+    const size_t q_s = tab2.add(out_s);
+    size_t q_u[3];
+    for (int d = 0; d < 3; ++d) q_u[d] = tab2.add(out_u[d]);
+    if ((rc = tab2.upload())) return rc;
+    Dense LS = L, LU[3];
+    LS.p = sbd.p;
+    LS.cs = ncell;
+    for (int q = 0; q < 3; ++q) {
+        LS.E[q] = D.nn[q];
+        LS.pad[q] = 0;
+    }
+    MX_LAUNCH(c, "k_box_xfer", k_box_xfer<false>, gx, dim3(NTB), tab2.dev<XTask>(q_s), LS, ncomp);
+    for (int d = 0; d < 3; ++d) {
+        LU[d] = LS;
+        LU[d].p = ubd[d].p;
+        LU[d].cs = nface[d];
+        LU[d].E[d] = D.nn[d] + (D.per[d] ? 0 : 1);
+        MX_LAUNCH(c, "k_box_xfer", k_box_xfer<false>, gx, dim3(NTB), tab2.dev<XTask>(q_u[d]),
+                  LU[d], 1);
+    }
+    return MX_OK;
 }
 
 }  // extern "C"
