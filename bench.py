@@ -450,6 +450,89 @@ def adjoint_main(args, emit=True):
     return out
 
 
+def sphere_adjoint(args, w):
+    """The spherical star's rollout gradient with the box-list step and its
+    adjoint (mx_adv_scal_step_boxes / _vjp_boxes): rho' and (rho h)' of the
+    star (384^3 in 216 uneven boxes, outflow sides, ppm_type 2) advected by
+    the MAC velocity of the pass just timed (VelPred with w0).  Times one
+    forward step, one adjoint step and the gradient of a 10-step rollout
+    with checkpoints every 5 steps (its forward, the recomputed segments and
+    the adjoint steps)."""
+    from maestroex_amd.adjoint import BoxStepper
+    from maestroex_amd.fabs import FaceMultiFab, MultiFab
+    n, nc, nsteps, every = args.n, 2, 10, 5
+    st = BoxStepper(w.boxes, [n] * 3, [0, 0, 0], nc, w.dt, w.rel_eps, ppm_type=2, spherical=1)
+    bk, ng = st.bk, st.NG[2]
+
+    def cells(ngrow):
+        return MultiFab(w.boxes, nc, ngrow, 3, bk)
+
+    s0 = cells(ng)
+    for b in range(len(w.boxes)):
+        s0.arrays[b].copy_(w.scal.arrays[b][0:nc])  # the workload's 4 ghosts
+    u = w.umac
+    a, b_ = cells(ng), cells(ng)
+    for x, y in zip(a.arrays, s0.arrays):
+        x.copy_(y)
+    fwd_ms = time_ms(lambda: st.step_mf(a, u, b_), args.steps, args.warmup)
+    g, sb = cells(0), cells(0)
+    for x in g.arrays:
+        x.fill_(1.0)
+    ub = FaceMultiFab(w.boxes, 1, 0, 3, bk)
+    adj_ms = time_ms(lambda: st.vjp_mf(a, u, g, sb, ub), args.steps, args.warmup)
+    fns = st.fns
+    fns["prof_enable"](st.m.ctx, 1)
+    st.step_mf(a, u, b_)
+    st.vjp_mf(a, u, g, sb, ub)
+    nk = fns["prof_read"](st.m.ctx, -1, None, None, None)
+    kern = {}
+    for idx in range(nk):
+        nm, ms, cnt = C.c_char_p(), C.c_double(), C.c_longlong()
+        fns["prof_read"](st.m.ctx, idx, C.byref(nm), C.byref(ms), C.byref(cnt))
+        kern[nm.value.decode()] = round(ms.value, 4)
+    fns["prof_enable"](st.m.ctx, 0)
+    print(f"[bench] spherical step kernels ms: {kern}", file=sys.stderr, flush=True)
+
+    # the rollout: states in preallocated MultiFabs, the loss gradient at
+    # the end taken as the final state (a zero target)
+    ckpt = [s0, cells(ng)]
+    seg_states = [cells(ng) for _ in range(every - 1)]
+    work = [cells(ng), cells(ng)]
+    gbs = [cells(0), cells(0)]
+
+    def rollout():
+        cur, nxt = work
+        src = s0
+        for k in range(nsteps):
+            st.step_mf(src, u, nxt)
+            cur, nxt = nxt, cur
+            src = cur
+            if (k + 1) % every == 0 and k + 1 < nsteps:
+                for x, y in zip(ckpt[(k + 1) // every].arrays, cur.arrays):
+                    x.copy_(y)
+        gb, sbn = gbs
+        for bi in range(len(w.boxes)):
+            gb.arrays[bi].copy_(cur.arrays[bi][(slice(None),) + cur.valid_region(bi)])
+        for seg in reversed(range(nsteps // every)):
+            states = [ckpt[seg]]
+            for q in range(every - 1):
+                st.step_mf(states[-1], u, seg_states[q])
+                states.append(seg_states[q])
+            for sx in reversed(states):
+                st.vjp_mf(sx, u, gb, sbn, ub)
+                gb, sbn = sbn, gb
+
+    roll_ms = time_ms(rollout, 1, 1)
+    out = {"workload": f"rho' and (rho h)' of the 384^3 star in {len(w.boxes)} uneven boxes, "
+                       "outflow sides, ppm_type 2, the pass's MAC velocity",
+           "forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4),
+           "adjoint_over_forward": round(adj_ms / fwd_ms, 3),
+           "rollout_10_steps_grad_ms": round(roll_ms, 2),
+           "rollout_over_forward_steps": round(roll_ms / (nsteps * fwd_ms), 3)}
+    print(f"[bench] spherical adjoint: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 # algorithmic bytes per cell of the update stages (3-D planar, NumSpec = 3,
 # each face counted once): UpdateScal species 2 + 6 NumSpec doubles (old and
 # new density and species, three fluxes and the force of each species),
@@ -1139,6 +1222,8 @@ def main():
     extras = {}
     if not args.headline_only and not args.spherical and args.dim == 3:
         extras = rest_of_metric(args, w, fields)
+    if not args.headline_only and args.spherical:
+        extras["sphere_adjoint"] = sphere_adjoint(args, w)
 
     cpu = None
     if not args.no_cpu:

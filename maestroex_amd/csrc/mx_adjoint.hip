@@ -1714,17 +1714,18 @@ int mx_adv_scal_step_boxes(mx_adv_ctx* c, mx_multifab* s, mx_face_multifab* umac
                                f.hi[2] - f.lo[2] + 1};
             se[d].push_back(dense_desc(ws_take(c, (size_t)npts(f) * ncomp), ex, f.lo, ncomp));
         }
-        const HBox gv = grow(v, 3, 1);
+        maxpts = std::max(maxpts, npts(v));
+    }
+    // the zero force of every box in one span (one fill)
+    for (int b = 0; b < nb; ++b) {
+        const HBox gv = grow(hbox(grids[b]), 3, 1);
         const int ex[3] = {gv.hi[0] - gv.lo[0] + 1, gv.hi[1] - gv.lo[1] + 1,
                            gv.hi[2] - gv.lo[2] + 1};
         fo[b] = dense_desc(ws_take(c, (size_t)npts(gv) * ncomp), ex, gv.lo, ncomp);
-        maxpts = std::max(maxpts, npts(v));
     }
     if (!ws_fits(c)) return MX_ERR_NOMEM;
-    for (int b = 0; b < nb; ++b) {
-        const long long np = npts(grow(hbox(grids[b]), 3, 1)) * ncomp;
-        MX_LAUNCH(c, "k_fill_zero", k_fill_zero, grid1(np), dim3(NTB), fo[b].p, np);
-    }
+    const long long span = (long long)(fo[nb - 1].p + fo[nb - 1].nstride * ncomp - fo[0].p);
+    MX_LAUNCH(c, "k_fill_zero", k_fill_zero, grid1(span), dim3(NTB), fo[0].p, span);
     mx_multifab fm{nb, fo.data()};
     mx_face_multifab sem;
     for (int d = 0; d < 3; ++d) sem.dir[d] = {nb, se[d].data()};
@@ -1814,12 +1815,33 @@ int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* c, const mx_multifab* s, const mx_fac
     for (int d = 0; d < 3; ++d)
         nface[d] = ncell / D.nn[d] * (D.nn[d] + (D.per[d] ? 0 : 1));
     const long long count = ab_arrays(c, A, ncomp, true, [](int) { return (double*)nullptr; });
-    size_t need = (size_t)(count + 2 * ncomp + 3) * ((size_t)cs * 8 + 256) +
+    size_t need = (size_t)(2 * ncomp + 3) * ((size_t)cs * 8 + 256) +
                   (size_t)ncell * ncomp * 8 + (nface[0] + nface[1] + nface[2]) * 8 + 8192;
     if ((rc = ws_reserve(c, need))) return rc;
+    // the intermediates live in the context's adjoint buffer, whose padding
+    // is zeroed when the buffer or the layout changes
+    const size_t abytes = (size_t)count * (size_t)cs * 8 + 256 * (size_t)(count + 1);
+    const long long layout[8] = {L.E[0], L.E[1], L.E[2], A.ext[0], A.ext[1], A.ext[2], count,
+                                 ncomp};
+    bool rezero = std::memcmp(layout, c->adj_layout, sizeof(layout)) != 0;
+    if (abytes > c->adj_cap) {
+        MX_HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (c->adj_buf) MX_HIP_CHECK(hipFree(c->adj_buf));
+        c->adj_buf = nullptr;
+        c->adj_cap = 0;
+        if (hipMalloc((void**)&c->adj_buf, abytes) != hipSuccess) {
+            c->adj_buf = nullptr;
+            return MX_ERR_NOMEM;
+        }
+        c->adj_cap = abytes;
+        rezero = true;
+    }
     std::vector<double*> padded;  // every array whose shell the sweeps read as zero
+    size_t aoff = 0;
     auto take = [&](int m) {
-        double* p = ws_take(c, (size_t)cs * m);
+        aoff = (aoff + 255) & ~(size_t)255;
+        double* p = reinterpret_cast<double*>(c->adj_buf + aoff);
+        aoff += (size_t)cs * m * sizeof(double);
         for (int q = 0; q < m; ++q) padded.push_back(p + (size_t)q * cs);
         return p;
     };
@@ -1837,7 +1859,7 @@ int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* c, const mx_multifab* s, const mx_fac
         ex[d] += D.per[d] ? 0 : 1;
         ubd[d] = dense_desc(ws_take(c, (size_t)nface[d]), ex, zero3, 1);
     }
-    if (!ws_fits(c)) return MX_ERR_NOMEM;
+    if (!ws_fits(c) || aoff > c->adj_cap) return MX_ERR_NOMEM;
     A.S = S;
     A.G = G;
     for (int d = 0; d < 3; ++d) A.U[d] = U[d];
@@ -1891,10 +1913,11 @@ int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* c, const mx_multifab* s, const mx_fac
     for (int d = 0; d < 3; ++d)
         MX_LAUNCH(c, "k_dense_outside", k_dense_outside, ga, dim3(NTB), dense_of(U[d]), 1, d, 0);
     const long long shell = cs - npt;
-    if (shell > 0)
+    if (shell > 0 && rezero)
         MX_LAUNCH(c, "k_shell_zero", k_shell_zero, dim3(grid1(shell).x, (unsigned)padded.size()),
                   dim3(NTB), tab.dev<double*>(o_p), L.E[0], L.E[1], L.E[2], A.ext[0], A.ext[1],
                   A.ext[2], L.pad[0], L.pad[1], L.pad[2]);
+    std::memcpy(c->adj_layout, layout, sizeof(layout));
 
     const dim3 g1 = grid1(npt);
     const dim3 gc(g1.x, (unsigned)ncomp);

@@ -929,6 +929,7 @@ int mx_adv_destroy(mx_adv_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (int lev = 0; lev < MX_MAX_LEV; ++lev) flux_reg_free(c, lev);
     if (c->ws) (void)hipFree(c->ws);
+    if (c->adj_buf) (void)hipFree(c->adj_buf);
     for (auto& t : c->tab_slots) {
         free_tab_slot(t);
         if (t.done) (void)hipEventDestroy(t.done);

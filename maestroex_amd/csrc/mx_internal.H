@@ -28,6 +28,13 @@ struct mx_adv_ctx {
     size_t ws_cap = 0;
     size_t ws_off = 0;
 
+    // the box-list adjoint's padded arrays: no kernel writes their padding,
+    // so it stays zero from one call to the next while the layout
+    // (adj_layout) is unchanged
+    char* adj_buf = nullptr;
+    size_t adj_cap = 0;
+    long long adj_layout[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
     // per-call box tables: built in htab; a new table is copied through a
     // ring of pinned staging buffers, so that an upload never waits for the
     // stream to drain (a pageable copy would).  A slot is reused after its
