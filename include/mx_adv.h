@@ -740,11 +740,17 @@ int mx_adv_update_vel(mx_adv_ctx* ctx, int lev, const mx_multifab* uold, mx_mult
 
 /* ---- differentiation ---------------------------------------------------- */
 /* A differentiable scalar advection step (no single counterpart: the
-   composition of MakeEdgeScal, MakeRhoXFlux and UpdateScal for a species
-   (MaestroDensityAdvance.cpp:115-122, MaestroMakeFlux.cpp:118-140,
-   MaestroAdvection.cpp:165-183) with a fixed MAC velocity and no forcing):
+   enthalpy step of EnthalpyAdvance with enthalpy_pred_type predict_rhohprime,
+   the default, for a perturbation (rho h)' with a zero base state, a fixed
+   MAC velocity and no forcing: MakeEdgeScal in the advective form
+   (MaestroEnthalpyAdvance.cpp:143-147), MakeRhoHFlux's
+   umac (rhoh0_edge + sedge) with rhoh0_edge = 0 (MaestroMakeFlux.cpp:699-708)
+   and UpdateScal's enthalpy update before its EOS branch
+   (MaestroAdvection.cpp:120-135)):
      sedge = MakeEdgeScal(s, umac, force 0, advective form), ppm_type 1 or 2
      snew  = s + dt (-div(umac sedge) + 0)
+   A non-zero base state adds a flux that does not depend on s, so the
+   state adjoint is the same; the velocity adjoint leaves that flux out.
    on a context whose level 0 is one fully periodic box [0, n)^3.  s holds
    ncomp components with >= 3 ghost cells (>= 4 for ppm_type 2; fewer is
    MX_ERR_BOX), umac[0..2] one face FAB per

@@ -1,14 +1,16 @@
 // mx_adjoint.hip -- a differentiable scalar advection step and its
 // hand-written adjoint (vector-Jacobian product).
 //
-// The step, on one fully periodic box [0, N)^3 with a fixed MAC velocity,
-// for every scalar component:
-//   sedge = MakeEdgeScal(s, umac, force = 0, advective form)   (ppm_type 1)
-//   F     = umac * sedge              (MakeRhoXFlux, predict_rhoX,
-//                                      MaestroMakeFlux.cpp:118-140)
-//   snew  = s + dt * (-div F + 0)     (UpdateScal, MaestroAdvection.cpp:165-183)
-// The forward runs the production kernels (periodic ghost fill, the fused
-// edge-state kernel) plus an update kernel.
+// The step, on one fully periodic box [0, N)^3 or over a box list with
+// periodic or outflow sides, with a fixed MAC velocity, for every scalar
+// component (the enthalpy step of EnthalpyAdvance, predict_rhohprime, for
+// (rho h)' with a zero base state):
+//   sedge = MakeEdgeScal(s, umac, force = 0, advective form)  (ppm_type 1, 2;
+//                                      MaestroEnthalpyAdvance.cpp:143-147)
+//   F     = umac * (0 + sedge)        (MakeRhoHFlux, MaestroMakeFlux.cpp:699-708)
+//   snew  = s + dt * (-div F + 0)     (UpdateScal, MaestroAdvection.cpp:120-135)
+// The forward runs the production kernels (ghost fill, the fused edge-state
+// kernel) plus an update kernel.
 //
 // The adjoint differentiates the same arithmetic with every limiter and
 // upwind branch frozen at its forward value (the function is piecewise

@@ -9,9 +9,13 @@
 //           (MaestroMakeEdgeScalars.cpp: PPM, Predictor :662-863, Transverse
 //            :865-1359, Edges :1361-1667; interior PPM formulas: no
 //            ext_dir or hoextrap sides)
-//   F     = umac * sedge                     (MakeRhoXFlux, predict_rhoX,
-//                                             MaestroMakeFlux.cpp:118-140)
-//   snew  = s + dt * (-div F + 0)            (UpdateScal, MaestroAdvection.cpp:165-183)
+//   F     = umac * sedge                     (MakeRhoHFlux, predict_rhohprime
+//                                             with rhoh0_edge = 0,
+//                                             MaestroMakeFlux.cpp:699-708)
+//   snew  = s + dt * (-div F + 0)            (UpdateScal, MaestroAdvection.cpp:120-135)
+// (the enthalpy step of MaestroEnthalpyAdvance.cpp:143-147; the same
+// arithmetic as a species step with species_pred_type 2 and an advective
+// prediction, which the oracle's stages are compared against)
 // written once for a number type T: double gives the step itself, Dual (a
 // value and one directional derivative) gives the Jacobian-vector product
 // with the limiter branches taken at the primal values.
