@@ -71,7 +71,8 @@ void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double r
               const double* const du[3], double* dsnew);
 void ScalStepBC(const int n[3], const int per[3], int ncomp, int ppm, double dt,
                 const double dx[3], double rel_eps, const double* s, const double* const u[3],
-                double* snew, const double* ds, const double* const du[3], double* dsnew);
+                double* snew, const double* ds, const double* const du[3], double* dsnew,
+                bool cons);
 void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const double* s,
                  const double* const u[3], double out[7]);
 int Put1dArrayOnCart(const Ctx& c, int lev, const BaseArr& s0, const std::vector<Arr>& s0_cart,
@@ -412,11 +413,13 @@ int mxo_scal_step_jvp(int n, int ncomp, int ppm, double dt, const double* dx, do
 
 // The step on n[0] x n[1] x n[2] cells, each direction periodic (per[d])
 // or outflow (face arrays one point longer along their outflow direction),
-// and its Jacobian-vector product when ds is given (du[d] may be null).
+// advective or conservative form, and its Jacobian-vector product when ds
+// is given (du[d] may be null).
 int mxo_scal_step_bc(const int* n, const int* per, int ncomp, int ppm, double dt,
                      const double* dx, double rel_eps, const double* s, const double* u0,
                      const double* u1, const double* u2, double* snew, const double* ds,
-                     const double* du0, const double* du1, const double* du2, double* dsnew) {
+                     const double* du0, const double* du1, const double* du2, double* dsnew,
+                     int is_conservative) {
     if (!n || !per || ncomp < 1 || (ppm != 1 && ppm != 2) || !dx || !s || !u0 || !u1 || !u2 ||
         !snew || (ds && !dsnew))
         return MX_ERR_ARG;
@@ -424,7 +427,8 @@ int mxo_scal_step_bc(const int* n, const int* per, int ncomp, int ppm, double dt
         if (n[d] < 4) return MX_ERR_ARG;
     const double* u[3] = {u0, u1, u2};
     const double* du[3] = {du0, du1, du2};
-    ScalStepBC(n, per, ncomp, ppm, dt, dx, rel_eps, s, u, snew, ds, du, dsnew);
+    ScalStepBC(n, per, ncomp, ppm, dt, dx, rel_eps, s, u, snew, ds, du, dsnew,
+               is_conservative != 0);
     return MX_OK;
 }
 

@@ -328,9 +328,12 @@ inline void outflow_face(int b, T& l, T& r) {
 
 // one step of one component; s and snew are cell arrays, u[d] face arrays
 // marks (op counting only): the counter after each of the five phases
+// cons: the conservative form of MakeEdgeScal (MaestroMakeEdgeScalars.cpp:
+// 906-929 transverse, :1411-1438 edges, MakeDivU :648-660)
 template <class T>
 void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double dt,
-               const double dx[3], double rel_eps, long long* marks = nullptr, int ppm = 1) {
+               const double dx[3], double rel_eps, long long* marks = nullptr, int ppm = 1,
+               bool cons = false) {
     const long long NC = g.cells();
     auto cell = [&](int i, int j, int k) { return g.at(i, j, k); };
     std::vector<T> Ip[3], Im[3], simh[3], corner[6], se[3];
@@ -388,14 +391,26 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
         });
     }
     if (marks) marks[1] = g_ops;
-    // corner-coupled states simh_dt on d-faces, corrected by t; the
-    // transverse term of cell (i,j,k) in direction t
+    const double dt2 = 0.5 * dt;
+    // corner-coupled states simh_dt on d-faces, corrected by t: the
+    // state `base` of cell (i,j,k) with its transverse term in direction t
     const double dt6 = dt / 6.0;
-    auto tterm = [&](int t, int i, int j, int k) {
+    const double dt3 = dt / 3.0;
+    // MakeDivU of cell (i,j,k): the six face differences over dx[0]
+    auto divu = [&](int i, int j, int k) {
+        T dv = u[0][g.face(0, i + 1, j, k)] - u[0][g.face(0, i, j, k)] + u[1][g.face(1, i, j + 1, k)] -
+               u[1][g.face(1, i, j, k)] + u[2][g.face(2, i, j, k + 1)] - u[2][g.face(2, i, j, k)];
+        return dv / dx[0];
+    };
+    auto corrected = [&](T base, int t, int i, int j, int k) {
         int p, q, r;
         step_e(t, p, q, r);
         const long long f0 = g.face(t, i, j, k), f1 = g.face(t, i + p, j + q, k + r);
-        return (dt6 / dx[t]) * (u[t][f1] + u[t][f0]) * (simh[t][f1] - simh[t][f0]);
+        if (!cons)
+            return base - (dt6 / dx[t]) * (u[t][f1] + u[t][f0]) * (simh[t][f1] - simh[t][f0]);
+        const T sc = s[cell(i, j, k)];
+        return base - (dt3 / dx[t]) * (simh[t][f1] * u[t][f1] - simh[t][f0] * u[t][f0]) -
+               dt3 * sc * divu(i, j, k) + (dt3 / dx[t]) * sc * (u[t][f1] - u[t][f0]);
     };
     for (int d = 0; d < 3; ++d)
         for (int t = 0; t < 3; ++t) {
@@ -404,8 +419,8 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
             step_e(d, a, b, c);
             faces(d, [&](int i, int j, int k, int bd) {
                 T l = cst<T>(0.0), rr = cst<T>(0.0);
-                if (bd != 1) l = Ip[d][cell(i - a, j - b, k - c)] - tterm(t, i - a, j - b, k - c);
-                if (bd != 2) rr = Im[d][cell(i, j, k)] - tterm(t, i, j, k);
+                if (bd != 1) l = corrected(Ip[d][cell(i - a, j - b, k - c)], t, i - a, j - b, k - c);
+                if (bd != 2) rr = corrected(Im[d][cell(i, j, k)], t, i, j, k);
                 outflow_face(bd, l, rr);
                 const long long f = g.face(d, i, j, k);
                 corner[pair_index(d, t)][f] = upwind(l, rr, u[d][f], rel_eps);
@@ -416,13 +431,20 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
     // y: x, z; z: x, y), force zero; the edge term of cell (i,j,k) from the
     // corner state on its t-faces corrected by o
     const double dt4 = 0.25 * dt;
-    const double dt2 = 0.5 * dt;
     auto eterm = [&](int t, int o, int i, int j, int k) {
         int p, q, r;
         step_e(t, p, q, r);
         const std::vector<T>& ct = corner[pair_index(t, o)];
         const long long f0 = g.face(t, i, j, k), f1 = g.face(t, i + p, j + q, k + r);
+        if (cons) return (dt2 / dx[t]) * (ct[f1] * u[t][f1] - ct[f0] * u[t][f0]);
         return (dt4 / dx[t]) * (u[t][f1] + u[t][f0]) * (ct[f1] - ct[f0]);
+    };
+    // the conservative form's normal term of cell (i,j,k) on its d-faces
+    auto nterm = [&](int d, int i, int j, int k) {
+        int a, b, c;
+        step_e(d, a, b, c);
+        return (dt2 / dx[d]) * s[cell(i, j, k)] *
+               (u[d][g.face(d, i + a, j + b, k + c)] - u[d][g.face(d, i, j, k)]);
     };
     for (int d = 0; d < 3; ++d) {
         int a, b, c;
@@ -433,11 +455,13 @@ void step_comp(const Grid& g, const T* s, const T* const u[3], T* snew, double d
                 l = Ip[d][cell(i - a, j - b, k - c)];
                 for (int t = 0; t < 3; ++t)
                     if (t != d) l = l - eterm(t, 3 - d - t, i - a, j - b, k - c);
+                if (cons) l = l - nterm(d, i - a, j - b, k - c);
             }
             if (bd != 2) {
                 rr = Im[d][cell(i, j, k)];
                 for (int t = 0; t < 3; ++t)
                     if (t != d) rr = rr - eterm(t, 3 - d - t, i, j, k);
+                if (cons) rr = rr - nterm(d, i, j, k);
             }
             T sedgel = l + dt2 * 0.0;
             T sedger = rr + dt2 * 0.0;
@@ -523,12 +547,14 @@ void StepOpCount(int n, double dt, const double dx[3], double rel_eps, const dou
 }
 
 // the step (T = double) or its directional derivative (ds, du != null) on
-// n[0] x n[1] x n[2] cells, direction d periodic (per[d]) or outflow:
+// n[0] x n[1] x n[2] cells, direction d periodic (per[d]) or outflow, in
+// the advective or (cons) the conservative form:
 // s, snew: ncomp cell arrays; u[d]: the face array of direction d (one more
 // point along d when d is outflow; du[d] may be null)
 void ScalStepBC(const int n[3], const int per[3], int ncomp, int ppm, double dt,
                 const double dx[3], double rel_eps, const double* s, const double* const u[3],
-                double* snew, const double* ds, const double* const du[3], double* dsnew) {
+                double* snew, const double* ds, const double* const du[3], double* dsnew,
+                bool cons) {
     const Grid g{{n[0], n[1], n[2]}, {per[0] ? 1 : 0, per[1] ? 1 : 0, per[2] ? 1 : 0}};
     const long long NC = g.cells();
     // components are independent: one OpenMP thread each (the double and
@@ -536,7 +562,8 @@ void ScalStepBC(const int n[3], const int per[3], int ncomp, int ppm, double dt,
     if (!ds) {
 #pragma omp parallel for schedule(dynamic, 1)
         for (int c = 0; c < ncomp; ++c)
-            step_comp<double>(g, s + c * NC, u, snew + c * NC, dt, dx, rel_eps, nullptr, ppm);
+            step_comp<double>(g, s + c * NC, u, snew + c * NC, dt, dx, rel_eps, nullptr, ppm,
+                              cons);
         return;
     }
     std::vector<Dual> ud[3];
@@ -551,7 +578,7 @@ void ScalStepBC(const int n[3], const int per[3], int ncomp, int ppm, double dt,
     for (int c = 0; c < ncomp; ++c) {
         std::vector<Dual> sd(NC), out(NC);
         for (long long x = 0; x < NC; ++x) sd[x] = {s[c * NC + x], ds[c * NC + x]};
-        step_comp<Dual>(g, sd.data(), up, out.data(), dt, dx, rel_eps, nullptr, ppm);
+        step_comp<Dual>(g, sd.data(), up, out.data(), dt, dx, rel_eps, nullptr, ppm, cons);
         for (long long x = 0; x < NC; ++x) {
             snew[c * NC + x] = out[x].v;
             dsnew[c * NC + x] = out[x].d;
@@ -565,7 +592,7 @@ void ScalStep(int n, int ncomp, int ppm, double dt, const double dx[3], double r
               const double* s, const double* const u[3], double* snew, const double* ds,
               const double* const du[3], double* dsnew) {
     const int nn[3] = {n, n, n}, per[3] = {1, 1, 1};
-    ScalStepBC(nn, per, ncomp, ppm, dt, dx, rel_eps, s, u, snew, ds, du, dsnew);
+    ScalStepBC(nn, per, ncomp, ppm, dt, dx, rel_eps, s, u, snew, ds, du, dsnew, false);
 }
 
 }  // namespace mxo

@@ -80,10 +80,12 @@ def step_op_count(s, u, dt, dx, rel_eps):
     return dict(zip(keys, out.tolist()))
 
 
-def scal_step_bc(s, u, dt, dx, rel_eps, periodic, ds=None, du=None, ppm_type=1, fast=False):
+def scal_step_bc(s, u, dt, dx, rel_eps, periodic, ds=None, du=None, ppm_type=1, fast=False,
+                 conservative=False):
     """The differentiable scalar step (oracle_step.cpp) on the cells of a
     domain whose direction d is periodic (periodic[d]) or outflow (foextrap),
-    ppm_type 1 or 2.
+    ppm_type 1 or 2, MakeEdgeScal in the advective or (conservative) the
+    conservative form.
     s: (ncomp, nz, ny, nx) indexed [c, k, j, i]; u: three face arrays, u[d]
     one point longer along d when d is outflow (faces 0..n[d]).  Returns
     snew, or (snew, dsnew) when the direction ds (and optionally du) is
@@ -121,7 +123,8 @@ def scal_step_bc(s, u, dt, dx, rel_eps, periodic, ds=None, du=None, ppm_type=1, 
     f.restype = C.c_int
     rc = f((C.c_int * 3)(*n), (C.c_int * 3)(*per), ncomp, ppm_type, C.c_double(dt),
            (C.c_double * 3)(*dx), C.c_double(rel_eps), _dptr(s), *[_dptr(x) for x in u],
-           _dptr(snew), dsp, *dup, _dptr(dsnew) if dsnew is not None else null)
+           _dptr(snew), dsp, *dup, _dptr(dsnew) if dsnew is not None else null,
+           1 if conservative else 0)
     if rc:
         raise _abi.MxError(rc, "oracle.scal_step_bc")
     return snew if ds is None else (snew, dsnew)

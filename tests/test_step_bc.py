@@ -7,6 +7,8 @@
   with the foextrap rules at the domain faces
   (MaestroMakeEdgeScalars.cpp:709-738, :931-961, :1446-1480), MakeRhoXFlux
   with species_pred_type 2 and UpdateScal of the species;
+* the same in the conservative form of MakeEdgeScal (is_conservative, the
+  form the reference uses when the flux is umac * sedge);
 * a constant field stays constant under a divergence-free velocity;
 * reflecting the domain in x maps the step onto itself (the high-side rule
   mirrors the low-side one);
@@ -41,9 +43,10 @@ def test_velocity_is_divergence_free():
         assert np.max(np.abs(divergence(u, per))) <= 1e-12 * 14
 
 
+@pytest.mark.parametrize("cons", [0, 1])
 @pytest.mark.parametrize("ppm", [1, 2])
-@pytest.mark.parametrize("per", [(0, 0, 0), (1, 0, 1)])
-def test_outflow_step_equals_the_pass_stages(ppm, per):
+@pytest.mark.parametrize("per", [(0, 0, 0), (1, 0, 1), (1, 1, 1)])
+def test_outflow_step_equals_the_pass_stages(ppm, per, cons):
     n, nspec, ng = 16, 2, 4
     nn = (n, n, n)
     boxes = uneven_boxes(n, 2, 6, 10, seed=4)
@@ -64,24 +67,51 @@ def test_outflow_step_equals_the_pass_stages(ppm, per):
     snew = MultiFab(boxes, nscal, 0, 3, NP)
     m = pyoracle.oracle_maestro(params, [boxes])
     bs, _ = m.bc_setup()
-    m.MakeEdgeScal(0, state, sedge, umac, force, 0, bs, first, first, nspec, 0)
+    m.MakeEdgeScal(0, state, sedge, umac, force, 0, bs, first, first, nspec, cons)
     zero = BaseState(np.zeros((1, n + 1)), NP)
     m.MakeRhoXFlux(0, sflux, eta, sedge, umac, zero, zero, None, zero, zero, None, zero, zero,
                    first, nspec)
     m.UpdateScal(0, state, snew, sflux, force, first, nspec)
     got = gather_cells(snew, nn)[first:first + nspec]
-    want = pyoracle.scal_step_bc(sp, u, dt, [1.0 / n] * 3, eps, per, ppm_type=ppm)
+    want = pyoracle.scal_step_bc(sp, u, dt, [1.0 / n] * 3, eps, per, ppm_type=ppm,
+                                 conservative=bool(cons))
     assert np.array_equal(got, want), float(np.max(np.abs(got - want)))
 
 
+@pytest.mark.parametrize("cons", [False, True])
 @pytest.mark.parametrize("ppm", [1, 2])
-def test_outflow_step_keeps_constants(ppm):
-    nn = (12, 10, 14)
+def test_outflow_step_keeps_constants(ppm, cons):
+    # the conservative form keeps constants only with equal spacings: its
+    # MakeDivU divides every face difference by dx[0]
+    # (MaestroMakeEdgeScalars.cpp:656-658)
+    nn = (12, 12, 12) if cons else (12, 10, 14)
     per = [0, 0, 0]
     u = mac_velocity(nn, per, seed=8)
     c = np.full((1, nn[2], nn[1], nn[0]), 0.7)
-    out = pyoracle.scal_step_bc(c, u, 0.4 / 14, [1.0 / q for q in nn], 1e-8, per, ppm_type=ppm)
+    out = pyoracle.scal_step_bc(c, u, 0.4 / nn[2], [1.0 / q for q in nn], 1e-8, per,
+                                ppm_type=ppm, conservative=cons)
     assert np.max(np.abs(out - 0.7)) <= 1e-13
+
+
+def test_conservative_form_differs_by_the_divergence_terms():
+    """Under a velocity with divergence the two forms differ; a constant
+    field c in the conservative form changes by the update of c div u (to
+    first order dt c div u), in the advective form it stays c."""
+    nn = (12, 10, 14)
+    per = [1, 1, 1]
+    rng = np.random.default_rng(14)
+    u = [0.3 * rng.standard_normal((nn[2], nn[1], nn[0])) for _ in range(3)]
+    c = np.full((1, nn[2], nn[1], nn[0]), 0.7)
+    dx = [1.0 / q for q in nn]
+    dt = 0.2 / 14
+    adv = pyoracle.scal_step_bc(c, u, dt, dx, 1e-8, per, ppm_type=1)
+    con = pyoracle.scal_step_bc(c, u, dt, dx, 1e-8, per, ppm_type=1, conservative=True)
+    from step_bc_cases import divergence
+    div = divergence(u, per)
+    # the update of a constant: s - dt div(u s_edge), and s_edge = c exactly
+    # in the advective form (edge states of a constant), so adv = c - dt c div
+    assert np.max(np.abs(adv - (0.7 - dt * 0.7 * div))) <= 1e-12
+    assert np.max(np.abs(con - adv)) > 1e-6
 
 
 @pytest.mark.parametrize("ppm", [1, 2])
@@ -111,17 +141,18 @@ def test_outflow_jvp_matches_finite_differences():
     du = [rng.standard_normal(x.shape) for x in u]
     dx = [1.0 / q for q in nn]
     dt, eps = 0.4 / 12, 1e-8
-    for ppm in (1, 2):
-        s1 = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, ppm_type=ppm)
-        s2, js = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, ds, du, ppm_type=ppm)
+    for ppm, cons in ((1, False), (2, False), (1, True), (2, True)):
+        kw = dict(ppm_type=ppm, conservative=cons)
+        s1 = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, **kw)
+        s2, js = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, ds, du, **kw)
         assert np.array_equal(s1, s2)
         # h = 1e-7: the Colella-Sekora limiter's square-root branch has a
         # large second derivative near its switch, so the O(h^2) error of
         # the central difference at 1e-6 reaches 1e-5 at a few cells
         h = 1e-7
         fp = pyoracle.scal_step_bc(s + h * ds, [u[d] + h * du[d] for d in range(3)], dt, dx, eps,
-                                   per, ppm_type=ppm)
+                                   per, **kw)
         fm = pyoracle.scal_step_bc(s - h * ds, [u[d] - h * du[d] for d in range(3)], dt, dx, eps,
-                                   per, ppm_type=ppm)
+                                   per, **kw)
         fd = (fp - fm) / (2 * h)
         assert np.max(np.abs(fd - js)) <= 1e-6 * np.max(np.abs(js))
