@@ -751,13 +751,19 @@ int mx_adv_update_vel(mx_adv_ctx* ctx, int lev, const mx_multifab* uold, mx_mult
      snew  = s + dt (-div(umac sedge) + 0)
    A non-zero base state adds a flux that does not depend on s, so the
    state adjoint is the same; the velocity adjoint leaves that flux out.
+   is_conservative != 0 predicts in the conservative form instead
+   (MaestroMakeEdgeScalars.cpp:906-929, 1411-1438, MakeDivU 648-660): the
+   species step with species_pred_type predict_rhoX
+   (MaestroDensityAdvance.cpp:118-122, MaestroMakeFlux.cpp:134-137) and the
+   enthalpy step with predict_rhoh (MaestroEnthalpyAdvance.cpp:139-142).
    on a context whose level 0 is one fully periodic box [0, n)^3.  s holds
    ncomp components with >= 3 ghost cells (>= 4 for ppm_type 2; fewer is
    MX_ERR_BOX), umac[0..2] one face FAB per
    direction with >= 1 ghost face.  The call fills the ghost cells of s and
    umac by periodic copy and sets face n of umac to face 0 (one periodic face
    carries one value). */
-int mx_adv_scal_step(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, mx_fab* snew, int ncomp);
+int mx_adv_scal_step(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, mx_fab* snew, int ncomp,
+                     int is_conservative);
 
 /* Hand-written adjoint of mx_adv_scal_step (no counterpart in the
    reference): given gbar = dL/dsnew on the valid cells, writes sbar = dL/ds
@@ -769,7 +775,7 @@ int mx_adv_scal_step(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, mx_fab* snew, int
    the forward step.  Deterministic (no atomics); the workspace holds about
    65 ncomp + 3 arrays of n^3 doubles. */
 int mx_adv_scal_step_vjp(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, const mx_fab* gbar,
-                         mx_fab* sbar, mx_fab* ubar, int ncomp);
+                         mx_fab* sbar, mx_fab* ubar, int ncomp, int is_conservative);
 
 /* The same step over the level-0 box list of a context (no single
    counterpart: the composition above on MultiFabs, as DensityAdvance runs
@@ -782,7 +788,7 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* ctx, mx_fab* s, mx_fab* umac, const mx_fab*
    FillBoundary + FillUmacGhost).  A face two boxes share must hold one
    value in both. */
 int mx_adv_scal_step_boxes(mx_adv_ctx* ctx, mx_multifab* s, mx_face_multifab* umac,
-                           mx_multifab* snew, int ncomp);
+                           mx_multifab* snew, int ncomp, int is_conservative);
 
 /* Hand-written adjoint of mx_adv_scal_step_boxes (no counterpart in the
    reference): gbar = dL/dsnew on the valid cells gives sbar = dL/ds on the
@@ -796,7 +802,8 @@ int mx_adv_scal_step_boxes(mx_adv_ctx* ctx, mx_multifab* s, mx_face_multifab* um
    arrays of the domain grown by 7 points per outflow direction. */
 int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* ctx, const mx_multifab* s,
                                const mx_face_multifab* umac, const mx_multifab* gbar,
-                               mx_multifab* sbar, mx_face_multifab* ubar, int ncomp);
+                               mx_multifab* sbar, mx_face_multifab* ubar, int ncomp,
+                               int is_conservative);
 
 #ifdef __cplusplus
 }

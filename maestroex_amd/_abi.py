@@ -265,11 +265,11 @@ LIB_ONLY = {
     "pass_graph_launch": [_vp],
     "pass_graph_destroy": [_vp],
     "autotune": [_vp, _i, P(PassArgs), P(TuneReport)],
-    "scal_step": [_vp, P(Fab), P(Fab), P(Fab), _i],
-    "scal_step_vjp": [_vp, P(Fab), P(Fab), P(Fab), P(Fab), P(Fab), _i],
-    "scal_step_boxes": [_vp, P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc), _i],
+    "scal_step": [_vp, P(Fab), P(Fab), P(Fab), _i, _i],
+    "scal_step_vjp": [_vp, P(Fab), P(Fab), P(Fab), P(Fab), P(Fab), _i, _i],
+    "scal_step_boxes": [_vp, P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc), _i, _i],
     "scal_step_vjp_boxes": [_vp, P(MultiFabDesc), P(FaceMultiFabDesc), P(MultiFabDesc),
-                            P(MultiFabDesc), P(FaceMultiFabDesc), _i],
+                            P(MultiFabDesc), P(FaceMultiFabDesc), _i, _i],
     "set_stream": [_vp, _vp],
     "sync": [_vp],
     "error_string": [_i],

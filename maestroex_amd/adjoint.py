@@ -1,12 +1,15 @@
 """Differentiable scalar advection (mx_adv_scal_step / mx_adv_scal_step_boxes
 and their hand-written adjoints), as torch.autograd.Functions.
 
-* `ScalStepper(n, ncomp, dt, rel_eps, ppm_type=1)`: a C-ABI context for one
-  fully periodic box [0, n)^3 (dx = 1/n, ppm_type 1 or 2) with device buffers of the
-  step; `step(s, u)` and `vjp(s, u, gbar)` take dense torch tensors:
+* `ScalStepper(n, ncomp, dt, rel_eps, ppm_type=1, conservative=False)`: a
+  C-ABI context for one fully periodic box [0, n)^3 (dx = 1/n, ppm_type 1 or
+  2, MakeEdgeScal in the advective or the conservative form) with device
+  buffers of the step; `step(s, u)` and `vjp(s, u, gbar)` take dense torch
+  tensors:
   s (ncomp, n, n, n) indexed [c, k, j, i] and u = (ux, uy, uz), each
   (n, n, n) holding the faces 0..n-1 of its direction (face n is face 0).
-* `BoxStepper(boxes, n, periodic, ncomp, dt, rel_eps, ppm_type=1)`: the same
+* `BoxStepper(boxes, n, periodic, ncomp, dt, rel_eps, ppm_type=1,
+  conservative=False)`: the same
   step over a box list that tiles the domain n = (nx, ny, nz), each
   direction periodic or outflow (foextrap); `step(s, u)` and
   `vjp(s, u, gbar)` take the same dense tensors as ScalStepper's, u[d]
@@ -70,12 +73,13 @@ class ScalStepper:
 
     NG = {1: 3, 2: 4}  # ghost cells of the state the edge stencils read, per ppm_type
 
-    def __init__(self, n, ncomp, dt, rel_eps, device="cuda:0", ppm_type=1):
+    def __init__(self, n, ncomp, dt, rel_eps, device="cuda:0", ppm_type=1, conservative=False):
         import torch
         import maestroex_amd as mx
         self.torch = torch
         self.n, self.ncomp, self.dt, self.rel_eps = n, ncomp, dt, rel_eps
         self.ppm_type = ppm_type
+        self.cons = 1 if conservative else 0
         g = Geometry(3, [n] * 3, [1.0 / n] * 3, [1, 1, 1], [0, 0, 0], [0, 0, 0])
         p = make_params(g, ppm_type=ppm_type, nspec=ncomp, dt=dt, rel_eps=rel_eps)
         self.m = mx.gpu_maestro(p, [[((0, 0, 0), (n - 1, n - 1, n - 1))]])
@@ -128,7 +132,7 @@ class ScalStepper:
         snew = self.torch.empty_like(self.snew)
         self._check(self.fns["scal_step"](self.m.ctx, C.byref(self._fab_s), self._fab_u,
                                           C.byref(self._fab(snew, [0] * 3, self.ncomp)),
-                                          self.ncomp), "scal_step")
+                                          self.ncomp, self.cons), "scal_step")
         return snew
 
     def vjp(self, s, u, gbar):
@@ -145,8 +149,8 @@ class ScalStepper:
         fub = (_abi.Fab * 3)(*[self._fab(ub[d], [0] * 3, 1) for d in range(3)])
         self._check(self.fns["scal_step_vjp"](self.m.ctx, C.byref(self._fab(s, [0] * 3, nc)), fu,
                                               C.byref(self._fab(gbar, [0] * 3, nc)),
-                                              C.byref(self._fab(sb, [0] * 3, nc)), fub, nc),
-                    "scal_step_vjp")
+                                              C.byref(self._fab(sb, [0] * 3, nc)), fub, nc,
+                                              self.cons), "scal_step_vjp")
         return sb, ub
 
 
@@ -156,7 +160,7 @@ class BoxStepper:
     NG = {1: 3, 2: 4}
 
     def __init__(self, boxes, n, periodic, ncomp, dt, rel_eps, ppm_type=1, device="cuda:0",
-                 spherical=0):
+                 spherical=0, conservative=False):
         import torch
         import maestroex_amd as mx
         from .fabs import Backend, FaceMultiFab, MultiFab
@@ -167,6 +171,7 @@ class BoxStepper:
         self.n = list(n)
         self.periodic = [1 if p else 0 for p in periodic]
         self.ncomp, self.dt, self.rel_eps, self.ppm_type = ncomp, dt, rel_eps, ppm_type
+        self.cons = 1 if conservative else 0
         lo = [_abi.PHYS_INTERIOR if p else _abi.PHYS_OUTFLOW for p in self.periodic]
         g = Geometry(3, self.n, [1.0 / q for q in self.n], self.periodic, lo, lo)
         p = make_params(g, ppm_type=ppm_type, nspec=ncomp, dt=dt, rel_eps=rel_eps,
@@ -190,14 +195,15 @@ class BoxStepper:
         """snew = step(s) on MultiFabs (s: >= 3 ghosts, 4 for ppm_type 2;
         u: >= 1 ghost face; both ghost layers are filled by the call)."""
         self._check(self.fns["scal_step_boxes"](self.m.ctx, C.byref(s.desc()), C.byref(u.desc()),
-                                                C.byref(snew.desc()), self.ncomp),
+                                                C.byref(snew.desc()), self.ncomp, self.cons),
                     "scal_step_boxes")
 
     def vjp_mf(self, s, u, gbar, sbar, ubar):
         self._check(self.fns["scal_step_vjp_boxes"](self.m.ctx, C.byref(s.desc()),
                                                     C.byref(u.desc()), C.byref(gbar.desc()),
                                                     C.byref(sbar.desc()), C.byref(ubar.desc()),
-                                                    self.ncomp), "scal_step_vjp_boxes")
+                                                    self.ncomp, self.cons),
+                    "scal_step_vjp_boxes")
 
     # dense global tensors <-> the box MultiFabs
     def _put_cells(self, mf, a):

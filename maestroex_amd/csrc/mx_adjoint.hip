@@ -4,7 +4,9 @@
 // The step, on one fully periodic box [0, N)^3 or over a box list with
 // periodic or outflow sides, with a fixed MAC velocity, for every scalar
 // component (the enthalpy step of EnthalpyAdvance, predict_rhohprime, for
-// (rho h)' with a zero base state):
+// (rho h)' with a zero base state; with is_conservative the conservative
+// prediction, as the reference's predict_rhoX species and predict_rhoh
+// enthalpy steps use it):
 //   sedge = MakeEdgeScal(s, umac, force = 0, advective form)  (ppm_type 1, 2;
 //                                      MaestroEnthalpyAdvance.cpp:143-147)
 //   F     = umac * (0 + sedge)        (MakeRhoHFlux, MaestroMakeFlux.cpp:699-708)
@@ -137,8 +139,11 @@ struct AB {
     int fold;          // ghost reach of the state adjoint (general layout)
     double dt, rel_eps, dx[3];
     // per-direction factors, evaluated once on the host by the same
-    // expressions: dt/h, (dt/4)/h (edge terms), (dt/6)/h (transverse terms)
+    // expressions: dt/h, (dt/4)/h (edge terms), (dt/6)/h (transverse terms);
+    // the conservative form's (dt/2)/h (edge terms) and (dt/3)/h (transverse
+    // terms), dt/3 and dx[0] (MakeDivU's divisor)
     double fdt[3], f4[3], f6[3];
+    double f2[3], f3[3], dt3, dx0;
 };
 
 template <bool P2>
@@ -252,17 +257,34 @@ __global__ void __launch_bounds__(NTB) k_tape_ppm(AB A) {
     }
 }
 
-// transverse term of cell c in direction t (dt/6h_t (u+ + u-) (simh+ - simh-))
+// MakeDivU of cell (i,j,k): the six face differences over dx[0]
+// (MaestroMakeEdgeScalars.cpp:655-658)
 template <class PT>
-__device__ __forceinline__ double tt(const AB& A, const PT& P, long long co, int i, int j, int k,
-                                     int t) {
+__device__ __forceinline__ double divu(const AB& A, const PT& P, int i, int j, int k) {
+    const long long c = P.at(i, j, k);
+    double dv = A.U[0][P.nb(i, j, k, 0, 1)] - A.U[0][c] + A.U[1][P.nb(i, j, k, 1, 1)] - A.U[1][c] +
+                A.U[2][P.nb(i, j, k, 2, 1)] - A.U[2][c];
+    return dv / A.dx0;
+}
+
+// the state `base` of cell c corrected by its transverse term in direction
+// t: advective base - dt/6h_t (u+ + u-) (simh+ - simh-); conservative
+// base - dt/3h_t (simh+ u+ - simh- u-) - dt/3 s divu + dt/3h_t s (u+ - u-)
+// (MaestroMakeEdgeScalars.cpp:906-929)
+template <bool CONS, class PT>
+__device__ __forceinline__ double tcorr(const AB& A, const PT& P, long long co, double base,
+                                        int i, int j, int k, int t) {
     const long long c = P.at(i, j, k);
     const long long c1 = P.nb(i, j, k, t, 1);
-    return A.f6[t] * (A.U[t][c1] + A.U[t][c]) * (A.SIMH[t][co + c1] - A.SIMH[t][co + c]);
+    const double* H = A.SIMH[t] + co;
+    if (!CONS) return base - A.f6[t] * (A.U[t][c1] + A.U[t][c]) * (H[c1] - H[c]);
+    const double sc = A.S[co + c];
+    return base - A.f3[t] * (H[c1] * A.U[t][c1] - H[c] * A.U[t][c]) -
+           A.dt3 * sc * divu(A, P, i, j, k) + A.f3[t] * sc * (A.U[t][c1] - A.U[t][c]);
 }
 
 // A3: corner-coupled states on d-faces, corrected by t
-template <class PT>
+template <class PT, bool CONS>
 __global__ void __launch_bounds__(NTB) k_tape_corner(AB A) {
     const PT P = PT::make(A);
     int i, j, k;
@@ -276,22 +298,34 @@ __global__ void __launch_bounds__(NTB) k_tape_corner(AB A) {
             if (t == d) continue;
             double l = A.IP[d][co + P.at(im, jm, km)];
             double r = A.IM[d][co + p];
-            l = l - tt(A, P, co, im, jm, km, t);
-            r = r - tt(A, P, co, i, j, k, t);
+            l = tcorr<CONS>(A, P, co, l, im, jm, km, t);
+            r = tcorr<CONS>(A, P, co, r, i, j, k, t);
             cor[pidx(d, t)] = upwind_b(l, r, A.U[d][p], A.rel_eps, P.bnd(d, posd(d, i, j, k)));
         }
     }
     for (int q = 0; q < 6; ++q) A.COR[q][co + p] = cor[q];
 }
 
-// edge term of cell c for the corner state on t-faces corrected by o
-template <class PT>
+// edge term of cell c for the corner state on t-faces corrected by o:
+// advective dt/4h_t (u+ + u-) (K+ - K-), conservative dt/2h_t (K+ u+ - K- u-)
+// (MaestroMakeEdgeScalars.cpp:1411-1438)
+template <bool CONS, class PT>
 __device__ __forceinline__ double et(const AB& A, const PT& P, long long co, int i, int j, int k,
                                      int t, int o) {
     const long long c = P.at(i, j, k);
     const long long c1 = P.nb(i, j, k, t, 1);
     const double* K = A.COR[pidx(t, o)] + co;
+    if (CONS) return A.f2[t] * (K[c1] * A.U[t][c1] - K[c] * A.U[t][c]);
     return A.f4[t] * (A.U[t][c1] + A.U[t][c]) * (K[c1] - K[c]);
+}
+
+// the conservative form's normal term of cell c on its d-faces,
+// dt/2h_d s (u_d+ - u_d-)
+template <class PT>
+__device__ __forceinline__ double nterm(const AB& A, const PT& P, long long co, int i, int j,
+                                        int k, int d) {
+    const long long c = P.at(i, j, k);
+    return A.f2[d] * A.S[co + c] * (A.U[d][P.nb(i, j, k, d, 1)] - A.U[d][c]);
 }
 
 // adjoint of the edge state on d-face c through the flux (Fb u) and its
@@ -308,7 +342,7 @@ __device__ __forceinline__ double edge_bar(const AB& A, const PT& P, long long c
 }
 
 // A4: edge states
-template <class PT>
+template <class PT, bool CONS>
 __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
     const PT P = PT::make(A);
     int i, j, k;
@@ -324,8 +358,12 @@ __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
         for (int t = 0; t < 3; ++t) {
             if (t == d) continue;
             const int o = 3 - d - t;
-            l = l - et(A, P, co, im, jm, km, t, o);
-            r = r - et(A, P, co, i, j, k, t, o);
+            l = l - et<CONS>(A, P, co, im, jm, km, t, o);
+            r = r - et<CONS>(A, P, co, i, j, k, t, o);
+        }
+        if (CONS) {
+            l = l - nterm(A, P, co, im, jm, km, d);
+            r = r - nterm(A, P, co, i, j, k, d);
         }
         const double sedgel = l + dt2 * 0.0;
         const double sedger = r + dt2 * 0.0;
@@ -361,8 +399,12 @@ __global__ void __launch_bounds__(NTB) k_tape_edge(AB A) {
 // states (slb, srb) and, with want_u, of its velocity for this component
 // (ub): each the sum, in this order, of the flux term (edge-state
 // upwinding), the corner-state terms (edge terms, corner upwinding) and the
-// transverse term (predictor upwinding)
-template <class PT>
+// transverse term (predictor upwinding).  The conservative form's flux
+// differences give the corner and upwind states the weights dt/2h u0 and
+// dt/3h u0, and its velocity adjoint also collects the state terms of the
+// transverse terms (s (u+ - u-) and s divu of both cells) and of the normal
+// edge terms (s (u+ - u-))
+template <bool CONS, class PT>
 __device__ __forceinline__ void face_totals(const AB& A, const PT& P, long long co, int i, int j,
                                             int k, int t, bool want_u, double& slb, double& srb,
                                             double& ub) {
@@ -381,7 +423,7 @@ __device__ __forceinline__ void face_totals(const AB& A, const PT& P, long long 
     srb = wr * seb;
     // corner states on t-faces, through the edge terms of the cells below
     // and above the face
-    const double b = A.f4[t];
+    const double b = CONS ? A.f2[t] : A.f4[t];
     double ubc = 0.0, slbc = 0.0, srbc = 0.0;
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
@@ -389,10 +431,14 @@ __device__ __forceinline__ void face_totals(const AB& A, const PT& P, long long 
         const int q = pidx(t, o);
         const double ebm = A.EB[q][co + fm];
         const double eb0 = A.EB[q][co + f];
-        const double kb = ebm * b * (u0 + um) - eb0 * b * (up + u0);
+        const double kb = CONS ? ebm * b * u0 - eb0 * b * u0
+                               : ebm * b * (u0 + um) - eb0 * b * (up + u0);
         if (want_u) {
             const double* K = A.COR[q] + co;
-            ubc += ebm * b * (K[f] - K[fm]) + eb0 * b * (K[fp] - K[f]);
+            if (CONS)
+                ubc += ebm * b * K[f] - eb0 * b * K[f];
+            else
+                ubc += ebm * b * (K[f] - K[fm]) + eb0 * b * (K[fp] - K[f]);
         }
         slbc += wl * kb;
         srbc += wr * kb;
@@ -402,12 +448,29 @@ __device__ __forceinline__ void face_totals(const AB& A, const PT& P, long long 
     srb += srbc;
     // upwind state simh, through the transverse terms of the cells below and
     // above the face
-    const double a = A.f6[t];
+    const double a = CONS ? A.f3[t] : A.f6[t];
     const double tbm = A.TB[t][co + fm], tb0 = A.TB[t][co + f];
-    const double hb = tbm * a * (u0 + um) - tb0 * a * (up + u0);
+    const double hb = CONS ? tbm * a * u0 - tb0 * a * u0
+                           : tbm * a * (u0 + um) - tb0 * a * (up + u0);
     if (want_u) {
         const double* H = A.SIMH[t] + co;
-        ub = ub + (tbm * a * (H[f] - H[fm]) + tb0 * a * (H[fp] - H[f]));
+        if (!CONS) {
+            ub = ub + (tbm * a * (H[f] - H[fm]) + tb0 * a * (H[fp] - H[f]));
+        } else {
+            // the cells below (fm) and above (f) the face: their transverse
+            // terms in t (simh u and s (u+ - u-)), the divergence in all
+            // three of their transverse terms, and their normal edge terms
+            // in t
+            const double sm = A.S[co + fm], s0 = A.S[co + f];
+            const double tsm = A.TB[0][co + fm] + A.TB[1][co + fm] + A.TB[2][co + fm];
+            const double ts0 = A.TB[0][co + f] + A.TB[1][co + f] + A.TB[2][co + f];
+            const int qn = t == 0 ? pidx(1, 2) : (t == 1 ? pidx(0, 2) : pidx(0, 1));
+            const double enm = A.EB[qn][co + fm], en0 = A.EB[qn][co + f];
+            ub = ub + (tbm * a * H[f] - tb0 * a * H[f]);
+            ub = ub + (tb0 * a * s0 - tbm * a * sm);
+            ub = ub + (A.dt3 / A.dx0) * (sm * tsm - s0 * ts0);
+            ub = ub + A.f2[t] * (enm * sm - en0 * s0);
+        }
     }
     double wl0, wr0;
     weights(u0, 0.0, wl0, wr0);
@@ -419,16 +482,17 @@ __device__ __forceinline__ void face_totals(const AB& A, const PT& P, long long 
 // adjoint of the corner state q on d-face (i,j,k) (through the edge terms of
 // the cells on both sides of it), with its upwinding weights: the left and
 // right predictor-state adjoints it passes on
-template <class PT>
+template <bool CONS, class PT>
 __device__ __forceinline__ void corner_bar(const AB& A, const PT& P, long long co, int i, int j,
                                            int k, int d, int q, double& lb, double& rb) {
-    const double b = A.f4[d];
+    const double b = CONS ? A.f2[d] : A.f4[d];
     const long long c = P.at(i, j, k);
     const long long fm = P.nb(i, j, k, d, -1), fp = P.nb(i, j, k, d, 1);
     const double u0 = A.U[d][c], um = A.U[d][fm], up = A.U[d][fp];
     const double ebm = A.EB[q][co + fm];
     const double eb0 = A.EB[q][co + c];
-    const double kb = ebm * b * (u0 + um) - eb0 * b * (up + u0);
+    const double kb = CONS ? ebm * b * u0 - eb0 * b * u0
+                           : ebm * b * (u0 + um) - eb0 * b * (up + u0);
     double wl, wr;
     weights(u0, A.rel_eps, wl, wr);
     bfix(P.bnd(d, posd(d, i, j, k)), wl, wr);
@@ -436,10 +500,28 @@ __device__ __forceinline__ void corner_bar(const AB& A, const PT& P, long long c
     rb = wr * kb;
 }
 
+// the conservative form's direct terms of the state of cell (i,j,k): its
+// transverse terms' dt/3 s divu - dt/3h_t s (u+ - u-) and its normal edge
+// terms' dt/2h_d s (u+ - u-), with their adjoints TB and EB
+template <class PT>
+__device__ __forceinline__ double cons_direct(const AB& A, const PT& P, long long co, int i,
+                                              int j, int k) {
+    const long long c = P.at(i, j, k);
+    const double dv = divu(A, P, i, j, k);
+    double r = 0.0;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const double du = A.U[t][P.nb(i, j, k, t, 1)] - A.U[t][c];
+        const int qn = t == 0 ? pidx(1, 2) : (t == 1 ? pidx(0, 2) : pidx(0, 1));
+        r += A.TB[t][co + c] * (A.dt3 * dv - A.f3[t] * du) + A.EB[qn][co + c] * (A.f2[t] * du);
+    }
+    return r;
+}
+
 // B4: per-cell transverse-term adjoints: the transverse term of cell p in
 // direction o enters the corner states on the d-faces p (right state) and
 // p + d (left state), d != o
-template <class PT>
+template <class PT, bool CONS>
 __global__ void __launch_bounds__(NTB) k_adj_tterm(AB A) {
     const PT P = PT::make(A);
     int i, j, k;
@@ -453,8 +535,8 @@ __global__ void __launch_bounds__(NTB) k_adj_tterm(AB A) {
             if (d == o) continue;
             const int q = pidx(d, o);
             double lb1, rb1, lb0, rb0;
-            corner_bar(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, q, lb1, rb1);
-            corner_bar(A, P, co, i, j, k, d, q, lb0, rb0);
+            corner_bar<CONS>(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, q, lb1, rb1);
+            corner_bar<CONS>(A, P, co, i, j, k, d, q, lb0, rb0);
             s += lb1 + rb0;
         }
         tb[o] = -s;
@@ -464,7 +546,7 @@ __global__ void __launch_bounds__(NTB) k_adj_tterm(AB A) {
 
 // B3 + B5 stored once per face: the left and right predictor-state adjoints
 // of every d-face (for the trace variant that reads them)
-template <class PT>
+template <class PT, bool CONS>
 __global__ void __launch_bounds__(NTB) k_adj_fsum(AB A) {
     const PT P = PT::make(A);
     int i, j, k;
@@ -473,7 +555,7 @@ __global__ void __launch_bounds__(NTB) k_adj_fsum(AB A) {
     const long long co = (long long)blockIdx.y * A.cs;
     double l[3], r[3], unused;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) face_totals(A, P, co, i, j, k, d, false, l[d], r[d], unused);
+    for (int d = 0; d < 3; ++d) face_totals<CONS>(A, P, co, i, j, k, d, false, l[d], r[d], unused);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         A.SLB[d][co + p] = l[d];
@@ -520,7 +602,7 @@ __device__ __forceinline__ TraceBar trace_bar(const AB& A, double s0, double sm,
 
 // B6: characteristic trace and CW limiter of cell p in direction d (FS: the
 // face adjoints are read from k_adj_fsum's arrays instead of recomputed)
-template <class PT, bool FS, bool ROLL>
+template <class PT, bool FS, bool ROLL, bool CONS>
 __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
     const PT P = PT::make(A);
     int i, j, k;
@@ -572,10 +654,10 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
             Imb = A.SRB[d][co + p];
         } else {
             double unused0, unused1, unused2;
-            face_totals(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb,
+            face_totals<CONS>(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb,
                         unused0, unused1);
             // face p's totals also give its velocity adjoint for this component
-            face_totals(A, P, co, i, j, k, d, true, unused2, Imb, ubd[ROLL ? 0 : d]);
+            face_totals<CONS>(A, P, co, i, j, k, d, true, unused2, Imb, ubd[ROLL ? 0 : d]);
         }
         const TraceBar tr = trace_bar(A, s0, sm, sp, s6, uhi, ulo, h, Ipb, Imb);
         const double smb = tr.smb, spb = tr.spb, uhib = tr.uhib, ulob = tr.ulob;
@@ -612,10 +694,13 @@ __global__ void __launch_bounds__(NTB) k_adj_trace(AB A) {
         r4[d] = ulob;
     }
     if (ROLL) {
+        if (CONS) s0s += cons_direct(A, P, co, i, j, k);
         A.S0S[co + p] = s0s;
         return;
     }
-    A.S0S[co + p] = (r0[0] + r0[1]) + r0[2];
+    double s0t = (r0[0] + r0[1]) + r0[2];
+    if (CONS) s0t += cons_direct(A, P, co, i, j, k);
+    A.S0S[co + p] = s0t;
     for (int d = 0; d < 3; ++d) {
         A.FLB[d][co + p] = r1[d];  // face p (below the cell)
         A.FRB[d][co + p] = r2[d];  // face above the cell
@@ -806,7 +891,7 @@ __device__ __forceinline__ void cs_cell_bar(const double* v, double fl, double f
 // p + d (FRB), the direct adjoints of the cells p - 2d .. p + 2d other than p
 // (DV[d][0..3] for offsets -2, -1, 1, 2), the velocity terms, and adds its
 // direct term of s(p) to S0S.
-template <class PT>
+template <class PT, bool CONS>
 __global__ void __launch_bounds__(NTB) k_adj_trace2(AB A) {
     const PT P = PT::make(A);
     int i, j, k;
@@ -832,9 +917,9 @@ __global__ void __launch_bounds__(NTB) k_adj_trace2(AB A) {
         const double uhi = A.U[d][fhi], ulo = A.U[d][p];
         const double s6 = 6.0 * s0 - 3.0 * (sm + sp);
         double Ipb, Imb, ubd, unused0, unused1, unused2;
-        face_totals(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb, unused0,
+        face_totals<CONS>(A, P, co, i + (d == 0), j + (d == 1), k + (d == 2), d, false, Ipb, unused0,
                     unused1);
-        face_totals(A, P, co, i, j, k, d, true, unused2, Imb, ubd);
+        face_totals<CONS>(A, P, co, i, j, k, d, true, unused2, Imb, ubd);
         const TraceBar tr = trace_bar(A, s0, sm, sp, s6, uhi, ulo, A.dx[d], Ipb, Imb);
         const double smb = tr.smb, spb = tr.spb, s0b = tr.s0b, uhib = tr.uhib, ulob = tr.ulob;
         double vb5[5], f0b, frb;
@@ -849,6 +934,7 @@ __global__ void __launch_bounds__(NTB) k_adj_trace2(AB A) {
         A.UHI[d][co + p] = uhib;
         A.UBC[d][co + p] = ubd + ulob;
     }
+    if (CONS) s0s += cons_direct(A, P, co, i, j, k);
     A.S0S[co + p] = s0s;
 }
 
@@ -1095,7 +1181,7 @@ __global__ void __launch_bounds__(NTB) k_adj_state(AB A) {
 // B10: velocity adjoint per face, summed over components in order
 // into the caller's face FABs; the duplicate face n of a periodic face FAB
 // carries no separate value and is set to zero
-template <class PT, bool STORED>
+template <class PT, bool STORED, bool CONS>
 __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, DA ub0, DA ub1, DA ub2, int has_n) {
     const PT P = PT::make(A);
     int i, j, k;
@@ -1115,7 +1201,7 @@ __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, DA ub0, DA ub1, DA ub2, i
                 ub = A.UBC[d][co + p];
             } else {
                 double slb, srb, ubt;
-                face_totals(A, P, co, i, j, k, d, true, slb, srb, ubt);
+                face_totals<CONS>(A, P, co, i, j, k, d, true, slb, srb, ubt);
                 ub = ubt + A.ULO[d][co + p];
             }
             s += ub + A.UHI[d][co + cm];
@@ -1214,13 +1300,9 @@ __global__ void __launch_bounds__(NTB) k_fill_zero(double* a, long long n) {
     if (p < n) a[p] = 0.0;
 }
 
-// the tape and reverse kernels of the step adjoint.  PT indexes the tape,
-// state and velocity sweeps, PC the transverse-term and trace gathers
-// (which run faster with compare indexing on MI355X, measured, whatever n
-// is); the velocity adjoint goes to ub0..ub2 (has_n: the periodic FABs
-// that hold face n)
-template <class PT, class PC>
-int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, DA ub0, DA ub1, DA ub2, int has_n) {
+template <class PT, class PC, bool CONS>
+int vjp_kernels_t(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, DA ub0, DA ub1, DA ub2,
+                  int has_n) {
     const bool ppm2 = c->p.ppm_type == 2;
     // the three-kernel state gather has no ghost folding: the general
     // layout takes the fused one
@@ -1229,21 +1311,21 @@ int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, DA ub0, DA ub1, DA
         MX_LAUNCH(c, "k_tape_ppm", (k_tape_ppm<2, PT>), gc, dim3(NTB), A);
     else
         MX_LAUNCH(c, "k_tape_ppm", (k_tape_ppm<1, PT>), gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_tape_corner", k_tape_corner<PT>, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_tape_edge", k_tape_edge<PT>, gc, dim3(NTB), A);
-    MX_LAUNCH(c, "k_adj_tterm", k_adj_tterm<PC>, gc, dim3(NTB), A);
+    MX_LAUNCH(c, "k_tape_corner", (k_tape_corner<PT, CONS>), gc, dim3(NTB), A);
+    MX_LAUNCH(c, "k_tape_edge", (k_tape_edge<PT, CONS>), gc, dim3(NTB), A);
+    MX_LAUNCH(c, "k_adj_tterm", (k_adj_tterm<PC, CONS>), gc, dim3(NTB), A);
     if (ppm2) {
         // ppm_type 2: the Colella-Sekora trace and state sweeps (the kernel
         // arrangement options apply to ppm_type 1)
-        MX_LAUNCH(c, "k_adj_trace", k_adj_trace2<PC>, gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace2<PC, CONS>), gc, dim3(NTB), A);
         MX_LAUNCH(c, "k_adj_state", k_adj_state2<PT>, gc, dim3(NTB), A);
     } else if (c->adj_face_sums) {
-        MX_LAUNCH(c, "k_adj_fsum", k_adj_fsum<PT>, gc, dim3(NTB), A);
-        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<PC, true, false>), gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_fsum", (k_adj_fsum<PT, CONS>), gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<PC, true, false, CONS>), gc, dim3(NTB), A);
     } else if (c->adj_trace_rolled) {
-        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<PC, false, true>), gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<PC, false, true, CONS>), gc, dim3(NTB), A);
     } else {
-        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<PC, false, false>), gc, dim3(NTB), A);
+        MX_LAUNCH(c, "k_adj_trace", (k_adj_trace<PC, false, false, CONS>), gc, dim3(NTB), A);
     }
     if (ppm2) {
         // the state sweep ran above
@@ -1255,10 +1337,24 @@ int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, DA ub0, DA ub1, DA
         MX_LAUNCH(c, "k_adj_state", k_adj_state<PT>, gc, dim3(NTB), A);
     }
     if (c->adj_face_sums && !ppm2)
-        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<PT, false>), g1, dim3(NTB), A, ub0, ub1, ub2, has_n);
+        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<PT, false, CONS>), g1, dim3(NTB), A, ub0, ub1, ub2,
+                  has_n);
     else
-        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<PT, true>), g1, dim3(NTB), A, ub0, ub1, ub2, has_n);
+        MX_LAUNCH(c, "k_adj_vel", (k_adj_vel<PT, true, CONS>), g1, dim3(NTB), A, ub0, ub1, ub2,
+                  has_n);
     return MX_OK;
+}
+
+// the tape and reverse kernels of the step adjoint.  PT indexes the tape,
+// state and velocity sweeps, PC the transverse-term and trace gathers
+// (which run faster with compare indexing on MI355X, measured, whatever n
+// is); the velocity adjoint goes to ub0..ub2 (has_n: the periodic FABs
+// that hold face n); cons: the conservative form
+template <class PT, class PC>
+int vjp_kernels(mx_adv_ctx* c, const AB& A, dim3 g1, dim3 gc, DA ub0, DA ub1, DA ub2, int has_n,
+                bool cons) {
+    return cons ? vjp_kernels_t<PT, PC, true>(c, A, g1, gc, ub0, ub1, ub2, has_n)
+                : vjp_kernels_t<PT, PC, false>(c, A, g1, gc, ub0, ub1, ub2, has_n);
 }
 
 // arrays of the tape and the reverse sweeps: `take(m)` hands out m
@@ -1313,12 +1409,16 @@ void ab_factors(const mx_adv_ctx* c, AB& A, int ncomp) {
     A.ncomp = ncomp;
     A.dt = c->p.dt;
     A.rel_eps = c->p.rel_eps;
+    const double dt2 = 0.5 * A.dt, dt3 = A.dt / 3.0, dt4 = 0.25 * A.dt, dt6 = A.dt / 6.0;
+    A.dt3 = dt3;
+    A.dx0 = c->p.dx[0][0];
     for (int d = 0; d < 3; ++d) {
         A.dx[d] = c->p.dx[0][d];
         A.fdt[d] = A.dt / A.dx[d];
-        const double dt4 = 0.25 * A.dt, dt6 = A.dt / 6.0;
         A.f4[d] = dt4 / A.dx[d];
         A.f6[d] = dt6 / A.dx[d];
+        A.f2[d] = dt2 / A.dx[d];
+        A.f3[d] = dt3 / A.dx[d];
     }
 }
 
@@ -1578,7 +1678,8 @@ mx_fab dense_desc(double* p, const int ext[3], const int begin[3], int ncomp) {
 
 extern "C" {
 
-int mx_adv_scal_step(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, mx_fab* snew, int ncomp) {
+int mx_adv_scal_step(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, mx_fab* snew, int ncomp,
+                     int is_conservative) {
     if (!c || !s || !umac || !snew) return MX_ERR_ARG;
     int n = 0;
     int rc = step_check(c, s, umac, ncomp, n);
@@ -1609,7 +1710,8 @@ int mx_adv_scal_step(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, mx_fab* snew, int n
         for (int d = 0; d < 3; ++d) b.lo[d] = b.hi[d] = MX_BC_INT_DIR;
     const int saved = c->fused_edge;
     c->fused_edge = 1;
-    rc = make_edge_scal(c, 0, &sm, &sem, &um, &fm, 0, bcs.data(), ncomp, 0, 0, ncomp, 0);
+    rc = make_edge_scal(c, 0, &sm, &sem, &um, &fm, 0, bcs.data(), ncomp, 0, 0, ncomp,
+                        is_conservative != 0);
     c->fused_edge = saved;
     if (rc) return rc;
     MX_LAUNCH(c, "k_step_update", k_step_update, grid1((long long)n * n * n), dim3(NTB), to_da(*s),
@@ -1620,7 +1722,7 @@ int mx_adv_scal_step(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, mx_fab* snew, int n
 }
 
 int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* gbar,
-                         mx_fab* sbar, mx_fab* ubar, int ncomp) {
+                         mx_fab* sbar, mx_fab* ubar, int ncomp, int is_conservative) {
     if (!c || !s || !umac || !gbar || !sbar || !ubar) return MX_ERR_ARG;
     int n = 0;
     int rc = step_check(c, s, umac, ncomp, n, false);
@@ -1668,12 +1770,13 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
         if (ubar[d].end[d] > n) has_n |= 1 << d;
     const DA u0 = to_da(ubar[0]), u1 = to_da(ubar[1]), u2 = to_da(ubar[2]);
     const bool p2 = c->adj_pow2 && (n & (n - 1)) == 0;
-    return p2 ? vjp_kernels<PerT<true>, PerT<false>>(c, A, g1, gc, u0, u1, u2, has_n)
-              : vjp_kernels<PerT<false>, PerT<false>>(c, A, g1, gc, u0, u1, u2, has_n);
+    const bool cons = is_conservative != 0;
+    return p2 ? vjp_kernels<PerT<true>, PerT<false>>(c, A, g1, gc, u0, u1, u2, has_n, cons)
+              : vjp_kernels<PerT<false>, PerT<false>>(c, A, g1, gc, u0, u1, u2, has_n, cons);
 }
 
 int mx_adv_scal_step_boxes(mx_adv_ctx* c, mx_multifab* s, mx_face_multifab* umac,
-                           mx_multifab* snew, int ncomp) {
+                           mx_multifab* snew, int ncomp, int is_conservative) {
     if (!c || !s || !umac || !snew) return MX_ERR_ARG;
     StepDomain D;
     int rc = box_domain(c, ncomp, D);
@@ -1733,7 +1836,8 @@ int mx_adv_scal_step_boxes(mx_adv_ctx* c, mx_multifab* s, mx_face_multifab* umac
     for (int d = 0; d < 3; ++d) sem.dir[d] = {nb, se[d].data()};
     const int saved = c->fused_edge;
     c->fused_edge = 1;
-    rc = make_edge_scal(c, 0, s, &sem, umac, &fm, 0, bcs.data(), ncomp, 0, 0, ncomp, 0);
+    rc = make_edge_scal(c, 0, s, &sem, umac, &fm, 0, bcs.data(), ncomp, 0, 0, ncomp,
+                        is_conservative != 0);
     c->fused_edge = saved;
     if (rc) return rc;
     std::vector<UpdTask> tasks(nb);
@@ -1760,7 +1864,7 @@ int mx_adv_scal_step_boxes(mx_adv_ctx* c, mx_multifab* s, mx_face_multifab* umac
 
 int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* c, const mx_multifab* s, const mx_face_multifab* umac,
                                const mx_multifab* gbar, mx_multifab* sbar,
-                               mx_face_multifab* ubar, int ncomp) {
+                               mx_face_multifab* ubar, int ncomp, int is_conservative) {
     if (!c || !s || !umac || !gbar || !sbar || !ubar) return MX_ERR_ARG;
     StepDomain D;
     int rc = box_domain(c, ncomp, D);
@@ -1924,7 +2028,7 @@ int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* c, const mx_multifab* s, const mx_fac
     const dim3 g1 = grid1(npt);
     const dim3 gc(g1.x, (unsigned)ncomp);
     if ((rc = vjp_kernels<GenT, GenT>(c, A, g1, gc, to_da(ubd[0]), to_da(ubd[1]), to_da(ubd[2]),
-                                      0)))
+                                      0, is_conservative != 0)))
         return rc;
 
     // scatter: sbar on every box's cells, ubar on every face a box holds

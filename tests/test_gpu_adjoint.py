@@ -2,7 +2,8 @@
 adjoint (mx_adv_scal_step_vjp) on a periodic box with a discretely
 divergence-free MAC velocity:
 
-* the GPU step equals the oracle's step bit for bit;
+* the GPU step equals the oracle's step bit for bit, in the advective and
+  the conservative form of MakeEdgeScal;
 * dot-product test: <v, J w> from the oracle's forward-mode derivative
   (dual numbers, oracle/oracle_step.cpp) equals <J^T v, w> from the GPU
   adjoint, for random w in the state and the velocity, to 1e-12 relative
@@ -79,6 +80,37 @@ def test_vjp_dot_product(n, ppm):
     assert abs(float(np.sum(v * jws)) - float(np.sum(sb * ws))) <= 1e-12 * abs(float(np.sum(v * jws)))
 
 
+@pytest.mark.parametrize("ppm", [1, 2])
+def test_conservative_step_and_dot_product(ppm):
+    """The conservative form (is_conservative = 1) under a velocity with
+    divergence: the GPU step equals the oracle's conservative step bit for
+    bit, and the adjoint passes the dot-product test against the oracle's
+    forward-mode derivative."""
+    from maestroex_amd.adjoint import ScalStepper
+    s, u, dt, eps = setup_case(n=16, seed=15)
+    rng = np.random.default_rng(16)
+    u = [x + 0.2 * rng.standard_normal(x.shape) for x in u]
+    n, nc = s.shape[1], s.shape[0]
+    print(f"[parity] conservative scal_step and vjp n={n} ppm_type={ppm}", flush=True)
+    dx = [1.0 / n] * 3
+    st = ScalStepper(n, nc, dt, eps, ppm_type=ppm, conservative=True)
+    g = st.step(to_dev(s), tuple(to_dev(x) for x in u)).cpu().numpy()
+    o = pyoracle.scal_step_bc(s, u, dt, dx, eps, [1, 1, 1], ppm_type=ppm, conservative=True)
+    assert np.array_equal(g, o), f"{int((g != o).sum())} values differ"
+    ws = rng.standard_normal(s.shape)
+    wu = [rng.standard_normal(x.shape) for x in u]
+    v = rng.standard_normal(s.shape)
+    _, jw = pyoracle.scal_step_bc(s, u, dt, dx, eps, [1, 1, 1], ws, wu, ppm_type=ppm,
+                                  conservative=True)
+    sb, ub = st.vjp(to_dev(s), tuple(to_dev(x) for x in u), to_dev(v))
+    sb = sb.cpu().numpy()
+    ub = [x.cpu().numpy() for x in ub]
+    lhs = float(np.sum(v * jw))
+    rhs = float(np.sum(sb * ws) + sum(np.sum(ub[d] * wu[d]) for d in range(3)))
+    print(f"[adjoint] conservative <v, J w> = {lhs!r}  <J^T v, w> = {rhs!r}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+
+
 VJP_VARIANTS = [
     {"adjoint_state_fused": 0},
     {"adjoint_face_sums": 1},
@@ -90,16 +122,18 @@ VJP_DEFAULTS = {"adjoint_state_fused": 1, "adjoint_face_sums": 0, "adjoint_pow2_
                 "adjoint_trace_rolled": 0}
 
 
+@pytest.mark.parametrize("cons", [False, True])
 @pytest.mark.parametrize("variant", VJP_VARIANTS)
-def test_vjp_kernel_variants_are_identical(variant):
+def test_vjp_kernel_variants_are_identical(variant, cons):
     """The adjoint's kernel arrangements (one- or three-kernel state gather,
     face adjoints stored or recomputed, mask or compare indexing) evaluate
-    the same expressions: equal bit for bit."""
+    the same expressions: equal bit for bit, in both forms."""
+    from maestroex_amd.adjoint import ScalStepper
     s, u, dt, eps = setup_case(seed=5)
     n, nc = s.shape[1], s.shape[0]
-    print(f"[parity] scal_step vjp variant {variant}", flush=True)
+    print(f"[parity] scal_step vjp variant {variant} cons={cons}", flush=True)
     v = np.random.default_rng(12).standard_normal(s.shape)
-    st = stepper_for(n, nc, dt, eps)
+    st = ScalStepper(n, nc, dt, eps, conservative=cons)
     args = (to_dev(s), tuple(to_dev(x) for x in u), to_dev(v))
     sb0, ub0 = st.vjp(*args)
     for key, val in variant.items():

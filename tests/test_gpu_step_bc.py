@@ -7,6 +7,8 @@
 * dot-product test: <v, J w> from the oracle's forward-mode derivative
   equals <J^T v, w> from the GPU adjoint, w in the state and the velocity,
   to 1e-12 relative;
+* both also in the conservative form of MakeEdgeScal, under a velocity
+  with divergence;
 * on an all-periodic domain the box-list adjoint equals the single-box
   adjoint (mx_adv_scal_step_vjp) bit for bit;
 * the spherical case: a star in an outflow box split into uneven boxes,
@@ -40,33 +42,47 @@ def to_dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
 
 
-def stepper(per, ppm, ncomp, dt, eps, boxes=None, n=NN, spherical=0):
+def stepper(per, ppm, ncomp, dt, eps, boxes=None, n=NN, spherical=0, cons=False):
     from maestroex_amd.adjoint import BoxStepper
     return BoxStepper(boxes or boxes_from_cuts(CUTS), n, per, ncomp, dt, eps, ppm_type=ppm,
-                      spherical=spherical)
+                      spherical=spherical, conservative=cons)
 
 
+def velocity(per, seed, cons):
+    """The divergence-free MAC velocity, plus noise with divergence for the
+    conservative form (whose s divu terms it exercises)."""
+    u = mac_velocity(NN, per, seed=seed)
+    if cons:
+        rng = np.random.default_rng(seed + 100)
+        u = tuple(x + 0.2 * rng.standard_normal(x.shape) for x in u)
+    return u
+
+
+@pytest.mark.parametrize("cons", [False, True])
 @pytest.mark.parametrize("ppm", [1, 2])
 @pytest.mark.parametrize("per", [(0, 0, 0), (1, 0, 1)])
-def test_box_step_forward_parity(ppm, per):
-    print(f"[parity] scal_step_boxes n={NN} periodic={per} ppm_type={ppm}", flush=True)
-    u = mac_velocity(NN, per, seed=21)
+def test_box_step_forward_parity(ppm, per, cons):
+    print(f"[parity] scal_step_boxes n={NN} periodic={per} ppm_type={ppm} cons={cons}",
+          flush=True)
+    u = velocity(per, 21, cons)
     s = scalar_fields(NN, 2, seed=22)
     dt, eps = 0.4 / 20, 1e-8
-    st = stepper(per, ppm, 2, dt, eps)
+    st = stepper(per, ppm, 2, dt, eps, cons=cons)
     g = st.step(to_dev(s), tuple(to_dev(x) for x in u)).cpu().numpy()
-    o = pyoracle.scal_step_bc(s, u, dt, [1.0 / q for q in NN], eps, per, ppm_type=ppm)
+    o = pyoracle.scal_step_bc(s, u, dt, [1.0 / q for q in NN], eps, per, ppm_type=ppm,
+                              conservative=cons)
     nd = int((g != o).sum())
     print(f"[parity] ndiff {nd} max abs {float(np.max(np.abs(g - o))):.3e}", flush=True)
     assert nd == 0
 
 
-def dot_test(st, s, u, dt, eps, per, ppm, seed, dx):
+def dot_test(st, s, u, dt, eps, per, ppm, seed, dx, cons=False):
     rng = np.random.default_rng(seed)
     ws = rng.standard_normal(s.shape)
     wu = [rng.standard_normal(x.shape) for x in u]
     v = rng.standard_normal(s.shape)
-    _, jw = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, ws, wu, ppm_type=ppm)
+    _, jw = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, ws, wu, ppm_type=ppm,
+                                  conservative=cons)
     sb, ub = st.vjp(to_dev(s), tuple(to_dev(x) for x in u), to_dev(v))
     sb = sb.cpu().numpy()
     ub = [x.cpu().numpy() for x in ub]
@@ -74,21 +90,23 @@ def dot_test(st, s, u, dt, eps, per, ppm, seed, dx):
     rhs = float(np.sum(sb * ws) + sum(np.sum(ub[d] * wu[d]) for d in range(3)))
     print(f"[adjoint] <v, J w> = {lhs!r}  <J^T v, w> = {rhs!r}", flush=True)
     assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
-    _, jws = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, ws, None, ppm_type=ppm)
+    _, jws = pyoracle.scal_step_bc(s, u, dt, dx, eps, per, ws, None, ppm_type=ppm,
+                                   conservative=cons)
     ls = float(np.sum(v * jws))
     assert abs(ls - float(np.sum(sb * ws))) <= 1e-12 * max(abs(ls), 1.0)
 
 
+@pytest.mark.parametrize("cons", [False, True])
 @pytest.mark.parametrize("ppm", [1, 2])
 @pytest.mark.parametrize("per", [(0, 0, 0), (1, 0, 1)])
-def test_box_vjp_dot_product(ppm, per):
-    print(f"[parity] scal_step_vjp_boxes dot product n={NN} periodic={per} ppm_type={ppm}",
-          flush=True)
-    u = mac_velocity(NN, per, seed=23)
+def test_box_vjp_dot_product(ppm, per, cons):
+    print(f"[parity] scal_step_vjp_boxes dot product n={NN} periodic={per} ppm_type={ppm} "
+          f"cons={cons}", flush=True)
+    u = velocity(per, 23, cons)
     s = scalar_fields(NN, 2, seed=24)
     dt, eps = 0.4 / 20, 1e-8
-    st = stepper(per, ppm, 2, dt, eps)
-    dot_test(st, s, u, dt, eps, per, ppm, 25, [1.0 / q for q in NN])
+    st = stepper(per, ppm, 2, dt, eps, cons=cons)
+    dot_test(st, s, u, dt, eps, per, ppm, 25, [1.0 / q for q in NN], cons=cons)
 
 
 @pytest.mark.parametrize("ppm", [1, 2])
