@@ -136,6 +136,8 @@ struct AB {
     int ext[3];
     int n, ncomp;
     GenT gen;          // the general layout (outflow sides)
+    int xcd;           // the grid's x is a multiple of 8 and each XCD walks one
+                       // contiguous eighth of it
     int fold;          // ghost reach of the state adjoint (general layout)
     double dt, rel_eps, dx[3];
     // per-direction factors, evaluated once on the host by the same
@@ -156,7 +158,12 @@ __device__ __forceinline__ GenT GenT::make(const AB& A) { return A.gen; }
 template <class PT>
 __device__ __forceinline__ bool point(const AB& A, const PT& P, int& i, int& j, int& k,
                                       long long& p) {
-    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    // workgroups go to the 8 XCDs round robin by linear id: with A.xcd the
+    // block of XCD x takes the x-th eighth of the points, in order, so that
+    // stencil neighbours stay in that XCD's L2
+    unsigned b = blockIdx.x;
+    if (A.xcd) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+    const unsigned q = b * blockDim.x + threadIdx.x;
     if (q >= (unsigned)A.npts) return false;
     const unsigned e0 = (unsigned)A.ext[0], e1 = (unsigned)A.ext[1];
     const unsigned r = q / e0;
@@ -1756,7 +1763,9 @@ int mx_adv_scal_step_vjp(mx_adv_ctx* c, mx_fab* s, mx_fab* umac, const mx_fab* g
     for (int d = 0; d < 3; ++d) A.ext[d] = n;
     ab_factors(c, A, ncomp);
 
-    const dim3 g1 = grid1(n3);
+    A.xcd = c->adj_xcd;
+    dim3 g1 = grid1(n3);
+    if (A.xcd) g1.x = (g1.x + 7u) & ~7u;
     const dim3 gc(g1.x, (unsigned)ncomp);
     if (!sd) MX_LAUNCH(c, "k_to_dense", k_to_dense, g1, dim3(NTB), to_da(*s), 0, ncomp, n, S);
     if (!gd)
@@ -2025,7 +2034,9 @@ int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* c, const mx_multifab* s, const mx_fac
                   A.ext[2], L.pad[0], L.pad[1], L.pad[2]);
     std::memcpy(c->adj_layout, layout, sizeof(layout));
 
-    const dim3 g1 = grid1(npt);
+    A.xcd = c->adj_xcd;
+    dim3 g1 = grid1(npt);
+    if (A.xcd) g1.x = (g1.x + 7u) & ~7u;
     const dim3 gc(g1.x, (unsigned)ncomp);
     if ((rc = vjp_kernels<GenT, GenT>(c, A, g1, gc, to_da(ubd[0]), to_da(ubd[1]), to_da(ubd[2]),
                                       0, is_conservative != 0)))

@@ -133,6 +133,7 @@ struct mx_adv_ctx {
     int adj_face_sums = 0;    // adjoint: predictor-state face adjoints stored once per face
     int adj_pow2 = 1;         // adjoint: mask indexing when n is a power of two
     int adj_trace_rolled = 0; // adjoint: trace kernel with one direction per loop trip
+    int adj_xcd = 0;          // adjoint: each XCD takes a contiguous range of blocks
 
     // calls that allocate nothing (mx_adv_advance_pass_noalloc).  With `dry`
     // set an entry point runs its host logic only: every launch is skipped,

@@ -117,9 +117,10 @@ VJP_VARIANTS = [
     {"adjoint_pow2_index": 0},
     {"adjoint_trace_rolled": 1},
     {"adjoint_state_fused": 0, "adjoint_face_sums": 1, "adjoint_pow2_index": 0},
+    {"adjoint_xcd": 1},
 ]
 VJP_DEFAULTS = {"adjoint_state_fused": 1, "adjoint_face_sums": 0, "adjoint_pow2_index": 1,
-                "adjoint_trace_rolled": 0}
+                "adjoint_trace_rolled": 0, "adjoint_xcd": 0}
 
 
 @pytest.mark.parametrize("cons", [False, True])
