@@ -490,6 +490,24 @@ __device__ __forceinline__ long long grid_point(const KGeom& g) {
     return (long long)b * blockDim.x + threadIdx.x;
 }
 
+// XCD-aware order of a per-component tile launch: the 1-D grid holds
+// 8 * nc * ceil(ntile / 8) blocks (xcd_tile_blocks); XCD x = blockIdx.x % 8
+// takes the tiles [x * per, (x + 1) * per), per = ceil(ntile / 8), in order,
+// and runs the nc component blocks of each tile back to back, so the planes
+// they all stage (the velocities, the halo) are read from HBM once into
+// that XCD's L2.  False for the blocks past the last tile.
+__device__ __forceinline__ bool xcd_tile(int ntile, int nc, int& t, int& v) {
+    const unsigned L = blockIdx.x;
+    const unsigned per = ((unsigned)ntile + 7u) / 8u;
+    const unsigned r = L >> 3;
+    v = (int)(r % (unsigned)nc);
+    t = (int)((L & 7u) * per + r / (unsigned)nc);
+    return t < ntile;
+}
+__host__ __device__ constexpr unsigned xcd_tile_blocks(long long ntile, int nc) {
+    return (unsigned)(8LL * nc * ((ntile + 7) / 8));
+}
+
 // region iteration: linear index -> (i, j, k) in box [lo, hi]
 __device__ __forceinline__ bool region_point(long long t, const int* lo, const int* hi, int& i,
                                              int& j, int& k) {

@@ -187,7 +187,7 @@ __device__ __forceinline__ double apply3(double x, double a, double b, double c)
 template <int PPM, bool CONS, int TX, int TY, bool XYF, int WPE>
 __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_per_eu(WPE)))
     k_es_fused(const FBox* __restrict__ boxes, const FTile* __restrict__ tiles,
-               const FComp* __restrict__ comps, KGeom g, FArgs A) {
+               const FComp* __restrict__ comps, KGeom g, FArgs A, int ntile) {
     constexpr int NT = fused_nt(TX, TY);
     constexpr int G = PPM == 2 ? 3 : 2;
     constexpr int W = TX + 2;
@@ -214,7 +214,11 @@ __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_
     __shared__ double P1[4 * NE];
     __shared__ double P2[NP2];
 
-    const FTile tl = tiles[blockIdx.x];
+    // ntile > 0: the XCD-aware order of xcd_tile over a 1-D grid, else tile
+    // blockIdx.x and component blockIdx.z
+    int tix = blockIdx.x, vc = blockIdx.z;
+    if (ntile > 0 && !xcd_tile(ntile, A.num_comp, tix, vc)) return;
+    const FTile tl = tiles[tix];
     const int bid = tl.box;
     const int blo0 = boxes[bid].lo[0], blo1 = boxes[bid].lo[1], blo2 = boxes[bid].lo[2];
     const int bhi0 = boxes[bid].hi[0], bhi1 = boxes[bid].hi[1], bhi2 = boxes[bid].hi[2];
@@ -231,7 +235,6 @@ __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_
     const bool last_y = (Y0 + ny - 1 == bhi1);
     const bool last_z = (z1 == bhi2);
 
-    const int vc = blockIdx.z;
     const int comp = comps[vc].comp;
     const int flux = comps[vc].flux;
     const int bcx0 = comps[vc].bclo[0], bcx1 = comps[vc].bchi[0];
@@ -799,18 +802,21 @@ template <int TX, int TY, int WPE, bool XYF>
 int launch_one(mx_adv_ctx* c, const FBox* db, const FTile* dtl, size_t ntile, const FComp* dc,
                const KGeom& kg, const FArgs& A, int num_comp, bool cons) {
     if (ntile == 0) return MX_OK;
-    dim3 grid((unsigned)ntile, 1, (unsigned)num_comp);
+    const bool xo = c->es_fused_xcd != 0;
+    const dim3 grid = xo ? dim3(xcd_tile_blocks((long long)ntile, num_comp))
+                         : dim3((unsigned)ntile, 1, (unsigned)num_comp);
+    const int nt = xo ? (int)ntile : 0;
     dim3 block(fused_nt(TX, TY));
     if (c->p.ppm_type == 1) {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt);
     } else {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt);
     }
     return MX_OK;
 }

@@ -145,6 +145,29 @@ def test_pass_xcd_remap(name, cfg):
     assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} etarhoflux")
 
 
+@pytest.mark.parametrize("name,cfg", XCD_CASES, ids=[c[0] for c in XCD_CASES])
+def test_pass_fused_xcd_order(name, cfg):
+    """The fused VelPred and MakeEdgeScal in the plain block order (tile
+    blockIdx.x, component blockIdx.z), against the default XCD-aware order
+    of every other pass test (each XCD a contiguous range of tiles, the
+    component blocks of a tile back to back, tile counts that are not
+    multiples of 8 included): the whole pass bit-exact either way."""
+    print(f"[parity] fused plain block order {name}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    # the default is the XCD-aware order: this runs the plain one
+    m.set_tuning("velpred_fused_xcd", 0)
+    m.set_tuning("edge_scal_xcd", 0)
+    _run_pass(m, wg)
+    assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
+    assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} sedge")
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} sflux")
+    assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} etarhoflux")
+
+
 VP_ALL_CASES = [CASES[0], CASES[1], CASES[7], CASES[9], CASES[10], CASES[16], CASES[8]]
 
 
