@@ -92,25 +92,35 @@ __device__ __forceinline__ PerT<P2> make_per(int n) {
 // array, so that reads up to pad[d] points outside need no test.
 struct GenT {
     static constexpr bool kGen = true;
-    int nn[3], per[3], pad[3];
+    // one scalar per direction (no arrays: an array indexed at run time goes
+    // to scratch memory)
+    int nn0, nn1, nn2, per0, per1, per2, pad0, pad1, pad2;
     int sj, sk;
-    __device__ __forceinline__ int w(int x, int d) const {
-        if (per[d]) return x < 0 ? x + nn[d] : (x >= nn[d] ? x - nn[d] : x);
-        return x + pad[d];
+    // the value of direction d by masks (a select chain is turned into a
+    // load from a scratch copy of the struct at a selected offset)
+    __device__ __forceinline__ static int pick(int d, int a, int b, int c) {
+        return (a & -(int)(d == 0)) | (b & -(int)(d == 1)) | (c & -(int)(d == 2));
+    }
+    __device__ __forceinline__ int nn_of(int d) const { return pick(d, nn0, nn1, nn2); }
+    __device__ __forceinline__ int per_of(int d) const { return pick(d, per0, per1, per2); }
+    __device__ __forceinline__ static int wrap(int x, int per, int n, int pad) {
+        if (per) return x < 0 ? x + n : (x >= n ? x - n : x);
+        return x + pad;
     }
     __device__ __forceinline__ int at(int i, int j, int k) const {
-        return w(i, 0) + sj * w(j, 1) + sk * w(k, 2);
+        return wrap(i, per0, nn0, pad0) + sj * wrap(j, per1, nn1, pad1) +
+               sk * wrap(k, per2, nn2, pad2);
     }
     __device__ __forceinline__ int nb(int i, int j, int k, int d, int o) const {
         return at(i + (d == 0 ? o : 0), j + (d == 1 ? o : 0), k + (d == 2 ? o : 0));
     }
     // 1: a d-face at the low outflow side, 2: at the high one, else 0
     __device__ __forceinline__ int bnd(int d, int pos) const {
-        if (per[d]) return 0;
-        return pos == 0 ? 1 : (pos == nn[d] ? 2 : 0);
+        if (per_of(d)) return 0;
+        return pos == 0 ? 1 : (pos == nn_of(d) ? 2 : 0);
     }
     __device__ __forceinline__ bool cell(int i, int j, int k) const {
-        return i < nn[0] && j < nn[1] && k < nn[2];
+        return i < nn0 && j < nn1 && k < nn2;
     }
     __device__ static GenT make(const AB& A);
 };
@@ -178,14 +188,14 @@ __device__ __forceinline__ bool point(const AB& A, const PT& P, int& i, int& j, 
 // corner-coupled and edge states (MaestroMakeEdgeScalars.cpp:709-738,
 // :931-961, :1446-1480): the outside state takes the inside one
 __device__ __forceinline__ double upwind_b(double l, double r, double u, double thr, int b);
+// (selects, not branches: assignments under branches to the two
+// references put them in scratch memory)
 __device__ __forceinline__ void bfix(int b, double& wl, double& wr) {
-    if (b == 1) {
-        wr = wl + wr;
-        wl = 0.0;
-    } else if (b == 2) {
-        wl = wl + wr;
-        wr = 0.0;
-    }
+    const double sum = wl + wr;
+    const double l = b == 1 ? 0.0 : (b == 2 ? sum : wl);
+    const double r = b == 1 ? sum : (b == 2 ? 0.0 : wr);
+    wl = l;
+    wr = r;
 }
 
 __device__ __forceinline__ void weights(double u, double thr, double& wl, double& wr) {
@@ -208,9 +218,9 @@ __device__ __forceinline__ double upwind(double l, double r, double u, double th
 }
 
 __device__ __forceinline__ double upwind_b(double l, double r, double u, double thr, int b) {
-    if (b == 1) l = r;
-    else if (b == 2) r = l;
-    return upwind(l, r, u, thr);
+    const double L = b == 1 ? r : l;
+    const double R = b == 2 ? l : r;
+    return upwind(L, R, u, thr);
 }
 
 __device__ __forceinline__ double dabs_d(double x) { return signbit(x) ? -1.0 : 1.0; }
@@ -955,21 +965,23 @@ __device__ __forceinline__ double fold_state(const AB& A, const PT& P, int i, in
     if constexpr (!PT::kGen) {
         return g(i, j, k);
     } else {
-        const int q[3] = {i, j, k};
-        int lo[3], hi[3];
-        bool edge = false;
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = hi[d] = q[d];
-            if (P.per[d]) continue;
-            if (q[d] == 0) lo[d] = -A.fold;
-            if (q[d] == P.nn[d] - 1) hi[d] = q[d] + A.fold;
-            edge = edge || lo[d] != q[d] || hi[d] != q[d];
-        }
-        if (!edge) return g(i, j, k);
+        // the range of positions per direction, without runtime-indexed
+        // arrays (they would go to scratch memory)
+        auto range = [&](int q, int d, int& lo, int& hi) {
+            lo = hi = q;
+            if (P.per_of(d)) return;
+            if (q == 0) lo = -A.fold;
+            if (q == P.nn_of(d) - 1) hi = q + A.fold;
+        };
+        int l0, h0, l1, h1, l2, h2;
+        range(i, 0, l0, h0);
+        range(j, 1, l1, h1);
+        range(k, 2, l2, h2);
+        if (l0 == h0 && l1 == h1 && l2 == h2) return g(i, j, k);
         double s = 0.0;
-        for (int c = lo[2]; c <= hi[2]; ++c)
-            for (int b = lo[1]; b <= hi[1]; ++b)
-                for (int a = lo[0]; a <= hi[0]; ++a) s += g(a, b, c);
+        for (int c = l2; c <= h2; ++c)
+            for (int b = l1; b <= h1; ++b)
+                for (int a = l0; a <= h0; ++a) s += g(a, b, c);
         return s;
     }
 }
@@ -1216,18 +1228,12 @@ __global__ void __launch_bounds__(NTB) k_adj_vel(AB A, DA ub0, DA ub1, DA ub2, i
         r[d] = s;
     }
     if constexpr (PT::kGen) {
-        // faces 0 .. nn[d] along d, cells across it
-        const int q[3] = {i, j, k};
-        bool ok[3];
-        for (int d = 0; d < 3; ++d) {
-            ok[d] = true;
-            for (int t = 0; t < 3; ++t)
-                if (t != d && q[t] >= P.nn[t]) ok[d] = false;
-            if (P.per[d] && q[d] >= P.nn[d]) ok[d] = false;
-        }
-        if (ok[0]) ub0(i, j, k) = r[0];
-        if (ok[1]) ub1(i, j, k) = r[1];
-        if (ok[2]) ub2(i, j, k) = r[2];
+        // faces 0 .. nn[d] along d (0 .. nn[d]-1 when periodic), cells
+        // across it
+        const bool ci = i < P.nn0, cj = j < P.nn1, ck = k < P.nn2;
+        if (cj && ck && (ci || !P.per0)) ub0(i, j, k) = r[0];
+        if (ci && ck && (cj || !P.per1)) ub1(i, j, k) = r[1];
+        if (ci && cj && (ck || !P.per2)) ub2(i, j, k) = r[2];
         return;
     }
     ub0(i, j, k) = r[0];
@@ -1913,9 +1919,9 @@ int mx_adv_scal_step_vjp_boxes(mx_adv_ctx* c, const mx_multifab* s, const mx_fac
         L.per[d] = D.per[d];
         cs *= L.E[d];
         npt *= A.ext[d];
-        A.gen.nn[d] = D.nn[d];
-        A.gen.per[d] = D.per[d];
-        A.gen.pad[d] = L.pad[d];
+        (d == 0 ? A.gen.nn0 : (d == 1 ? A.gen.nn1 : A.gen.nn2)) = D.nn[d];
+        (d == 0 ? A.gen.per0 : (d == 1 ? A.gen.per1 : A.gen.per2)) = D.per[d];
+        (d == 0 ? A.gen.pad0 : (d == 1 ? A.gen.pad1 : A.gen.pad2)) = L.pad[d];
     }
     if (cs > 0x7fffffffLL || npt > 0xffffffffLL) return MX_ERR_UNSUPPORTED;
     L.cs = cs;
