@@ -490,18 +490,25 @@ __device__ __forceinline__ long long grid_point(const KGeom& g) {
     return (long long)b * blockDim.x + threadIdx.x;
 }
 
-// XCD-aware order of a per-component tile launch: the 1-D grid holds
-// 8 * nc * ceil(ntile / 8) blocks (xcd_tile_blocks); XCD x = blockIdx.x % 8
-// takes the tiles [x * per, (x + 1) * per), per = ceil(ntile / 8), in order,
-// and runs the nc component blocks of each tile back to back, so the planes
-// they all stage (the velocities, the halo) are read from HBM once into
-// that XCD's L2.  False for the blocks past the last tile.
-__device__ __forceinline__ bool xcd_tile(int ntile, int nc, int& t, int& v) {
+// XCD-aware orders of a per-component tile launch over a 1-D grid of
+// 8 * nc * ceil(ntile / 8) blocks (xcd_tile_blocks).  XCD x =
+// blockIdx.x % 8 runs the nc component blocks of each of its tiles back to
+// back, so the planes they all stage (the velocities) are read from HBM
+// once into that XCD's L2.  Its tiles are
+//   mode 1: x, x + 8, x + 16, ... (the round robin of the launch order,
+//           which balances tiles of unequal cost);
+//   mode 2: the contiguous range [x * per, (x + 1) * per), per =
+//           ceil(ntile / 8) (neighbouring tiles share their halo in L2).
+// False for the blocks past the last tile.
+__device__ __forceinline__ bool xcd_tile(int ntile, int nc, int mode, int& t, int& v) {
     const unsigned L = blockIdx.x;
-    const unsigned per = ((unsigned)ntile + 7u) / 8u;
     const unsigned r = L >> 3;
     v = (int)(r % (unsigned)nc);
-    t = (int)((L & 7u) * per + r / (unsigned)nc);
+    const unsigned local = r / (unsigned)nc;
+    if (mode == 2)
+        t = (int)((L & 7u) * (((unsigned)ntile + 7u) / 8u) + local);
+    else
+        t = (int)(local * 8u + (L & 7u));
     return t < ntile;
 }
 __host__ __device__ constexpr unsigned xcd_tile_blocks(long long ntile, int nc) {

@@ -187,7 +187,7 @@ __device__ __forceinline__ double apply3(double x, double a, double b, double c)
 template <int PPM, bool CONS, int TX, int TY, bool XYF, int WPE>
 __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_per_eu(WPE)))
     k_es_fused(const FBox* __restrict__ boxes, const FTile* __restrict__ tiles,
-               const FComp* __restrict__ comps, KGeom g, FArgs A, int ntile) {
+               const FComp* __restrict__ comps, KGeom g, FArgs A, int ntile, int xmode) {
     constexpr int NT = fused_nt(TX, TY);
     constexpr int G = PPM == 2 ? 3 : 2;
     constexpr int W = TX + 2;
@@ -214,10 +214,10 @@ __global__ void __launch_bounds__(fused_nt(TX, TY)) __attribute__((amdgpu_waves_
     __shared__ double P1[4 * NE];
     __shared__ double P2[NP2];
 
-    // ntile > 0: the XCD-aware order of xcd_tile over a 1-D grid, else tile
+    // xmode 1, 2: the XCD-aware orders of xcd_tile over a 1-D grid; 0: tile
     // blockIdx.x and component blockIdx.z
     int tix = blockIdx.x, vc = blockIdx.z;
-    if (ntile > 0 && !xcd_tile(ntile, A.num_comp, tix, vc)) return;
+    if (xmode != 0 && !xcd_tile(ntile, A.num_comp, xmode, tix, vc)) return;
     const FTile tl = tiles[tix];
     const int bid = tl.box;
     const int blo0 = boxes[bid].lo[0], blo1 = boxes[bid].lo[1], blo2 = boxes[bid].lo[2];
@@ -802,21 +802,21 @@ template <int TX, int TY, int WPE, bool XYF>
 int launch_one(mx_adv_ctx* c, const FBox* db, const FTile* dtl, size_t ntile, const FComp* dc,
                const KGeom& kg, const FArgs& A, int num_comp, bool cons) {
     if (ntile == 0) return MX_OK;
-    const bool xo = c->es_fused_xcd != 0;
-    const dim3 grid = xo ? dim3(xcd_tile_blocks((long long)ntile, num_comp))
+    const int xm = c->es_fused_xcd;
+    const dim3 grid = xm ? dim3(xcd_tile_blocks((long long)ntile, num_comp))
                          : dim3((unsigned)ntile, 1, (unsigned)num_comp);
-    const int nt = xo ? (int)ntile : 0;
+    const int nt = (int)ntile;
     dim3 block(fused_nt(TX, TY));
     if (c->p.ppm_type == 1) {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt, xm);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<1, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt, xm);
     } else {
         if (cons)
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, true, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt, xm);
         else
-            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt);
+            MX_LAUNCH(c, "k_es_fused", (k_es_fused<2, false, TX, TY, XYF, WPE>), grid, block, db, dtl, dc, kg, A, nt, xm);
     }
     return MX_OK;
 }

@@ -1284,15 +1284,15 @@ __device__ __forceinline__ void vp_comp(const VFBox& B, const VFTile& tl, const 
     }
 }
 
-// ntile > 0: the XCD-aware order of xcd_tile over a 1-D grid, else tile
+// xmode 1, 2: the XCD-aware orders of xcd_tile over a 1-D grid; 0: tile
 // blockIdx.x and component blockIdx.z
 template <int PPM, int TX, int TY, bool XYF, int WPE>
 __global__ void __launch_bounds__(vf_nt(TX, TY)) __attribute__((amdgpu_waves_per_eu(WPE)))
     k_vp_fused_comp(const VFBox* __restrict__ boxes, const VFTile* __restrict__ tiles, KGeom g,
-                    VFArgs A, int ntile) {
+                    VFArgs A, int ntile, int xmode) {
     __shared__ VPShared<PPM, TX, TY> sh;
     int t = blockIdx.x, n = blockIdx.z;
-    if (ntile > 0 && !xcd_tile(ntile, 3, t, n)) return;
+    if (xmode != 0 && !xcd_tile(ntile, 3, xmode, t, n)) return;
     const VFTile tl = tiles[t];
     const VFBox& B = boxes[tl.box];
     if (n == 0)
@@ -1335,15 +1335,15 @@ void vf_launch_one(mx_adv_ctx* c, const VFBox* db, const VFTile* dt, size_t n, c
     if (n == 0 || rc) return;
     auto launch = [&]() -> int {
         if constexpr (COMP) {
-            const bool xo = c->vp_fused_xcd != 0;
-            const dim3 grid = xo ? dim3(xcd_tile_blocks((long long)n, 3)) : dim3((unsigned)n, 1, 3);
-            const int nt = xo ? (int)n : 0;
+            const int xm = c->vp_fused_xcd;
+            const dim3 grid = xm ? dim3(xcd_tile_blocks((long long)n, 3)) : dim3((unsigned)n, 1, 3);
+            const int nt = (int)n;
             if (c->p.ppm_type == 1)
                 MX_LAUNCH(c, "k_vp_fused", (k_vp_fused_comp<1, TX, TY, XYF, WPE>), grid,
-                          dim3(vf_nt(TX, TY)), db, dt, kg, A, nt);
+                          dim3(vf_nt(TX, TY)), db, dt, kg, A, nt, xm);
             else
                 MX_LAUNCH(c, "k_vp_fused", (k_vp_fused_comp<2, TX, TY, XYF, WPE>), grid,
-                          dim3(vf_nt(TX, TY)), db, dt, kg, A, nt);
+                          dim3(vf_nt(TX, TY)), db, dt, kg, A, nt, xm);
         } else {
             if (c->p.ppm_type == 1)
                 MX_LAUNCH(c, "k_vp_fused", (k_vp_fused<1, TX, TY, XYF, WPE>), dim3((unsigned)n),

@@ -145,20 +145,21 @@ def test_pass_xcd_remap(name, cfg):
     assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} etarhoflux")
 
 
+@pytest.mark.parametrize("es_mode,vp_mode", [(0, 0), (2, 2)])
 @pytest.mark.parametrize("name,cfg", XCD_CASES, ids=[c[0] for c in XCD_CASES])
-def test_pass_fused_xcd_order(name, cfg):
-    """The fused VelPred and MakeEdgeScal in the plain block order (tile
-    blockIdx.x, component blockIdx.z), against the default XCD-aware order
-    of every other pass test (each XCD a contiguous range of tiles, the
-    component blocks of a tile back to back, tile counts that are not
-    multiples of 8 included): the whole pass bit-exact either way."""
-    print(f"[parity] fused plain block order {name}", flush=True)
+def test_pass_fused_xcd_order(name, cfg, es_mode, vp_mode):
+    """The block orders of the fused MakeEdgeScal and VelPred other than
+    the default (1, which every other pass test runs):
+    0 the plain order (tile blockIdx.x, component blockIdx.z), 1 tiles round
+    robin over the XCDs and 2 contiguous tile ranges per XCD, each with the
+    component blocks of a tile back to back (tile counts that are not
+    multiples of 8 included): the whole pass bit-exact."""
+    print(f"[parity] fused block orders {name} edge {es_mode} velpred {vp_mode}", flush=True)
     wc, wg = pair(**cfg)
     _run_pass(oracle_for(wc), wc)
     m = gpu_for(wg)
-    # the default is the XCD-aware order: this runs the plain one
-    m.set_tuning("velpred_fused_xcd", 0)
-    m.set_tuning("edge_scal_xcd", 0)
+    m.set_tuning("edge_scal_xcd", es_mode)
+    m.set_tuning("velpred_fused_xcd", vp_mode)
     _run_pass(m, wg)
     assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
     assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
