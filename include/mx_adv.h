@@ -213,8 +213,8 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
      "velpred_fused_comp" 1 (default): one velocity component per block of
                         the fused VelPred; 0: all three in one block.
      "velpred_fused_tile" tile and register budget of the per-component
-                        fused VelPred: 0 (default): 16 x 11 columns, 3 waves
-                        per SIMD; 1: 16 x 8, 3; 2: 16 x 11, 4; 3: 32 x 8, 3.
+                        fused VelPred: 0: 16 x 11 columns, 3 waves per SIMD;
+                        1: 16 x 8, 3; 2 (default): 16 x 11, 4; 3: 32 x 8, 3.
      "rhoh_flux_epilogue" 1 (default): in the pass, MakeRhoHFlux's planar
                         branches that read only the RhoH edge state
                         (enthalpy_pred_type 0, 1, 3) run as the fused edge

@@ -127,7 +127,7 @@ struct mx_adv_ctx {
     int vp_fused_comp = 1; // fused VelPred: 1 = one velocity component per block
     int rhoh_flux_epilogue = 1;  // the pass: MakeRhoHFlux (planar, ept 0/1/3) in the
                                  // fused edge kernel's epilogue
-    int vp_fused_tile = 0; // its tile (waves per SIMD): 0 16x11 (3), 1 16x8 (3), 2 16x11 (4), 3 32x8 (3)
+    int vp_fused_tile = 2; // its tile (waves per SIMD): 0 16x11 (3), 1 16x8 (3), 2 16x11 (4), 3 32x8 (3)
     int fused_tile = -1;    // 0: 32 x 8 columns, 1: 16 x 16, 2: 16 x 12, 3: 16 x 8,
                             // 4: 16 x 11; -1: 16 x 11 for ppm_type 1, else 16 x 12
     int fused_per_box = 0;  // 1: one fused launch per box (comparison mode)

@@ -430,8 +430,8 @@ __global__ void __launch_bounds__(vf_nt(TX, TY)) __attribute__((amdgpu_waves_per
         const double* ULp = UL[slot_p];
         const double* VLp = VL[slot_p];
 
-        // ---- A: stage plane q; shift the column windows; save plane p's
-        //      cell values and forces; issue the loads of plane q + 1
+        // ---- A: stage plane q; shift the column windows; issue the loads
+        //      of plane q + 1
 #pragma unroll
         for (int k = 0; k < NS; ++k)
             if (okS[k]) {
@@ -444,16 +444,16 @@ __global__ void __launch_bounds__(vf_nt(TX, TY)) __attribute__((amdgpu_waves_per
 #pragma unroll
         for (int k = 0; k < NV; ++k)
             if (okV[k]) VLq[tid + k * NT] = pV[k];
+        double flx = 0.0, fly = 0.0;  // plane q - 1 forces, x and y
         if (col) {
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
-                SC[n][tid] = v[n][4];  // plane q - 1 (before the shift)
 #pragma unroll
                 for (int jj = 0; jj < 8; ++jj) v[n][jj] = v[n][jj + 1];
                 v[n][4 + G] = pcs[n];
             }
-            FL[0][tid] = fo_q[0];  // plane q - 1
-            FL[1][tid] = fo_q[1];
+            flx = fo_q[0];
+            fly = fo_q[1];
             foz_p = fo_q[2];
 #pragma unroll
             for (int n = 0; n < 3; ++n) {
@@ -494,6 +494,16 @@ __global__ void __launch_bounds__(vf_nt(TX, TY)) __attribute__((amdgpu_waves_per
             }
         }
         __syncthreads();
+
+        // plane q - 1's cell values and forces, which D2 and E2 read across
+        // columns: stored after the barrier above, which no thread passes
+        // before every thread is done with the previous plane's E2
+        if (col) {
+#pragma unroll
+            for (int n = 0; n < 3; ++n) SC[n][tid] = v[n][3];
+            FL[0][tid] = flx;
+            FL[1][tid] = fly;
+        }
 
         // ---- B1: face values of plane q; PPM of cell q in z
 #pragma unroll
@@ -997,8 +1007,6 @@ __device__ __forceinline__ void vp_comp(const VFBox& B, const VFTile& tl, const 
             ufx = ufCol[0];
             ufy = ufCol[ufns];
             ufz = ufCol[2 * ufns];
-            SC[tid] = v[4];  // plane q - 1 (before the shift)
-            FL[tid] = fo_q;
 #pragma unroll
             for (int jj = 0; jj < 8; ++jj) v[jj] = v[jj + 1];
             v[4 + G] = pcs;
@@ -1031,6 +1039,14 @@ __device__ __forceinline__ void vp_comp(const VFBox& B, const VFTile& tl, const 
         }
         if (col) ufCol += ufks;
         __syncthreads();
+
+        // plane q - 1's cell value and force, which D2 and E2 read across
+        // columns: stored after the barrier above, which no thread passes
+        // before every thread is done with the previous plane's E2
+        if (col) {
+            SC[tid] = v[3];
+            FL[tid] = fo_p;
+        }
 
         // ---- B1: face values of plane q; PPM of cell q in z
         for (int idx = tid; idx < FXW * H; idx += NT) {
@@ -1456,6 +1472,9 @@ int velpred_fused(mx_adv_ctx* c, const std::vector<VpLevel>& levs) {
     // (256 VGPRs); one component per block: the tile shape (cell columns)
     // and register budget (waves per SIMD) of `velpred_fused_tile`
     if (!c->vp_fused_comp) return vf_launch<16, 11, 2, false>(c, levs, tz);
+    // the default, 2 (16 x 11 at 4 waves per SIMD: 128 VGPRs, 4 spilled for
+    // ppm_type 1, 22 for ppm_type 2), takes 1.57 against 1.80 ms of tile 0
+    // per 256^3 ppm_type 1 pass, and is level with it for ppm_type 2
     switch (c->vp_fused_tile) {
         case 1: return vf_launch<16, 8, 3, true>(c, levs, tz);
         case 2: return vf_launch<16, 11, 4, true>(c, levs, tz);
