@@ -214,7 +214,9 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
                         the fused VelPred; 0: all three in one block.
      "velpred_fused_tile" tile and register budget of the per-component
                         fused VelPred: 0: 16 x 11 columns, 3 waves per SIMD;
-                        1: 16 x 8, 3; 2 (default): 16 x 11, 4; 3: 32 x 8, 3.
+                        1: 16 x 8, 3; 2 (default): 16 x 11, 4; 3: 32 x 8, 3;
+                        4: 16 x 8, 4; 5: 16 x 12, 4 (2, 4 and 5: tiles at
+                        an acting x/y domain side at 3).
      "rhoh_flux_epilogue" 1 (default): in the pass, MakeRhoHFlux's planar
                         branches that read only the RhoH edge state
                         (enthalpy_pred_type 0, 1, 3) run as the fused edge

@@ -867,7 +867,7 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->vp_fused_comp = (int)value;
     } else if (k == "velpred_fused_tile") {
-        if (value < 0 || value > 3) return MX_ERR_ARG;
+        if (value < 0 || value > 5) return MX_ERR_ARG;
         c->vp_fused_tile = (int)value;
     } else if (k == "velpred_fused_tz") {
         if (value < 1 || value > 4096) return MX_ERR_ARG;

@@ -1373,7 +1373,9 @@ void vf_launch_one(mx_adv_ctx* c, const VFBox* db, const VFTile* dt, size_t n, c
     rc = launch();
 }
 
-template <int TX, int TY, int WPE, bool COMP>
+// WPE: the register budget (waves per SIMD) of the x/y-free tiles, WPEB that
+// of the tiles at an acting x/y side, whose variant carries the side rules
+template <int TX, int TY, int WPE, bool COMP, int WPEB = WPE>
 int vf_launch(mx_adv_ctx* c, const std::vector<VpLevel>& levs, int tz) {
     const long long lim = 0x7fffffffLL;
     const bool sph = c->p.spherical != 0;
@@ -1457,7 +1459,7 @@ int vf_launch(mx_adv_ctx* c, const std::vector<VpLevel>& levs, int tz) {
     const VFTile* dtl = tab.dev<VFTile>(ot);
     const KGeom kg = kgeom(c, levs[0].lev);
     vf_launch_one<TX, TY, WPE, COMP, true>(c, db, dtl, nfree, kg, A, rc);
-    vf_launch_one<TX, TY, WPE, COMP, false>(c, db, dtl + nfree, tiles.size() - nfree, kg, A, rc);
+    vf_launch_one<TX, TY, WPEB, COMP, false>(c, db, dtl + nfree, tiles.size() - nfree, kg, A, rc);
     return rc;
 }
 
@@ -1472,13 +1474,16 @@ int velpred_fused(mx_adv_ctx* c, const std::vector<VpLevel>& levs) {
     // (256 VGPRs); one component per block: the tile shape (cell columns)
     // and register budget (waves per SIMD) of `velpred_fused_tile`
     if (!c->vp_fused_comp) return vf_launch<16, 11, 2, false>(c, levs, tz);
-    // the default, 2 (16 x 11 at 4 waves per SIMD: 128 VGPRs, 4 spilled for
-    // ppm_type 1, 22 for ppm_type 2), takes 1.57 against 1.80 ms of tile 0
-    // per 256^3 ppm_type 1 pass, and is level with it for ppm_type 2
+    // the default, 2 (16 x 11, x/y-free tiles at 4 waves per SIMD: 128
+    // VGPRs, 8 spilled for ppm_type 1, 26 for ppm_type 2), takes 1.57
+    // against 1.80 ms of tile 0 per 256^3 ppm_type 1 pass; 2, 4 and 5 keep
+    // the tiles at an acting x/y side at 3 waves, where 4 spill up to 107
     switch (c->vp_fused_tile) {
         case 1: return vf_launch<16, 8, 3, true>(c, levs, tz);
-        case 2: return vf_launch<16, 11, 4, true>(c, levs, tz);
+        case 2: return vf_launch<16, 11, 4, true, 3>(c, levs, tz);
         case 3: return vf_launch<32, 8, 3, true>(c, levs, tz);
+        case 4: return vf_launch<16, 8, 4, true, 3>(c, levs, tz);
+        case 5: return vf_launch<16, 12, 4, true, 3>(c, levs, tz);
         default: return vf_launch<16, 11, 3, true>(c, levs, tz);
     }
 }

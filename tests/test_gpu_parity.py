@@ -216,7 +216,7 @@ VP_FUSED_CASES = VP_ALL_CASES + [CASES[11], CASES[17], KERNEL_CHOICE_CASES[3]]
 @pytest.mark.parametrize("fused,tz,comp,tile", [(0, 32, 1, 0), (1, 1, 1, 0), (1, 5, 1, 0),
                                                 (1, 32, 1, 0), (1, 5, 0, 0), (1, 5, 1, 1),
                                                 (1, 5, 1, 2), (1, 32, 1, 2), (1, 5, 1, 3),
-                                                (1, 32, 0, 0)])
+                                                (1, 5, 1, 4), (1, 5, 1, 5), (1, 32, 0, 0)])
 @pytest.mark.parametrize("name,cfg", VP_FUSED_CASES, ids=[c[0] for c in VP_FUSED_CASES])
 def test_pass_velpred_fused(name, cfg, fused, tz, comp, tile):
     """VelPred as the fused z-marching kernel (the default for 3-D
