@@ -198,6 +198,12 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
                         block size: 0: 32 x 8 (384 threads), 1: 16 x 16 (384),
                         2: 16 x 12 (256), 3: 16 x 8 (192), 4: 16 x 11 (256);
                         -1 (default): 16 x 11 for ppm_type 1, else 16 x 12.
+     "utrans_line"      1 (default): 3-D MakeUtrans with ppm_type 1/2 traces
+                        each cell once (x-faces from rows of cells through
+                        LDS, y/z-faces by columns marching along y or z);
+                        0: one face per thread, each cell traced for both
+                        of its faces.
+     "utrans_chunks"    chunks per y/z column of that form (1-64, default 8).
      "velpred_ppm_all"  1 (default): VelPred's PPM kernel computes every
                         velocity component of a cell in one thread (3-D,
                         ppm_type 1/2, no traced forces); 0: one component

@@ -851,6 +851,12 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
     } else if (k == "edge_scal_per_box") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->fused_per_box = (int)value;
+    } else if (k == "utrans_line") {
+        if (value != 0 && value != 1) return MX_ERR_ARG;
+        c->utrans_line = (int)value;
+    } else if (k == "utrans_chunks") {
+        if (value < 1 || value > 64) return MX_ERR_ARG;
+        c->utrans_chunks = (int)value;
     } else if (k == "velpred_trans_all") {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->vp_trans_all = (int)value;

@@ -121,6 +121,8 @@ struct mx_adv_ctx {
     int es_fused_xcd = 1;  // fused MakeEdgeScal: XCD-aware block order (xcd_tile mode)
     int xcd_remap = 0;  // KGeom::xcd of the staged pass kernels
     int vp_ppm_all = 1;  // VelPred's PPM kernel: 1 = all components per thread
+    int utrans_line = 1;   // MakeUtrans (3-D, ppm_type 1/2): 1 = each cell traced once
+    int utrans_chunks = 8; // chunks per y/z column of that form
     int vp_trans_all = 1;  // VelPred's transverse kernel: 1 = all six states per thread
     int vp_fused = 1;      // VelPred: 1 = the fused z-marching kernel where it applies
     int vp_fused_tz = 32;  // planes per chunk of the fused VelPred kernel

@@ -172,6 +172,142 @@ __global__ void __launch_bounds__(256) k_utrans(const UtBox* __restrict__ boxes,
         utrans_face<PPM, 2>(B, g, A, i, j, k);
 }
 
+// MakeUtrans with each cell traced once (3-D, ppm_type 1 and 2).  The state
+// on a D-face depends only on velocity component D along direction D: the
+// face takes Ip of its low cell and Im of its high cell, the same
+// ppm_edges / ppm_trace evaluations that utrans_face makes per face, so the
+// results are bit-identical.
+//   blockIdx.z = 0, x-faces: a block's threads take consecutive cells of the
+//     box's rows, each row with its cells lo - 1 .. hi + 1 (nx + 2 cells,
+//     flattened; lanes consecutive in x).  Each thread traces its cell and
+//     keeps Ip in LDS; the face at the low side of a thread's cell takes the
+//     previous thread's Ip.  Block b starts at cell b * (UT_NT - 1), so the
+//     last cell of one block is the first of the next, whose face the
+//     earlier block forms.
+//   blockIdx.z = 1, 2: each thread takes one x column (lanes consecutive in
+//     x, so loads and stores coalesce) and one of nch chunks of the faces
+//     lo .. hi + 1 along y or z, marching a register window along its column
+//     and tracing each cell once (one extra cell per chunk).  The march also
+//     carries the slopes (ppm_type 1) or face values (ppm_type 2) from cell
+//     to cell, each evaluated once as in the fused VelPred's z direction, and
+//     takes ppm_edges in full where the cell has a boundary stencil.
+constexpr int UT_NT = 256;
+
+template <int PPM>
+__device__ __forceinline__ void utrans_rows(const UtBox& B, const KGeom& g, const UtArgs& A) {
+    __shared__ double sIp[UT_NT];
+    const unsigned nx = (unsigned)(B.hi[0] - B.lo[0] + 1);
+    const unsigned ny = (unsigned)(B.hi[1] - B.lo[1] + 1);
+    const unsigned nz = (unsigned)(B.hi[2] - B.lo[2] + 1);
+    const unsigned rw = nx + 2;
+    const unsigned t = blockIdx.x * (unsigned)(UT_NT - 1) + threadIdx.x;
+    const bool on = t < rw * ny * nz;
+    unsigned c = 0;
+    int i = 0, j = 0, k = 0;
+    double Ip = 0.0, Im = 0.0;
+    if (on) {
+        const unsigned row = t / rw;
+        c = t - row * rw;
+        i = B.lo[0] - 1 + (int)c;
+        j = B.lo[1] + (int)(row % ny);
+        k = B.lo[2] + (int)(row / ny);
+        double v[9], sm, sp;
+        load_stencil<PPM>(B.ut.p + B.ut.off(i, j, k, 0), 1, v);
+        ppm_edges<PPM, 0>(v, i, g.domlo[0], g.domhi[0], A.bclo[0][0], A.bchi[0][0], sm, sp);
+        const double uc = B.uf(i, j, k, 0);
+        ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[0], g.rel_eps, Ip, Im);
+    }
+    sIp[threadIdx.x] = Ip;
+    __syncthreads();
+    if (on && threadIdx.x > 0 && c > 0) utrans_riemann<0>(B, g, i, j, k, sIp[threadIdx.x - 1], Im);
+}
+
+template <int PPM, int D>
+__device__ __forceinline__ void utrans_march(const UtBox& B, const KGeom& g, const UtArgs& A,
+                                             int nch) {
+    constexpr int G = PPM == 2 ? 3 : 2;
+    constexpr int T = D == 1 ? 2 : 1;  // the transverse direction other than x
+    const unsigned nx = (unsigned)(B.hi[0] - B.lo[0] + 1);
+    const unsigned nt = (unsigned)(B.hi[T] - B.lo[T] + 1);
+    const int L = (B.hi[D] - B.lo[D] + 2 + nch - 1) / nch;
+    const unsigned t = blockIdx.x * (unsigned)UT_NT + threadIdx.x;
+    if (t >= nx * nt * (unsigned)nch) return;
+    const unsigned r = t / nx;
+    const int i = B.lo[0] + (int)(t - r * nx);
+    const int tt = B.lo[T] + (int)(r % nt);
+    const int f0 = B.lo[D] + (int)(r / nt) * L;
+    const int f1 = min(f0 + L - 1, B.hi[D] + 1);
+    if (f0 > f1) return;
+    const long long st = B.ut.stride(D), fst = B.uf.stride(D);
+    const double* up = B.ut.p + B.ut.off(i, D == 1 ? f0 - 1 : tt, D == 2 ? f0 - 1 : tt, D);
+    const double* fp = B.uf.p + B.uf.off(i, D == 1 ? f0 - 1 : tt, D == 2 ? f0 - 1 : tt, D);
+    const int dlo = g.domlo[D], dhi = g.domhi[D];
+    const int bclo = A.bclo[D][D], bchi = A.bchi[D][D];
+    double v[9];
+    load_stencil<PPM>(up, st, v);
+    double uc = *fp;
+    double Ipl = 0.0;
+    double zf0 = 0.0, zf1 = 0.0, zf2 = 0.0, zsl = 0.0;
+    for (int cpos = f0 - 1;; ++cpos) {
+        // the next cell's window entry and velocity, loaded ahead
+        double vn = 0.0, un = 0.0;
+        if (cpos < f1) {
+            vn = up[(G + 1) * st];
+            un = fp[fst];
+        }
+        double sm, sp, Ip, Im;
+        if (PPM == 1) {
+            if (cpos == f0 - 1) {
+                const double d0 = vl_slope(v[2], v[3], v[4]);
+                zsl = vl_slope(v[3], v[4], v[5]);
+                zf0 = face1(v[3], v[4], d0, zsl);
+            }
+            const double d1 = vl_slope(v[4], v[5], v[6]);
+            zf1 = face1(v[4], v[5], zsl, d1);
+            zsl = d1;
+            sm = zf0;
+            sp = zf1;
+            cw_limit(v[4], sm, sp);
+        } else {
+            if (cpos == f0 - 1) {
+                zf0 = cs_face(v[1], v[2], v[3], v[4]);
+                zf1 = cs_face(v[2], v[3], v[4], v[5]);
+                zf2 = cs_face(v[3], v[4], v[5], v[6]);
+            }
+            const double f3 = cs_face(v[4], v[5], v[6], v[7]);
+            cs_cell(v, zf0, zf1, zf2, f3, 1.e-10, sm, sp);
+            zf0 = zf1;
+            zf1 = zf2;
+            zf2 = f3;
+        }
+        if (ppm_special<PPM>(cpos, dlo, dhi, bclo, bchi))
+            ppm_edges<PPM, D>(v, cpos, dlo, dhi, bclo, bchi, sm, sp);
+        if (PPM == 1) zf0 = zf1;
+        ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[D], g.rel_eps, Ip, Im);
+        if (cpos >= f0) utrans_riemann<D>(B, g, i, D == 1 ? cpos : tt, D == 2 ? cpos : tt, Ipl, Im);
+        if (cpos == f1) break;
+        Ipl = Ip;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = v[q + 1];
+        v[4 + G] = vn;
+        uc = un;
+        up += st;
+        fp += fst;
+    }
+}
+
+template <int PPM>
+__global__ void __launch_bounds__(UT_NT) k_utrans_line(const UtBox* __restrict__ boxes, KGeom g,
+                                                      UtArgs A, int nch) {
+    const UtBox& B = boxes[blockIdx.y];
+    if (blockIdx.z == 0)
+        utrans_rows<PPM>(B, g, A);
+    else if (blockIdx.z == 1)
+        utrans_march<PPM, 1>(B, g, A, nch);
+    else
+        utrans_march<PPM, 2>(B, g, A, nch);
+}
+
 // ---------------------------------------------------------------------------
 // VelPred
 
@@ -700,6 +836,21 @@ int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mult
         for (int d = 0; d < dim; ++d) maxpts = std::max(maxpts, npts(nodal(g, d)));
         boxes.push_back(B);
     }
+    // the line form: 3-D, ppm_type 1 and 2; blocks for the larger of the
+    // x-face rows and the y/z column chunks of any box
+    const int nch = c->utrans_chunks;
+    const bool line = c->utrans_line && dim == 3 && c->p.ppm_type != 0;
+    long long lblocks = 1;
+    if (line)
+        for (const UtBox& B : boxes) {
+            const long long n0 = B.hi[0] - B.lo[0] + 1, n1 = B.hi[1] - B.lo[1] + 1,
+                            n2 = B.hi[2] - B.lo[2] + 1;
+            const long long rows = (n0 + 2) * n1 * n2;
+            if (rows >= 0x7fffffffLL || n0 * std::max(n1, n2) * nch >= 0x7fffffffLL)
+                return MX_ERR_UNSUPPORTED;
+            lblocks = std::max(lblocks, (rows - 1 + UT_NT - 2) / (UT_NT - 1));
+            lblocks = std::max(lblocks, (n0 * std::max(n1, n2) * nch + UT_NT - 1) / UT_NT);
+        }
     UtArgs A;
     for (int n = 0; n < 3; ++n)
         for (int d = 0; d < 3; ++d) {
@@ -714,7 +865,12 @@ int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mult
     const int block = 256;
     const UtBox* dp = tab.dev<UtBox>(off);
     dim3 gr = grid_for(maxpts, (int)boxes.size(), dim, block);
-    if (c->p.ppm_type == 0)
+    const dim3 gl((unsigned)lblocks, (unsigned)boxes.size(), 3);
+    if (line && c->p.ppm_type == 1)
+        MX_LAUNCH(c, "k_utrans", k_utrans_line<1>, gl, dim3(UT_NT), dp, kg, A, nch);
+    else if (line)
+        MX_LAUNCH(c, "k_utrans", k_utrans_line<2>, gl, dim3(UT_NT), dp, kg, A, nch);
+    else if (c->p.ppm_type == 0)
         MX_LAUNCH(c, "k_utrans", k_utrans<0>,  gr, dim3(block), dp, kg, A);
     else if (c->p.ppm_type == 1)
         MX_LAUNCH(c, "k_utrans", k_utrans<1>,  gr, dim3(block), dp, kg, A);

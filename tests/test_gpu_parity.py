@@ -171,6 +171,31 @@ def test_pass_fused_xcd_order(name, cfg, es_mode, vp_mode):
 
 VP_ALL_CASES = [CASES[0], CASES[1], CASES[7], CASES[9], CASES[10], CASES[16], CASES[8]]
 
+UT_CASES = [CASES[0], CASES[1], CASES[6], CASES[7], CASES[9], CASES[10], CASES[17],
+            KERNEL_CHOICE_CASES[3]]
+
+
+@pytest.mark.parametrize("line,chunks", [(0, 4), (1, 1), (1, 3), (1, 4), (1, 7)])
+@pytest.mark.parametrize("name,cfg", UT_CASES, ids=[c[0] for c in UT_CASES])
+def test_pass_utrans_forms(name, cfg, line, chunks):
+    """MakeUtrans one face per thread (each cell traced for both of its
+    faces) and in the line form (x-face rows through LDS, y/z columns
+    marched in 1, 3, 4 and 7 chunks, so chunks of one face, chunk ends
+    inside the box and empty chunks of small boxes all occur): utrans and
+    the whole pass bit-exact, walls, inflow, spherical w0mac and uneven boxes
+    included."""
+    print(f"[parity] utrans line={line} chunks={chunks} {name}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    m = gpu_for(wg)
+    m.set_tuning("utrans_line", line)
+    m.set_tuning("utrans_chunks", chunks)
+    _run_pass(m, wg)
+    assert_bitexact(compare_face(wc.utrans, wg.utrans, region="all"), f"{name} utrans")
+    assert_bitexact(compare_face(wc.umac, wg.umac, region="all"), f"{name} umac")
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} sedge")
+
 
 @pytest.mark.parametrize("name,cfg", VP_ALL_CASES, ids=[c[0] for c in VP_ALL_CASES])
 def test_pass_velpred_ppm_per_component(name, cfg):
