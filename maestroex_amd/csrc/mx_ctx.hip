@@ -513,14 +513,17 @@ int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge, bool
     const int dim = c->p.dim;
     const auto& grids = c->grids[lev];
     const int block = 256;
-    // FillBoundary per direction
+    // FillBoundary: every direction's ghost faces are filled from that
+    // direction's valid faces, so the slabs of all directions go in one
+    // launch
+    std::vector<SlabDesc> slabs;
+    std::vector<CandDesc> cands;
+    long long maxpts = 0;
     for (int d = 0; d < dim; ++d) {
         const mx_multifab& m = uedge->dir[d];
         const auto& plan = ghost_plan(c, lev, d, m);
         if (plan.slabs.empty()) continue;
-        std::vector<SlabDesc> slabs;
-        std::vector<CandDesc> cands;
-        long long maxpts = 0;
+        const int cbase = (int)cands.size();
         for (const auto& q : plan.cands) {
             CandDesc cd;
             cd.src = to_da(m.fab[q.src_box]);
@@ -540,7 +543,7 @@ int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge, bool
                 sd.lo[q] = r.lo[q] = sl.lo[q];
                 sd.hi[q] = r.hi[q] = sl.hi[q];
             }
-            sd.cand_off = sl.cand_off;
+            sd.cand_off = cbase + sl.cand_off;
             sd.cand_n = sl.cand_n;
             sd.ncomp = m.fab[sl.box].ncomp;
             HBox vb = nodal(hbox(grids[sl.box]), d);
@@ -559,14 +562,21 @@ int fill_face_ghosts(mx_adv_ctx* c, int lev, const mx_face_multifab* uedge, bool
             maxpts = std::max(maxpts, npts(r));
             slabs.push_back(sd);
         }
+    }
+    if (!slabs.empty()) {
         Table tab(c);
         size_t os = tab.add(slabs);
         size_t oc = tab.add(cands);
         int rc = tab.upload();
         if (rc) return rc;
-        MX_LAUNCH(c, "k_fill_ghost_faces", k_fill_ghost_faces,
-                  grid_for(maxpts, (int)slabs.size(), 1, block), dim3(block),
-                  tab.dev<SlabDesc>(os), tab.dev<CandDesc>(oc));
+        // grid y takes the slabs, at most 65535 per launch
+        const SlabDesc* ds = tab.dev<SlabDesc>(os);
+        for (size_t s0 = 0; s0 < slabs.size(); s0 += 65535) {
+            const int ns = (int)std::min<size_t>(65535, slabs.size() - s0);
+            MX_LAUNCH(c, "k_fill_ghost_faces", k_fill_ghost_faces,
+                      grid_for(maxpts, ns, 1, block), dim3(block), ds + s0,
+                      tab.dev<CandDesc>(oc));
+        }
     }
     // FillUmacGhost: x, then y, then z (each reads what the previous wrote)
     for (int d = 0; d < dim; ++d) {

@@ -171,6 +171,25 @@ def test_pass_fused_xcd_order(name, cfg, es_mode, vp_mode):
 
 VP_ALL_CASES = [CASES[0], CASES[1], CASES[7], CASES[9], CASES[10], CASES[16], CASES[8]]
 
+def test_set_tuning_rejects_bad_values():
+    """A tuning value outside a key's range, and an unknown key, are
+    MX_ERR_ARG and change nothing; the range ends are accepted."""
+    from maestroex_amd import _abi
+    print("[parity] set_tuning ranges", flush=True)
+    _, wg = pair(**CASES[0][1])
+    m = gpu_for(wg)
+    bad = [("utrans_line", 2), ("utrans_line", -1), ("utrans_chunks", 0), ("utrans_chunks", 65),
+           ("velpred_fused_tile", -1), ("velpred_fused_tile", 6), ("velpred_fused_xcd", 3),
+           ("edge_scal_xcd", -1), ("edge_scal_fused", 2), ("no_such_key", 1)]
+    for key, val in bad:
+        with pytest.raises(_abi.MxError) as e:
+            m.set_tuning(key, val)
+        assert e.value.code == _abi.MX_ERR_ARG, key
+    for key, val in [("utrans_chunks", 1), ("utrans_chunks", 64), ("velpred_fused_tile", 0),
+                     ("velpred_fused_tile", 5), ("utrans_line", 0), ("velpred_fused_xcd", 2)]:
+        m.set_tuning(key, val)
+
+
 UT_CASES = [CASES[0], CASES[1], CASES[6], CASES[7], CASES[9], CASES[10], CASES[17],
             KERNEL_CHOICE_CASES[3]]
 
