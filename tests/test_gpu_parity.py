@@ -123,6 +123,28 @@ def test_pass_enthalpy_pred_types(ept, spt, epilogue):
     assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"ept {ept} eta")
 
 
+SPH_EPT_CASES = [CASES[14], CASES[16], CASES[17]]
+
+
+@pytest.mark.parametrize("ept", [0, 1, 2, 3, 5])
+@pytest.mark.parametrize("name,cfg", SPH_EPT_CASES, ids=[c[0] for c in SPH_EPT_CASES])
+def test_pass_enthalpy_pred_types_spherical(name, cfg, ept):
+    """Spherical passes through MakeRhoHFlux's spherical branches with the
+    face base states (umac (rho0_edge h0_edge + sedge), umac sedge, umac
+    rho_edge h_edge, umac (rho'_edge + rho0_edge)(h_edge + h0_edge)) and,
+    with the exact base state, its branches that leave the x-faces as they
+    are: every output bit-exact, uneven boxes included."""
+    cfg = dict(cfg, enthalpy_pred_type=ept)
+    print(f"[parity] spherical pass {name} ept={ept}", flush=True)
+    wc, wg = pair(**cfg)
+    _run_pass(oracle_for(wc), wc)
+    _run_pass(gpu_for(wg), wg)
+    comps = [0, 1] + list(range(4, wc.nscal))
+    assert_bitexact(compare_face(wc.sedge, wg.sedge, comps=comps), f"{name} ept {ept} sedge")
+    assert_bitexact(compare_face(wc.sflux, wg.sflux, comps=comps), f"{name} ept {ept} sflux")
+    assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} eta")
+
+
 XCD_CASES = [CASES[0], CASES[3], CASES[7], CASES[10], CASES[16], CASES[17]]
 
 
