@@ -1115,7 +1115,7 @@ def main():
     fields = w.pass_fields()
     tuned = None
     # the library's defaults
-    shape = {"tile": "16x11" if args.ppm == 1 else "16x12", "tz": 32}
+    shape = {"tile": "16x11", "tz": 32}
     if args.autotune and not args.no_autotune:
         # the autotuner picks the fused edge kernel's tile and z-chunk on
         # this workload (untimed, before the warm-up)

@@ -197,7 +197,7 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
      "edge_scal_tile"   cell columns per block of the fused kernel, and the
                         block size: 0: 32 x 8 (384 threads), 1: 16 x 16 (384),
                         2: 16 x 12 (256), 3: 16 x 8 (192), 4: 16 x 11 (256);
-                        -1 (default): 16 x 11 for ppm_type 1, else 16 x 12.
+                        -1 (default): 16 x 11.
      "utrans_line"      1 (default): 3-D MakeUtrans with ppm_type 1/2 traces
                         each cell once (x-faces from rows of cells through
                         LDS, y/z-faces by columns marching along y or z);

@@ -922,13 +922,14 @@ int launch_fused(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int 
 //   0: 32 x 8 (384 threads), 1: 16 x 16 (384), 2: 16 x 12 (256), 3: 16 x 8 (192),
 //   4: 16 x 11 (256)
 // all at 3 waves per SIMD (168 VGPRs), so a CU holds 2, 2, 3, 4 or 3 blocks.
-// -1 (the default) is 16 x 11 for ppm_type 1 and 16 x 12 otherwise: with
-// ppm_type 1 the 16 x 11 tile's face phases (19 x 13 x-faces, 18 x 14
-// y-faces) fit one pass of 256 threads, where 16 x 12 leaves one wave a
-// second pass while the others wait at the barrier
+// -1 (the default) is 16 x 11: with ppm_type 1 its face phases (19 x 13
+// x-faces, 18 x 14 y-faces) fit one pass of 256 threads, where 16 x 12
+// leaves one wave a second pass while the others wait at the barrier; with
+// ppm_type 2 its x/y-free variant spills 1 VGPR against 7 for 16 x 12
+// (5.50 against 5.57 ms per 256^3 pass)
 int fused_tile_of(const mx_adv_ctx* c) {
     if (c->fused_tile >= 0) return c->fused_tile;
-    return c->p.ppm_type == 1 ? 4 : 2;
+    return 4;
 }
 int launch_tile(mx_adv_ctx* c, const std::vector<EdgeLevel>& levs, int tz, int is_vel,
                 const mx_bcrec* bcs, const std::vector<int>& sc, const std::vector<int>& bc,
