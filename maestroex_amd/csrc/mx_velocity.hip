@@ -177,13 +177,13 @@ __global__ void __launch_bounds__(256) k_utrans(const UtBox* __restrict__ boxes,
 // face takes Ip of its low cell and Im of its high cell, the same
 // ppm_edges / ppm_trace evaluations that utrans_face makes per face, so the
 // results are bit-identical.
-//   blockIdx.z = 0, x-faces: a block's threads take consecutive cells of the
-//     box's rows, each row with its cells lo - 1 .. hi + 1 (nx + 2 cells,
-//     flattened; lanes consecutive in x).  Each thread traces its cell and
-//     keeps Ip in LDS; the face at the low side of a thread's cell takes the
-//     previous thread's Ip.  Block b starts at cell b * (UT_NT - 1), so the
-//     last cell of one block is the first of the next, whose face the
-//     earlier block forms.
+//   blockIdx.z = 0, x-faces: a wavefront's 64 lanes take consecutive cells
+//     of the box's rows, each row with its cells lo - 1 .. hi + 1 (nx + 2
+//     cells, flattened; lanes consecutive in x).  Each lane traces its cell,
+//     and the face at the low side of a lane's cell takes the previous
+//     lane's Ip by a wavefront shuffle (no LDS, no barrier).  Wavefront w
+//     starts at cell 63 w, so its last cell is the next one's first, whose
+//     face the earlier wavefront forms.
 //   blockIdx.z = 1, 2: each thread takes one x column (lanes consecutive in
 //     x, so loads and stores coalesce) and one of nch chunks of the faces
 //     lo .. hi + 1 along y or z, marching a register window along its column
@@ -195,12 +195,13 @@ constexpr int UT_NT = 256;
 
 template <int PPM>
 __device__ __forceinline__ void utrans_rows(const UtBox& B, const KGeom& g, const UtArgs& A) {
-    __shared__ double sIp[UT_NT];
     const unsigned nx = (unsigned)(B.hi[0] - B.lo[0] + 1);
     const unsigned ny = (unsigned)(B.hi[1] - B.lo[1] + 1);
     const unsigned nz = (unsigned)(B.hi[2] - B.lo[2] + 1);
     const unsigned rw = nx + 2;
-    const unsigned t = blockIdx.x * (unsigned)(UT_NT - 1) + threadIdx.x;
+    const unsigned lane = threadIdx.x & 63u;
+    const unsigned wave = (blockIdx.x * (unsigned)UT_NT + threadIdx.x) >> 6;
+    const unsigned t = wave * 63u + lane;
     const bool on = t < rw * ny * nz;
     unsigned c = 0;
     int i = 0, j = 0, k = 0;
@@ -217,9 +218,9 @@ __device__ __forceinline__ void utrans_rows(const UtBox& B, const KGeom& g, cons
         const double uc = B.uf(i, j, k, 0);
         ppm_trace(v[4], sm, sp, uc, uc, g.dt, g.dx[0], g.rel_eps, Ip, Im);
     }
-    sIp[threadIdx.x] = Ip;
-    __syncthreads();
-    if (on && threadIdx.x > 0 && c > 0) utrans_riemann<0>(B, g, i, j, k, sIp[threadIdx.x - 1], Im);
+    // every lane takes part in the shuffle; lane 0's result is unused
+    const double Ipl = __shfl_up(Ip, 1u, 64);
+    if (on && lane > 0 && c > 0) utrans_riemann<0>(B, g, i, j, k, Ipl, Im);
 }
 
 template <int PPM, int D>
@@ -848,7 +849,9 @@ int make_utrans(mx_adv_ctx* c, int lev, const mx_multifab* utilde, const mx_mult
             const long long rows = (n0 + 2) * n1 * n2;
             if (rows >= 0x7fffffffLL || n0 * std::max(n1, n2) * nch >= 0x7fffffffLL)
                 return MX_ERR_UNSUPPORTED;
-            lblocks = std::max(lblocks, (rows - 1 + UT_NT - 2) / (UT_NT - 1));
+            // wavefronts of 63 new cells each, 4 per block
+            const long long waves = (rows - 1 + 62) / 63;
+            lblocks = std::max(lblocks, (waves + UT_NT / 64 - 1) / (UT_NT / 64));
             lblocks = std::max(lblocks, (n0 * std::max(n1, n2) * nch + UT_NT - 1) / UT_NT);
         }
     UtArgs A;
