@@ -232,12 +232,14 @@ int mx_adv_prof_read(mx_adv_ctx* ctx, int idx, const char** name, double* total_
                         box's blocks; 0 (default): blocks in launch order.
      "edge_scal_xcd", "velpred_fused_xcd"  block order of the fused edge
                         kernel and of the per-component fused VelPred: 1
-                        (default): each XCD runs the component blocks of a
-                        tile back to back (they share the velocity planes
-                        they stage in that XCD's L2), tiles round robin over
-                        the XCDs; 2: the same with a contiguous eighth of
-                        the tiles per XCD; 0: tile x component in launch
-                        order.
+                        (the edge kernel's default): each XCD runs the
+                        component blocks of a tile back to back (they share
+                        the velocity planes they stage in that XCD's L2),
+                        tiles round robin over the XCDs; 2: the same with a
+                        contiguous eighth of the tiles per XCD; 0: tile x
+                        component in launch order; -1 (VelPred's default,
+                        VelPred only): 2 for one level of equal boxes, else
+                        1.
      "adjoint_face_sums", "adjoint_state_fused", "adjoint_pow2_index",
      "adjoint_trace_rolled", "adjoint_xcd"  arrangements of the step
                         adjoint's kernels (defaults 0, 1, 1, 0, 0); every

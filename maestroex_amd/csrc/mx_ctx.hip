@@ -898,7 +898,7 @@ int mx_adv_set_tuning(mx_adv_ctx* c, const char* key, long long value) {
         if (value != 0 && value != 1) return MX_ERR_ARG;
         c->adj_face_sums = (int)value;
     } else if (k == "velpred_fused_xcd") {
-        if (value < 0 || value > 2) return MX_ERR_ARG;
+        if (value < -1 || value > 2) return MX_ERR_ARG;
         c->vp_fused_xcd = (int)value;
     } else if (k == "edge_scal_xcd") {
         if (value < 0 || value > 2) return MX_ERR_ARG;

@@ -117,7 +117,8 @@ struct mx_adv_ctx {
     // kernel selection (mx_adv_set_tuning)
     int fused_edge = 1;
     int fused_tz = 32;
-    int vp_fused_xcd = 1;  // fused VelPred: XCD-aware block order (xcd_tile mode)
+    int vp_fused_xcd = -1; // fused VelPred: XCD-aware block order (xcd_tile mode); -1: 2 for
+                           // one level of equal boxes, else 1
     int es_fused_xcd = 1;  // fused MakeEdgeScal: XCD-aware block order (xcd_tile mode)
     int xcd_remap = 0;  // KGeom::xcd of the staged pass kernels
     int vp_ppm_all = 1;  // VelPred's PPM kernel: 1 = all components per thread

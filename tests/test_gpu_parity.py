@@ -167,11 +167,12 @@ def test_pass_xcd_remap(name, cfg):
     assert_bitexact(compare_mf(wc.etarhoflux, wg.etarhoflux, region="all"), f"{name} etarhoflux")
 
 
-@pytest.mark.parametrize("es_mode,vp_mode", [(0, 0), (2, 2)])
+@pytest.mark.parametrize("es_mode,vp_mode", [(0, 0), (2, 2), (1, 1)])
 @pytest.mark.parametrize("name,cfg", XCD_CASES, ids=[c[0] for c in XCD_CASES])
 def test_pass_fused_xcd_order(name, cfg, es_mode, vp_mode):
-    """The block orders of the fused MakeEdgeScal and VelPred other than
-    the default (1, which every other pass test runs):
+    """The block orders of the fused MakeEdgeScal and VelPred set
+    explicitly (the other pass tests run the defaults: 1 for the edge
+    kernel; for VelPred 2 on one level of equal boxes, else 1):
     0 the plain order (tile blockIdx.x, component blockIdx.z), 1 tiles round
     robin over the XCDs and 2 contiguous tile ranges per XCD, each with the
     component blocks of a tile back to back (tile counts that are not
@@ -201,14 +202,15 @@ def test_set_tuning_rejects_bad_values():
     _, wg = pair(**CASES[0][1])
     m = gpu_for(wg)
     bad = [("utrans_line", 2), ("utrans_line", -1), ("utrans_chunks", 0), ("utrans_chunks", 65),
-           ("velpred_fused_tile", -1), ("velpred_fused_tile", 6), ("velpred_fused_xcd", 3),
+           ("velpred_fused_tile", -1), ("velpred_fused_tile", 6), ("velpred_fused_xcd", 3), ("velpred_fused_xcd", -2),
            ("edge_scal_xcd", -1), ("edge_scal_fused", 2), ("no_such_key", 1)]
     for key, val in bad:
         with pytest.raises(_abi.MxError) as e:
             m.set_tuning(key, val)
         assert e.value.code == _abi.MX_ERR_ARG, key
     for key, val in [("utrans_chunks", 1), ("utrans_chunks", 64), ("velpred_fused_tile", 0),
-                     ("velpred_fused_tile", 5), ("utrans_line", 0), ("velpred_fused_xcd", 2)]:
+                     ("velpred_fused_tile", 5), ("utrans_line", 0), ("velpred_fused_xcd", 2),
+                     ("velpred_fused_xcd", -1)]:
         m.set_tuning(key, val)
 
 
